@@ -1,0 +1,124 @@
+/*
+ * slq.h -- C ABI of the MI355X (gfx950) stochastic Lanczos quadrature library.
+ *
+ * The library replaces the device side of matfree's SLQ log-determinant path
+ * (Rademacher probes -> decomp.tridiag_sym(reortho="full") ->
+ * funm.monte_carlo_funm_sym_logdet -> stochtrace.estimator_monte_carlo).
+ * In matfree that path is pure JAX; the functions below are what a JAX FFI
+ * custom call (or any other host) binds in its place.
+ *
+ * Conventions
+ *  - Every array argument is a DEVICE pointer on the context's device,
+ *    caller-owned, row-major, float32 unless stated otherwise.
+ *  - Every call is asynchronous and ordered on the context's stream.
+ *  - The return value is an slq_status; on failure slq_last_error() gives a
+ *    message.  Argument errors are detected on the host before any launch.
+ *  - Scratch memory (partial sums, work vectors, the Krylov basis of
+ *    slq_logdet) is owned and pooled by the context.  A context is used by
+ *    one host thread at a time.
+ *  - Layouts: a batch of P vectors of length n is P x n (vector p at
+ *    p * n).  A Krylov basis is k x P x n (vector j of probe p at
+ *    (j * P + p) * n).  A Hessenberg matrix batch is P x k x k stored column
+ *    by column (H[p][col][row] at (p * k + col) * k + row), the order in
+ *    which the Arnoldi iteration writes it.
+ */
+#ifndef SLQ_H_
+#define SLQ_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum slq_status {
+  SLQ_OK = 0,
+  SLQ_EINVAL = -1,       /* bad argument (null pointer, bad enum, bad size)      */
+  SLQ_EINVAL_K = -2,     /* num_matvecs outside [0, n]: matfree's "exceeds" error */
+  SLQ_ENOMEM = -3,       /* device allocation failed                              */
+  SLQ_EHIP = -4,         /* HIP runtime error                                     */
+  SLQ_EUNSUPPORTED = -5, /* valid request this build does not implement           */
+  SLQ_EADJOINT_K0 = -6   /* adjoint requested for num_matvecs = 0                 */
+} slq_status;
+
+typedef enum slq_operator_kind {
+  SLQ_OP_DENSE = 0, /* stored symmetric matrix A (n x n): v -> A v             */
+  SLQ_OP_RBF = 1    /* K_ij = exp(-|x_i - x_j|^2 / (2 l^2)) + s2 delta_ij      */
+} slq_operator_kind;
+
+typedef enum slq_precision {
+  SLQ_PREC_FP32 = 0,   /* FP32 MFMA, FP32 operands                                */
+  SLQ_PREC_BF16 = 1,   /* one BF16 MFMA product, FP32 accumulation                 */
+  SLQ_PREC_BF16X3 = 2  /* split hi+lo BF16: hi*hi + hi*lo + lo*hi, FP32 accumulation */
+} slq_precision;
+
+/* Operator descriptor: the reference's matvec(v, *params) plugin
+ * (decomp.py:460, user code such as tests/test_decomp/test_tridiag_sym.py:16-18
+ * and tutorials/8_gaussian_logpdf.py:21-31), restricted to the two operator
+ * families this library runs on the GPU. */
+typedef struct slq_operator {
+  int32_t kind;        /* slq_operator_kind                                   */
+  int32_t precision;   /* slq_precision of the block matvec                    */
+  int64_t n;           /* dimension                                            */
+  const float* matrix; /* DENSE: n x n symmetric, row-major                    */
+  const float* points; /* RBF: n x dim points                                  */
+  int32_t dim;         /* RBF: point dimension, 1..4                           */
+  int32_t reserved;
+  double lengthscale;  /* RBF: l > 0                                          */
+  double noise;        /* RBF: s2 (added on the diagonal in FP32)             */
+} slq_operator;
+
+/* Context (device, stream, pooled scratch). stream is a hipStream_t or NULL. */
+typedef struct slq_context slq_context;
+int slq_context_create(int device, void* stream, slq_context** out);
+int slq_context_destroy(slq_context* ctx);
+int slq_context_set_stream(slq_context* ctx, void* stream);
+const char* slq_last_error(const slq_context* ctx);
+/* Bytes of device scratch the context currently holds. */
+int64_t slq_context_scratch_bytes(const slq_context* ctx);
+
+/* Rademacher probes, bit-identical to jax.random.rademacher(key, (num, n),
+ * float32) with the partitionable threefry2x32 layout.
+ * Replaces stochtrace.sampler_signs -> _sampler_from_jax_random ->
+ * _uniform_signs -> prng.rademacher (stochtrace.py:932-937, 957-964, 974-977;
+ * backend/prng.py:26-29).  key = {key[0], key[1]} as in jax.random.PRNGKey
+ * (backend/prng.py:6-7).  out: num x n. */
+int slq_sampler_signs(slq_context* ctx, uint32_t key0, uint32_t key1, int64_t num, int64_t n,
+                      float* out);
+
+/* Block matvec W = A V over P vectors (rows of V): the reference's matvec
+ * (decomp.py:460) batched by vmap (stochtrace.py:49).  V, W: P x n. */
+int slq_matvec(slq_context* ctx, const slq_operator* op, const float* V, int64_t P, float* W);
+
+/* Arnoldi with full reorthogonalisation (classical Gram-Schmidt twice) for P
+ * start vectors: decomp.hessenberg(k, reortho="full") forward pass,
+ * decomp.py:426-477.
+ *   V0: P x n start vectors (normalised internally, as decomp.py:435, 456)
+ *   Q:  k x P x n Krylov basis (output)
+ *   H:  P x k x k Hessenberg matrices, column-major per probe (output)
+ *   R:  P x n unnormalised residuals (output)
+ *   c:  P values 1 / |v0| (output; decomp.py:450)
+ * Fails with SLQ_EINVAL_K unless 0 <= k <= n (decomp.py:427-429). */
+int slq_hessenberg(slq_context* ctx, const slq_operator* op, const float* V0, int64_t P, int64_t k,
+                   float* Q, float* H, float* R, float* c);
+
+/* SLQ log-determinant integrand and estimator for P probes z_p:
+ *   per_probe[p] = |z_p|^2 [log T_p]_00 with T_p = (H_p + H_p^T)/2 from
+ *   slq_hessenberg on z_p / |z_p|, evaluated in FP64 (eigenvalues and first
+ *   eigenvector components of the tridiagonal T_p);
+ *   mean = (1/P) sum_p per_probe[p].
+ * Replaces funm.monte_carlo_funm_sym_logdet -> monte_carlo_funm_sym.quadform
+ * (funm.py:186-243), dense_funm_sym_eigh(log) (funm.py:322-335) and
+ * stochtrace.estimator_monte_carlo (stochtrace.py:47-50).
+ *   Z: P x n probes; per_probe: P doubles; mean: 1 double (either may be
+ *   NULL); alpha: P x k and beta: P x (k-1) tridiagonal coefficients (may be
+ *   NULL).  The Krylov basis lives in context scratch (k x P x n floats). */
+int slq_logdet(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P, int64_t k,
+               double* per_probe, double* mean, float* alpha, float* beta);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SLQ_H_ */

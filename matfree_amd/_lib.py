@@ -1,0 +1,159 @@
+"""ctypes binding of the C ABI declared in ``include/slq.h``.
+
+The shared library ``libslq.so`` is built in-tree by ``matfree_amd._build``.
+There is no fallback: if the library is missing, every GPU entry point raises.
+"""
+
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libslq.so")
+
+# status codes (slq_status in include/slq.h)
+SLQ_OK = 0
+SLQ_EINVAL = -1
+SLQ_EINVAL_K = -2
+SLQ_ENOMEM = -3
+SLQ_EHIP = -4
+SLQ_EUNSUPPORTED = -5
+SLQ_EADJOINT_K0 = -6
+
+# operator kinds / precisions
+SLQ_OP_DENSE = 0
+SLQ_OP_RBF = 1
+PRECISIONS = {"fp32": 0, "bf16": 1, "bf16x3": 2}
+
+c_void_p = ctypes.c_void_p
+c_i64 = ctypes.c_int64
+c_int = ctypes.c_int
+c_u32 = ctypes.c_uint32
+
+
+class OperatorStruct(ctypes.Structure):
+    """Mirror of ``slq_operator``."""
+
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("n", ctypes.c_int64),
+        ("matrix", c_void_p),
+        ("points", c_void_p),
+        ("dim", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("lengthscale", ctypes.c_double),
+        ("noise", ctypes.c_double),
+    ]
+
+
+# name -> (restype, argtypes); every function declared in include/slq.h
+SIGNATURES = {
+    "slq_context_create": (c_int, [c_int, c_void_p, ctypes.POINTER(c_void_p)]),
+    "slq_context_destroy": (c_int, [c_void_p]),
+    "slq_context_set_stream": (c_int, [c_void_p, c_void_p]),
+    "slq_last_error": (ctypes.c_char_p, [c_void_p]),
+    "slq_context_scratch_bytes": (c_i64, [c_void_p]),
+    "slq_sampler_signs": (c_int, [c_void_p, c_u32, c_u32, c_i64, c_i64, c_void_p]),
+    "slq_matvec": (c_int, [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_void_p]),
+    "slq_hessenberg": (
+        c_int,
+        [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_i64, c_void_p, c_void_p,
+         c_void_p, c_void_p],
+    ),
+    "slq_logdet": (
+        c_int,
+        [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_i64, c_void_p, c_void_p,
+         c_void_p, c_void_p],
+    ),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """Load libslq.so (raises ImportError with the build command if missing)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise ImportError(
+                    f"{LIB_PATH} is not built; run `python -m matfree_amd._build` "
+                    "(matfree_amd has no CPU fallback)"
+                )
+            lib = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+class SLQError(RuntimeError):
+    """A HIP runtime failure inside libslq."""
+
+
+def raise_for(rc, ctx_ptr):
+    """Translate an slq_status into the reference's exception classes."""
+    if rc == SLQ_OK:
+        return
+    msg = load().slq_last_error(ctx_ptr) if ctx_ptr else b""
+    msg = (msg or b"").decode() or f"libslq status {rc}"
+    if rc in (SLQ_EINVAL, SLQ_EINVAL_K, SLQ_EADJOINT_K0):
+        raise ValueError(msg)
+    if rc == SLQ_EUNSUPPORTED:
+        raise NotImplementedError(msg)
+    if rc == SLQ_ENOMEM:
+        raise MemoryError(msg)
+    raise SLQError(msg)
+
+
+class Context:
+    """One ``slq_context`` per (device, thread)."""
+
+    def __init__(self, device):
+        lib = load()
+        ptr = c_void_p()
+        rc = lib.slq_context_create(int(device), None, ctypes.byref(ptr))
+        if rc != SLQ_OK:
+            raise SLQError(f"slq_context_create(device={device}) failed with status {rc}")
+        self.ptr = ptr
+        self.device = int(device)
+
+    def set_stream(self, stream_handle):
+        raise_for(load().slq_context_set_stream(self.ptr, c_void_p(stream_handle)), self.ptr)
+
+    def call(self, name, *args):
+        rc = getattr(load(), name)(self.ptr, *args)
+        raise_for(rc, self.ptr)
+
+    def scratch_bytes(self):
+        return int(load().slq_context_scratch_bytes(self.ptr))
+
+    def close(self):
+        if self.ptr:
+            load().slq_context_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_contexts = threading.local()
+
+
+def context(device, stream_handle):
+    """The calling thread's context for ``device``, bound to ``stream_handle``."""
+    table = getattr(_contexts, "table", None)
+    if table is None:
+        table = _contexts.table = {}
+    ctx = table.get(device)
+    if ctx is None:
+        ctx = table[device] = Context(device)
+    ctx.set_stream(stream_handle)
+    return ctx

@@ -1,0 +1,401 @@
+// Streaming kernels of the Arnoldi step with full reorthogonalisation
+// (classical Gram-Schmidt applied twice), decomp.py:454-477:
+//
+//   v /= length; Q[:, j] = v; w = A v                    (normalize + matvec)
+//   h = Q^T w; w = w - Q h                               (pass 1: dots, pass 2: project)
+//   w = w - Q (Q^T w)                                     (pass 2: dots, pass 3: project)
+//   length = sqrt(<w, w>); h[j + 1] = length; H[:, j] = h
+//
+// Only the j + 1 live basis columns are read (the reference multiplies by the
+// full-width zero-padded Q; the extra products are exact zeros).  All kernels
+// are HBM-bound; a block owns one chunk of kChunk elements of one probe's
+// vector, and cross-block sums go through per-chunk partials that the consumer
+// reduces in a fixed order (deterministic results).
+//
+// Basis layout: Q[c][p][i] at (c * P + p) * n + i.
+#include "slq_internal.h"
+
+namespace slq {
+namespace {
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int V>
+struct VecIO;
+
+template <>
+struct VecIO<4> {
+  __device__ static __forceinline__ void load(const float* p, float (&r)[4]) {
+    const float4 t = *reinterpret_cast<const float4*>(p);
+    r[0] = t.x; r[1] = t.y; r[2] = t.z; r[3] = t.w;
+  }
+  __device__ static __forceinline__ void store(float* p, const float (&r)[4]) {
+    *reinterpret_cast<float4*>(p) = make_float4(r[0], r[1], r[2], r[3]);
+  }
+};
+
+template <>
+struct VecIO<1> {
+  __device__ static __forceinline__ void load(const float* p, float (&r)[1]) { r[0] = *p; }
+  __device__ static __forceinline__ void store(float* p, const float (&r)[1]) { *p = r[0]; }
+};
+
+// Elements handled by one thread: NE groups of V contiguous floats; group e
+// starts at chunk_base + (e * kStreamThreads + t) * V.
+template <int V>
+struct Layout {
+  static constexpr int NE = int(kChunk / (kStreamThreads * V));
+};
+
+// Block-wide sum of one value per thread; every thread gets the result.
+__device__ __forceinline__ float block_sum(float v, float* red) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  float s = 0.f;
+#pragma unroll
+  for (int w = 0; w < kStreamThreads / 64; ++w) s += red[w];
+  __syncthreads();
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// dots: partial[p][b][c] = sum over chunk b of Q[c][p][:] . x[p][:]
+// ---------------------------------------------------------------------------
+template <int V>
+__global__ void __launch_bounds__(kStreamThreads)
+    k_dots(const float* __restrict__ Q, const float* __restrict__ x, int64_t P, int64_t n,
+           int ncols, float* __restrict__ partial, int pstride) {
+  constexpr int NE = Layout<V>::NE;
+  constexpr int CB = 16;
+  __shared__ float red[kStreamThreads / 64][CB];
+  const int64_t p = blockIdx.y;
+  const int64_t b = blockIdx.x;
+  const int64_t nb = gridDim.x;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int64_t base = b * kChunk;
+
+  float xv[NE][V];
+#pragma unroll
+  for (int e = 0; e < NE; ++e) {
+    const int64_t i = base + (int64_t(e) * kStreamThreads + t) * V;
+    if (i < n) {
+      VecIO<V>::load(x + p * n + i, xv[e]);
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; ++v) xv[e][v] = 0.f;
+    }
+  }
+
+  for (int c0 = 0; c0 < ncols; c0 += CB) {
+    float acc[CB];
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) {
+      acc[cb] = 0.f;
+      if (c0 + cb < ncols) {
+        const float* q = Q + (int64_t(c0 + cb) * P + p) * n;
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+          const int64_t i = base + (int64_t(e) * kStreamThreads + t) * V;
+          if (i < n) {
+            float qv[V];
+            VecIO<V>::load(q + i, qv);
+#pragma unroll
+            for (int v = 0; v < V; ++v) acc[cb] = fmaf(qv[v], xv[e][v], acc[cb]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) {
+      const float s = wave_sum(acc[cb]);
+      if (lane == 0) red[wave][cb] = s;
+    }
+    __syncthreads();
+    if (t < CB && c0 + t < ncols) {
+      float s = 0.f;
+#pragma unroll
+      for (int w = 0; w < kStreamThreads / 64; ++w) s += red[w][t];
+      partial[(p * nb + b) * pstride + c0 + t] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// sum of squares: partial[p][b] = sum over chunk b of x[p][:]^2
+template <int V>
+__global__ void __launch_bounds__(kStreamThreads)
+    k_sumsq(const float* __restrict__ x, int64_t n, float* __restrict__ partial) {
+  constexpr int NE = Layout<V>::NE;
+  __shared__ float red[kStreamThreads / 64];
+  const int64_t p = blockIdx.y, b = blockIdx.x, nb = gridDim.x;
+  const int t = threadIdx.x;
+  const int64_t base = b * kChunk;
+  float acc = 0.f;
+#pragma unroll
+  for (int e = 0; e < NE; ++e) {
+    const int64_t i = base + (int64_t(e) * kStreamThreads + t) * V;
+    if (i < n) {
+      float xv[V];
+      VecIO<V>::load(x + p * n + i, xv);
+#pragma unroll
+      for (int v = 0; v < V; ++v) acc = fmaf(xv[v], xv[v], acc);
+    }
+  }
+  const float s = block_sum(acc, red);
+  if (t == 0) partial[p * nb + b] = s;
+}
+
+// Reduce coefficient partials of probe p into LDS (fixed order over chunks).
+__device__ __forceinline__ void load_coeffs(const float* __restrict__ hpart, int64_t p,
+                                            int64_t nb, int ncols, int pstride, float* hs,
+                                            float* hout, int64_t hout_stride, bool write_out) {
+  for (int c = threadIdx.x; c < ncols; c += blockDim.x) {
+    float s = 0.f;
+    for (int64_t bb = 0; bb < nb; ++bb) s += hpart[(p * nb + bb) * pstride + c];
+    hs[c] = s;
+    if (write_out && hout) hout[p * hout_stride + c] = s;
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// project: y = x - Q h (h from partials); optional norm^2 partials of y
+// ---------------------------------------------------------------------------
+template <int V>
+__global__ void __launch_bounds__(kStreamThreads)
+    k_project(const float* __restrict__ Q, const float* __restrict__ x, float* __restrict__ y,
+              int64_t P, int64_t n, int ncols, const float* __restrict__ hpart, int pstride,
+              float* __restrict__ norm2part, float* __restrict__ hout, int64_t hout_stride) {
+  constexpr int NE = Layout<V>::NE;
+  extern __shared__ float hs[];
+  __shared__ float red[kStreamThreads / 64];
+  const int64_t p = blockIdx.y, b = blockIdx.x, nb = gridDim.x;
+  const int t = threadIdx.x;
+  const int64_t base = b * kChunk;
+  load_coeffs(hpart, p, nb, ncols, pstride, hs, hout, hout_stride, b == 0);
+
+  float s[NE][V];
+#pragma unroll
+  for (int e = 0; e < NE; ++e)
+#pragma unroll
+    for (int v = 0; v < V; ++v) s[e][v] = 0.f;
+
+  for (int c = 0; c < ncols; ++c) {
+    const float hc = hs[c];
+    const float* q = Q + (int64_t(c) * P + p) * n;
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      const int64_t i = base + (int64_t(e) * kStreamThreads + t) * V;
+      if (i < n) {
+        float qv[V];
+        VecIO<V>::load(q + i, qv);
+#pragma unroll
+        for (int v = 0; v < V; ++v) s[e][v] = fmaf(qv[v], hc, s[e][v]);
+      }
+    }
+  }
+
+  float acc = 0.f;
+#pragma unroll
+  for (int e = 0; e < NE; ++e) {
+    const int64_t i = base + (int64_t(e) * kStreamThreads + t) * V;
+    if (i < n) {
+      float xv[V];
+      VecIO<V>::load(x + p * n + i, xv);
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        xv[v] = xv[v] - s[e][v];
+        acc = fmaf(xv[v], xv[v], acc);
+      }
+      VecIO<V>::store(y + p * n + i, xv);
+    }
+  }
+  if (norm2part) {
+    const float tot = block_sum(acc, red);
+    if (t == 0) norm2part[p * nb + b] = tot;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// fused project + dots (ncols <= NC): y = x - Q h1, h2part = Q^T y.
+// One element per lane per iteration with the NC basis values in registers,
+// so Q is read once for both operations.
+// ---------------------------------------------------------------------------
+template <int NC>
+__global__ void __launch_bounds__(kStreamThreads)
+    k_project_dots(const float* __restrict__ Q, const float* __restrict__ x, float* __restrict__ y,
+                   int64_t P, int64_t n, int ncols, const float* __restrict__ h1part,
+                   float* __restrict__ h2part, int pstride, float* __restrict__ hout,
+                   int64_t hout_stride) {
+  __shared__ float hs[NC];
+  __shared__ float red[kStreamThreads / 64][NC];
+  const int64_t p = blockIdx.y, b = blockIdx.x, nb = gridDim.x;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int64_t base = b * kChunk;
+  load_coeffs(h1part, p, nb, ncols, pstride, hs, hout, hout_stride, b == 0);
+
+  float h1[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) h1[c] = (c < ncols) ? hs[c] : 0.f;
+  float acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = 0.f;
+
+  const float* qp = Q + p * n;
+  const int64_t qstride = P * n;
+  for (int64_t it = 0; it < kChunk / kStreamThreads; ++it) {
+    const int64_t i = base + it * kStreamThreads + t;
+    if (i >= n) break;
+    float q[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) q[c] = (c < ncols) ? qp[int64_t(c) * qstride + i] : 0.f;
+    float s = 0.f;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) s = fmaf(q[c], h1[c], s);
+    const float yi = x[p * n + i] - s;
+    y[p * n + i] = yi;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = fmaf(q[c], yi, acc[c]);
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    if (c < ncols) {
+      const float v = wave_sum(acc[c]);
+      if (lane == 0) red[wave][c] = v;
+    }
+  }
+  __syncthreads();
+  for (int c = t; c < ncols; c += kStreamThreads) {
+    float v = 0.f;
+#pragma unroll
+    for (int w = 0; w < kStreamThreads / 64; ++w) v += red[w][c];
+    h2part[(p * nb + b) * pstride + c] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// normalize: out = x / sqrt(sum_b norm2part), plus bookkeeping of the norm
+// ---------------------------------------------------------------------------
+constexpr int kNormVec = 4;  // float4 groups per thread
+
+template <int V>
+__global__ void __launch_bounds__(kStreamThreads)
+    k_normalize(const float* __restrict__ x, float* __restrict__ out, int64_t n, int64_t nbn,
+                const float* __restrict__ norm2part, float* __restrict__ norm_out,
+                float* __restrict__ hslot, int64_t hslot_stride, float* __restrict__ inv_out) {
+  __shared__ float nrm;
+  const int64_t p = blockIdx.y;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    float s = 0.f;
+    for (int64_t bb = 0; bb < nbn; ++bb) s += norm2part[p * nbn + bb];
+    const float len = sqrtf(s);
+    nrm = len;
+    if (blockIdx.x == 0) {
+      if (norm_out) norm_out[p] = len;
+      if (hslot) hslot[p * hslot_stride] = len;
+      if (inv_out) inv_out[p] = 1.0f / len;
+    }
+  }
+  __syncthreads();
+  const float len = nrm;
+  const int64_t per_block = int64_t(kStreamThreads) * V * kNormVec;
+  const int64_t base = int64_t(blockIdx.x) * per_block;
+#pragma unroll
+  for (int e = 0; e < kNormVec; ++e) {
+    const int64_t i = base + (int64_t(e) * kStreamThreads + t) * V;
+    if (i < n) {
+      float v[V];
+      VecIO<V>::load(x + p * n + i, v);
+#pragma unroll
+      for (int u = 0; u < V; ++u) v[u] = v[u] / len;
+      VecIO<V>::store(out + p * n + i, v);
+    }
+  }
+}
+
+inline bool aligned16(const void* ptr) { return (reinterpret_cast<uintptr_t>(ptr) & 15) == 0; }
+
+}  // namespace
+
+hipError_t launch_dots(hipStream_t s, const float* Q, const float* x, int64_t P, int64_t n,
+                       int ncols, float* partial, int pstride) {
+  const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
+  const bool vec = (n % 4 == 0) && aligned16(x) && (Q == nullptr || aligned16(Q));
+  if (Q == nullptr) {
+    if (vec)
+      hipLaunchKernelGGL(k_sumsq<4>, grid, dim3(kStreamThreads), 0, s, x, n, partial);
+    else
+      hipLaunchKernelGGL(k_sumsq<1>, grid, dim3(kStreamThreads), 0, s, x, n, partial);
+  } else {
+    if (vec)
+      hipLaunchKernelGGL(k_dots<4>, grid, dim3(kStreamThreads), 0, s, Q, x, P, n, ncols, partial,
+                         pstride);
+    else
+      hipLaunchKernelGGL(k_dots<1>, grid, dim3(kStreamThreads), 0, s, Q, x, P, n, ncols, partial,
+                         pstride);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_project(hipStream_t s, const float* Q, const float* x, float* y, int64_t P,
+                          int64_t n, int ncols, const float* hpart, int pstride, float* norm2part,
+                          float* hout, int64_t hout_stride) {
+  const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
+  const size_t shmem = size_t(ncols > 0 ? ncols : 1) * sizeof(float);
+  const bool vec = (n % 4 == 0) && aligned16(x) && aligned16(y) && aligned16(Q);
+  if (vec)
+    hipLaunchKernelGGL(k_project<4>, grid, dim3(kStreamThreads), shmem, s, Q, x, y, P, n, ncols,
+                       hpart, pstride, norm2part, hout, hout_stride);
+  else
+    hipLaunchKernelGGL(k_project<1>, grid, dim3(kStreamThreads), shmem, s, Q, x, y, P, n, ncols,
+                       hpart, pstride, norm2part, hout, hout_stride);
+  return hipGetLastError();
+}
+
+hipError_t launch_project_dots(hipStream_t s, const float* Q, const float* x, float* y, int64_t P,
+                               int64_t n, int ncols, const float* h1part, float* h2part,
+                               int pstride, float* hout, int64_t hout_stride) {
+  const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
+  const dim3 block(kStreamThreads);
+  if (ncols <= 8)
+    hipLaunchKernelGGL(k_project_dots<8>, grid, block, 0, s, Q, x, y, P, n, ncols, h1part, h2part,
+                       pstride, hout, hout_stride);
+  else if (ncols <= 16)
+    hipLaunchKernelGGL(k_project_dots<16>, grid, block, 0, s, Q, x, y, P, n, ncols, h1part, h2part,
+                       pstride, hout, hout_stride);
+  else if (ncols <= 32)
+    hipLaunchKernelGGL(k_project_dots<32>, grid, block, 0, s, Q, x, y, P, n, ncols, h1part, h2part,
+                       pstride, hout, hout_stride);
+  else if (ncols <= kFusedMaxCols)
+    hipLaunchKernelGGL(k_project_dots<kFusedMaxCols>, grid, block, 0, s, Q, x, y, P, n, ncols,
+                       h1part, h2part, pstride, hout, hout_stride);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_normalize(hipStream_t s, const float* x, float* out, int64_t P, int64_t n,
+                            const float* norm2part, float* norm_out, float* hslot,
+                            int64_t hslot_stride, float* inv_out) {
+  const bool vec = (n % 4 == 0) && aligned16(x) && aligned16(out);
+  const int V = vec ? 4 : 1;
+  const int64_t per_block = int64_t(kStreamThreads) * V * kNormVec;
+  const dim3 grid(unsigned((n + per_block - 1) / per_block), unsigned(P));
+  const int64_t nbn = num_chunks(n);
+  if (vec)
+    hipLaunchKernelGGL(k_normalize<4>, grid, dim3(kStreamThreads), 0, s, x, out, n, nbn, norm2part,
+                       norm_out, hslot, hslot_stride, inv_out);
+  else
+    hipLaunchKernelGGL(k_normalize<1>, grid, dim3(kStreamThreads), 0, s, x, out, n, nbn, norm2part,
+                       norm_out, hslot, hslot_stride, inv_out);
+  return hipGetLastError();
+}
+
+}  // namespace slq

@@ -1,0 +1,291 @@
+// C ABI of libslq: argument checking, scratch pooling and the launch sequence
+// of the SLQ log-determinant path.  Kernels live in k_*.hip.
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "slq_internal.h"
+
+struct slq_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string error;
+  struct Buf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+  };
+  // pooled scratch, grown on demand (never shrunk until destroy)
+  Buf partial1, partial2, norm2, work_w, work_w2, basis, hmat, resid, vec0, scal1, scal2, eig,
+      per_probe, mean;
+};
+
+namespace slq {
+namespace {
+
+int fail(slq_context* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->error = msg;
+  return code;
+}
+
+int hip_fail(slq_context* ctx, hipError_t err, const char* where) {
+  std::string msg = std::string(where) + ": " + hipGetErrorString(err);
+  return fail(ctx, SLQ_EHIP, msg);
+}
+
+#define SLQ_HIP(ctx, expr)                                     \
+  do {                                                         \
+    hipError_t _e = (expr);                                    \
+    if (_e != hipSuccess) return hip_fail((ctx), _e, #expr);   \
+  } while (0)
+
+// Grow a pooled buffer to at least `bytes`.  The stream is synchronised
+// before an old allocation is released so in-flight work never sees it freed.
+int ensure(slq_context* ctx, slq_context::Buf& b, size_t bytes, void** out) {
+  if (bytes == 0) bytes = 16;
+  if (b.bytes < bytes) {
+    if (b.ptr) {
+      SLQ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      SLQ_HIP(ctx, hipFree(b.ptr));
+      b.ptr = nullptr;
+      b.bytes = 0;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(ctx, SLQ_ENOMEM,
+                  "device allocation of " + std::to_string(bytes) + " bytes failed: " +
+                      hipGetErrorString(e));
+    }
+    b.ptr = p;
+    b.bytes = bytes;
+  }
+  *out = b.ptr;
+  return SLQ_OK;
+}
+
+template <typename T>
+int ensure_t(slq_context* ctx, slq_context::Buf& b, size_t count, T** out) {
+  void* p = nullptr;
+  int rc = ensure(ctx, b, count * sizeof(T), &p);
+  *out = static_cast<T*>(p);
+  return rc;
+}
+
+#define SLQ_TRY(expr)              \
+  do {                             \
+    int _rc = (expr);              \
+    if (_rc != SLQ_OK) return _rc; \
+  } while (0)
+
+std::string exceeds_msg(int64_t k, int64_t n) {
+  // Same wording as matfree's decomp._error_num_matvecs (decomp.py:753-756).
+  return "Parameter 'num_matvecs'=" + std::to_string(k) +
+         " exceeds the acceptable range. Expected: 0 <= num_matvecs <= " + std::to_string(n) + ".";
+}
+
+int check_operator(slq_context* ctx, const slq_operator* op) {
+  if (!op) return fail(ctx, SLQ_EINVAL, "operator descriptor is NULL");
+  if (op->n <= 0) return fail(ctx, SLQ_EINVAL, "operator dimension n must be positive");
+  if (op->precision != SLQ_PREC_FP32)
+    return fail(ctx, SLQ_EUNSUPPORTED,
+                "precision " + std::to_string(op->precision) + " is not available in this build");
+  if (op->kind == SLQ_OP_DENSE) {
+    if (!op->matrix) return fail(ctx, SLQ_EINVAL, "dense operator without a matrix");
+  } else if (op->kind == SLQ_OP_RBF) {
+    if (!op->points) return fail(ctx, SLQ_EINVAL, "RBF operator without points");
+    if (op->dim < 1 || op->dim > 4)
+      return fail(ctx, SLQ_EINVAL, "RBF point dimension must be in 1..4");
+    if (!(op->lengthscale > 0.0)) return fail(ctx, SLQ_EINVAL, "RBF lengthscale must be > 0");
+  } else {
+    return fail(ctx, SLQ_EINVAL, "unknown operator kind " + std::to_string(op->kind));
+  }
+  return SLQ_OK;
+}
+
+int set_device(slq_context* ctx) {
+  SLQ_HIP(ctx, hipSetDevice(ctx->device));
+  return SLQ_OK;
+}
+
+// Forward Arnoldi (decomp.py:426-477) for P start vectors; see slq.h.
+int hessenberg_core(slq_context* ctx, const slq_operator& op, const float* V0, int64_t P,
+                    int64_t k, float* Q, float* H, float* R, float* c) {
+  const int64_t n = op.n;
+  hipStream_t s = ctx->stream;
+  const int64_t nb = num_chunks(n);
+  const int pstride = int(k > 0 ? k : 1);
+  float *part1, *part2, *norm2, *w, *w2;
+  SLQ_TRY(ensure_t(ctx, ctx->partial1, size_t(P * nb * pstride), &part1));
+  SLQ_TRY(ensure_t(ctx, ctx->partial2, size_t(P * nb * pstride), &part2));
+  SLQ_TRY(ensure_t(ctx, ctx->norm2, size_t(P * nb), &norm2));
+  SLQ_TRY(ensure_t(ctx, ctx->work_w, size_t(P * n), &w));
+  SLQ_TRY(ensure_t(ctx, ctx->work_w2, size_t(P * n), &w2));
+
+  if (k > 0) SLQ_HIP(ctx, hipMemsetAsync(H, 0, size_t(P * k * k) * sizeof(float), s));
+
+  // initlength = sqrt(<v, v>) (decomp.py:435); c = 1 / initlength (:450)
+  SLQ_HIP(ctx, launch_dots(s, nullptr, V0, P, n, 1, norm2, 1));
+  if (k == 0) {
+    SLQ_HIP(ctx, launch_normalize(s, V0, w, P, n, norm2, nullptr, nullptr, 0, c));
+    SLQ_HIP(ctx, hipMemcpyAsync(R, V0, size_t(P * n) * sizeof(float), hipMemcpyDeviceToDevice, s));
+    return SLQ_OK;
+  }
+  // step 0: v /= length; Q[:, 0] = v (decomp.py:456-457)
+  SLQ_HIP(ctx, launch_normalize(s, V0, Q, P, n, norm2, nullptr, nullptr, 0, c));
+  for (int64_t j = 0; j < k; ++j) {
+    const float* qj = Q + j * P * n;
+    const int ncols = int(j + 1);
+    float* hcol = H + j * k;  // column j of every probe, stride k * k
+    // w = A v (decomp.py:460)
+    SLQ_HIP(ctx, launch_matvec(s, op, qj, P, w));
+    // h = Q^T w (decomp.py:463)
+    SLQ_HIP(ctx, launch_dots(s, Q, w, P, n, ncols, part1, pstride));
+    // w = w - Q h (decomp.py:464), then the second pass's Q^T w (decomp.py:468)
+    if (ncols <= kFusedMaxCols) {
+      SLQ_HIP(ctx, launch_project_dots(s, Q, w, w2, P, n, ncols, part1, part2, pstride, hcol, k * k));
+    } else {
+      SLQ_HIP(ctx, launch_project(s, Q, w, w2, P, n, ncols, part1, pstride, nullptr, hcol, k * k));
+      SLQ_HIP(ctx, launch_dots(s, Q, w2, P, n, ncols, part2, pstride));
+    }
+    // w = w - Q (Q^T w) and <w, w> (decomp.py:468, 471)
+    const bool last = (j + 1 == k);
+    float* dst = last ? R : w;
+    SLQ_HIP(ctx, launch_project(s, Q, w2, dst, P, n, ncols, part2, pstride, norm2, nullptr, 0));
+    if (!last) {
+      // length = sqrt(<w, w>); h[j+1] = length (decomp.py:471-475); next
+      // step's v /= length; Q[:, j+1] = v (decomp.py:456-457)
+      SLQ_HIP(ctx, launch_normalize(s, w, Q + (j + 1) * P * n, P, n, norm2, nullptr, hcol + j + 1,
+                                    k * k, nullptr));
+    }
+  }
+  return SLQ_OK;
+}
+
+}  // namespace
+}  // namespace slq
+
+using namespace slq;
+
+extern "C" {
+
+int slq_context_create(int device, void* stream, slq_context** out) {
+  if (!out) return SLQ_EINVAL;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+    (void)hipGetLastError();
+    return SLQ_EINVAL;
+  }
+  if (hipSetDevice(device) != hipSuccess) return SLQ_EHIP;
+  slq_context* ctx = new slq_context();
+  ctx->device = device;
+  ctx->stream = static_cast<hipStream_t>(stream);
+  *out = ctx;
+  return SLQ_OK;
+}
+
+int slq_context_destroy(slq_context* ctx) {
+  if (!ctx) return SLQ_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  slq_context::Buf* bufs[] = {&ctx->partial1, &ctx->partial2, &ctx->norm2,  &ctx->work_w,
+                              &ctx->work_w2,  &ctx->basis,    &ctx->hmat,   &ctx->resid,
+                              &ctx->vec0,     &ctx->scal1,    &ctx->scal2,  &ctx->eig,
+                              &ctx->per_probe, &ctx->mean};
+  for (auto* b : bufs)
+    if (b->ptr) (void)hipFree(b->ptr);
+  delete ctx;
+  return SLQ_OK;
+}
+
+int slq_context_set_stream(slq_context* ctx, void* stream) {
+  if (!ctx) return SLQ_EINVAL;
+  ctx->stream = static_cast<hipStream_t>(stream);
+  return SLQ_OK;
+}
+
+const char* slq_last_error(const slq_context* ctx) { return ctx ? ctx->error.c_str() : ""; }
+
+int64_t slq_context_scratch_bytes(const slq_context* ctx) {
+  if (!ctx) return 0;
+  const slq_context::Buf* bufs[] = {&ctx->partial1, &ctx->partial2, &ctx->norm2,  &ctx->work_w,
+                                    &ctx->work_w2,  &ctx->basis,    &ctx->hmat,   &ctx->resid,
+                                    &ctx->vec0,     &ctx->scal1,    &ctx->scal2,  &ctx->eig,
+                                    &ctx->per_probe, &ctx->mean};
+  int64_t total = 0;
+  for (auto* b : bufs) total += int64_t(b->bytes);
+  return total;
+}
+
+int slq_sampler_signs(slq_context* ctx, uint32_t key0, uint32_t key1, int64_t num, int64_t n,
+                      float* out) {
+  if (!ctx) return SLQ_EINVAL;
+  if (num < 0 || n < 0) return fail(ctx, SLQ_EINVAL, "num and n must be non-negative");
+  if (num * n > 0 && !out) return fail(ctx, SLQ_EINVAL, "output pointer is NULL");
+  SLQ_TRY(set_device(ctx));
+  SLQ_HIP(ctx, launch_rademacher(ctx->stream, key0, key1, num * n, out));
+  return SLQ_OK;
+}
+
+int slq_matvec(slq_context* ctx, const slq_operator* op, const float* V, int64_t P, float* W) {
+  if (!ctx) return SLQ_EINVAL;
+  SLQ_TRY(check_operator(ctx, op));
+  if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of vectors P must be positive");
+  if (!V || !W) return fail(ctx, SLQ_EINVAL, "input or output pointer is NULL");
+  SLQ_TRY(set_device(ctx));
+  SLQ_HIP(ctx, launch_matvec(ctx->stream, *op, V, P, W));
+  return SLQ_OK;
+}
+
+int slq_hessenberg(slq_context* ctx, const slq_operator* op, const float* V0, int64_t P,
+                   int64_t k, float* Q, float* H, float* R, float* c) {
+  if (!ctx) return SLQ_EINVAL;
+  SLQ_TRY(check_operator(ctx, op));
+  if (k < 0 || k > op->n) return fail(ctx, SLQ_EINVAL_K, exceeds_msg(k, op->n));
+  if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of vectors P must be positive");
+  if (!V0 || !R || !c || (k > 0 && (!Q || !H)))
+    return fail(ctx, SLQ_EINVAL, "input or output pointer is NULL");
+  SLQ_TRY(set_device(ctx));
+  return hessenberg_core(ctx, *op, V0, P, k, Q, H, R, c);
+}
+
+int slq_logdet(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P, int64_t k,
+               double* per_probe, double* mean, float* alpha, float* beta) {
+  if (!ctx) return SLQ_EINVAL;
+  SLQ_TRY(check_operator(ctx, op));
+  const int64_t n = op->n;
+  if (k < 0 || k > n) return fail(ctx, SLQ_EINVAL_K, exceeds_msg(k, n));
+  if (k == 0)
+    return fail(ctx, SLQ_EINVAL, "the log-determinant integrand needs num_matvecs >= 1");
+  if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of probes P must be positive");
+  if (!Z) return fail(ctx, SLQ_EINVAL, "probe pointer is NULL");
+  SLQ_TRY(set_device(ctx));
+  hipStream_t s = ctx->stream;
+  float *v0, *Q, *H, *R, *c, *ell, *norm2;
+  double *eig, *pp;
+  SLQ_TRY(ensure_t(ctx, ctx->vec0, size_t(P * n), &v0));
+  SLQ_TRY(ensure_t(ctx, ctx->basis, size_t(k * P * n), &Q));
+  SLQ_TRY(ensure_t(ctx, ctx->hmat, size_t(P * k * k), &H));
+  SLQ_TRY(ensure_t(ctx, ctx->resid, size_t(P * n), &R));
+  SLQ_TRY(ensure_t(ctx, ctx->scal1, size_t(P), &c));
+  SLQ_TRY(ensure_t(ctx, ctx->scal2, size_t(P), &ell));
+  SLQ_TRY(ensure_t(ctx, ctx->norm2, size_t(P * num_chunks(n)), &norm2));
+  SLQ_TRY(ensure_t(ctx, ctx->eig, size_t(3 * k * P), &eig));
+  if (per_probe) {
+    pp = per_probe;
+  } else {
+    SLQ_TRY(ensure_t(ctx, ctx->per_probe, size_t(P), &pp));
+  }
+  // length = |z|; v0 = z / length (funm.py:228-229)
+  SLQ_HIP(ctx, launch_dots(s, nullptr, Z, P, n, 1, norm2, 1));
+  SLQ_HIP(ctx, launch_normalize(s, Z, v0, P, n, norm2, ell, nullptr, 0, nullptr));
+  // tridiag_sym(matvec, v0) (funm.py:237 -> decomp.py:130-143)
+  SLQ_TRY(hessenberg_core(ctx, *op, v0, P, k, Q, H, R, c));
+  // |z|^2 [log T]_00 (funm.py:239-241) and the probe mean (stochtrace.py:50)
+  SLQ_HIP(ctx, launch_tridiag_quadrature(s, H, ell, P, k, eig, pp, mean, alpha, beta));
+  return SLQ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,61 @@
+// Internal declarations shared by the HIP translation units of libslq.
+// Kernels live in k_*.hip; each exposes host launch functions declared here.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/slq.h"
+
+namespace slq {
+
+// Threads per block of the streaming (HBM-bound) kernels.
+constexpr int kStreamThreads = 256;
+// float4 groups per thread per block in the streaming kernels: one block
+// covers kStreamThreads * 4 * kStreamVec = 8192 elements of one vector.
+constexpr int kStreamVec = 8;
+constexpr int64_t kChunk = int64_t(kStreamThreads) * 4 * kStreamVec;
+// Basis columns held in registers by the fused project+dots kernel.
+constexpr int kFusedMaxCols = 64;
+
+inline int64_t num_chunks(int64_t n) { return (n + kChunk - 1) / kChunk; }
+
+// ---- probes (k_probes.hip) -------------------------------------------------
+hipError_t launch_rademacher(hipStream_t s, uint32_t k0, uint32_t k1, int64_t count, float* out);
+
+// ---- streaming Gram-Schmidt kernels (k_lanczos.hip) ------------------------
+// partial[p][b][c] = sum_{i in chunk b} Q[c][p][i] * x[p][i], c < ncols.
+// Q may be null with ncols = 1 to mean "sum of squares of x" (partial[p][b][0]).
+hipError_t launch_dots(hipStream_t s, const float* Q, const float* x, int64_t P, int64_t n,
+                       int ncols, float* partial, int pstride);
+// y = x - sum_c Q[c] h[c] with h[c] = sum_b hpart[p][b][c]; optional
+// norm2part[p][b] = sum over chunk of y^2.  hout (if non-null) receives h for
+// probe p at hout + p * hout_stride (written by chunk 0 only).
+hipError_t launch_project(hipStream_t s, const float* Q, const float* x, float* y, int64_t P,
+                          int64_t n, int ncols, const float* hpart, int pstride, float* norm2part,
+                          float* hout, int64_t hout_stride);
+// Fused: y = x - Q h1 (h1 reduced from h1part), then h2part = Q^T y per chunk.
+hipError_t launch_project_dots(hipStream_t s, const float* Q, const float* x, float* y, int64_t P,
+                               int64_t n, int ncols, const float* h1part, float* h2part,
+                               int pstride, float* hout, int64_t hout_stride);
+// out[p][i] = x[p][i] / sqrt(sum_b norm2part[p][b]); optionally writes the
+// norm to norm_out[p] (float) and to hslot[p * hslot_stride].
+hipError_t launch_normalize(hipStream_t s, const float* x, float* out, int64_t P, int64_t n,
+                            const float* norm2part, float* norm_out, float* hslot,
+                            int64_t hslot_stride, float* inv_out);
+
+// ---- block matvec (k_matvec.hip) -------------------------------------------
+hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, int64_t P,
+                         float* W);
+
+// ---- tridiagonal eigen + quadrature (k_tridiag.hip) ------------------------
+// For each probe p: T = (H + H^T)/2 restricted to its tridiagonal; returns
+// per_probe[p] = scale[p]^2 * sum_j U_0j^2 log(lambda_j) in FP64, optional
+// alpha/beta extraction, and the mean over probes.
+hipError_t launch_tridiag_quadrature(hipStream_t s, const float* H, const float* scale,
+                                     int64_t P, int64_t k, double* work, double* per_probe,
+                                     double* mean, float* alpha, float* beta);
+
+}  // namespace slq

@@ -1,0 +1,125 @@
+"""SLQ integrands on the GPU (matfree.funm subset on the log-determinant path).
+
+* :func:`monte_carlo_funm_sym_logdet` (funm.py:186-202), also exported as
+  :data:`integrand_funm_sym_logdet`;
+* :func:`monte_carlo_funm_sym` (funm.py:205-243) and
+  :func:`dense_funm_sym_eigh` (funm.py:322-335) for ``matfun = log``.
+
+The integrand ``quadform(matvec, v0, *params) = |v0|^2 [f(T)]_00`` runs as one
+libslq call (``slq_logdet``): normalise, Arnoldi with full
+reorthogonalisation, FP64 tridiagonal eigen-solve and Gauss quadrature.
+"""
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _runtime, _tree, decomp, operators
+
+_LOG_FUNCTIONS = (np.log, math.log, torch.log)
+
+
+class _DenseFunmSymEigh:
+    """``dense_funm_sym_eigh(matfun)`` (funm.py:322-335)."""
+
+    def __init__(self, matfun):
+        self.matfun = matfun
+
+    @property
+    def is_log(self):
+        return any(self.matfun is f for f in _LOG_FUNCTIONS)
+
+    def __call__(self, dense_matrix):
+        raise NotImplementedError(
+            "matfree_amd fuses f(T) into its FP64 tridiagonal eigen-solve kernel; "
+            "use it through monte_carlo_funm_sym / monte_carlo_funm_sym_logdet"
+        )
+
+
+def dense_funm_sym_eigh(matfun):
+    """Matrix function via symmetric eigendecomposition (funm.py:322-335)."""
+    return _DenseFunmSymEigh(matfun)
+
+
+def logdet_per_probe(matvec, Z, num_matvecs, params, want_coeffs=False):
+    """Run ``slq_logdet`` on probes ``Z`` (P, n).
+
+    Returns ``(per_probe (P,) float64, mean () float64)`` and, with
+    ``want_coeffs``, the tridiagonal ``(alpha (P, k), beta (P, k-1))``.
+    """
+    op = operators.resolve(matvec)
+    shape = tuple(Z.shape)
+    P, n = (1, shape[0]) if len(shape) == 1 else shape
+    k = int(num_matvecs)
+    if k < 0 or k > n:
+        raise ValueError(decomp._error_num_matvecs(k, maxval=n, minval=0))
+    device = _runtime.target_device(Z, *params)
+    Z = _runtime.as_f32(Z, device).reshape(P, n)
+    desc, keep = op.descriptor(n, params, device)
+    per_probe = torch.empty((P,), dtype=torch.float64, device=device)
+    mean = torch.empty((), dtype=torch.float64, device=device)
+    alpha = beta = None
+    if want_coeffs:
+        alpha = torch.empty((P, k), dtype=torch.float32, device=device)
+        beta = torch.empty((P, max(k - 1, 0)), dtype=torch.float32, device=device)
+    ctx = _runtime.context(device)
+    ctx.call(
+        "slq_logdet",
+        ctypes.byref(desc),
+        _runtime.ptr(Z),
+        P,
+        k,
+        _runtime.ptr(per_probe),
+        _runtime.ptr(mean),
+        _runtime.ptr(alpha) if want_coeffs else None,
+        _runtime.ptr(beta) if want_coeffs and k > 1 else None,
+    )
+    del keep
+    if want_coeffs:
+        return per_probe, mean, alpha, beta
+    return per_probe, mean
+
+
+class _QuadFormSym:
+    """``monte_carlo_funm_sym(...).quadform`` (funm.py:226-241)."""
+
+    def __init__(self, dense_funm, tridiag_sym):
+        if not isinstance(dense_funm, _DenseFunmSymEigh) or not dense_funm.is_log:
+            raise NotImplementedError(
+                "matfree_amd's GPU integrand implements dense_funm_sym_eigh(log) only"
+            )
+        if not isinstance(tridiag_sym, decomp._TridiagSymFull):
+            raise TypeError("pass decomp.tridiag_sym(k) of matfree_amd (reortho='full')")
+        self.tridiag_sym = tridiag_sym
+        self.num_matvecs = tridiag_sym.num_matvecs
+
+    def batched(self, matvec, samples, *parameters):
+        """Per-probe values ``|z_p|^2 [log T_p]_00`` for the rows of ``samples``."""
+        per_probe, _ = logdet_per_probe(matvec, samples, self.num_matvecs, parameters)
+        return per_probe
+
+    def estimate(self, matvec, samples, *parameters):
+        """Mean over probes, reduced on the device."""
+        _, mean = logdet_per_probe(matvec, samples, self.num_matvecs, parameters)
+        return mean
+
+    def __call__(self, matvec, v0, *parameters):
+        flat, _ = _tree.ravel(v0, decomp._host_or_gpu(v0, *parameters))
+        per_probe, _ = logdet_per_probe(matvec, flat[None, :], self.num_matvecs, parameters)
+        return per_probe[0]
+
+
+def monte_carlo_funm_sym(dense_funm, tridiag_sym, /):
+    """Integrand for matrix-function trace estimation (funm.py:205-243)."""
+    return _QuadFormSym(dense_funm, tridiag_sym)
+
+
+def monte_carlo_funm_sym_logdet(tridiag_sym, /):
+    """Integrand for the log-determinant (funm.py:186-202)."""
+    return monte_carlo_funm_sym(dense_funm_sym_eigh(np.log), tridiag_sym)
+
+
+# The name used by newer matfree API generations.
+integrand_funm_sym_logdet = monte_carlo_funm_sym_logdet

@@ -1,0 +1,82 @@
+"""Host-side behaviour of the drop-in API that needs no GPU: factory
+validation and error classes/messages of the reference (decomp.py:112-122,
+375-378, 427-429, 753-756), and rejection of foreign matvecs."""
+
+import numpy as np
+import pytest
+
+from matfree_amd import decomp, funm, operators, prng, stochtrace
+
+
+def test_tridiag_sym_bad_reortho_raises_valueerror():
+    with pytest.raises(ValueError, match="unsupported"):
+        decomp.tridiag_sym(3, reortho="partial")
+
+
+def test_tridiag_sym_reortho_none_is_out_of_scope():
+    with pytest.raises(NotImplementedError):
+        decomp.tridiag_sym(3, reortho="none")
+
+
+@pytest.mark.parametrize("reortho_wrong", [True, "full_with_sparsity", "None"])
+def test_hessenberg_bad_flag_raises_typeerror(reortho_wrong):
+    # tests/test_decomp/test_hessenberg.py:72-75
+    with pytest.raises(TypeError, match="Unexpected input"):
+        decomp.hessenberg(1, reortho=reortho_wrong)
+
+
+@pytest.mark.parametrize("k", [-1, 14])
+def test_num_matvecs_out_of_range_raises_exceeds(k):
+    # tests/test_decomp/test_consistency.py:50-63 -- raised before any device use
+    A = np.eye(13, dtype=np.float32)
+    v0 = np.ones(13, dtype=np.float32)
+    with pytest.raises(ValueError, match="exceeds"):
+        decomp.tridiag_sym(k)(operators.dense_matvec, v0, A)
+    with pytest.raises(ValueError, match="exceeds"):
+        decomp.hessenberg(k, reortho="full")(operators.dense_matvec, v0, A)
+    integrand = funm.monte_carlo_funm_sym_logdet(decomp.tridiag_sym(k))
+    with pytest.raises(ValueError, match="exceeds"):
+        integrand(operators.dense_matvec, v0, A)
+
+
+def test_python_callable_matvec_is_rejected():
+    A = np.eye(4, dtype=np.float32)
+    with pytest.raises(TypeError, match="operators"):
+        decomp.tridiag_sym(2)(lambda v: A @ v, np.ones(4, np.float32))
+
+
+def test_integrand_aliases():
+    assert funm.integrand_funm_sym_logdet is funm.monte_carlo_funm_sym_logdet
+    integrand = funm.integrand_funm_sym_logdet(decomp.tridiag_sym(4))
+    assert integrand.num_matvecs == 4
+
+
+def test_non_log_dense_funm_is_out_of_scope():
+    with pytest.raises(NotImplementedError):
+        funm.monte_carlo_funm_sym(funm.dense_funm_sym_eigh(np.exp), decomp.tridiag_sym(4))
+
+
+def test_estimator_rejects_foreign_integrands():
+    with pytest.raises(TypeError):
+        stochtrace.estimator_monte_carlo(lambda m, v: 0.0, stochtrace.sampler_signs(np.ones(3), num=2))
+
+
+def test_sampler_shapes_and_complex_rejected():
+    s = stochtrace.sampler_signs({"a": np.ones(3), "b": np.ones((2, 2))}, num=5)
+    assert s.n == 7 and s.num == 5
+    with pytest.raises(NotImplementedError):
+        stochtrace.sampler_signs(np.ones(3, dtype=complex), num=2)
+
+
+def test_prng_key_and_words():
+    assert prng.PRNGKey(1).tolist() == [0, 1]
+    assert prng.key_words(prng.PRNGKey(2**33 + 7)) == (2, 7)
+    with pytest.raises(ValueError):
+        prng.key_words(np.zeros(3, np.uint32))
+
+
+def test_rbf_operator_validation():
+    with pytest.raises(ValueError):
+        operators.RBF(np.zeros((4, 5)))
+    with pytest.raises(ValueError):
+        operators.RBF(np.zeros((4, 3)), precision="fp64")
