@@ -1,0 +1,262 @@
+#!/usr/bin/env python3
+"""Throughput of the SLQ log-determinant path on one MI355X.
+
+One step = one full SLQ log-determinant solve: Rademacher probes ->
+tridiag_sym(k, reortho="full") -> log-det integrand -> probe mean, with all
+inputs resident in HBM.  The metric counts probe-Lanczos-steps (one probe x one
+Arnoldi step), so ``value = probes * depth * steps / seconds``.
+
+Default workload: the matrix-free RBF log-determinant, n = 32768 points
+uniform in [0, 1]^3 (seeded), K_ij = exp(-|x_i - x_j|^2 / (2 * 0.5^2)) +
+1e-2 delta_ij, 128 probes, depth 64.  ``--workload dense`` runs the stored
+dense SPD case (n = 16384, 256 probes, depth 64, A = B B^T / n + 0.1 I).
+
+Printed: one JSON line with the throughput, the relative error against an
+FP64 dense-Cholesky log-determinant of the same operator, the roofline of the
+dominant kernel (device time from HIP events on the launch stream during the
+timed region) and a CPU baseline (NumPy restatement of matfree's algorithm on
+a bounded sample of the same workload).
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SLQ probe-Lanczos-steps/s at ≤1e-3 rel err vs FP64; serve p50/p99 latency"
+UNIT = "probe-Lanczos-steps/s"
+PEAK = {  # MI355X published dense peaks
+    "hbm_GBs": 8000.0,
+    "fp32_mfma_TFs": 157.3,
+    "bf16_mfma_TFs": 2500.0,
+}
+WORKLOADS = {
+    "rbf": dict(n=32768, probes=128, depth=64, lengthscale=0.5, noise=1e-2),
+    "dense": dict(n=16384, probes=256, depth=64, shift=0.1),
+}
+DEFAULT_PRECISION = {"rbf": "bf16x2", "dense": "fp32"}
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="rbf")
+    ap.add_argument("--precision", default=None, help="fp32 | bf16 | bf16x3")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-reference", action="store_true", help="skip the FP64 Cholesky check")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def setup(args, device):
+    import numpy as np
+    import torch
+
+    from matfree_amd import operators, problems, prng, stochtrace
+
+    w = WORKLOADS[args.workload]
+    n, P, k = w["n"], w["probes"], w["depth"]
+    prec = args.precision or DEFAULT_PRECISION[args.workload]
+    if args.workload == "rbf":
+        X = problems.rbf_points(n, seed=args.seed)
+        op = operators.RBF(X, precision=prec)
+        params = (w["lengthscale"], w["noise"])
+        host = {"points": X}
+    else:
+        A = problems.dense_spd_device(n, seed=args.seed, shift=w["shift"], device=device)
+        op = operators.DenseMatvec(precision=prec)
+        params = (A,)
+        host = {"matrix": A}
+    Z = stochtrace.sampler_signs(np.zeros(n, np.float32), num=P).flat(prng.PRNGKey(args.seed),
+                                                                       device)
+    torch.cuda.synchronize()
+    return op, params, Z, (n, P, k), prec, host
+
+
+def fp64_reference(args, host, device):
+    import torch
+
+    from matfree_amd import problems
+
+    w = WORKLOADS[args.workload]
+    if args.workload == "rbf":
+        K = problems.rbf_dense64_device(host["points"], w["lengthscale"], w["noise"], device)
+    else:
+        K = host["matrix"].to(torch.float64)
+    val = problems.logdet_cholesky_device(K)
+    del K
+    torch.cuda.empty_cache()
+    return val
+
+
+def timed_solves(op, params, Z, k, steps, warmup, ctx):
+    import torch
+
+    from matfree_amd import funm
+
+    for _ in range(warmup):
+        funm.logdet_per_probe(op, Z, k, params)
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    for kc in ("matvec", "dots", "project_dots", "project", "normalize", "quadrature"):
+        ctx.profile_read(kc)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mean = None
+    for _ in range(steps):
+        _, mean = funm.logdet_per_probe(op, Z, k, params)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kernels = {}
+    for kc in ("matvec", "dots", "project_dots", "project", "normalize", "quadrature"):
+        ms, cnt = ctx.profile_read(kc)
+        kernels[kc] = {"total_ms": ms, "launches": cnt}
+    ctx.profile(False)
+    return t1 - t0, float(mean), kernels
+
+
+def roofline(kernels, n, P, k, prec, workload):
+    """Roofline entry of the kernel class with the largest device time."""
+    name = max(kernels, key=lambda kc: kernels[kc]["total_ms"])
+    kk = kernels[name]
+    avg_s = kk["total_ms"] / max(kk["launches"], 1) / 1e3
+    if name == "matvec":
+        flops = 2.0 * n * n * P  # algorithmic flops of one block matvec
+        peak = PEAK["fp32_mfma_TFs"] if prec == "fp32" else PEAK["bf16_mfma_TFs"]
+        achieved = flops / avg_s / 1e12
+        entry = {"kernel": "block_matvec_" + workload + "_" + prec, "bound": "mfma",
+                 "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                 "algorithmic_per_launch": flops, "avg_launch_ms": avg_s * 1e3}
+    else:
+        # Streaming Gram-Schmidt kernels: bytes per launch averaged over the
+        # k steps (column j reads j + 1 basis vectors of n floats per probe).
+        mean_cols = (k + 1) / 2.0
+        per_launch = {
+            "dots": 4.0 * n * P * (mean_cols + 1),
+            "project_dots": 4.0 * n * P * (mean_cols + 2),
+            "project": 4.0 * n * P * (mean_cols + 2),
+            "normalize": 8.0 * n * P,
+            "quadrature": 8.0 * P * k * k,
+        }[name]
+        achieved = per_launch / avg_s / 1e9
+        entry = {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": PEAK["hbm_GBs"],
+                 "unit": "GB/s", "frac": achieved / PEAK["hbm_GBs"],
+                 "algorithmic_per_launch": per_launch, "avg_launch_ms": avg_s * 1e3}
+    entry["traffic"] = measured_traffic(entry["kernel"])
+    return entry
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+        return table.get(kernel, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(args, host, device, n, k):
+    """NumPy restatement of matfree's algorithm (the oracle) on a bounded sample."""
+    import numpy as np
+    import torch
+
+    from oracle import prng as oprng
+    from oracle import slq as oslq
+
+    w = WORKLOADS[args.workload]
+    probes = 8 if args.workload == "rbf" else 16
+    if args.workload == "rbf":
+        from matfree_amd import problems
+
+        K = problems.rbf_dense64_device(host["points"], w["lengthscale"], w["noise"], device)
+        A32 = K.to(torch.float32).cpu().numpy()
+        del K
+        torch.cuda.empty_cache()
+    else:
+        A32 = host["matrix"].cpu().numpy()
+    op = oslq.DenseOperator(A32, np.float32)
+    Z = oprng.rademacher(oprng.prng_key(args.seed), (probes, n), np.float32)
+    try:
+        from threadpoolctl import threadpool_info
+
+        cores = max(int(i.get("num_threads", 1)) for i in threadpool_info()) if threadpool_info() else 1
+    except Exception:
+        cores = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    oslq.quadform_logdet(op, k, Z, np.float32)
+    dt = time.perf_counter() - t0
+    sample = (f"{probes} probes x {k} steps of the same n={n} {args.workload} operator, "
+              "FP32 NumPy restatement of matfree's formulation (batched A@V, full-width CGS2, "
+              "eigh); operator materialised on the host before timing")
+    return {"value": probes * k / dt, "unit": UNIT, "cores": cores, "kind": "port",
+            "sample": sample, "seconds": dt}
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are out of scope "
+                                   "(see DESIGN.md for how the path would shard)",
+                          "n_gpus": args.gpus}), flush=True)
+        sys.exit(2)
+
+    import torch
+
+    from matfree_amd import _runtime
+
+    device = torch.device("cuda", 0)
+    torch.cuda.set_device(device)
+    op, params, Z, (n, P, k), prec, host = setup(args, device)
+    ctx = _runtime.context(device)
+    log(f"bench: workload={args.workload} n={n} P={P} k={k} precision={prec}")
+    dt, est, kernels = timed_solves(op, params, Z, k, args.steps, args.warmup, ctx)
+    value = P * k * args.steps / dt
+    log(f"bench: {dt / args.steps * 1e3:.2f} ms/solve, {value:.0f} {UNIT}, logdet {est:.8e}")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": UNIT,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if prec == "fp32" else prec + "+fp32",
+        "data": "synthetic (seeded points / matrix, Rademacher probes from PRNGKey(seed))",
+        "config": {"workload": f"{args.workload}-logdet n={n} probes={P} depth={k}",
+                   "n": n, "probes": P, "depth": k, "precision": prec,
+                   **{kk: v for kk, v in WORKLOADS[args.workload].items()
+                      if kk not in ("n", "probes", "depth")}},
+        "logdet": est,
+    }
+    if not args.no_reference:
+        ref = fp64_reference(args, host, device)
+        out["logdet_fp64_cholesky"] = ref
+        out["rel_err_vs_fp64"] = abs(est - ref) / abs(ref)
+        log(f"bench: FP64 Cholesky {ref:.8e}, rel err {out['rel_err_vs_fp64']:.2e}")
+    out["roofline"] = roofline(kernels, n, P, k, prec, args.workload)
+    out["kernels_ms_per_solve"] = {kc: v["total_ms"] / args.steps for kc, v in kernels.items()}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args, host, device, n, k)
+        log(f"bench: cpu baseline {out['cpu_baseline']['value']:.1f} {UNIT}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -50,7 +50,8 @@ typedef enum slq_operator_kind {
 typedef enum slq_precision {
   SLQ_PREC_FP32 = 0,   /* FP32 MFMA, FP32 operands                                */
   SLQ_PREC_BF16 = 1,   /* one BF16 MFMA product, FP32 accumulation                 */
-  SLQ_PREC_BF16X3 = 2  /* split hi+lo BF16: hi*hi + hi*lo + lo*hi, FP32 accumulation */
+  SLQ_PREC_BF16X3 = 2, /* split hi+lo BF16: hi*hi + hi*lo + lo*hi, FP32 accumulation */
+  SLQ_PREC_BF16X2 = 3  /* matrix split hi+lo, vectors BF16: hi*hi + lo*hi           */
 } slq_precision;
 
 /* Operator descriptor: the reference's matvec(v, *params) plugin
@@ -116,6 +117,23 @@ int slq_hessenberg(slq_context* ctx, const slq_operator* op, const float* V0, in
  *   NULL).  The Krylov basis lives in context scratch (k x P x n floats). */
 int slq_logdet(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P, int64_t k,
                double* per_probe, double* mean, float* alpha, float* beta);
+
+/* ---- measurement hooks (no counterpart in the reference) -------------------
+ * With profiling on, every launch of the listed kernel classes made through
+ * this context is bracketed by a pair of HIP events on the context's stream.
+ * slq_profile_read synchronises the stream, returns the summed device time
+ * (ms) and launch count of one class since the last read, and resets it. */
+typedef enum slq_kernel_class {
+  SLQ_KC_MATVEC = 0,       /* block matvec                              */
+  SLQ_KC_DOTS = 1,         /* Gram-Schmidt coefficients Q^T w            */
+  SLQ_KC_PROJECT_DOTS = 2, /* fused w - Q h and Q^T (w - Q h)            */
+  SLQ_KC_PROJECT = 3,      /* w - Q h (+ norm partials)                  */
+  SLQ_KC_NORMALIZE = 4,    /* v / |v| and basis store                    */
+  SLQ_KC_QUADRATURE = 5,   /* FP64 tridiagonal eigen + quadrature        */
+  SLQ_KC_COUNT = 6
+} slq_kernel_class;
+int slq_profile_enable(slq_context* ctx, int on);
+int slq_profile_read(slq_context* ctx, int kernel_class, double* total_ms, int64_t* launches);
 
 #ifdef __cplusplus
 }

@@ -23,7 +23,7 @@ SLQ_EADJOINT_K0 = -6
 # operator kinds / precisions
 SLQ_OP_DENSE = 0
 SLQ_OP_RBF = 1
-PRECISIONS = {"fp32": 0, "bf16": 1, "bf16x3": 2}
+PRECISIONS = {"fp32": 0, "bf16": 1, "bf16x3": 2, "bf16x2": 3}
 
 c_void_p = ctypes.c_void_p
 c_i64 = ctypes.c_int64
@@ -66,7 +66,14 @@ SIGNATURES = {
         [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_i64, c_void_p, c_void_p,
          c_void_p, c_void_p],
     ),
+    "slq_profile_enable": (c_int, [c_void_p, c_int]),
+    "slq_profile_read": (c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(c_i64)]),
 }
+
+# kernel classes of the measurement hooks (slq_kernel_class)
+KERNEL_CLASSES = {"matvec": 0, "dots": 1, "project_dots": 2, "project": 3, "normalize": 4,
+                  "quadrature": 5}
 
 _lib = None
 _lock = threading.Lock()
@@ -128,6 +135,18 @@ class Context:
     def call(self, name, *args):
         rc = getattr(load(), name)(self.ptr, *args)
         raise_for(rc, self.ptr)
+
+    def profile(self, on):
+        raise_for(load().slq_profile_enable(self.ptr, 1 if on else 0), self.ptr)
+
+    def profile_read(self, kernel_class):
+        """(total device ms, launches) of one kernel class since the last read."""
+        ms = ctypes.c_double()
+        cnt = c_i64()
+        rc = load().slq_profile_read(self.ptr, KERNEL_CLASSES[kernel_class], ctypes.byref(ms),
+                                     ctypes.byref(cnt))
+        raise_for(rc, self.ptr)
+        return float(ms.value), int(cnt.value)
 
     def scratch_bytes(self):
         return int(load().slq_context_scratch_bytes(self.ptr))

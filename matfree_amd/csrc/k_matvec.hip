@@ -2,18 +2,29 @@
 // (decomp.py:460), vmapped over probes (stochtrace.py:49), becomes one dense
 // contraction W[p][i] = sum_m A[i][m] V[p][m] on the matrix cores.
 //
-// FP32 path: v_mfma_f32_32x32x2f32.  A workgroup (4 waves) owns a tile of
-// BI = 128 output elements x BP = 64 probes and walks the contraction index m
-// in steps of BK = 32 through LDS.  Each wave owns 64 x 32 outputs (two 32x32
-// accumulators).  Within an MFMA the two lane halves carry two different m;
-// lane half h walks m in [h * BK/2, (h + 1) * BK/2) of the tile so that each
-// lane reads its operands as contiguous float4s from LDS.
+// FP32 path: v_mfma_f32_32x32x2f32 with D[i][p] (rows = output elements,
+// columns = probes).  A workgroup of 4 waves owns 128 output elements; each
+// wave owns 32 of them for 32 * NACC probes (NACC 32x32 accumulators).
 //
-// Operators:
-//  * DENSE: the A tile is read from the stored row-major matrix;
-//  * RBF:   the A tile is generated in place, K_im = exp2(c |x_i - x_m|^2)
-//           with c = -log2(e) / (2 l^2); the nugget s2 * v is added in FP32
-//           in the epilogue (never rounded into a low-precision tile).
+//  * A operand: each lane builds its own fragment in registers -- row
+//    i = i0 + 32 * wave + (lane & 31), contraction indices
+//    m0 + 16 * (lane >> 5) + s, s = 0..15 of the current BK = 32 tile.  For a
+//    stored matrix the 16 values are one contiguous 64-byte load (prefetched a
+//    tile ahead); for the RBF kernel they are generated in place from the
+//    lane's own point and the tile's 32 points, so every kernel entry is
+//    evaluated exactly once per matvec and never touches LDS.
+//  * B operand: the probe tile V[p0 .. p0 + 32 NACC)[m0 .. m0 + 32) is shared
+//    by the four waves through a double-buffered LDS tile (one barrier per
+//    tile; the next tile's global loads are in flight during the MFMAs).
+//  * Blocked summation: every tile accumulates into fresh registers that are
+//    then added to the running sums, so rounding grows with BK + n / BK rather
+//    than n (as in a blocked CPU GEMM).
+//
+// Within an MFMA the two lane halves carry two different contraction indices;
+// lane half h walks m in [16 h, 16 h + 16) of the tile, for both operands.
+//
+// RBF: K_im = exp2(c |x_i - x_m|^2), c = -log2(e) / (2 l^2); the nugget
+// s2 * v is added in FP32 in the epilogue (never folded into a tile).
 #include "slq_internal.h"
 
 namespace slq {
@@ -21,9 +32,8 @@ namespace {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-constexpr int BI = 128;
-constexpr int BP = 64;
-constexpr int BK = 32;
+constexpr int BI = 128;      // output elements per workgroup
+constexpr int BK = 32;       // contraction tile
 constexpr int LDK = BK + 4;  // padded LDS row (floats)
 constexpr int kThreads = 256;
 
@@ -34,13 +44,13 @@ struct RBFParams {
   float noise;
 };
 
-// Loads of the V tile: 64 probes x 32 m = 512 float4, two per thread.
-template <bool VEC>
+// V tile: (32 NACC) probes x 32 m = 8 NACC float4, NACC per thread.
+template <int NACC, bool VEC>
 __device__ __forceinline__ void load_v_tile(const float* __restrict__ V, int64_t P, int64_t n,
-                                            int64_t p0, int64_t m0, float4 (&reg)[2]) {
+                                            int64_t p0, int64_t m0, float4 (&reg)[NACC]) {
   const int t = threadIdx.x;
 #pragma unroll
-  for (int u = 0; u < 2; ++u) {
+  for (int u = 0; u < NACC; ++u) {
     const int f = t + u * kThreads;
     const int row = f >> 3, c4 = f & 7;
     const int64_t p = p0 + row, m = m0 + 4 * c4;
@@ -60,128 +70,75 @@ __device__ __forceinline__ void load_v_tile(const float* __restrict__ V, int64_t
   }
 }
 
-__device__ __forceinline__ void store_v_tile(float* Vs, const float4 (&reg)[2]) {
+template <int NACC>
+__device__ __forceinline__ void store_v_tile(float* Vs, const float4 (&reg)[NACC]) {
   const int t = threadIdx.x;
 #pragma unroll
-  for (int u = 0; u < 2; ++u) {
+  for (int u = 0; u < NACC; ++u) {
     const int f = t + u * kThreads;
     const int row = f >> 3, c4 = f & 7;
     *reinterpret_cast<float4*>(Vs + row * LDK + 4 * c4) = reg[u];
   }
 }
 
-// Loads of a stored A tile: 128 rows x 32 m = 1024 float4, four per thread.
+// The lane's 16 stored-matrix values A[row][m0 + 16 h + 0..15].
 template <bool VEC>
-__device__ __forceinline__ void load_a_tile(const float* __restrict__ A, int64_t n, int64_t i0,
-                                            int64_t m0, float4 (&reg)[4]) {
-  const int t = threadIdx.x;
+__device__ __forceinline__ void load_a_frag(const float* __restrict__ A, int64_t n, int64_t row,
+                                            int64_t m0, int h, float (&a)[16]) {
+  const int64_t m = m0 + 16 * h;
+  if (row < n) {
+    const float* src = A + row * n + m;
+    if (VEC && m + 16 <= n) {
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int f = t + u * kThreads;
-    const int row = f >> 3, c4 = f & 7;
-    const int64_t i = i0 + row, m = m0 + 4 * c4;
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (i < n) {
-      const float* src = A + i * n + m;
-      if (VEC) {
-        if (m < n) v = *reinterpret_cast<const float4*>(src);
-      } else {
-        if (m + 0 < n) v.x = src[0];
-        if (m + 1 < n) v.y = src[1];
-        if (m + 2 < n) v.z = src[2];
-        if (m + 3 < n) v.w = src[3];
+      for (int q = 0; q < 4; ++q) {
+        const float4 v = *reinterpret_cast<const float4*>(src + 4 * q);
+        a[4 * q + 0] = v.x;
+        a[4 * q + 1] = v.y;
+        a[4 * q + 2] = v.z;
+        a[4 * q + 3] = v.w;
       }
+    } else {
+#pragma unroll
+      for (int s = 0; s < 16; ++s) a[s] = (m + s < n) ? src[s] : 0.f;
     }
-    reg[u] = v;
+  } else {
+#pragma unroll
+    for (int s = 0; s < 16; ++s) a[s] = 0.f;
   }
 }
 
-__device__ __forceinline__ void store_a_tile(float* As, const float4 (&reg)[4]) {
-  const int t = threadIdx.x;
+// The lane's 16 generated RBF values exp2(c |x_row - x_m|^2).
+template <int DIM>
+__device__ __forceinline__ void gen_rbf_frag(const float4* Xs, const float (&xr)[4], float coef,
+                                             int64_t n, int64_t m0, int h, float (&a)[16]) {
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int f = t + u * kThreads;
-    const int row = f >> 3, c4 = f & 7;
-    *reinterpret_cast<float4*>(As + row * LDK + 4 * c4) = reg[u];
-  }
-}
-
-// Generate an RBF tile: thread t owns row (t >> 1) and 16 columns starting at
-// 16 * (t & 1).  xi holds the thread's own point (fixed for the kernel).
-__device__ __forceinline__ void gen_rbf_tile(float* As, const float* Xs, const float (&xi)[4],
-                                             int dim, float coef, int64_t n, int64_t m0) {
-  const int t = threadIdx.x;
-  const int row = t >> 1, cbase = 16 * (t & 1);
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    float vals[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int col = cbase + 4 * q + u;
-      float d2 = 0.f;
-      for (int d = 0; d < dim; ++d) {
-        const float diff = xi[d] - Xs[col * 4 + d];
-        d2 = fmaf(diff, diff, d2);
-      }
-      const float kv = __builtin_amdgcn_exp2f(coef * d2);
-      vals[u] = (m0 + col < n) ? kv : 0.f;
-    }
-    *reinterpret_cast<float4*>(As + row * LDK + cbase + 4 * q) =
-        make_float4(vals[0], vals[1], vals[2], vals[3]);
-  }
-}
-
-// One BK-deep step of MFMAs from the LDS tiles.  The tile's partial sums
-// start from zero and are added to the running total afterwards (blocked
-// summation): the rounding error of an n-term contraction then grows with
-// BK + n / BK instead of n, as in a blocked CPU GEMM.
-__device__ __forceinline__ void mfma_tile(const float* As, const float* Vs, int wi, int wp,
-                                          floatx16 (&acc)[2]) {
-  const int lane = threadIdx.x & 63;
-  const int r = lane & 31, h = lane >> 5;
-  const float* a0p = As + (wi + r) * LDK + h * (BK / 2);
-  const float* a1p = As + (wi + 32 + r) * LDK + h * (BK / 2);
-  const float* bp = Vs + (wp + r) * LDK + h * (BK / 2);
-  floatx16 t0, t1;
-#pragma unroll
-  for (int i = 0; i < 16; ++i) { t0[i] = 0.f; t1[i] = 0.f; }
-#pragma unroll
-  for (int q = 0; q < BK / 8; ++q) {
-    const float4 a0 = *reinterpret_cast<const float4*>(a0p + 4 * q);
-    const float4 a1 = *reinterpret_cast<const float4*>(a1p + 4 * q);
-    const float4 b = *reinterpret_cast<const float4*>(bp + 4 * q);
-    t0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b.x, t0, 0, 0, 0);
-    t1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b.x, t1, 0, 0, 0);
-    t0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b.y, t0, 0, 0, 0);
-    t1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b.y, t1, 0, 0, 0);
-    t0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, b.z, t0, 0, 0, 0);
-    t1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, b.z, t1, 0, 0, 0);
-    t0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, b.w, t0, 0, 0, 0);
-    t1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b.w, t1, 0, 0, 0);
-  }
-#pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    acc[0][i] += t0[i];
-    acc[1][i] += t1[i];
+  for (int s = 0; s < 16; ++s) {
+    const float4 xm = Xs[16 * h + s];
+    float d = xr[0] - xm.x;
+    float d2 = d * d;
+    if (DIM > 1) { d = xr[1] - xm.y; d2 = fmaf(d, d, d2); }
+    if (DIM > 2) { d = xr[2] - xm.z; d2 = fmaf(d, d, d2); }
+    if (DIM > 3) { d = xr[3] - xm.w; d2 = fmaf(d, d, d2); }
+    const float kv = __builtin_amdgcn_exp2f(coef * d2);
+    a[s] = (m0 + 16 * h + s < n) ? kv : 0.f;
   }
 }
 
 // Epilogue: lane l of a 32x32 accumulator holds column (l & 31) and rows
 // 8 * (r / 4) + 4 * (l / 32) + (r % 4), r = 0..15.
-template <bool VEC, bool NUGGET>
+template <int NACC, bool VEC, bool NUGGET>
 __device__ __forceinline__ void store_out(float* __restrict__ W, const float* __restrict__ V,
-                                          int64_t P, int64_t n, int64_t i0, int64_t p0, int wi,
-                                          int wp, const floatx16 (&acc)[2], float noise) {
+                                          int64_t P, int64_t n, int64_t i0w, int64_t p0,
+                                          const floatx16 (&acc)[NACC], float noise) {
   const int lane = threadIdx.x & 63;
-  const int64_t p = p0 + wp + (lane & 31);
-  if (p >= P) return;
 #pragma unroll
-  for (int mi = 0; mi < 2; ++mi) {
+  for (int c = 0; c < NACC; ++c) {
+    const int64_t p = p0 + 32 * c + (lane & 31);
+    if (p >= P) continue;
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
-      const int64_t i = i0 + wi + mi * 32 + 8 * g + 4 * (lane >> 5);
-      float o[4] = {acc[mi][4 * g + 0], acc[mi][4 * g + 1], acc[mi][4 * g + 2],
-                    acc[mi][4 * g + 3]};
+      const int64_t i = i0w + 8 * g + 4 * (lane >> 5);
+      float o[4] = {acc[c][4 * g + 0], acc[c][4 * g + 1], acc[c][4 * g + 2], acc[c][4 * g + 3]};
       if (VEC) {
         if (i < n) {
           if (NUGGET) {
@@ -207,106 +164,658 @@ __device__ __forceinline__ void store_out(float* __restrict__ W, const float* __
   }
 }
 
-template <bool VEC>
+template <int NACC, bool RBF, int DIM, bool VEC>
 __global__ void __launch_bounds__(kThreads)
-    k_matvec_dense_f32(const float* __restrict__ A, const float* __restrict__ V, int64_t P,
-                       int64_t n, float* __restrict__ W) {
-  __shared__ __attribute__((aligned(16))) float As[BI * LDK];
-  __shared__ __attribute__((aligned(16))) float Vs[BP * LDK];
-  const int64_t i0 = int64_t(blockIdx.x) * BI;
-  const int64_t p0 = int64_t(blockIdx.y) * BP;
-  const int wave = threadIdx.x >> 6;
-  const int wi = (wave & 1) * 64, wp = (wave >> 1) * 32;
-  floatx16 acc[2];
-#pragma unroll
-  for (int r = 0; r < 16; ++r) { acc[0][r] = 0.f; acc[1][r] = 0.f; }
+    k_matvec_f32(const float* __restrict__ A, RBFParams prm, const float* __restrict__ V,
+                 int64_t P, int64_t n, float* __restrict__ W) {
+  constexpr int BPW = 32 * NACC;  // probes per workgroup
+  __shared__ __attribute__((aligned(16))) float Vs[2][BPW * LDK];
+  __shared__ float4 Xs[2][BK];
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6, r = lane & 31, h = lane >> 5;
+  const int64_t i0w = int64_t(blockIdx.x) * BI + 32 * wave;
+  const int64_t row = i0w + r;
+  const int64_t p0 = int64_t(blockIdx.y) * BPW;
+  // split-K: slice z of gridDim.z covers tiles [kt_begin, kt_end)
+  const int64_t all_tiles = (n + BK - 1) / BK;
+  const int64_t kt_begin = all_tiles * blockIdx.z / gridDim.z;
+  const int64_t kt_end = all_tiles * (blockIdx.z + 1) / gridDim.z;
+  const int64_t ntiles = kt_end - kt_begin;
+  const int64_t mbase = kt_begin * BK;
+  const bool split = gridDim.z > 1;
+  float* Wout = split ? W + int64_t(blockIdx.z) * P * n : W;
 
-  float4 areg[4], vreg[2];
-  load_a_tile<VEC>(A, n, i0, 0, areg);
-  load_v_tile<VEC>(V, P, n, p0, 0, vreg);
-  for (int64_t m0 = 0; m0 < n; m0 += BK) {
-    store_a_tile(As, areg);
-    store_v_tile(Vs, vreg);
-    __syncthreads();
-    if (m0 + BK < n) {  // prefetch the next tile while the MFMAs run
-      load_a_tile<VEC>(A, n, i0, m0 + BK, areg);
-      load_v_tile<VEC>(V, P, n, p0, m0 + BK, vreg);
+  floatx16 acc[NACC];
+#pragma unroll
+  for (int c = 0; c < NACC; ++c)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[c][e] = 0.f;
+
+  float xr[4] = {0.f, 0.f, 0.f, 0.f};
+  if (RBF && row < n) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) xr[d] = prm.X[row * DIM + d];
+  }
+  auto load_x = [&](int64_t m0) {
+    float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int64_t m = m0 + t;
+    if (m < n) {
+      const float* src = prm.X + m * DIM;
+      x.x = src[0];
+      if (DIM > 1) x.y = src[1];
+      if (DIM > 2) x.z = src[2];
+      if (DIM > 3) x.w = src[3];
     }
-    mfma_tile(As, Vs, wi, wp, acc);
+    return x;
+  };
+
+  float4 vreg[NACC];
+  float areg[16];
+  float4 xnext = make_float4(0.f, 0.f, 0.f, 0.f);
+  load_v_tile<NACC, VEC>(V, P, n, p0, mbase, vreg);
+  store_v_tile<NACC>(Vs[0], vreg);
+  if (RBF) {
+    if (t < BK) Xs[0][t] = load_x(mbase);
+  } else {
+    load_a_frag<VEC>(A, n, row, mbase, h, areg);
+  }
+  __syncthreads();
+
+  for (int64_t kt = 0; kt < ntiles; ++kt) {
+    const int buf = int(kt & 1);
+    const int64_t m0 = mbase + kt * BK;
+    const bool more = kt + 1 < ntiles;
+    if (more) {
+      load_v_tile<NACC, VEC>(V, P, n, p0, m0 + BK, vreg);
+      if (RBF && t < BK) xnext = load_x(m0 + BK);
+    }
+    float a[16];
+    if (RBF) {
+      gen_rbf_frag<DIM>(Xs[buf], xr, prm.coef, n, m0, h, a);
+    } else {
+#pragma unroll
+      for (int s = 0; s < 16; ++s) a[s] = areg[s];
+      if (more) load_a_frag<VEC>(A, n, row, m0 + BK, h, areg);
+    }
+
+    floatx16 tmp[NACC];
+#pragma unroll
+    for (int c = 0; c < NACC; ++c)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) tmp[c][e] = 0.f;
+    const float* vb = Vs[buf] + r * LDK + 16 * h;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      float4 b[NACC];
+#pragma unroll
+      for (int c = 0; c < NACC; ++c)
+        b[c] = *reinterpret_cast<const float4*>(vb + 32 * c * LDK + 4 * q);
+#pragma unroll
+      for (int c = 0; c < NACC; ++c)
+        tmp[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[4 * q + 0], b[c].x, tmp[c], 0, 0, 0);
+#pragma unroll
+      for (int c = 0; c < NACC; ++c)
+        tmp[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[4 * q + 1], b[c].y, tmp[c], 0, 0, 0);
+#pragma unroll
+      for (int c = 0; c < NACC; ++c)
+        tmp[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[4 * q + 2], b[c].z, tmp[c], 0, 0, 0);
+#pragma unroll
+      for (int c = 0; c < NACC; ++c)
+        tmp[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[4 * q + 3], b[c].w, tmp[c], 0, 0, 0);
+    }
+#pragma unroll
+    for (int c = 0; c < NACC; ++c)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[c][e] += tmp[c][e];
+
+    if (more) {
+      store_v_tile<NACC>(Vs[buf ^ 1], vreg);
+      if (RBF && t < BK) Xs[buf ^ 1][t] = xnext;
+    }
     __syncthreads();
   }
-  store_out<VEC, false>(W, V, P, n, i0, p0, wi, wp, acc, 0.f);
+  if (split)
+    store_out<NACC, VEC, false>(Wout, V, P, n, i0w, p0, acc, 0.f);
+  else
+    store_out<NACC, VEC, RBF>(Wout, V, P, n, i0w, p0, acc, prm.noise);
 }
 
-template <bool VEC>
-__global__ void __launch_bounds__(kThreads)
-    k_matvec_rbf_f32(RBFParams prm, const float* __restrict__ V, int64_t P, int64_t n,
-                     float* __restrict__ W) {
-  __shared__ __attribute__((aligned(16))) float As[BI * LDK];
-  __shared__ __attribute__((aligned(16))) float Vs[BP * LDK];
-  __shared__ float Xs[BK * 4];
-  const int64_t i0 = int64_t(blockIdx.x) * BI;
-  const int64_t p0 = int64_t(blockIdx.y) * BP;
-  const int t = threadIdx.x;
-  const int wave = t >> 6;
-  const int wi = (wave & 1) * 64, wp = (wave >> 1) * 32;
-  floatx16 acc[2];
-#pragma unroll
-  for (int r = 0; r < 16; ++r) { acc[0][r] = 0.f; acc[1][r] = 0.f; }
+// ---------------------------------------------------------------------------
+// BF16 paths: v_mfma_f32_32x32x16_bf16 (gfx950), FP32 accumulation.
+// Lane (r, h) supplies A[i = r][k = 8 h .. 8 h + 7] of each 16-deep MFMA, so
+// within a BK = 32 tile it owns 16 contraction slots (8 per MFMA).
+//   NPROD = 1: hi*hi with x_hi = bf16(x);
+//   NPROD = 2: lo*hi + hi*hi -- the matrix split x = hi + lo (lo = bf16(x -
+//              hi)), the probes rounded once to BF16;
+//   NPROD = 3: lo*hi + hi*lo + hi*hi -- both operands split, ~16 significant
+//              bits recovered.
+// BF16 products are exact in FP32, so the accumulation chain needs no
+// blocking: its rounding (~sqrt(n / 16) ulp) stays below the representation
+// error of the split operands.
+//
+// The probes are split into BF16 hi/lo planes once per matvec by
+// k_split_probes (not once per workgroup), into a tile-major layout that is
+// zero-padded to whole tiles and probe blocks, so the tile loads carry no
+// bounds tests.  For the RBF kernel each 32-column tile is stored in the
+// contraction order produced by the MFMA-generated fragments (column groups
+// of four with bits 0 and 1 of the group index swapped).
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-  float xi[4] = {0.f, 0.f, 0.f, 0.f};
-  {
-    const int64_t i = i0 + (t >> 1);
-    if (i < n)
-      for (int d = 0; d < prm.dim; ++d) xi[d] = prm.X[i * prm.dim + d];
-  }
-  float4 vreg[2];
-  load_v_tile<VEC>(V, P, n, p0, 0, vreg);
-  for (int64_t m0 = 0; m0 < n; m0 += BK) {
-    if (t < BK * 4) {
-      const int col = t >> 2, d = t & 3;
-      const int64_t m = m0 + col;
-      Xs[t] = (m < n && d < prm.dim) ? prm.X[m * prm.dim + d] : 0.f;
+constexpr int LDB = BK + 8;  // padded LDS row of bf16 (80 bytes)
+
+__device__ __forceinline__ float bf16_to_f32(__bf16 b) { return float(b); }
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// Split two floats into packed BF16 hi and lo words (one v_cvt_pk_bf16_f32
+// each): hi = bf16(x), lo = bf16(x - hi).
+__device__ __forceinline__ void split_pair(float x0, float x1, uint32_t& hi, uint32_t& lo) {
+  const f32x2 x = {x0, x1};
+  hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2));
+  const f32x2 h = {__builtin_bit_cast(float, hi << 16),
+                   __builtin_bit_cast(float, hi & 0xffff0000u)};
+  lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(x - h, bf16x2));
+}
+
+__device__ __forceinline__ int perm_group(int c4) {
+  return (c4 & 4) | ((c4 & 1) << 1) | ((c4 & 2) >> 1);
+}
+
+// Split BF16 planes of V, zero-padded, tile-major: the 32 columns of tile kt
+// of probe p sit at (kt * Ppad + p) * 32, so one workgroup's probe tile is a
+// contiguous block of 64 * BPW bytes.
+template <bool PERM>
+__global__ void __launch_bounds__(256)
+    k_split_probes(const float* __restrict__ V, int64_t P, int64_t n, int64_t Ppad, int64_t npad,
+                   __bf16* __restrict__ Vhi, __bf16* __restrict__ Vlo) {
+  const int64_t groups = Ppad * (npad / 4);
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < groups; g += stride) {
+    const int64_t p = g / (npad / 4);
+    const int64_t m = 4 * (g % (npad / 4));
+    float x[4] = {0.f, 0.f, 0.f, 0.f};
+    if (p < P) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (m + e < n) x[e] = V[p * n + m + e];
     }
-    store_v_tile(Vs, vreg);
-    __syncthreads();
-    gen_rbf_tile(As, Xs, xi, prm.dim, prm.coef, n, m0);
-    if (m0 + BK < n) load_v_tile<VEC>(V, P, n, p0, m0 + BK, vreg);
-    __syncthreads();
-    mfma_tile(As, Vs, wi, wp, acc);
-    __syncthreads();
+    uint32_t h0, l0, h1, l1;
+    split_pair(x[0], x[1], h0, l0);
+    split_pair(x[2], x[3], h1, l1);
+    const int64_t tile = m / BK;
+    const int c4 = int((m % BK) / 4);
+    const int64_t dst = (tile * Ppad + p) * BK + 4 * (PERM ? perm_group(c4) : c4);
+    *reinterpret_cast<uint2*>(Vhi + dst) = make_uint2(h0, h1);
+    *reinterpret_cast<uint2*>(Vlo + dst) = make_uint2(l0, l1);
   }
-  store_out<VEC, true>(W, V, P, n, i0, p0, wi, wp, acc, prm.noise);
+}
+
+// One probe tile of one plane: BPW = 32 NACC rows x 32 bf16 = 4 BPW pieces of
+// 16 bytes, spread over the 256 threads (NACC / 2 pieces each, or one piece
+// for the first 128 threads when NACC = 1).
+template <int NACC>
+struct PlaneTile {
+  static constexpr int kPieces = 4 * 32 * NACC;
+  static constexpr int kPerThread = (kPieces + kThreads - 1) / kThreads;
+};
+
+template <int NACC>
+__device__ __forceinline__ void load_plane_tile(const __bf16* __restrict__ plane, int64_t Ppad,
+                                                int64_t p0, int64_t kt,
+                                                u32x4 (&reg)[PlaneTile<NACC>::kPerThread]) {
+  const int t = threadIdx.x;
+  const __bf16* base = plane + (kt * Ppad + p0) * BK;  // contiguous BPW x 32 block
+#pragma unroll
+  for (int u = 0; u < PlaneTile<NACC>::kPerThread; ++u) {
+    const int f = t + u * kThreads;
+    if (f < PlaneTile<NACC>::kPieces) reg[u] = *reinterpret_cast<const u32x4*>(base + 8 * f);
+  }
+}
+
+template <int NACC>
+__device__ __forceinline__ void store_plane_tile(__bf16* lds,
+                                                 const u32x4 (&reg)[PlaneTile<NACC>::kPerThread]) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < PlaneTile<NACC>::kPerThread; ++u) {
+    const int f = t + u * kThreads;
+    if (f < PlaneTile<NACC>::kPieces) {
+      const int row = f >> 2, q = f & 3;
+      *reinterpret_cast<u32x4*>(lds + row * LDB + 8 * q) = reg[u];
+    }
+  }
+}
+
+// Contraction index of fragment slot s (0..15) for lane half h (natural order).
+__device__ __forceinline__ int frag_k(int s, int h) { return (s < 8) ? 8 * h + s : 16 + 8 * h + (s - 8); }
+
+template <bool VEC>
+__device__ __forceinline__ void load_a_frag16(const float* __restrict__ A, int64_t n, int64_t row,
+                                              int64_t m0, int h, float (&a)[16]) {
+  if (row < n) {
+    const float* src = A + row * n + m0;
+    if (VEC && m0 + BK <= n) {
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        const float* q = src + 16 * half + 8 * h;
+        const float4 v0 = *reinterpret_cast<const float4*>(q);
+        const float4 v1 = *reinterpret_cast<const float4*>(q + 4);
+        a[8 * half + 0] = v0.x; a[8 * half + 1] = v0.y; a[8 * half + 2] = v0.z;
+        a[8 * half + 3] = v0.w; a[8 * half + 4] = v1.x; a[8 * half + 5] = v1.y;
+        a[8 * half + 6] = v1.z; a[8 * half + 7] = v1.w;
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        const int64_t m = m0 + frag_k(s, h);
+        a[s] = (m < n) ? src[frag_k(s, h)] : 0.f;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < 16; ++s) a[s] = 0.f;
+  }
+}
+
+// RBF fragment for the BF16 kernels.  The base-2 exponent of a 32 x 32 tile,
+//   arg[m][i] = c |x_i|^2 + c |x_m|^2 - 2 c x_i . x_m,
+// is the rank-6 product L R with L[m] = (x_m, y_m, z_m, c |x_m|^2, 1, 0) and
+// R[:, i] = (-2c x_i, -2c y_i, -2c z_i, 1, c |x_i|^2, 0), evaluated by three
+// FP32 MFMAs (32x32x2).  The MFMA output gives lane (r, h) the exponents of
+// its own row i = r at the 16 points m = 8 (s / 4) + 4 h + (s % 4),
+// s = 0..15 -- exactly the BF16 A-fragment of that lane once the contraction
+// index is ordered that way, which the permuted probe planes provide.
+// The lane reads one tile point from LDS per tile, and the VALU work per
+// entry is the exp2 and the hi/lo split.  Points past n carry
+// c |x_m|^2 = -inf, so their entries are exactly 0.  The exponent's absolute
+// rounding error (~1e-6 for points in the unit cube and l = 0.5) is below
+// the 2^-17 relative error of the split BF16 representation.
+__device__ __forceinline__ void gen_rbf_frag_mfma(const float4* Xs, const float (&rb)[3],
+                                                  float (&a)[16]) {
+  const int lane = threadIdx.x & 63;
+  const int h = lane >> 5;
+  const float4 xm = Xs[lane & 31];
+  // A operand of MFMA t: L[m][2t + h]
+  const float a0 = h ? xm.y : xm.x;
+  const float a1 = h ? xm.w : xm.z;
+  const float a2 = h ? 0.f : 1.f;
+  floatx16 d;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) d[e] = 0.f;
+  d = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, rb[0], d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, rb[1], d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, rb[2], d, 0, 0, 0);
+#pragma unroll
+  for (int e = 0; e < 16; ++e) a[e] = __builtin_amdgcn_exp2f(d[e]);
+}
+
+template <int NPROD>
+__device__ __forceinline__ void split_frag(const float (&a)[16], u32x4 (&hi)[2], u32x4 (&lo)[2]) {
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      uint32_t h, l;
+      if (NPROD >= 2) {
+        split_pair(a[8 * kb + 2 * e], a[8 * kb + 2 * e + 1], h, l);
+        lo[kb][e] = l;
+      } else {
+        const f32x2 x = {a[8 * kb + 2 * e], a[8 * kb + 2 * e + 1]};
+        h = __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2));
+      }
+      hi[kb][e] = h;
+    }
+}
+
+__device__ __forceinline__ floatx16 mfma_bf16(const u32x4& a, const u32x4& b, const floatx16& c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
+                                                 __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+// Software-pipelined BF16 matvec: the fragment of tile kt + 1 is generated
+// (or converted) after the MFMAs of tile kt are issued, and the probe tiles
+// are fetched two tiles ahead into alternating register sets (the loop is
+// unrolled by two so the sets keep fixed registers).  RBF kernels here need
+// dim <= 3 (the per-point constant rides in the fourth lane of the float4 of
+// staged points).
+template <int NACC, bool RBF, bool VEC, int NPROD>
+__global__ void __launch_bounds__(kThreads)
+    k_matvec_bf16(const float* __restrict__ A, RBFParams prm, const __bf16* __restrict__ Vhi_g,
+                  const __bf16* __restrict__ Vlo_g, int64_t Ppad, const float* __restrict__ V,
+                  int64_t P, int64_t n, float* __restrict__ W) {
+  constexpr int BPW = 32 * NACC;
+  constexpr bool LO = NPROD == 3;  // probes' low plane used
+  constexpr int PT = PlaneTile<NACC>::kPerThread;
+  __shared__ __attribute__((aligned(16))) __bf16 Vhi[2][BPW * LDB];
+  __shared__ __attribute__((aligned(16))) __bf16 Vlo[LO ? 2 : 1][LO ? BPW * LDB : 8];
+  __shared__ float4 Xs[2][BK];
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6, r = lane & 31, h = lane >> 5;
+  const int64_t i0w = int64_t(blockIdx.x) * BI + 32 * wave;
+  const int64_t row = i0w + r;
+  const int64_t p0 = int64_t(blockIdx.y) * BPW;
+  const int64_t all_tiles = (n + BK - 1) / BK;
+  const int64_t kt_begin = all_tiles * blockIdx.z / gridDim.z;
+  const int64_t kt_end = all_tiles * (blockIdx.z + 1) / gridDim.z;
+  const int64_t ntiles = kt_end - kt_begin;
+  const int64_t mbase = kt_begin * BK;
+  const bool split = gridDim.z > 1;
+  float* Wout = split ? W + int64_t(blockIdx.z) * P * n : W;
+  const int dim = prm.dim;
+
+  floatx16 acc[NACC];
+#pragma unroll
+  for (int c = 0; c < NACC; ++c)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[c][e] = 0.f;
+
+  // lane constants: B operand R[2t + h][row] of the exponent MFMAs
+  float rb[3] = {0.f, 0.f, 0.f};
+  if (RBF) {
+    float u[3] = {0.f, 0.f, 0.f};
+    float ai = 0.f;
+    if (row < n) {
+      for (int d = 0; d < dim; ++d) {
+        const float x = prm.X[row * dim + d];
+        u[d] = -2.f * prm.coef * x;
+        ai = fmaf(x * x, prm.coef, ai);
+      }
+    }
+    rb[0] = h ? u[1] : u[0];
+    rb[1] = h ? 1.f : u[2];
+    rb[2] = h ? 0.f : ai;
+  }
+  auto prep_x = [&](int64_t m0) {
+    float4 x = make_float4(0.f, 0.f, 0.f, -__builtin_inff());
+    const int64_t m = m0 + t;
+    if (m < n) {
+      const float* src = prm.X + m * dim;
+      float b = 0.f;
+      x.x = src[0];
+      b = fmaf(x.x * x.x, prm.coef, b);
+      if (dim > 1) { x.y = src[1]; b = fmaf(x.y * x.y, prm.coef, b); }
+      if (dim > 2) { x.z = src[2]; b = fmaf(x.z * x.z, prm.coef, b); }
+      x.w = b;
+    }
+    return x;
+  };
+
+  // prologue: probe tile 0 staged, tile 1 in flight, points of tiles 0 and 1
+  u32x4 hA[PT], hB[PT], lA[PT], lB[PT];
+  float areg[16];
+  float4 xnext = make_float4(0.f, 0.f, 0.f, -__builtin_inff());
+  load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin, hA);
+  if (LO) load_plane_tile<NACC>(Vlo_g, Ppad, p0, kt_begin, lA);
+  if (ntiles > 1) {
+    load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin + 1, hB);
+    if (LO) load_plane_tile<NACC>(Vlo_g, Ppad, p0, kt_begin + 1, lB);
+  }
+  store_plane_tile<NACC>(Vhi[0], hA);
+  if (LO) store_plane_tile<NACC>(&Vlo[0][0], lA);
+  if (RBF) {
+    if (t < BK) {
+      Xs[0][t] = prep_x(mbase);
+      Xs[1][t] = (ntiles > 1) ? prep_x(mbase + BK) : prep_x(n);
+    }
+  } else {
+    load_a_frag16<VEC>(A, n, row, mbase, h, areg);
+  }
+  __syncthreads();
+  u32x4 ahi[2], alo[2];
+  {
+    float a[16];
+    if (RBF) {
+      gen_rbf_frag_mfma(Xs[0], rb, a);
+    } else {
+#pragma unroll
+      for (int s = 0; s < 16; ++s) a[s] = areg[s];
+      if (ntiles > 1) load_a_frag16<VEC>(A, n, row, mbase + BK, h, areg);
+    }
+    split_frag<NPROD>(a, ahi, alo);
+  }
+
+  // One tile: issue the loads of tile kt + 2 into (ih, il) -- they held tile
+  // kt, staged one iteration ago -- run the MFMAs of tile kt, build the next
+  // fragment, and stage tile kt + 1 from (sh, sl) into the other LDS buffer.
+  auto step = [&](int64_t kt, u32x4 (&ih)[PT], u32x4 (&il)[PT], const u32x4 (&sh)[PT],
+                  const u32x4 (&sl)[PT]) {
+    const int buf = int(kt & 1);
+    const int64_t m0 = mbase + kt * BK;
+    const bool more = kt + 1 < ntiles;
+    if (kt + 2 < ntiles) {
+      load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin + kt + 2, ih);
+      if (LO) load_plane_tile<NACC>(Vlo_g, Ppad, p0, kt_begin + kt + 2, il);
+    }
+    if (RBF && t < BK) xnext = (kt + 2 < ntiles) ? prep_x(m0 + 2 * BK) : prep_x(n);
+
+    const __bf16* vh = Vhi[buf] + r * LDB + 8 * h;
+    const __bf16* vl = &Vlo[LO ? buf : 0][0] + r * LDB + 8 * h;
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+      u32x4 bh[NACC], bl[NACC];
+#pragma unroll
+      for (int c = 0; c < NACC; ++c) {
+        bh[c] = *reinterpret_cast<const u32x4*>(vh + 32 * c * LDB + 16 * kb);
+        if (LO) bl[c] = *reinterpret_cast<const u32x4*>(vl + 32 * c * LDB + 16 * kb);
+      }
+      if (NPROD >= 2) {
+#pragma unroll
+        for (int c = 0; c < NACC; ++c) acc[c] = mfma_bf16(alo[kb], bh[c], acc[c]);
+      }
+      if (LO) {
+#pragma unroll
+        for (int c = 0; c < NACC; ++c) acc[c] = mfma_bf16(ahi[kb], bl[c], acc[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < NACC; ++c) acc[c] = mfma_bf16(ahi[kb], bh[c], acc[c]);
+    }
+
+    // next tile's fragment while the MFMAs drain
+    u32x4 nhi[2], nlo[2];
+    if (more) {
+      float a[16];
+      if (RBF) {
+        gen_rbf_frag_mfma(Xs[buf ^ 1], rb, a);
+      } else {
+#pragma unroll
+        for (int s = 0; s < 16; ++s) a[s] = areg[s];
+        if (kt + 2 < ntiles) load_a_frag16<VEC>(A, n, row, m0 + 2 * BK, h, areg);
+      }
+      split_frag<NPROD>(a, nhi, nlo);
+      store_plane_tile<NACC>(Vhi[buf ^ 1], sh);
+      if (LO) store_plane_tile<NACC>(&Vlo[LO ? (buf ^ 1) : 0][0], sl);
+    }
+    if (RBF && t < BK) Xs[buf][t] = xnext;  // points of tile kt + 2
+    __syncthreads();
+    if (more) {
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb) {
+        ahi[kb] = nhi[kb];
+        if (NPROD >= 2) alo[kb] = nlo[kb];
+      }
+    }
+  };
+
+  for (int64_t kt = 0; kt < ntiles; kt += 2) {
+    step(kt, hA, lA, hB, lB);
+    if (kt + 1 < ntiles) step(kt + 1, hB, lB, hA, lA);
+  }
+  if (split)
+    store_out<NACC, VEC, false>(Wout, V, P, n, i0w, p0, acc, 0.f);
+  else
+    store_out<NACC, VEC, RBF>(Wout, V, P, n, i0w, p0, acc, prm.noise);
 }
 
 inline bool aligned16(const void* ptr) { return (reinterpret_cast<uintptr_t>(ptr) & 15) == 0; }
 
+// W = sum_z partial[z] (+ s2 V), summed in the fixed order z = 0..S-1.
+template <bool NUGGET>
+__global__ void __launch_bounds__(256)
+    k_splitk_reduce(const float* __restrict__ partial, int S, int64_t count,
+                    const float* __restrict__ V, float noise, float* __restrict__ W) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
+    float v = partial[i];
+    for (int z = 1; z < S; ++z) v += partial[int64_t(z) * count + i];
+    if (NUGGET) v = fmaf(noise, V[i], v);
+    W[i] = v;
+  }
+}
+
+template <int NACC, bool RBF, int NPROD>
+hipError_t launch_tpl_bf16_n(hipStream_t s, const float* A, const RBFParams& prm,
+                             const __bf16* Vhi, const __bf16* Vlo, int64_t Ppad, const float* V,
+                             int64_t P, int64_t n, float* W, bool vec, int S) {
+  const dim3 grid(unsigned((n + BI - 1) / BI), unsigned((P + 32 * NACC - 1) / (32 * NACC)),
+                  unsigned(S));
+  const dim3 block(kThreads);
+  if (vec)
+    hipLaunchKernelGGL((k_matvec_bf16<NACC, RBF, true, NPROD>), grid, block, 0, s, A, prm, Vhi,
+                       Vlo, Ppad, V, P, n, W);
+  else
+    hipLaunchKernelGGL((k_matvec_bf16<NACC, RBF, false, NPROD>), grid, block, 0, s, A, prm, Vhi,
+                       Vlo, Ppad, V, P, n, W);
+  return hipGetLastError();
+}
+
+template <int NACC, bool RBF>
+hipError_t launch_tpl_bf16(hipStream_t s, const float* A, const RBFParams& prm,
+                           const __bf16* Vhi, const __bf16* Vlo, int64_t Ppad, const float* V,
+                           int64_t P, int64_t n, float* W, bool vec, int nprod, int S) {
+  if (nprod == 3)
+    return launch_tpl_bf16_n<NACC, RBF, 3>(s, A, prm, Vhi, Vlo, Ppad, V, P, n, W, vec, S);
+  if (nprod == 2)
+    return launch_tpl_bf16_n<NACC, RBF, 2>(s, A, prm, Vhi, Vlo, Ppad, V, P, n, W, vec, S);
+  return launch_tpl_bf16_n<NACC, RBF, 1>(s, A, prm, Vhi, Vlo, Ppad, V, P, n, W, vec, S);
+}
+
+template <int NACC, bool RBF, int DIM>
+hipError_t launch_tpl(hipStream_t s, const float* A, const RBFParams& prm, const float* V,
+                      int64_t P, int64_t n, float* W, bool vec, int S) {
+  const dim3 grid(unsigned((n + BI - 1) / BI), unsigned((P + 32 * NACC - 1) / (32 * NACC)),
+                  unsigned(S));
+  if (vec)
+    hipLaunchKernelGGL((k_matvec_f32<NACC, RBF, DIM, true>), grid, dim3(kThreads), 0, s, A, prm,
+                       V, P, n, W);
+  else
+    hipLaunchKernelGGL((k_matvec_f32<NACC, RBF, DIM, false>), grid, dim3(kThreads), 0, s, A, prm,
+                       V, P, n, W);
+  return hipGetLastError();
+}
+
+template <bool RBF, int DIM>
+hipError_t launch_f32(hipStream_t s, const float* A, const RBFParams& prm, const float* V,
+                      int64_t P, int64_t n, float* W, bool vec, int S) {
+  if (P <= 32) return launch_tpl<1, RBF, DIM>(s, A, prm, V, P, n, W, vec, S);
+  if (P <= 64) return launch_tpl<2, RBF, DIM>(s, A, prm, V, P, n, W, vec, S);
+  return launch_tpl<4, RBF, DIM>(s, A, prm, V, P, n, W, vec, S);
+}
+
+int nacc_for(int64_t P) { return P <= 32 ? 1 : (P <= 64 ? 2 : 4); }
+
+template <bool RBF>
+hipError_t launch_bf16(hipStream_t s, const float* A, const RBFParams& prm, const __bf16* Vhi,
+                       const __bf16* Vlo, int64_t Ppad, const float* V, int64_t P, int64_t n,
+                       float* W, bool vec, int nprod, int S) {
+  switch (nacc_for(P)) {
+    case 1: return launch_tpl_bf16<1, RBF>(s, A, prm, Vhi, Vlo, Ppad, V, P, n, W, vec, nprod, S);
+    case 2: return launch_tpl_bf16<2, RBF>(s, A, prm, Vhi, Vlo, Ppad, V, P, n, W, vec, nprod, S);
+    default: return launch_tpl_bf16<4, RBF>(s, A, prm, Vhi, Vlo, Ppad, V, P, n, W, vec, nprod, S);
+  }
+}
+
+// Split-K factor: enough workgroups for two per CU (256 CUs), as long as
+// every slice keeps at least 8 tiles and the partial sums fit the scratch.
+int choose_splitk(int64_t n, int64_t P, int64_t avail_floats) {
+  const int64_t nacc = nacc_for(P);
+  const int64_t wgs = ((n + BI - 1) / BI) * ((P + 32 * nacc - 1) / (32 * nacc));
+  const int64_t tiles = (n + BK - 1) / BK;
+  int S = 1;
+  while (S < kMaxSplitK && wgs * S < 512 && tiles / (2 * S) >= 8 &&
+         int64_t(2 * S) * P * n <= avail_floats)
+    S *= 2;
+  return S;
+}
+
 }  // namespace
 
+int64_t matvec_scratch_floats(int64_t n, int64_t P) {
+  const int64_t bpw = 32 * nacc_for(P);
+  const int64_t Ppad = (P + bpw - 1) / bpw * bpw;
+  const int64_t npad = (n + BK - 1) / BK * BK;
+  return kMaxSplitK * P * n + Ppad * npad;  // partial sums + two BF16 planes
+}
+
 hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, int64_t P,
-                         float* W) {
+                         float* W, float* scratch, int64_t scratch_floats) {
   const int64_t n = op.n;
   if (n <= 0 || P <= 0) return hipSuccess;
-  const dim3 grid(unsigned((n + BI - 1) / BI), unsigned((P + BP - 1) / BP));
-  const dim3 block(kThreads);
   const bool vec = (n % 4 == 0) && aligned16(V) && aligned16(W);
+  const bool fp32 = op.precision == SLQ_PREC_FP32;
+  const int nprod = op.precision == SLQ_PREC_BF16X3 ? 3 : (op.precision == SLQ_PREC_BF16X2 ? 2 : 1);
+  // scratch layout: [BF16 planes (bf16 path only)] [split-K partial sums]
+  int64_t planes_floats = 0;
+  __bf16 *Vhi = nullptr, *Vlo = nullptr;
+  const int64_t npad = (n + BK - 1) / BK * BK;
+  const int64_t bpw = 32 * nacc_for(P);
+  const int64_t Ppad = (P + bpw - 1) / bpw * bpw;
+  if (!fp32) {
+    if (!scratch) return hipErrorInvalidValue;
+    planes_floats = Ppad * npad;
+    if (planes_floats > scratch_floats) return hipErrorInvalidValue;
+    Vhi = reinterpret_cast<__bf16*>(scratch);
+    Vlo = Vhi + Ppad * npad;
+    int64_t blocks = (Ppad * npad / 4 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (op.kind == SLQ_OP_RBF)
+      hipLaunchKernelGGL(k_split_probes<true>, dim3(unsigned(blocks)), dim3(256), 0, s, V, P, n,
+                         Ppad, npad, Vhi, Vlo);
+    else
+      hipLaunchKernelGGL(k_split_probes<false>, dim3(unsigned(blocks)), dim3(256), 0, s, V, P, n,
+                         Ppad, npad, Vhi, Vlo);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  float* partial = scratch ? scratch + planes_floats : nullptr;
+  const int S = partial ? choose_splitk(n, P, scratch_floats - planes_floats) : 1;
+  float* out = (S > 1) ? partial : W;
+  RBFParams prm{};
+  hipError_t err;
   if (op.kind == SLQ_OP_DENSE) {
     const bool v2 = vec && aligned16(op.matrix);
-    if (v2)
-      hipLaunchKernelGGL(k_matvec_dense_f32<true>, grid, block, 0, s, op.matrix, V, P, n, W);
-    else
-      hipLaunchKernelGGL(k_matvec_dense_f32<false>, grid, block, 0, s, op.matrix, V, P, n, W);
+    err = fp32 ? launch_f32<false, 1>(s, op.matrix, prm, V, P, n, out, v2, S)
+               : launch_bf16<false>(s, op.matrix, prm, Vhi, Vlo, Ppad, V, P, n, out, v2, nprod, S);
   } else {
-    RBFParams prm;
     prm.X = op.points;
     prm.dim = op.dim;
     prm.coef = float(-1.4426950408889634 / (2.0 * op.lengthscale * op.lengthscale));
     prm.noise = float(op.noise);
-    if (vec)
-      hipLaunchKernelGGL(k_matvec_rbf_f32<true>, grid, block, 0, s, prm, V, P, n, W);
-    else
-      hipLaunchKernelGGL(k_matvec_rbf_f32<false>, grid, block, 0, s, prm, V, P, n, W);
+    if (!fp32) {
+      if (op.dim > 3) return hipErrorInvalidValue;  // rejected by the API before launch
+      err = launch_bf16<true>(s, nullptr, prm, Vhi, Vlo, Ppad, V, P, n, out, vec, nprod, S);
+    } else {
+      switch (op.dim) {
+        case 1: err = launch_f32<true, 1>(s, nullptr, prm, V, P, n, out, vec, S); break;
+        case 2: err = launch_f32<true, 2>(s, nullptr, prm, V, P, n, out, vec, S); break;
+        case 3: err = launch_f32<true, 3>(s, nullptr, prm, V, P, n, out, vec, S); break;
+        default: err = launch_f32<true, 4>(s, nullptr, prm, V, P, n, out, vec, S); break;
+      }
+    }
   }
+  if (err != hipSuccess || S == 1) return err;
+  const int64_t count = P * n;
+  int64_t blocks = (count + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (op.kind == SLQ_OP_RBF)
+    hipLaunchKernelGGL(k_splitk_reduce<true>, dim3(unsigned(blocks)), dim3(256), 0, s, partial, S,
+                       count, V, prm.noise, W);
+  else
+    hipLaunchKernelGGL(k_splitk_reduce<false>, dim3(unsigned(blocks)), dim3(256), 0, s, partial,
+                       S, count, V, 0.f, W);
   return hipGetLastError();
 }
 

@@ -10,8 +10,8 @@
 // Algorithm: implicit QL with Wilkinson-type shifts on the symmetric
 // tridiagonal (d, e), applying every plane rotation to the tracked row.
 // The bulge chase is sequential, so one lane owns one probe; the (d, e, z)
-// arrays live in a global workspace laid out [index][probe] so that the
-// lanes of a wave touch consecutive addresses.
+// arrays live in LDS (a global workspace for depths too large for it), laid
+// out [index][lane] so that the lanes of a wave touch consecutive addresses.
 //
 // The tridiagonal is extracted from the Arnoldi Hessenberg matrix exactly as
 // decomp.py:133-135 does it, in FP32: alpha_i = H_ii and
@@ -26,29 +26,32 @@ namespace slq {
 namespace {
 
 constexpr int kMaxSweeps = 64;
+constexpr int kLdsBytes = 64 * 1024;
 
-__global__ void __launch_bounds__(64)
-    k_tridiag_quadrature(const float* __restrict__ H, const float* __restrict__ scale, int64_t P,
-                         int64_t k, double* __restrict__ work, double* __restrict__ per_probe,
-                         float* __restrict__ alpha, float* __restrict__ beta) {
-  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= P) return;
-  double* d = work;
-  double* e = work + k * P;
-  double* z = work + 2 * k * P;
-#define D(i) d[int64_t(i) * P + p]
-#define E(i) e[int64_t(i) * P + p]
-#define Z(i) z[int64_t(i) * P + p]
-  const float* Hp = H + p * k * k;  // column-major: H[row][col] at col * k + row
+// The (d, e, z) arrays of lane `lane` among `lanes` probes: element i at
+// base[i * lanes + lane], so the lanes of a wave touch consecutive addresses.
+struct Arrays {
+  double* d;
+  double* e;
+  double* z;
+  int64_t stride;
+};
+
+__device__ void ql_quadrature(const float* __restrict__ Hp, int64_t k, const Arrays& a,
+                              float* alpha, float* beta, double len, double* out) {
+#define D(i) a.d[int64_t(i) * a.stride]
+#define E(i) a.e[int64_t(i) * a.stride]
+#define Z(i) a.z[int64_t(i) * a.stride]
+  // tridiagonal of T = (H + H^T)/2 in FP32 (decomp.py:133-135); H column-major
   for (int64_t i = 0; i < k; ++i) {
-    const float a = Hp[i * k + i];
-    D(i) = double(a);
+    const float av = Hp[i * k + i];
+    D(i) = double(av);
     Z(i) = (i == 0) ? 1.0 : 0.0;
-    if (alpha) alpha[p * k + i] = a;
+    if (alpha) alpha[i] = av;
     if (i + 1 < k) {
       const float b = 0.5f * (Hp[(i + 1) * k + i] + Hp[i * k + i + 1]);
       E(i) = double(b);
-      if (beta) beta[p * (k - 1) + i] = b;
+      if (beta) beta[i] = b;
     } else {
       E(i) = 0.0;
     }
@@ -104,15 +107,39 @@ __global__ void __launch_bounds__(64)
   }
 
   double acc = 0.0;
-  for (int64_t j = 0; j < k; ++j) {
-    const double w = Z(j) * Z(j);
-    acc += w * log(D(j));
-  }
-  const double len = double(scale[p]);
-  per_probe[p] = failed ? __builtin_nan("") : len * len * acc;
+  for (int64_t j = 0; j < k; ++j) acc += Z(j) * Z(j) * log(D(j));
+  *out = failed ? __builtin_nan("") : len * len * acc;
 #undef D
 #undef E
 #undef Z
+}
+
+// Arrays in LDS: `lanes` probes per block, 3 k doubles each.
+__global__ void __launch_bounds__(64)
+    k_tridiag_quadrature_lds(const float* __restrict__ H, const float* __restrict__ scale,
+                             int64_t P, int64_t k, double* __restrict__ per_probe,
+                             float* __restrict__ alpha, float* __restrict__ beta) {
+  extern __shared__ double lds[];
+  const int lanes = blockDim.x;
+  const int lane = threadIdx.x;
+  const int64_t p = int64_t(blockIdx.x) * lanes + lane;
+  if (p >= P) return;
+  Arrays a{lds + lane, lds + k * lanes + lane, lds + 2 * k * lanes + lane, lanes};
+  ql_quadrature(H + p * k * k, k, a, alpha ? alpha + p * k : nullptr,
+                beta ? beta + p * (k - 1) : nullptr, double(scale[p]), per_probe + p);
+}
+
+// Arrays in a global workspace (3 k P doubles) for depths too large for LDS.
+__global__ void __launch_bounds__(64)
+    k_tridiag_quadrature_global(const float* __restrict__ H, const float* __restrict__ scale,
+                                int64_t P, int64_t k, double* __restrict__ work,
+                                double* __restrict__ per_probe, float* __restrict__ alpha,
+                                float* __restrict__ beta) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  Arrays a{work + p, work + k * P + p, work + 2 * k * P + p, P};
+  ql_quadrature(H + p * k * k, k, a, alpha ? alpha + p * k : nullptr,
+                beta ? beta + p * (k - 1) : nullptr, double(scale[p]), per_probe + p);
 }
 
 // Probe mean in a fixed (sequential) order: deterministic for a given P.
@@ -132,9 +159,19 @@ hipError_t launch_tridiag_quadrature(hipStream_t s, const float* H, const float*
                                      float* alpha, float* beta) {
   if (P <= 0) return hipSuccess;
   if (k > 0) {
-    const unsigned blocks = unsigned((P + 63) / 64);
-    hipLaunchKernelGGL(k_tridiag_quadrature, dim3(blocks), dim3(64), 0, s, H, scale, P, k, work,
-                       per_probe, alpha, beta);
+    // probes per block: as many as fit 3 k doubles each into 64 KB of LDS
+    int lanes = 64;
+    while (lanes > 1 && int64_t(lanes) * 3 * k * 8 > kLdsBytes) lanes /= 2;
+    if (int64_t(lanes) * 3 * k * 8 <= kLdsBytes) {
+      const unsigned blocks = unsigned((P + lanes - 1) / lanes);
+      const size_t shmem = size_t(lanes) * 3 * size_t(k) * sizeof(double);
+      hipLaunchKernelGGL(k_tridiag_quadrature_lds, dim3(blocks), dim3(lanes), shmem, s, H, scale,
+                         P, k, per_probe, alpha, beta);
+    } else {
+      const unsigned blocks = unsigned((P + 63) / 64);
+      hipLaunchKernelGGL(k_tridiag_quadrature_global, dim3(blocks), dim3(64), 0, s, H, scale, P,
+                         k, work, per_probe, alpha, beta);
+    }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
   }

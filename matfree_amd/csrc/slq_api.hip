@@ -17,7 +17,11 @@ struct slq_context {
   };
   // pooled scratch, grown on demand (never shrunk until destroy)
   Buf partial1, partial2, norm2, work_w, work_w2, basis, hmat, resid, vec0, scal1, scal2, eig,
-      per_probe, mean;
+      per_probe, mean, splitk;
+  // measurement hooks: event pairs per kernel class, reused across reads
+  bool profiling = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events[SLQ_KC_COUNT];
+  size_t used[SLQ_KC_COUNT] = {};
 };
 
 namespace slq {
@@ -88,7 +92,8 @@ std::string exceeds_msg(int64_t k, int64_t n) {
 int check_operator(slq_context* ctx, const slq_operator* op) {
   if (!op) return fail(ctx, SLQ_EINVAL, "operator descriptor is NULL");
   if (op->n <= 0) return fail(ctx, SLQ_EINVAL, "operator dimension n must be positive");
-  if (op->precision != SLQ_PREC_FP32)
+  if (op->precision != SLQ_PREC_FP32 && op->precision != SLQ_PREC_BF16 &&
+      op->precision != SLQ_PREC_BF16X2 && op->precision != SLQ_PREC_BF16X3)
     return fail(ctx, SLQ_EUNSUPPORTED,
                 "precision " + std::to_string(op->precision) + " is not available in this build");
   if (op->kind == SLQ_OP_DENSE) {
@@ -98,11 +103,38 @@ int check_operator(slq_context* ctx, const slq_operator* op) {
     if (op->dim < 1 || op->dim > 4)
       return fail(ctx, SLQ_EINVAL, "RBF point dimension must be in 1..4");
     if (!(op->lengthscale > 0.0)) return fail(ctx, SLQ_EINVAL, "RBF lengthscale must be > 0");
+    if (op->precision != SLQ_PREC_FP32 && op->dim > 3)
+      return fail(ctx, SLQ_EUNSUPPORTED, "BF16 RBF kernels support point dimension <= 3");
   } else {
     return fail(ctx, SLQ_EINVAL, "unknown operator kind " + std::to_string(op->kind));
   }
   return SLQ_OK;
 }
+
+// Bracket one launch with an event pair when profiling is on.
+struct Timed {
+  slq_context* ctx;
+  hipEvent_t end = nullptr;
+  Timed(slq_context* c, int kc) : ctx(c) {
+    if (!ctx->profiling) return;
+    auto& vec = ctx->events[kc];
+    size_t& u = ctx->used[kc];
+    if (u == vec.size()) {
+      hipEvent_t a = nullptr, b = nullptr;
+      if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+      }
+      vec.emplace_back(a, b);
+    }
+    (void)hipEventRecord(vec[u].first, ctx->stream);
+    end = vec[u].second;
+    ++u;
+  }
+  ~Timed() {
+    if (end) (void)hipEventRecord(end, ctx->stream);
+  }
+};
 
 int set_device(slq_context* ctx) {
   SLQ_HIP(ctx, hipSetDevice(ctx->device));
@@ -116,7 +148,9 @@ int hessenberg_core(slq_context* ctx, const slq_operator& op, const float* V0, i
   hipStream_t s = ctx->stream;
   const int64_t nb = num_chunks(n);
   const int pstride = int(k > 0 ? k : 1);
-  float *part1, *part2, *norm2, *w, *w2;
+  float *part1, *part2, *norm2, *w, *w2, *mvs;
+  const int64_t mv_floats = matvec_scratch_floats(n, P);
+  SLQ_TRY(ensure_t(ctx, ctx->splitk, size_t(mv_floats), &mvs));
   SLQ_TRY(ensure_t(ctx, ctx->partial1, size_t(P * nb * pstride), &part1));
   SLQ_TRY(ensure_t(ctx, ctx->partial2, size_t(P * nb * pstride), &part2));
   SLQ_TRY(ensure_t(ctx, ctx->norm2, size_t(P * nb), &norm2));
@@ -139,11 +173,18 @@ int hessenberg_core(slq_context* ctx, const slq_operator& op, const float* V0, i
     const int ncols = int(j + 1);
     float* hcol = H + j * k;  // column j of every probe, stride k * k
     // w = A v (decomp.py:460)
-    SLQ_HIP(ctx, launch_matvec(s, op, qj, P, w));
+    {
+      Timed tm(ctx, SLQ_KC_MATVEC);
+      SLQ_HIP(ctx, launch_matvec(s, op, qj, P, w, mvs, mv_floats));
+    }
     // h = Q^T w (decomp.py:463)
-    SLQ_HIP(ctx, launch_dots(s, Q, w, P, n, ncols, part1, pstride));
+    {
+      Timed tm(ctx, SLQ_KC_DOTS);
+      SLQ_HIP(ctx, launch_dots(s, Q, w, P, n, ncols, part1, pstride));
+    }
     // w = w - Q h (decomp.py:464), then the second pass's Q^T w (decomp.py:468)
     if (ncols <= kFusedMaxCols) {
+      Timed tm(ctx, SLQ_KC_PROJECT_DOTS);
       SLQ_HIP(ctx, launch_project_dots(s, Q, w, w2, P, n, ncols, part1, part2, pstride, hcol, k * k));
     } else {
       SLQ_HIP(ctx, launch_project(s, Q, w, w2, P, n, ncols, part1, pstride, nullptr, hcol, k * k));
@@ -152,10 +193,14 @@ int hessenberg_core(slq_context* ctx, const slq_operator& op, const float* V0, i
     // w = w - Q (Q^T w) and <w, w> (decomp.py:468, 471)
     const bool last = (j + 1 == k);
     float* dst = last ? R : w;
-    SLQ_HIP(ctx, launch_project(s, Q, w2, dst, P, n, ncols, part2, pstride, norm2, nullptr, 0));
+    {
+      Timed tm(ctx, SLQ_KC_PROJECT);
+      SLQ_HIP(ctx, launch_project(s, Q, w2, dst, P, n, ncols, part2, pstride, norm2, nullptr, 0));
+    }
     if (!last) {
       // length = sqrt(<w, w>); h[j+1] = length (decomp.py:471-475); next
       // step's v /= length; Q[:, j+1] = v (decomp.py:456-457)
+      Timed tm(ctx, SLQ_KC_NORMALIZE);
       SLQ_HIP(ctx, launch_normalize(s, w, Q + (j + 1) * P * n, P, n, norm2, nullptr, hcol + j + 1,
                                     k * k, nullptr));
     }
@@ -190,10 +235,15 @@ int slq_context_destroy(slq_context* ctx) {
   if (!ctx) return SLQ_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  for (auto& vec : ctx->events)
+    for (auto& ev : vec) {
+      (void)hipEventDestroy(ev.first);
+      (void)hipEventDestroy(ev.second);
+    }
   slq_context::Buf* bufs[] = {&ctx->partial1, &ctx->partial2, &ctx->norm2,  &ctx->work_w,
                               &ctx->work_w2,  &ctx->basis,    &ctx->hmat,   &ctx->resid,
                               &ctx->vec0,     &ctx->scal1,    &ctx->scal2,  &ctx->eig,
-                              &ctx->per_probe, &ctx->mean};
+                              &ctx->per_probe, &ctx->mean, &ctx->splitk};
   for (auto* b : bufs)
     if (b->ptr) (void)hipFree(b->ptr);
   delete ctx;
@@ -213,7 +263,7 @@ int64_t slq_context_scratch_bytes(const slq_context* ctx) {
   const slq_context::Buf* bufs[] = {&ctx->partial1, &ctx->partial2, &ctx->norm2,  &ctx->work_w,
                                     &ctx->work_w2,  &ctx->basis,    &ctx->hmat,   &ctx->resid,
                                     &ctx->vec0,     &ctx->scal1,    &ctx->scal2,  &ctx->eig,
-                                    &ctx->per_probe, &ctx->mean};
+                                    &ctx->per_probe, &ctx->mean, &ctx->splitk};
   int64_t total = 0;
   for (auto* b : bufs) total += int64_t(b->bytes);
   return total;
@@ -235,7 +285,10 @@ int slq_matvec(slq_context* ctx, const slq_operator* op, const float* V, int64_t
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of vectors P must be positive");
   if (!V || !W) return fail(ctx, SLQ_EINVAL, "input or output pointer is NULL");
   SLQ_TRY(set_device(ctx));
-  SLQ_HIP(ctx, launch_matvec(ctx->stream, *op, V, P, W));
+  float* mvs;
+  const int64_t mv_floats = matvec_scratch_floats(op->n, P);
+  SLQ_TRY(ensure_t(ctx, ctx->splitk, size_t(mv_floats), &mvs));
+  SLQ_HIP(ctx, launch_matvec(ctx->stream, *op, V, P, W, mvs, mv_floats));
   return SLQ_OK;
 }
 
@@ -284,7 +337,40 @@ int slq_logdet(slq_context* ctx, const slq_operator* op, const float* Z, int64_t
   // tridiag_sym(matvec, v0) (funm.py:237 -> decomp.py:130-143)
   SLQ_TRY(hessenberg_core(ctx, *op, v0, P, k, Q, H, R, c));
   // |z|^2 [log T]_00 (funm.py:239-241) and the probe mean (stochtrace.py:50)
-  SLQ_HIP(ctx, launch_tridiag_quadrature(s, H, ell, P, k, eig, pp, mean, alpha, beta));
+  {
+    Timed tm(ctx, SLQ_KC_QUADRATURE);
+    SLQ_HIP(ctx, launch_tridiag_quadrature(s, H, ell, P, k, eig, pp, mean, alpha, beta));
+  }
+  return SLQ_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int slq_profile_enable(slq_context* ctx, int on) {
+  if (!ctx) return SLQ_EINVAL;
+  ctx->profiling = (on != 0);
+  return SLQ_OK;
+}
+
+int slq_profile_read(slq_context* ctx, int kernel_class, double* total_ms, int64_t* launches) {
+  if (!ctx) return SLQ_EINVAL;
+  if (kernel_class < 0 || kernel_class >= SLQ_KC_COUNT || !total_ms || !launches)
+    return fail(ctx, SLQ_EINVAL, "bad kernel class or NULL output");
+  SLQ_TRY(set_device(ctx));
+  SLQ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  double sum = 0.0;
+  auto& vec = ctx->events[kernel_class];
+  const size_t used = ctx->used[kernel_class];
+  for (size_t i = 0; i < used; ++i) {
+    float ms = 0.f;
+    SLQ_HIP(ctx, hipEventElapsedTime(&ms, vec[i].first, vec[i].second));
+    sum += ms;
+  }
+  ctx->used[kernel_class] = 0;
+  *total_ms = sum;
+  *launches = int64_t(used);
   return SLQ_OK;
 }
 

@@ -47,8 +47,14 @@ hipError_t launch_normalize(hipStream_t s, const float* x, float* out, int64_t P
                             int64_t hslot_stride, float* inv_out);
 
 // ---- block matvec (k_matvec.hip) -------------------------------------------
+// Largest split-K factor of the block matvec; its partial sums need
+// kMaxSplitK * P * n floats of scratch (split-K is off when scratch is null).
+constexpr int kMaxSplitK = 4;
+// Scratch (in floats) the block matvec can use for P vectors of length n:
+// the split-K partial sums and the BF16 hi/lo planes of the vectors.
+int64_t matvec_scratch_floats(int64_t n, int64_t P);
 hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, int64_t P,
-                         float* W);
+                         float* W, float* scratch, int64_t scratch_floats);
 
 // ---- tridiagonal eigen + quadrature (k_tridiag.hip) ------------------------
 // For each probe p: T = (H + H^T)/2 restricted to its tridiagonal; returns

@@ -88,6 +88,28 @@ def test_rbf_matvec(cuda, n, P, dim):
     assert rel(W.cpu().numpy(), V.astype(np.float64) @ K) < 2e-6
 
 
+@pytest.mark.parametrize("precision,tol", [("bf16x3", 3e-5), ("bf16x2", 4e-3), ("bf16", 6e-3)])
+@pytest.mark.parametrize("n,P,dim", [(1000, 16, 3), (256, 128, 3), (130, 70, 3), (333, 40, 1),
+                                     (96, 200, 2)])
+def test_rbf_matvec_bf16(cuda, precision, tol, n, P, dim):
+    # split hi+lo BF16 recovers ~16 significant bits; plain BF16 keeps 8
+    X = problems.rbf_points(n, seed=3, dim=dim)
+    V = np.random.default_rng(2).standard_normal((P, n)).astype(np.float32)
+    W = operators.RBF(X, precision=precision)(torch.as_tensor(V, device=cuda), 0.5, 1e-2)
+    K = problems.rbf_dense64(X, 0.5, 1e-2)
+    assert rel(W.cpu().numpy(), V.astype(np.float64) @ K) < tol
+
+
+@pytest.mark.parametrize("precision,tol", [("bf16x3", 3e-5), ("bf16", 6e-3)])
+def test_dense_matvec_bf16(cuda, precision, tol):
+    n, P = 520, 40
+    A = problems.dense_spd(n, seed=1, shift=1.0)
+    V = np.random.default_rng(4).standard_normal((P, n)).astype(np.float32)
+    op = operators.DenseMatvec(precision=precision)
+    W = op(torch.as_tensor(V, device=cuda), torch.as_tensor(A, device=cuda))
+    assert rel(W.cpu().numpy(), V.astype(np.float64) @ A.astype(np.float64).T) < tol
+
+
 def test_single_vector_matvec(cuda):
     A = sym_from_eigs(np.linspace(1, 2, 33), 0)
     v = np.arange(33, dtype=np.float32)
