@@ -372,7 +372,8 @@ __device__ __forceinline__ void load_plane_tile(const __bf16* __restrict__ plane
 #pragma unroll
   for (int u = 0; u < PlaneTile<NACC>::kPerThread; ++u) {
     const int f = t + u * kThreads;
-    if (f < PlaneTile<NACC>::kPieces) reg[u] = *reinterpret_cast<const u32x4*>(base + 8 * f);
+    if (PlaneTile<NACC>::kPieces % kThreads == 0 || f < PlaneTile<NACC>::kPieces)
+      reg[u] = *reinterpret_cast<const u32x4*>(base + 8 * f);
   }
 }
 
@@ -383,7 +384,7 @@ __device__ __forceinline__ void store_plane_tile(__bf16* lds,
 #pragma unroll
   for (int u = 0; u < PlaneTile<NACC>::kPerThread; ++u) {
     const int f = t + u * kThreads;
-    if (f < PlaneTile<NACC>::kPieces) {
+    if (PlaneTile<NACC>::kPieces % kThreads == 0 || f < PlaneTile<NACC>::kPieces) {
       const int row = f >> 2, q = f & 3;
       *reinterpret_cast<u32x4*>(lds + row * LDB + 8 * q) = reg[u];
     }
@@ -434,8 +435,7 @@ __device__ __forceinline__ void load_a_frag16(const float* __restrict__ A, int64
 // c |x_m|^2 = -inf, so their entries are exactly 0.  The exponent's absolute
 // rounding error (~1e-6 for points in the unit cube and l = 0.5) is below
 // the 2^-17 relative error of the split BF16 representation.
-__device__ __forceinline__ void gen_rbf_frag_mfma(const float4* Xs, const float (&rb)[3],
-                                                  float (&a)[16]) {
+__device__ __forceinline__ floatx16 rbf_exponents_mfma(const float4* Xs, const float (&rb)[3]) {
   const int lane = threadIdx.x & 63;
   const int h = lane >> 5;
   const float4 xm = Xs[lane & 31];
@@ -449,6 +449,12 @@ __device__ __forceinline__ void gen_rbf_frag_mfma(const float4* Xs, const float 
   d = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, rb[0], d, 0, 0, 0);
   d = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, rb[1], d, 0, 0, 0);
   d = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, rb[2], d, 0, 0, 0);
+  return d;
+}
+
+__device__ __forceinline__ void gen_rbf_frag_mfma(const float4* Xs, const float (&rb)[3],
+                                                  float (&a)[16]) {
+  const floatx16 d = rbf_exponents_mfma(Xs, rb);
 #pragma unroll
   for (int e = 0; e < 16; ++e) a[e] = __builtin_amdgcn_exp2f(d[e]);
 }
@@ -529,19 +535,18 @@ __global__ void __launch_bounds__(kThreads)
     rb[1] = h ? 1.f : u[2];
     rb[2] = h ? 0.f : ai;
   }
+  // Point m0 + (t & 31) of a tile as (x, y, z, c |x|^2), or (0, 0, 0, -inf)
+  // past n; branch-free (clamped address, selects) so that it does not split
+  // the scheduling region of the tile loop.
   auto prep_x = [&](int64_t m0) {
-    float4 x = make_float4(0.f, 0.f, 0.f, -__builtin_inff());
-    const int64_t m = m0 + t;
-    if (m < n) {
-      const float* src = prm.X + m * dim;
-      float b = 0.f;
-      x.x = src[0];
-      b = fmaf(x.x * x.x, prm.coef, b);
-      if (dim > 1) { x.y = src[1]; b = fmaf(x.y * x.y, prm.coef, b); }
-      if (dim > 2) { x.z = src[2]; b = fmaf(x.z * x.z, prm.coef, b); }
-      x.w = b;
-    }
-    return x;
+    const int64_t m = m0 + (t & (BK - 1));
+    const bool valid = m < n;
+    const float* src = prm.X + (valid ? m : n - 1) * dim;
+    const float x0 = src[0];
+    const float x1 = dim > 1 ? src[dim > 1 ? 1 : 0] : 0.f;
+    const float x2 = dim > 2 ? src[dim > 2 ? 2 : 0] : 0.f;
+    const float b = fmaf(x2 * x2, prm.coef, fmaf(x1 * x1, prm.coef, x0 * x0 * prm.coef));
+    return valid ? make_float4(x0, x1, x2, b) : make_float4(0.f, 0.f, 0.f, -__builtin_inff());
   };
 
   // prologue: probe tile 0 staged, tile 1 in flight, points of tiles 0 and 1
@@ -565,7 +570,7 @@ __global__ void __launch_bounds__(kThreads)
     load_a_frag16<VEC>(A, n, row, mbase, h, areg);
   }
   __syncthreads();
-  u32x4 ahi[2], alo[2];
+  u32x4 fAh[2], fAl[2], fBh[2], fBl[2];  // fragments of the current / next tile
   {
     float a[16];
     if (RBF) {
@@ -575,22 +580,30 @@ __global__ void __launch_bounds__(kThreads)
       for (int s = 0; s < 16; ++s) a[s] = areg[s];
       if (ntiles > 1) load_a_frag16<VEC>(A, n, row, mbase + BK, h, areg);
     }
-    split_frag<NPROD>(a, ahi, alo);
+    split_frag<NPROD>(a, fAh, fAl);
   }
 
   // One tile: issue the loads of tile kt + 2 into (ih, il) -- they held tile
-  // kt, staged one iteration ago -- run the MFMAs of tile kt, build the next
-  // fragment, and stage tile kt + 1 from (sh, sl) into the other LDS buffer.
+  // kt, staged one iteration ago -- run the MFMAs of tile kt with fragment
+  // (ahi, alo), build the next fragment into (nhi, nlo), and stage tile kt + 1
+  // from (sh, sl) into the other LDS buffer.
   auto step = [&](int64_t kt, u32x4 (&ih)[PT], u32x4 (&il)[PT], const u32x4 (&sh)[PT],
-                  const u32x4 (&sl)[PT]) {
+                  const u32x4 (&sl)[PT], const u32x4 (&ahi)[2], const u32x4 (&alo)[2],
+                  u32x4 (&nhi)[2], u32x4 (&nlo)[2]) {
     const int buf = int(kt & 1);
     const int64_t m0 = mbase + kt * BK;
     const bool more = kt + 1 < ntiles;
-    if (kt + 2 < ntiles) {
-      load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin + kt + 2, ih);
-      if (LO) load_plane_tile<NACC>(Vlo_g, Ppad, p0, kt_begin + kt + 2, il);
-    }
-    if (RBF && t < BK) xnext = (kt + 2 < ntiles) ? prep_x(m0 + 2 * BK) : prep_x(n);
+    // tile kt + 2 (clamped: the last two steps reload the final tile, which
+    // is never staged); no branch before the MFMA/VALU region below
+    const int64_t kt2 = (kt + 2 < ntiles) ? kt + 2 : ntiles - 1;
+    load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin + kt2, ih);
+    if (LO) load_plane_tile<NACC>(Vlo_g, Ppad, p0, kt_begin + kt2, il);
+    if (RBF) xnext = (kt + 2 < ntiles) ? prep_x(m0 + 2 * BK) : prep_x(n);
+
+    // exponents of tile kt + 1 first, so their MFMA latency hides behind the
+    // BF16 MFMAs of tile kt (on the last tile they are computed and unused)
+    floatx16 dnext;
+    if (RBF) dnext = rbf_exponents_mfma(Xs[buf ^ 1], rb);
 
     const __bf16* vh = Vhi[buf] + r * LDB + 8 * h;
     const __bf16* vl = &Vlo[LO ? buf : 0][0] + r * LDB + 8 * h;
@@ -614,36 +627,54 @@ __global__ void __launch_bounds__(kThreads)
       for (int c = 0; c < NACC; ++c) acc[c] = mfma_bf16(ahi[kb], bh[c], acc[c]);
     }
 
-    // next tile's fragment while the MFMAs drain
-    u32x4 nhi[2], nlo[2];
-    if (more) {
+    // next tile's fragment (exp2 and split), interleaved by the scheduler
+    // with the MFMAs above
+    {
       float a[16];
       if (RBF) {
-        gen_rbf_frag_mfma(Xs[buf ^ 1], rb, a);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) a[e] = __builtin_amdgcn_exp2f(dnext[e]);
       } else {
 #pragma unroll
         for (int s = 0; s < 16; ++s) a[s] = areg[s];
         if (kt + 2 < ntiles) load_a_frag16<VEC>(A, n, row, m0 + 2 * BK, h, areg);
       }
       split_frag<NPROD>(a, nhi, nlo);
+    }
+    // keep the fragment's VALU work ahead of the barrier (instead of being
+    // sunk past it), where it can overlap the MFMAs
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+      asm volatile("" : : "v"(nhi[kb]));
+      if (NPROD >= 2) asm volatile("" : : "v"(nlo[kb]));
+    }
+    if (RBF) {
+      // scheduling: the exponent MFMAs, the B-fragment reads, then each BF16
+      // MFMA followed by a group of the fragment's VALU work
+      __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, (LO ? 16 : 8), 0);
+#pragma unroll
+      for (int g = 0; g < 2 * NACC * NPROD; ++g) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
+      }
+    }
+    if (more) {
       store_plane_tile<NACC>(Vhi[buf ^ 1], sh);
       if (LO) store_plane_tile<NACC>(&Vlo[LO ? (buf ^ 1) : 0][0], sl);
     }
     if (RBF && t < BK) Xs[buf][t] = xnext;  // points of tile kt + 2
     __syncthreads();
-    if (more) {
-#pragma unroll
-      for (int kb = 0; kb < 2; ++kb) {
-        ahi[kb] = nhi[kb];
-        if (NPROD >= 2) alo[kb] = nlo[kb];
-      }
-    }
   };
 
-  for (int64_t kt = 0; kt < ntiles; kt += 2) {
-    step(kt, hA, lA, hB, lB);
-    if (kt + 1 < ntiles) step(kt + 1, hB, lB, hA, lA);
+  // pairs of tiles, then the odd remainder (no branch inside the loop body,
+  // so the accumulators stay in place)
+  int64_t kt = 0;
+  for (; kt + 1 < ntiles; kt += 2) {
+    step(kt, hA, lA, hB, lB, fAh, fAl, fBh, fBl);
+    step(kt + 1, hB, lB, hA, lA, fBh, fBl, fAh, fAl);
   }
+  if (kt < ntiles) step(kt, hA, lA, hB, lB, fAh, fAl, fBh, fBl);
   if (split)
     store_out<NACC, VEC, false>(Wout, V, P, n, i0w, p0, acc, 0.f);
   else
