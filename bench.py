@@ -176,7 +176,7 @@ def cpu_baseline(args, host, device, n, k):
     from oracle import slq as oslq
 
     w = WORKLOADS[args.workload]
-    probes = 8 if args.workload == "rbf" else 16
+    probes = 32
     if args.workload == "rbf":
         from matfree_amd import problems
 
