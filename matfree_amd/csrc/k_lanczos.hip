@@ -8,20 +8,71 @@
 //
 // Only the j + 1 live basis columns are read (the reference multiplies by the
 // full-width zero-padded Q; the extra products are exact zeros).  All kernels
-// are HBM-bound; a block owns one chunk of kChunk elements of one probe's
-// vector, and cross-block sums go through per-chunk partials that the consumer
+// are HBM-bound.  A block owns one chunk of kChunk elements of one probe's
+// vector; cross-block sums go through per-chunk partials that the consumer
 // reduces in a fixed order (deterministic results).
 //
-// Basis layout: Q[c][p][i] at (c * P + p) * n + i.
+// Column reductions use a wave-level reduce-scatter: with NC partial sums
+// per lane, log2(NC) exchange steps halve the live values each time
+// (NC/2 + NC/4 + ... + 1 shuffles in total instead of 6 per column), after
+// which lane l holds the wave total of column l >> (6 - log2 NC).
+//
+// Basis layout: column c of probe p at Q + c * ldc + p * n (ldc >= P * n).
 #include "slq_internal.h"
 
 namespace slq {
 namespace {
 
+constexpr int kWaves = kStreamThreads / 64;
+
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
+}
+
+// Reduce-scatter of NC per-lane values (NC a power of two, <= 64).
+template <int L, int OFF>
+struct ReduceScatter {
+  __device__ static __forceinline__ void run(float* acc, int lane) {
+    if constexpr (OFF == 0) {
+      return;
+    } else if constexpr (L > 1) {
+      const bool upper = (lane & OFF) != 0;
+#pragma unroll
+      for (int i = 0; i < L / 2; ++i) {
+        const float lo = acc[i], hi = acc[i + L / 2];
+        const float send = upper ? lo : hi;
+        const float keep = upper ? hi : lo;
+        acc[i] = keep + __shfl_xor(send, OFF, 64);
+      }
+      ReduceScatter<L / 2, OFF / 2>::run(acc, lane);
+    } else {
+      acc[0] += __shfl_xor(acc[0], OFF, 64);
+      ReduceScatter<1, OFF / 2>::run(acc, lane);
+    }
+  }
+};
+
+constexpr int log2i(int v) { return v <= 1 ? 0 : 1 + log2i(v / 2); }
+
+// Block-wide column sums of acc[0..NC) (one set per thread); the sums of the
+// columns c < ncols are written to out[c] (out is global memory).
+template <int NC>
+__device__ __forceinline__ void block_column_sums(float (&acc)[NC], int ncols, float* red,
+                                                  float* out) {
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  ReduceScatter<NC, 32>::run(acc, lane);
+  constexpr int shift = 6 - log2i(NC);
+  if ((lane & ((1 << shift) - 1)) == 0) red[wave * NC + (lane >> shift)] = acc[0];
+  __syncthreads();
+  if (t < ncols) {
+    float s = 0.f;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) s += red[w * NC + t];
+    out[t] = s;
+  }
+  __syncthreads();
 }
 
 template <int V>
@@ -59,25 +110,25 @@ __device__ __forceinline__ float block_sum(float v, float* red) {
   __syncthreads();
   float s = 0.f;
 #pragma unroll
-  for (int w = 0; w < kStreamThreads / 64; ++w) s += red[w];
+  for (int w = 0; w < kWaves; ++w) s += red[w];
   __syncthreads();
   return s;
 }
 
 // ---------------------------------------------------------------------------
 // dots: partial[p][b][c] = sum over chunk b of Q[c][p][:] . x[p][:]
+// Columns in batches of NC (all held as per-lane partial sums).
 // ---------------------------------------------------------------------------
-template <int V>
+template <int V, int NC>
 __global__ void __launch_bounds__(kStreamThreads)
-    k_dots(const float* __restrict__ Q, const float* __restrict__ x, int64_t P, int64_t n,
+    k_dots(const float* __restrict__ Q, int64_t ldc, const float* __restrict__ x, int64_t n,
            int ncols, float* __restrict__ partial, int pstride) {
   constexpr int NE = Layout<V>::NE;
-  constexpr int CB = 16;
-  __shared__ float red[kStreamThreads / 64][CB];
+  __shared__ float red[kWaves * NC];
   const int64_t p = blockIdx.y;
   const int64_t b = blockIdx.x;
   const int64_t nb = gridDim.x;
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int t = threadIdx.x;
   const int64_t base = b * kChunk;
 
   float xv[NE][V];
@@ -92,13 +143,14 @@ __global__ void __launch_bounds__(kStreamThreads)
     }
   }
 
-  for (int c0 = 0; c0 < ncols; c0 += CB) {
-    float acc[CB];
+  const float* qp = Q + p * n;
+  for (int c0 = 0; c0 < ncols; c0 += NC) {
+    float acc[NC];
 #pragma unroll
-    for (int cb = 0; cb < CB; ++cb) {
-      acc[cb] = 0.f;
-      if (c0 + cb < ncols) {
-        const float* q = Q + (int64_t(c0 + cb) * P + p) * n;
+    for (int c = 0; c < NC; ++c) {
+      acc[c] = 0.f;
+      if (c0 + c < ncols) {
+        const float* q = qp + int64_t(c0 + c) * ldc;
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
           const int64_t i = base + (int64_t(e) * kStreamThreads + t) * V;
@@ -106,24 +158,13 @@ __global__ void __launch_bounds__(kStreamThreads)
             float qv[V];
             VecIO<V>::load(q + i, qv);
 #pragma unroll
-            for (int v = 0; v < V; ++v) acc[cb] = fmaf(qv[v], xv[e][v], acc[cb]);
+            for (int v = 0; v < V; ++v) acc[c] = fmaf(qv[v], xv[e][v], acc[c]);
           }
         }
       }
     }
-#pragma unroll
-    for (int cb = 0; cb < CB; ++cb) {
-      const float s = wave_sum(acc[cb]);
-      if (lane == 0) red[wave][cb] = s;
-    }
-    __syncthreads();
-    if (t < CB && c0 + t < ncols) {
-      float s = 0.f;
-#pragma unroll
-      for (int w = 0; w < kStreamThreads / 64; ++w) s += red[w][t];
-      partial[(p * nb + b) * pstride + c0 + t] = s;
-    }
-    __syncthreads();
+    const int live = ncols - c0 < NC ? ncols - c0 : NC;
+    block_column_sums<NC>(acc, live, red, partial + (p * nb + b) * pstride + c0);
   }
 }
 
@@ -132,7 +173,7 @@ template <int V>
 __global__ void __launch_bounds__(kStreamThreads)
     k_sumsq(const float* __restrict__ x, int64_t n, float* __restrict__ partial) {
   constexpr int NE = Layout<V>::NE;
-  __shared__ float red[kStreamThreads / 64];
+  __shared__ float red[kWaves];
   const int64_t p = blockIdx.y, b = blockIdx.x, nb = gridDim.x;
   const int t = threadIdx.x;
   const int64_t base = b * kChunk;
@@ -169,12 +210,13 @@ __device__ __forceinline__ void load_coeffs(const float* __restrict__ hpart, int
 // ---------------------------------------------------------------------------
 template <int V>
 __global__ void __launch_bounds__(kStreamThreads)
-    k_project(const float* __restrict__ Q, const float* __restrict__ x, float* __restrict__ y,
-              int64_t P, int64_t n, int ncols, const float* __restrict__ hpart, int pstride,
-              float* __restrict__ norm2part, float* __restrict__ hout, int64_t hout_stride) {
+    k_project(const float* __restrict__ Q, int64_t ldc, const float* __restrict__ x,
+              float* __restrict__ y, int64_t n, int ncols, const float* __restrict__ hpart,
+              int pstride, float* __restrict__ norm2part, float* __restrict__ hout,
+              int64_t hout_stride) {
   constexpr int NE = Layout<V>::NE;
   extern __shared__ float hs[];
-  __shared__ float red[kStreamThreads / 64];
+  __shared__ float red[kWaves];
   const int64_t p = blockIdx.y, b = blockIdx.x, nb = gridDim.x;
   const int t = threadIdx.x;
   const int64_t base = b * kChunk;
@@ -186,9 +228,10 @@ __global__ void __launch_bounds__(kStreamThreads)
 #pragma unroll
     for (int v = 0; v < V; ++v) s[e][v] = 0.f;
 
+  const float* qp = Q + p * n;
   for (int c = 0; c < ncols; ++c) {
     const float hc = hs[c];
-    const float* q = Q + (int64_t(c) * P + p) * n;
+    const float* q = qp + int64_t(c) * ldc;
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
       const int64_t i = base + (int64_t(e) * kStreamThreads + t) * V;
@@ -224,65 +267,99 @@ __global__ void __launch_bounds__(kStreamThreads)
 
 // ---------------------------------------------------------------------------
 // fused project + dots (ncols <= NC): y = x - Q h1, h2part = Q^T y.
-// One element per lane per iteration with the NC basis values in registers,
-// so Q is read once for both operations.
+// E consecutive elements per lane per iteration with their NC basis values
+// in registers, so Q is read once for both operations; the element loop is
+// not unrolled (one iteration's values live at a time), and h1 is read from
+// LDS.
 // ---------------------------------------------------------------------------
-template <int NC>
+template <int E>
+struct VecE;
+template <>
+struct VecE<1> {
+  using T = float;
+  __device__ static __forceinline__ float get(const T& v, int) { return v; }
+  __device__ static __forceinline__ void set(T& v, int, float f) { v = f; }
+};
+template <>
+struct VecE<2> {
+  using T = float2;
+  __device__ static __forceinline__ float get(const T& v, int e) { return e ? v.y : v.x; }
+  __device__ static __forceinline__ void set(T& v, int e, float f) { (e ? v.y : v.x) = f; }
+};
+template <>
+struct VecE<4> {
+  using T = float4;
+  __device__ static __forceinline__ float get(const T& v, int e) {
+    return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w;
+  }
+  __device__ static __forceinline__ void set(T& v, int e, float f) {
+    (e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w) = f;
+  }
+};
+
+template <int NC, int E>
 __global__ void __launch_bounds__(kStreamThreads)
-    k_project_dots(const float* __restrict__ Q, const float* __restrict__ x, float* __restrict__ y,
-                   int64_t P, int64_t n, int ncols, const float* __restrict__ h1part,
-                   float* __restrict__ h2part, int pstride, float* __restrict__ hout,
-                   int64_t hout_stride) {
-  __shared__ float hs[NC];
-  __shared__ float red[kStreamThreads / 64][NC];
+    k_project_dots(const float* __restrict__ Q, int64_t ldc, const float* __restrict__ x,
+                   float* __restrict__ y, int64_t n, int ncols,
+                   const float* __restrict__ h1part, float* __restrict__ h2part, int pstride,
+                   float* __restrict__ hout, int64_t hout_stride) {
+  using VT = typename VecE<E>::T;
+  __shared__ __attribute__((aligned(16))) float hs[NC];
+  __shared__ float red[kWaves * NC];
   const int64_t p = blockIdx.y, b = blockIdx.x, nb = gridDim.x;
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int t = threadIdx.x;
   const int64_t base = b * kChunk;
+  if (t < NC) hs[t] = 0.f;
+  __syncthreads();
   load_coeffs(h1part, p, nb, ncols, pstride, hs, hout, hout_stride, b == 0);
 
-  float h1[NC];
-#pragma unroll
-  for (int c = 0; c < NC; ++c) h1[c] = (c < ncols) ? hs[c] : 0.f;
   float acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = 0.f;
 
   const float* qp = Q + p * n;
-  const int64_t qstride = P * n;
-  for (int64_t it = 0; it < kChunk / kStreamThreads; ++it) {
-    const int64_t i = base + it * kStreamThreads + t;
+#pragma unroll 1
+  for (int64_t it = 0; it < kChunk / (kStreamThreads * E); ++it) {
+    const int64_t i = base + (it * kStreamThreads + t) * E;
     if (i >= n) break;
-    float q[NC];
+    // Column addresses are formed by a pointer chain that the compiler may
+    // not re-associate: otherwise it keeps NC loop-invariant 64-bit column
+    // offsets live across the loop (2 NC registers).
+    VT q[NC];
+    const float* col = qp + i;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) q[c] = (c < ncols) ? qp[int64_t(c) * qstride + i] : 0.f;
-    float s = 0.f;
+    for (int c = 0; c < NC; ++c) {
+      if (c < ncols) {
+        q[c] = *reinterpret_cast<const VT*>(col);
+      } else {
 #pragma unroll
-    for (int c = 0; c < NC; ++c) s = fmaf(q[c], h1[c], s);
-    const float yi = x[p * n + i] - s;
-    y[p * n + i] = yi;
-#pragma unroll
-    for (int c = 0; c < NC; ++c) acc[c] = fmaf(q[c], yi, acc[c]);
-  }
-#pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    if (c < ncols) {
-      const float v = wave_sum(acc[c]);
-      if (lane == 0) red[wave][c] = v;
+        for (int e = 0; e < E; ++e) VecE<E>::set(q[c], e, 0.f);
+      }
+      col += ldc;
+      asm volatile("" : "+v"(col));
     }
-  }
-  __syncthreads();
-  for (int c = t; c < ncols; c += kStreamThreads) {
-    float v = 0.f;
+    VT yv = *reinterpret_cast<const VT*>(x + p * n + i);
 #pragma unroll
-    for (int w = 0; w < kStreamThreads / 64; ++w) v += red[w][c];
-    h2part[(p * nb + b) * pstride + c] = v;
+    for (int e = 0; e < E; ++e) {
+      float s = 0.f;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) s = fmaf(VecE<E>::get(q[c], e), hs[c], s);
+      VecE<E>::set(yv, e, VecE<E>::get(yv, e) - s);
+    }
+    *reinterpret_cast<VT*>(y + p * n + i) = yv;
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        acc[c] = fmaf(VecE<E>::get(q[c], e), VecE<E>::get(yv, e), acc[c]);
   }
+  block_column_sums<NC>(acc, ncols, red, h2part + (p * nb + b) * pstride);
 }
 
 // ---------------------------------------------------------------------------
 // normalize: out = x / sqrt(sum_b norm2part), plus bookkeeping of the norm
 // ---------------------------------------------------------------------------
-constexpr int kNormVec = 4;  // float4 groups per thread
+constexpr int kNormVec = 2;  // float4 groups per thread
 
 template <int V>
 __global__ void __launch_bounds__(kStreamThreads)
@@ -322,62 +399,76 @@ __global__ void __launch_bounds__(kStreamThreads)
 
 inline bool aligned16(const void* ptr) { return (reinterpret_cast<uintptr_t>(ptr) & 15) == 0; }
 
+template <int V>
+hipError_t launch_dots_v(hipStream_t s, const float* Q, int64_t ldc, const float* x, int64_t P,
+                         int64_t n, int ncols, float* partial, int pstride) {
+  const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
+  const dim3 block(kStreamThreads);
+  if (ncols <= 8)
+    hipLaunchKernelGGL((k_dots<V, 8>), grid, block, 0, s, Q, ldc, x, n, ncols, partial, pstride);
+  else if (ncols <= 16)
+    hipLaunchKernelGGL((k_dots<V, 16>), grid, block, 0, s, Q, ldc, x, n, ncols, partial, pstride);
+  else if (ncols <= 32)
+    hipLaunchKernelGGL((k_dots<V, 32>), grid, block, 0, s, Q, ldc, x, n, ncols, partial, pstride);
+  else
+    hipLaunchKernelGGL((k_dots<V, 64>), grid, block, 0, s, Q, ldc, x, n, ncols, partial, pstride);
+  return hipGetLastError();
+}
+
 }  // namespace
 
-hipError_t launch_dots(hipStream_t s, const float* Q, const float* x, int64_t P, int64_t n,
-                       int ncols, float* partial, int pstride) {
-  const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
-  const bool vec = (n % 4 == 0) && aligned16(x) && (Q == nullptr || aligned16(Q));
+hipError_t launch_dots(hipStream_t s, const float* Q, int64_t ldc, const float* x, int64_t P,
+                       int64_t n, int ncols, float* partial, int pstride) {
+  const bool vec = (n % 4 == 0) && aligned16(x) && (Q == nullptr || (aligned16(Q) && ldc % 4 == 0));
   if (Q == nullptr) {
+    const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
     if (vec)
       hipLaunchKernelGGL(k_sumsq<4>, grid, dim3(kStreamThreads), 0, s, x, n, partial);
     else
       hipLaunchKernelGGL(k_sumsq<1>, grid, dim3(kStreamThreads), 0, s, x, n, partial);
-  } else {
-    if (vec)
-      hipLaunchKernelGGL(k_dots<4>, grid, dim3(kStreamThreads), 0, s, Q, x, P, n, ncols, partial,
-                         pstride);
-    else
-      hipLaunchKernelGGL(k_dots<1>, grid, dim3(kStreamThreads), 0, s, Q, x, P, n, ncols, partial,
-                         pstride);
+    return hipGetLastError();
   }
-  return hipGetLastError();
+  return vec ? launch_dots_v<4>(s, Q, ldc, x, P, n, ncols, partial, pstride)
+             : launch_dots_v<1>(s, Q, ldc, x, P, n, ncols, partial, pstride);
 }
 
-hipError_t launch_project(hipStream_t s, const float* Q, const float* x, float* y, int64_t P,
-                          int64_t n, int ncols, const float* hpart, int pstride, float* norm2part,
-                          float* hout, int64_t hout_stride) {
+hipError_t launch_project(hipStream_t s, const float* Q, int64_t ldc, const float* x, float* y,
+                          int64_t P, int64_t n, int ncols, const float* hpart, int pstride,
+                          float* norm2part, float* hout, int64_t hout_stride) {
   const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
   const size_t shmem = size_t(ncols > 0 ? ncols : 1) * sizeof(float);
-  const bool vec = (n % 4 == 0) && aligned16(x) && aligned16(y) && aligned16(Q);
+  const bool vec = (n % 4 == 0) && aligned16(x) && aligned16(y) && aligned16(Q) && ldc % 4 == 0;
   if (vec)
-    hipLaunchKernelGGL(k_project<4>, grid, dim3(kStreamThreads), shmem, s, Q, x, y, P, n, ncols,
+    hipLaunchKernelGGL(k_project<4>, grid, dim3(kStreamThreads), shmem, s, Q, ldc, x, y, n, ncols,
                        hpart, pstride, norm2part, hout, hout_stride);
   else
-    hipLaunchKernelGGL(k_project<1>, grid, dim3(kStreamThreads), shmem, s, Q, x, y, P, n, ncols,
+    hipLaunchKernelGGL(k_project<1>, grid, dim3(kStreamThreads), shmem, s, Q, ldc, x, y, n, ncols,
                        hpart, pstride, norm2part, hout, hout_stride);
   return hipGetLastError();
 }
 
-hipError_t launch_project_dots(hipStream_t s, const float* Q, const float* x, float* y, int64_t P,
-                               int64_t n, int ncols, const float* h1part, float* h2part,
-                               int pstride, float* hout, int64_t hout_stride) {
+hipError_t launch_project_dots(hipStream_t s, const float* Q, int64_t ldc, const float* x,
+                               float* y, int64_t P, int64_t n, int ncols, const float* h1part,
+                               float* h2part, int pstride, float* hout, int64_t hout_stride) {
   const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
   const dim3 block(kStreamThreads);
-  if (ncols <= 8)
-    hipLaunchKernelGGL(k_project_dots<8>, grid, block, 0, s, Q, x, y, P, n, ncols, h1part, h2part,
-                       pstride, hout, hout_stride);
-  else if (ncols <= 16)
-    hipLaunchKernelGGL(k_project_dots<16>, grid, block, 0, s, Q, x, y, P, n, ncols, h1part, h2part,
-                       pstride, hout, hout_stride);
-  else if (ncols <= 32)
-    hipLaunchKernelGGL(k_project_dots<32>, grid, block, 0, s, Q, x, y, P, n, ncols, h1part, h2part,
-                       pstride, hout, hout_stride);
-  else if (ncols <= kFusedMaxCols)
-    hipLaunchKernelGGL(k_project_dots<kFusedMaxCols>, grid, block, 0, s, Q, x, y, P, n, ncols,
-                       h1part, h2part, pstride, hout, hout_stride);
-  else
+  const bool v2 = (n % 2 == 0) && ldc % 2 == 0 && ((reinterpret_cast<uintptr_t>(Q) |
+                   reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y)) & 7) == 0;
+#define SLQ_PD(NC, E)                                                                        \
+  hipLaunchKernelGGL((k_project_dots<NC, E>), grid, block, 0, s, Q, ldc, x, y, n, ncols, h1part, \
+                     h2part, pstride, hout, hout_stride)
+  if (ncols <= 8) {
+    if (v2) SLQ_PD(8, 2); else SLQ_PD(8, 1);
+  } else if (ncols <= 16) {
+    if (v2) SLQ_PD(16, 2); else SLQ_PD(16, 1);
+  } else if (ncols <= 32) {
+    if (v2) SLQ_PD(32, 2); else SLQ_PD(32, 1);
+  } else if (ncols <= kFusedMaxCols) {
+    if (v2) SLQ_PD(kFusedMaxCols, 2); else SLQ_PD(kFusedMaxCols, 1);
+  } else {
     return hipErrorInvalidValue;
+  }
+#undef SLQ_PD
   return hipGetLastError();
 }
 

@@ -141,9 +141,15 @@ int set_device(slq_context* ctx) {
   return SLQ_OK;
 }
 
+// Column stride of the internal Krylov basis: P * n rounded up to 64 floats
+// plus a 1088-float skew.
+inline int64_t basis_stride(int64_t P, int64_t n) {
+  return (P * n + 63) / 64 * 64 + kBasisSkew;
+}
+
 // Forward Arnoldi (decomp.py:426-477) for P start vectors; see slq.h.
 int hessenberg_core(slq_context* ctx, const slq_operator& op, const float* V0, int64_t P,
-                    int64_t k, float* Q, float* H, float* R, float* c) {
+                    int64_t k, float* Q, int64_t ldc, float* H, float* R, float* c) {
   const int64_t n = op.n;
   hipStream_t s = ctx->stream;
   const int64_t nb = num_chunks(n);
@@ -160,7 +166,7 @@ int hessenberg_core(slq_context* ctx, const slq_operator& op, const float* V0, i
   if (k > 0) SLQ_HIP(ctx, hipMemsetAsync(H, 0, size_t(P * k * k) * sizeof(float), s));
 
   // initlength = sqrt(<v, v>) (decomp.py:435); c = 1 / initlength (:450)
-  SLQ_HIP(ctx, launch_dots(s, nullptr, V0, P, n, 1, norm2, 1));
+  SLQ_HIP(ctx, launch_dots(s, nullptr, 0, V0, P, n, 1, norm2, 1));
   if (k == 0) {
     SLQ_HIP(ctx, launch_normalize(s, V0, w, P, n, norm2, nullptr, nullptr, 0, c));
     SLQ_HIP(ctx, hipMemcpyAsync(R, V0, size_t(P * n) * sizeof(float), hipMemcpyDeviceToDevice, s));
@@ -169,7 +175,7 @@ int hessenberg_core(slq_context* ctx, const slq_operator& op, const float* V0, i
   // step 0: v /= length; Q[:, 0] = v (decomp.py:456-457)
   SLQ_HIP(ctx, launch_normalize(s, V0, Q, P, n, norm2, nullptr, nullptr, 0, c));
   for (int64_t j = 0; j < k; ++j) {
-    const float* qj = Q + j * P * n;
+    const float* qj = Q + j * ldc;
     const int ncols = int(j + 1);
     float* hcol = H + j * k;  // column j of every probe, stride k * k
     // w = A v (decomp.py:460)
@@ -180,28 +186,29 @@ int hessenberg_core(slq_context* ctx, const slq_operator& op, const float* V0, i
     // h = Q^T w (decomp.py:463)
     {
       Timed tm(ctx, SLQ_KC_DOTS);
-      SLQ_HIP(ctx, launch_dots(s, Q, w, P, n, ncols, part1, pstride));
+      SLQ_HIP(ctx, launch_dots(s, Q, ldc, w, P, n, ncols, part1, pstride));
     }
     // w = w - Q h (decomp.py:464), then the second pass's Q^T w (decomp.py:468)
     if (ncols <= kFusedMaxCols) {
       Timed tm(ctx, SLQ_KC_PROJECT_DOTS);
-      SLQ_HIP(ctx, launch_project_dots(s, Q, w, w2, P, n, ncols, part1, part2, pstride, hcol, k * k));
+      SLQ_HIP(ctx, launch_project_dots(s, Q, ldc, w, w2, P, n, ncols, part1, part2, pstride, hcol,
+                                        k * k));
     } else {
-      SLQ_HIP(ctx, launch_project(s, Q, w, w2, P, n, ncols, part1, pstride, nullptr, hcol, k * k));
-      SLQ_HIP(ctx, launch_dots(s, Q, w2, P, n, ncols, part2, pstride));
+      SLQ_HIP(ctx, launch_project(s, Q, ldc, w, w2, P, n, ncols, part1, pstride, nullptr, hcol, k * k));
+      SLQ_HIP(ctx, launch_dots(s, Q, ldc, w2, P, n, ncols, part2, pstride));
     }
     // w = w - Q (Q^T w) and <w, w> (decomp.py:468, 471)
     const bool last = (j + 1 == k);
     float* dst = last ? R : w;
     {
       Timed tm(ctx, SLQ_KC_PROJECT);
-      SLQ_HIP(ctx, launch_project(s, Q, w2, dst, P, n, ncols, part2, pstride, norm2, nullptr, 0));
+      SLQ_HIP(ctx, launch_project(s, Q, ldc, w2, dst, P, n, ncols, part2, pstride, norm2, nullptr, 0));
     }
     if (!last) {
       // length = sqrt(<w, w>); h[j+1] = length (decomp.py:471-475); next
       // step's v /= length; Q[:, j+1] = v (decomp.py:456-457)
       Timed tm(ctx, SLQ_KC_NORMALIZE);
-      SLQ_HIP(ctx, launch_normalize(s, w, Q + (j + 1) * P * n, P, n, norm2, nullptr, hcol + j + 1,
+      SLQ_HIP(ctx, launch_normalize(s, w, Q + (j + 1) * ldc, P, n, norm2, nullptr, hcol + j + 1,
                                     k * k, nullptr));
     }
   }
@@ -301,7 +308,7 @@ int slq_hessenberg(slq_context* ctx, const slq_operator* op, const float* V0, in
   if (!V0 || !R || !c || (k > 0 && (!Q || !H)))
     return fail(ctx, SLQ_EINVAL, "input or output pointer is NULL");
   SLQ_TRY(set_device(ctx));
-  return hessenberg_core(ctx, *op, V0, P, k, Q, H, R, c);
+  return hessenberg_core(ctx, *op, V0, P, k, Q, P * op->n, H, R, c);
 }
 
 int slq_logdet(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P, int64_t k,
@@ -319,7 +326,10 @@ int slq_logdet(slq_context* ctx, const slq_operator* op, const float* Z, int64_t
   float *v0, *Q, *H, *R, *c, *ell, *norm2;
   double *eig, *pp;
   SLQ_TRY(ensure_t(ctx, ctx->vec0, size_t(P * n), &v0));
-  SLQ_TRY(ensure_t(ctx, ctx->basis, size_t(k * P * n), &Q));
+  // basis columns padded so that consecutive columns do not sit a power of
+  // two apart (all columns of a step are streamed together)
+  const int64_t ldc = basis_stride(P, n);
+  SLQ_TRY(ensure_t(ctx, ctx->basis, size_t(k * ldc), &Q));
   SLQ_TRY(ensure_t(ctx, ctx->hmat, size_t(P * k * k), &H));
   SLQ_TRY(ensure_t(ctx, ctx->resid, size_t(P * n), &R));
   SLQ_TRY(ensure_t(ctx, ctx->scal1, size_t(P), &c));
@@ -332,10 +342,10 @@ int slq_logdet(slq_context* ctx, const slq_operator* op, const float* Z, int64_t
     SLQ_TRY(ensure_t(ctx, ctx->per_probe, size_t(P), &pp));
   }
   // length = |z|; v0 = z / length (funm.py:228-229)
-  SLQ_HIP(ctx, launch_dots(s, nullptr, Z, P, n, 1, norm2, 1));
+  SLQ_HIP(ctx, launch_dots(s, nullptr, 0, Z, P, n, 1, norm2, 1));
   SLQ_HIP(ctx, launch_normalize(s, Z, v0, P, n, norm2, ell, nullptr, 0, nullptr));
   // tridiag_sym(matvec, v0) (funm.py:237 -> decomp.py:130-143)
-  SLQ_TRY(hessenberg_core(ctx, *op, v0, P, k, Q, H, R, c));
+  SLQ_TRY(hessenberg_core(ctx, *op, v0, P, k, Q, ldc, H, R, c));
   // |z|^2 [log T]_00 (funm.py:239-241) and the probe mean (stochtrace.py:50)
   {
     Timed tm(ctx, SLQ_KC_QUADRATURE);

@@ -13,10 +13,14 @@ namespace slq {
 
 // Threads per block of the streaming (HBM-bound) kernels.
 constexpr int kStreamThreads = 256;
-// float4 groups per thread per block in the streaming kernels: one block
-// covers kStreamThreads * 4 * kStreamVec = 8192 elements of one vector.
-constexpr int kStreamVec = 8;
-constexpr int64_t kChunk = int64_t(kStreamThreads) * 4 * kStreamVec;
+// Elements of one vector covered by one block of the streaming kernels
+// (a multiple of 4 * kStreamThreads; 2048 = two float4 groups per thread).
+#ifndef SLQ_STREAM_CHUNK
+#define SLQ_STREAM_CHUNK 2048
+#endif
+constexpr int64_t kChunk = SLQ_STREAM_CHUNK;
+// Extra floats between consecutive columns of the internal Krylov basis.
+constexpr int64_t kBasisSkew = 1088;
 // Basis columns held in registers by the fused project+dots kernel.
 constexpr int kFusedMaxCols = 64;
 
@@ -26,20 +30,21 @@ inline int64_t num_chunks(int64_t n) { return (n + kChunk - 1) / kChunk; }
 hipError_t launch_rademacher(hipStream_t s, uint32_t k0, uint32_t k1, int64_t count, float* out);
 
 // ---- streaming Gram-Schmidt kernels (k_lanczos.hip) ------------------------
+// Basis column c of probe p starts at Q + c * ldc + p * n (ldc >= P * n).
 // partial[p][b][c] = sum_{i in chunk b} Q[c][p][i] * x[p][i], c < ncols.
 // Q may be null with ncols = 1 to mean "sum of squares of x" (partial[p][b][0]).
-hipError_t launch_dots(hipStream_t s, const float* Q, const float* x, int64_t P, int64_t n,
-                       int ncols, float* partial, int pstride);
+hipError_t launch_dots(hipStream_t s, const float* Q, int64_t ldc, const float* x, int64_t P,
+                       int64_t n, int ncols, float* partial, int pstride);
 // y = x - sum_c Q[c] h[c] with h[c] = sum_b hpart[p][b][c]; optional
 // norm2part[p][b] = sum over chunk of y^2.  hout (if non-null) receives h for
 // probe p at hout + p * hout_stride (written by chunk 0 only).
-hipError_t launch_project(hipStream_t s, const float* Q, const float* x, float* y, int64_t P,
-                          int64_t n, int ncols, const float* hpart, int pstride, float* norm2part,
-                          float* hout, int64_t hout_stride);
+hipError_t launch_project(hipStream_t s, const float* Q, int64_t ldc, const float* x, float* y,
+                          int64_t P, int64_t n, int ncols, const float* hpart, int pstride,
+                          float* norm2part, float* hout, int64_t hout_stride);
 // Fused: y = x - Q h1 (h1 reduced from h1part), then h2part = Q^T y per chunk.
-hipError_t launch_project_dots(hipStream_t s, const float* Q, const float* x, float* y, int64_t P,
-                               int64_t n, int ncols, const float* h1part, float* h2part,
-                               int pstride, float* hout, int64_t hout_stride);
+hipError_t launch_project_dots(hipStream_t s, const float* Q, int64_t ldc, const float* x,
+                               float* y, int64_t P, int64_t n, int ncols, const float* h1part,
+                               float* h2part, int pstride, float* hout, int64_t hout_stride);
 // out[p][i] = x[p][i] / sqrt(sum_b norm2part[p][b]); optionally writes the
 // norm to norm_out[p] (float) and to hslot[p * hslot_stride].
 hipError_t launch_normalize(hipStream_t s, const float* x, float* out, int64_t P, int64_t n,
