@@ -30,6 +30,9 @@ HIPCC_FLAGS = [
     "-target-feature",
     "-Xclang",
     "-packed-fp32-ops",
+    # MFMA results and accumulators in ordinary VGPRs (no AGPR copies)
+    "-mllvm",
+    "-amdgpu-mfma-vgpr-form=1",
 ]
 
 

@@ -430,33 +430,43 @@ __device__ __forceinline__ void load_a_frag16(const float* __restrict__ A, int64
 // its own row i = r at the 16 points m = 8 (s / 4) + 4 h + (s % 4),
 // s = 0..15 -- exactly the BF16 A-fragment of that lane once the contraction
 // index is ordered that way, which the permuted probe planes provide.
-// The lane reads one tile point from LDS per tile, and the VALU work per
-// entry is the exp2 and the hi/lo split.  Points past n carry
+// The A operands of lane (r, h) are L[m][2t + h], t = 0..2, i.e. (x, z, 1)
+// for h = 0 and (y, c |x|^2, 0) for h = 1; the first two come from the
+// prepared point records below, the third is a lane constant.  The VALU work
+// per entry is the exp2 and the hi/lo split.  Points past n carry
 // c |x_m|^2 = -inf, so their entries are exactly 0.  The exponent's absolute
 // rounding error (~1e-6 for points in the unit cube and l = 0.5) is below
 // the 2^-17 relative error of the split BF16 representation.
-__device__ __forceinline__ floatx16 rbf_exponents_mfma(const float4* Xs, const float (&rb)[3]) {
-  const int lane = threadIdx.x & 63;
-  const int h = lane >> 5;
-  const float4 xm = Xs[lane & 31];
-  // A operand of MFMA t: L[m][2t + h]
-  const float a0 = h ? xm.y : xm.x;
-  const float a1 = h ? xm.w : xm.z;
-  const float a2 = h ? 0.f : 1.f;
+__device__ __forceinline__ floatx16 rbf_exponents_mfma(float2 xm, float a2, const float (&rb)[3]) {
   floatx16 d;
 #pragma unroll
   for (int e = 0; e < 16; ++e) d[e] = 0.f;
-  d = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, rb[0], d, 0, 0, 0);
-  d = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, rb[1], d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_32x32x2f32(xm.x, rb[0], d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_32x32x2f32(xm.y, rb[1], d, 0, 0, 0);
   d = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, rb[2], d, 0, 0, 0);
   return d;
 }
 
-__device__ __forceinline__ void gen_rbf_frag_mfma(const float4* Xs, const float (&rb)[3],
-                                                  float (&a)[16]) {
-  const floatx16 d = rbf_exponents_mfma(Xs, rb);
-#pragma unroll
-  for (int e = 0; e < 16; ++e) a[e] = __builtin_amdgcn_exp2f(d[e]);
+// Point records of the BF16 RBF kernel: for every m < npad two float2,
+// rec[2 m] = (x, z) and rec[2 m + 1] = (y, c |x|^2) (coordinates past dim are
+// 0), and (0, 0), (0, -inf) for the padding m >= n.
+__global__ void __launch_bounds__(256)
+    k_prep_points(const float* __restrict__ X, int dim, float coef, int64_t n, int64_t npad,
+                  float2* __restrict__ rec) {
+  const int64_t m = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (m >= npad) return;
+  if (m < n) {
+    const float* src = X + m * dim;
+    const float x0 = src[0];
+    const float x1 = dim > 1 ? src[1] : 0.f;
+    const float x2 = dim > 2 ? src[2] : 0.f;
+    const float b = fmaf(x2 * x2, coef, fmaf(x1 * x1, coef, x0 * x0 * coef));
+    rec[2 * m] = make_float2(x0, x2);
+    rec[2 * m + 1] = make_float2(x1, b);
+  } else {
+    rec[2 * m] = make_float2(0.f, 0.f);
+    rec[2 * m + 1] = make_float2(0.f, -__builtin_inff());
+  }
 }
 
 template <int NPROD>
@@ -482,23 +492,49 @@ __device__ __forceinline__ floatx16 mfma_bf16(const u32x4& a, const u32x4& b, co
                                                  __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
 
-// Software-pipelined BF16 matvec: the fragment of tile kt + 1 is generated
-// (or converted) after the MFMAs of tile kt are issued, and the probe tiles
-// are fetched two tiles ahead into alternating register sets (the loop is
-// unrolled by two so the sets keep fixed registers).  RBF kernels here need
-// dim <= 3 (the per-point constant rides in the fourth lane of the float4 of
-// staged points).
-template <int NACC, bool RBF, bool VEC, int NPROD>
+// The BF16 MFMAs of one tile: acc[c] += A-fragment (ahi, alo) x probe block c
+// of the staged tile (vh / vl: this lane's row of the hi / lo plane).
+template <int NACC, int NPROD>
+__device__ __forceinline__ void tile_mfmas(const __bf16* vh, const __bf16* vl,
+                                           const u32x4 (&ahi)[2], const u32x4 (&alo)[2],
+                                           floatx16 (&acc)[NACC]) {
+  constexpr bool LO = NPROD == 3;
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb) {
+    u32x4 bh[NACC], bl[NACC];
+#pragma unroll
+    for (int c = 0; c < NACC; ++c) {
+      bh[c] = *reinterpret_cast<const u32x4*>(vh + 32 * c * LDB + 16 * kb);
+      if (LO) bl[c] = *reinterpret_cast<const u32x4*>(vl + 32 * c * LDB + 16 * kb);
+    }
+    if (NPROD >= 2) {
+#pragma unroll
+      for (int c = 0; c < NACC; ++c) acc[c] = mfma_bf16(alo[kb], bh[c], acc[c]);
+    }
+    if (LO) {
+#pragma unroll
+      for (int c = 0; c < NACC; ++c) acc[c] = mfma_bf16(ahi[kb], bl[c], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < NACC; ++c) acc[c] = mfma_bf16(ahi[kb], bh[c], acc[c]);
+  }
+}
+
+// Stored-matrix BF16 matvec: the fragment of tile kt + 1 is loaded a tile
+// ahead and split after the MFMAs of tile kt are issued; the probe tiles are
+// fetched two tiles ahead into alternating register sets (the loop is
+// unrolled by two so the sets keep fixed registers).
+template <int NACC, bool VEC, int NPROD>
 __global__ void __launch_bounds__(kThreads)
-    k_matvec_bf16(const float* __restrict__ A, RBFParams prm, const __bf16* __restrict__ Vhi_g,
-                  const __bf16* __restrict__ Vlo_g, int64_t Ppad, const float* __restrict__ V,
-                  int64_t P, int64_t n, float* __restrict__ W) {
+    k_matvec_bf16_dense(const float* __restrict__ A, const __bf16* __restrict__ Vhi_g,
+                        const __bf16* __restrict__ Vlo_g, int64_t Ppad,
+                        const float* __restrict__ V, int64_t P, int64_t n,
+                        float* __restrict__ W) {
   constexpr int BPW = 32 * NACC;
   constexpr bool LO = NPROD == 3;  // probes' low plane used
   constexpr int PT = PlaneTile<NACC>::kPerThread;
   __shared__ __attribute__((aligned(16))) __bf16 Vhi[2][BPW * LDB];
   __shared__ __attribute__((aligned(16))) __bf16 Vlo[LO ? 2 : 1][LO ? BPW * LDB : 8];
-  __shared__ float4 Xs[2][BK];
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6, r = lane & 31, h = lane >> 5;
   const int64_t i0w = int64_t(blockIdx.x) * BI + 32 * wave;
@@ -511,6 +547,94 @@ __global__ void __launch_bounds__(kThreads)
   const int64_t mbase = kt_begin * BK;
   const bool split = gridDim.z > 1;
   float* Wout = split ? W + int64_t(blockIdx.z) * P * n : W;
+
+  floatx16 acc[NACC];
+#pragma unroll
+  for (int c = 0; c < NACC; ++c)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[c][e] = 0.f;
+
+  // prologue: probe tile 0 staged, tile 1 in flight
+  u32x4 hA[PT], hB[PT], lA[PT], lB[PT];
+  float areg[16];
+  const int64_t kt1 = ntiles > 1 ? 1 : 0;
+  load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin, hA);
+  if (LO) load_plane_tile<NACC>(Vlo_g, Ppad, p0, kt_begin, lA);
+  load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin + kt1, hB);
+  if (LO) load_plane_tile<NACC>(Vlo_g, Ppad, p0, kt_begin + kt1, lB);
+  store_plane_tile<NACC>(Vhi[0], hA);
+  if (LO) store_plane_tile<NACC>(&Vlo[0][0], lA);
+  load_a_frag16<VEC>(A, n, row, mbase, h, areg);
+  __syncthreads();
+  u32x4 fAh[2], fAl[2], fBh[2], fBl[2];  // fragments of the current / next tile
+  {
+    float a[16];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) a[s] = areg[s];
+    if (ntiles > 1) load_a_frag16<VEC>(A, n, row, mbase + BK, h, areg);
+    split_frag<NPROD>(a, fAh, fAl);
+  }
+
+  auto step = [&](int64_t kt, u32x4 (&ih)[PT], u32x4 (&il)[PT], const u32x4 (&sh)[PT],
+                  const u32x4 (&sl)[PT], const u32x4 (&ahi)[2], const u32x4 (&alo)[2],
+                  u32x4 (&nhi)[2], u32x4 (&nlo)[2]) {
+    const int buf = int(kt & 1);
+    const int64_t m0 = mbase + kt * BK;
+    const int64_t kt2 = (kt + 2 < ntiles) ? kt + 2 : ntiles - 1;
+    load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin + kt2, ih);
+    if (LO) load_plane_tile<NACC>(Vlo_g, Ppad, p0, kt_begin + kt2, il);
+    tile_mfmas<NACC, NPROD>(Vhi[buf] + r * LDB + 8 * h, &Vlo[LO ? buf : 0][0] + r * LDB + 8 * h,
+                            ahi, alo, acc);
+    {
+      float a[16];
+#pragma unroll
+      for (int s = 0; s < 16; ++s) a[s] = areg[s];
+      if (kt + 2 < ntiles) load_a_frag16<VEC>(A, n, row, m0 + 2 * BK, h, areg);
+      split_frag<NPROD>(a, nhi, nlo);
+    }
+    store_plane_tile<NACC>(Vhi[buf ^ 1], sh);
+    if (LO) store_plane_tile<NACC>(&Vlo[LO ? (buf ^ 1) : 0][0], sl);
+    __syncthreads();
+  };
+
+  int64_t kt = 0;
+  for (; kt + 1 < ntiles; kt += 2) {
+    step(kt, hA, lA, hB, lB, fAh, fAl, fBh, fBl);
+    step(kt + 1, hB, lB, hA, lA, fBh, fBl, fAh, fAl);
+  }
+  if (kt < ntiles) step(kt, hA, lA, hB, lB, fAh, fAl, fBh, fBl);
+  store_out<NACC, VEC, false>(Wout, V, P, n, i0w, p0, acc, 0.f);
+}
+
+// RBF BF16 matvec.  Per tile and wave: three FP32 MFMAs give the exponents of
+// the NEXT tile, the 2 NACC NPROD BF16 MFMAs of this tile follow, and the
+// exp2 / split of the next tile's fragment is interleaved between them.  The
+// probe tiles and the point records are fetched two tiles ahead into
+// alternating register sets; the loop body is one basic block (clamped tile
+// indices, unconditional staging of the other LDS buffer), so that the
+// scheduling groups below hold.
+template <int NACC, bool VEC, int NPROD>
+__global__ void __launch_bounds__(kThreads)
+    k_matvec_rbf_bf16(const float2* __restrict__ rec, RBFParams prm,
+                      const __bf16* __restrict__ Vhi_g, const __bf16* __restrict__ Vlo_g,
+                      int64_t Ppad, const float* __restrict__ V, int64_t P, int64_t n,
+                      float* __restrict__ W) {
+  constexpr int BPW = 32 * NACC;
+  constexpr bool LO = NPROD == 3;  // probes' low plane used
+  constexpr int PT = PlaneTile<NACC>::kPerThread;
+  __shared__ __attribute__((aligned(16))) __bf16 Vhi[2][BPW * LDB];
+  __shared__ __attribute__((aligned(16))) __bf16 Vlo[LO ? 2 : 1][LO ? BPW * LDB : 8];
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6, r = lane & 31, h = lane >> 5;
+  const int64_t i0w = int64_t(blockIdx.x) * BI + 32 * wave;
+  const int64_t row = i0w + r;
+  const int64_t p0 = int64_t(blockIdx.y) * BPW;
+  const int64_t all_tiles = (n + BK - 1) / BK;
+  const int64_t kt_begin = all_tiles * blockIdx.z / gridDim.z;
+  const int64_t kt_end = all_tiles * (blockIdx.z + 1) / gridDim.z;
+  const int64_t ntiles = kt_end - kt_begin;
+  const bool split = gridDim.z > 1;
+  float* Wout = split ? W + int64_t(blockIdx.z) * P * n : W;
   const int dim = prm.dim;
 
   floatx16 acc[NACC];
@@ -519,9 +643,10 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[c][e] = 0.f;
 
-  // lane constants: B operand R[2t + h][row] of the exponent MFMAs
+  // lane constants: B operand R[2t + h][row] of the exponent MFMAs, and the
+  // third A operand L[m][4 + h]
   float rb[3] = {0.f, 0.f, 0.f};
-  if (RBF) {
+  {
     float u[3] = {0.f, 0.f, 0.f};
     float ai = 0.f;
     if (row < n) {
@@ -535,150 +660,110 @@ __global__ void __launch_bounds__(kThreads)
     rb[1] = h ? 1.f : u[2];
     rb[2] = h ? 0.f : ai;
   }
-  // Point m0 + (t & 31) of a tile as (x, y, z, c |x|^2), or (0, 0, 0, -inf)
-  // past n; branch-free (clamped address, selects) so that it does not split
-  // the scheduling region of the tile loop.
-  auto prep_x = [&](int64_t m0) {
-    const int64_t m = m0 + (t & (BK - 1));
-    const bool valid = m < n;
-    const float* src = prm.X + (valid ? m : n - 1) * dim;
-    const float x0 = src[0];
-    const float x1 = dim > 1 ? src[dim > 1 ? 1 : 0] : 0.f;
-    const float x2 = dim > 2 ? src[dim > 2 ? 2 : 0] : 0.f;
-    const float b = fmaf(x2 * x2, prm.coef, fmaf(x1 * x1, prm.coef, x0 * x0 * prm.coef));
-    return valid ? make_float4(x0, x1, x2, b) : make_float4(0.f, 0.f, 0.f, -__builtin_inff());
-  };
+  const float a2 = h ? 0.f : 1.f;
+  // point record of this lane for tile kt (relative to kt_begin)
+  const float2* xrow = rec + 2 * (kt_begin * BK + r) + h;
+  auto load_pts = [&](int64_t kt) { return xrow[2 * BK * kt]; };
 
-  // prologue: probe tile 0 staged, tile 1 in flight, points of tiles 0 and 1
+  // prologue: probe tile 0 staged, tile 1 in flight, fragment of tile 0
   u32x4 hA[PT], hB[PT], lA[PT], lB[PT];
-  float areg[16];
-  float4 xnext = make_float4(0.f, 0.f, 0.f, -__builtin_inff());
+  const int64_t kt1 = ntiles > 1 ? 1 : 0;
   load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin, hA);
   if (LO) load_plane_tile<NACC>(Vlo_g, Ppad, p0, kt_begin, lA);
-  if (ntiles > 1) {
-    load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin + 1, hB);
-    if (LO) load_plane_tile<NACC>(Vlo_g, Ppad, p0, kt_begin + 1, lB);
-  }
+  load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin + kt1, hB);
+  if (LO) load_plane_tile<NACC>(Vlo_g, Ppad, p0, kt_begin + kt1, lB);
+  float2 xA = load_pts(kt1), xB;
   store_plane_tile<NACC>(Vhi[0], hA);
   if (LO) store_plane_tile<NACC>(&Vlo[0][0], lA);
-  if (RBF) {
-    if (t < BK) {
-      Xs[0][t] = prep_x(mbase);
-      Xs[1][t] = (ntiles > 1) ? prep_x(mbase + BK) : prep_x(n);
-    }
-  } else {
-    load_a_frag16<VEC>(A, n, row, mbase, h, areg);
-  }
-  __syncthreads();
   u32x4 fAh[2], fAl[2], fBh[2], fBl[2];  // fragments of the current / next tile
   {
+    const floatx16 d = rbf_exponents_mfma(load_pts(0), a2, rb);
     float a[16];
-    if (RBF) {
-      gen_rbf_frag_mfma(Xs[0], rb, a);
-    } else {
 #pragma unroll
-      for (int s = 0; s < 16; ++s) a[s] = areg[s];
-      if (ntiles > 1) load_a_frag16<VEC>(A, n, row, mbase + BK, h, areg);
-    }
+    for (int e = 0; e < 16; ++e) a[e] = __builtin_amdgcn_exp2f(d[e]);
     split_frag<NPROD>(a, fAh, fAl);
   }
+  __syncthreads();
 
-  // One tile: issue the loads of tile kt + 2 into (ih, il) -- they held tile
-  // kt, staged one iteration ago -- run the MFMAs of tile kt with fragment
-  // (ahi, alo), build the next fragment into (nhi, nlo), and stage tile kt + 1
-  // from (sh, sl) into the other LDS buffer.
+  // One tile: issue the loads of tile kt + 2 into (ih, il, xi) -- they held
+  // tile kt, staged one iteration ago -- run the exponent MFMAs of tile kt + 1
+  // from xs and the BF16 MFMAs of tile kt with fragment (ahi, alo), build the
+  // next fragment into (nhi, nlo), and stage tile kt + 1 from (sh, sl) into
+  // the other LDS buffer (on the last tile this copy is unused).
+  //
+  // Instruction order is fixed with scheduling barriers: the loads, the
+  // exponent MFMAs, the B-fragment reads, kLead BF16 MFMAs while the
+  // exponents complete, then the remaining BF16 MFMAs each followed by a
+  // share of the eight exp2 / split chunks of the next fragment (so the VALU
+  // work issues while the matrix core runs), and the LDS stores last.
+  constexpr int kM = 2 * NPROD * NACC;  // BF16 MFMAs per tile
+  constexpr int kLead = kM > 3 ? 3 : kM - 1;
   auto step = [&](int64_t kt, u32x4 (&ih)[PT], u32x4 (&il)[PT], const u32x4 (&sh)[PT],
-                  const u32x4 (&sl)[PT], const u32x4 (&ahi)[2], const u32x4 (&alo)[2],
-                  u32x4 (&nhi)[2], u32x4 (&nlo)[2]) {
+                  const u32x4 (&sl)[PT], float2& xi, const float2& xs, const u32x4 (&ahi)[2],
+                  const u32x4 (&alo)[2], u32x4 (&nhi)[2], u32x4 (&nlo)[2]) {
     const int buf = int(kt & 1);
-    const int64_t m0 = mbase + kt * BK;
-    const bool more = kt + 1 < ntiles;
-    // tile kt + 2 (clamped: the last two steps reload the final tile, which
-    // is never staged); no branch before the MFMA/VALU region below
     const int64_t kt2 = (kt + 2 < ntiles) ? kt + 2 : ntiles - 1;
+    // points (needed one step later) ahead of the planes (needed at the end
+    // of the next step)
+    xi = load_pts(kt2);
     load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin + kt2, ih);
     if (LO) load_plane_tile<NACC>(Vlo_g, Ppad, p0, kt_begin + kt2, il);
-    if (RBF) xnext = (kt + 2 < ntiles) ? prep_x(m0 + 2 * BK) : prep_x(n);
-
-    // exponents of tile kt + 1 first, so their MFMA latency hides behind the
-    // BF16 MFMAs of tile kt (on the last tile they are computed and unused)
-    floatx16 dnext;
-    if (RBF) dnext = rbf_exponents_mfma(Xs[buf ^ 1], rb);
-
+    __builtin_amdgcn_sched_barrier(0);
+    const floatx16 dnext = rbf_exponents_mfma(xs, a2, rb);
     const __bf16* vh = Vhi[buf] + r * LDB + 8 * h;
     const __bf16* vl = &Vlo[LO ? buf : 0][0] + r * LDB + 8 * h;
+    u32x4 bh[2][NACC], bl[2][NACC];
 #pragma unroll
-    for (int kb = 0; kb < 2; ++kb) {
-      u32x4 bh[NACC], bl[NACC];
+    for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
       for (int c = 0; c < NACC; ++c) {
-        bh[c] = *reinterpret_cast<const u32x4*>(vh + 32 * c * LDB + 16 * kb);
-        if (LO) bl[c] = *reinterpret_cast<const u32x4*>(vl + 32 * c * LDB + 16 * kb);
+        bh[kb][c] = *reinterpret_cast<const u32x4*>(vh + 32 * c * LDB + 16 * kb);
+        if (LO) bl[kb][c] = *reinterpret_cast<const u32x4*>(vl + 32 * c * LDB + 16 * kb);
       }
-      if (NPROD >= 2) {
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int c = 0; c < NACC; ++c) acc[c] = mfma_bf16(alo[kb], bh[c], acc[c]);
+    for (int i = 0; i < kM; ++i) {
+      // MFMA i: kb outer, then the product (lo*hi, hi*lo, hi*hi), then c
+      const int kb = i / (NPROD * NACC);
+      const int prod = (i % (NPROD * NACC)) / NACC;
+      const int c = i % NACC;
+      const bool a_lo = NPROD >= 2 && prod == 0;
+      const bool b_lo = LO && prod == 1;
+      acc[c] = mfma_bf16(a_lo ? alo[kb] : ahi[kb], b_lo ? bl[kb][c] : bh[kb][c], acc[c]);
+      if (i >= kLead) {
+        const int j = i - kLead, slots = kM - kLead;
+#pragma unroll
+        for (int e = j * 8 / slots; e < (j + 1) * 8 / slots; ++e) {
+          const float x0 = __builtin_amdgcn_exp2f(dnext[2 * e]);
+          const float x1 = __builtin_amdgcn_exp2f(dnext[2 * e + 1]);
+          uint32_t hw, lw;
+          if (NPROD >= 2) {
+            split_pair(x0, x1, hw, lw);
+            nlo[e / 4][e % 4] = lw;
+          } else {
+            const f32x2 x = {x0, x1};
+            hw = __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2));
+          }
+          nhi[e / 4][e % 4] = hw;
+        }
       }
-      if (LO) {
-#pragma unroll
-        for (int c = 0; c < NACC; ++c) acc[c] = mfma_bf16(ahi[kb], bl[c], acc[c]);
-      }
-#pragma unroll
-      for (int c = 0; c < NACC; ++c) acc[c] = mfma_bf16(ahi[kb], bh[c], acc[c]);
+      __builtin_amdgcn_sched_barrier(0);
     }
-
-    // next tile's fragment (exp2 and split), interleaved by the scheduler
-    // with the MFMAs above
-    {
-      float a[16];
-      if (RBF) {
-#pragma unroll
-        for (int e = 0; e < 16; ++e) a[e] = __builtin_amdgcn_exp2f(dnext[e]);
-      } else {
-#pragma unroll
-        for (int s = 0; s < 16; ++s) a[s] = areg[s];
-        if (kt + 2 < ntiles) load_a_frag16<VEC>(A, n, row, m0 + 2 * BK, h, areg);
-      }
-      split_frag<NPROD>(a, nhi, nlo);
-    }
-    // keep the fragment's VALU work ahead of the barrier (instead of being
-    // sunk past it), where it can overlap the MFMAs
-#pragma unroll
-    for (int kb = 0; kb < 2; ++kb) {
-      asm volatile("" : : "v"(nhi[kb]));
-      if (NPROD >= 2) asm volatile("" : : "v"(nlo[kb]));
-    }
-    if (RBF) {
-      // scheduling: the exponent MFMAs, the B-fragment reads, then each BF16
-      // MFMA followed by a group of the fragment's VALU work
-      __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
-      __builtin_amdgcn_sched_group_barrier(0x100, (LO ? 16 : 8), 0);
-#pragma unroll
-      for (int g = 0; g < 2 * NACC * NPROD; ++g) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-        __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
-      }
-    }
-    if (more) {
-      store_plane_tile<NACC>(Vhi[buf ^ 1], sh);
-      if (LO) store_plane_tile<NACC>(&Vlo[LO ? (buf ^ 1) : 0][0], sl);
-    }
-    if (RBF && t < BK) Xs[buf][t] = xnext;  // points of tile kt + 2
+    store_plane_tile<NACC>(Vhi[buf ^ 1], sh);
+    if (LO) store_plane_tile<NACC>(&Vlo[LO ? (buf ^ 1) : 0][0], sl);
     __syncthreads();
   };
 
-  // pairs of tiles, then the odd remainder (no branch inside the loop body,
-  // so the accumulators stay in place)
   int64_t kt = 0;
   for (; kt + 1 < ntiles; kt += 2) {
-    step(kt, hA, lA, hB, lB, fAh, fAl, fBh, fBl);
-    step(kt + 1, hB, lB, hA, lA, fBh, fBl, fAh, fAl);
+    step(kt, hA, lA, hB, lB, xB, xA, fAh, fAl, fBh, fBl);
+    step(kt + 1, hB, lB, hA, lA, xA, xB, fBh, fBl, fAh, fAl);
   }
-  if (kt < ntiles) step(kt, hA, lA, hB, lB, fAh, fAl, fBh, fBl);
+  if (kt < ntiles) step(kt, hA, lA, hB, lB, xB, xA, fAh, fAl, fBh, fBl);
   if (split)
     store_out<NACC, VEC, false>(Wout, V, P, n, i0w, p0, acc, 0.f);
   else
-    store_out<NACC, VEC, RBF>(Wout, V, P, n, i0w, p0, acc, prm.noise);
+    store_out<NACC, VEC, true>(Wout, V, P, n, i0w, p0, acc, prm.noise);
 }
 
 inline bool aligned16(const void* ptr) { return (reinterpret_cast<uintptr_t>(ptr) & 15) == 0; }
@@ -698,30 +783,40 @@ __global__ void __launch_bounds__(256)
 }
 
 template <int NACC, bool RBF, int NPROD>
-hipError_t launch_tpl_bf16_n(hipStream_t s, const float* A, const RBFParams& prm,
-                             const __bf16* Vhi, const __bf16* Vlo, int64_t Ppad, const float* V,
-                             int64_t P, int64_t n, float* W, bool vec, int S) {
+hipError_t launch_tpl_bf16_n(hipStream_t s, const float* A, const float2* rec,
+                             const RBFParams& prm, const __bf16* Vhi, const __bf16* Vlo,
+                             int64_t Ppad, const float* V, int64_t P, int64_t n, float* W, bool vec,
+                             int S) {
   const dim3 grid(unsigned((n + BI - 1) / BI), unsigned((P + 32 * NACC - 1) / (32 * NACC)),
                   unsigned(S));
   const dim3 block(kThreads);
-  if (vec)
-    hipLaunchKernelGGL((k_matvec_bf16<NACC, RBF, true, NPROD>), grid, block, 0, s, A, prm, Vhi,
-                       Vlo, Ppad, V, P, n, W);
-  else
-    hipLaunchKernelGGL((k_matvec_bf16<NACC, RBF, false, NPROD>), grid, block, 0, s, A, prm, Vhi,
-                       Vlo, Ppad, V, P, n, W);
+  if constexpr (RBF) {
+    if (vec)
+      hipLaunchKernelGGL((k_matvec_rbf_bf16<NACC, true, NPROD>), grid, block, 0, s, rec, prm, Vhi,
+                         Vlo, Ppad, V, P, n, W);
+    else
+      hipLaunchKernelGGL((k_matvec_rbf_bf16<NACC, false, NPROD>), grid, block, 0, s, rec, prm,
+                         Vhi, Vlo, Ppad, V, P, n, W);
+  } else {
+    if (vec)
+      hipLaunchKernelGGL((k_matvec_bf16_dense<NACC, true, NPROD>), grid, block, 0, s, A, Vhi, Vlo,
+                         Ppad, V, P, n, W);
+    else
+      hipLaunchKernelGGL((k_matvec_bf16_dense<NACC, false, NPROD>), grid, block, 0, s, A, Vhi,
+                         Vlo, Ppad, V, P, n, W);
+  }
   return hipGetLastError();
 }
 
 template <int NACC, bool RBF>
-hipError_t launch_tpl_bf16(hipStream_t s, const float* A, const RBFParams& prm,
+hipError_t launch_tpl_bf16(hipStream_t s, const float* A, const float2* rec, const RBFParams& prm,
                            const __bf16* Vhi, const __bf16* Vlo, int64_t Ppad, const float* V,
                            int64_t P, int64_t n, float* W, bool vec, int nprod, int S) {
   if (nprod == 3)
-    return launch_tpl_bf16_n<NACC, RBF, 3>(s, A, prm, Vhi, Vlo, Ppad, V, P, n, W, vec, S);
+    return launch_tpl_bf16_n<NACC, RBF, 3>(s, A, rec, prm, Vhi, Vlo, Ppad, V, P, n, W, vec, S);
   if (nprod == 2)
-    return launch_tpl_bf16_n<NACC, RBF, 2>(s, A, prm, Vhi, Vlo, Ppad, V, P, n, W, vec, S);
-  return launch_tpl_bf16_n<NACC, RBF, 1>(s, A, prm, Vhi, Vlo, Ppad, V, P, n, W, vec, S);
+    return launch_tpl_bf16_n<NACC, RBF, 2>(s, A, rec, prm, Vhi, Vlo, Ppad, V, P, n, W, vec, S);
+  return launch_tpl_bf16_n<NACC, RBF, 1>(s, A, rec, prm, Vhi, Vlo, Ppad, V, P, n, W, vec, S);
 }
 
 template <int NACC, bool RBF, int DIM>
@@ -749,13 +844,16 @@ hipError_t launch_f32(hipStream_t s, const float* A, const RBFParams& prm, const
 int nacc_for(int64_t P) { return P <= 32 ? 1 : (P <= 64 ? 2 : 4); }
 
 template <bool RBF>
-hipError_t launch_bf16(hipStream_t s, const float* A, const RBFParams& prm, const __bf16* Vhi,
-                       const __bf16* Vlo, int64_t Ppad, const float* V, int64_t P, int64_t n,
-                       float* W, bool vec, int nprod, int S) {
+hipError_t launch_bf16(hipStream_t s, const float* A, const float2* rec, const RBFParams& prm,
+                       const __bf16* Vhi, const __bf16* Vlo, int64_t Ppad, const float* V,
+                       int64_t P, int64_t n, float* W, bool vec, int nprod, int S) {
   switch (nacc_for(P)) {
-    case 1: return launch_tpl_bf16<1, RBF>(s, A, prm, Vhi, Vlo, Ppad, V, P, n, W, vec, nprod, S);
-    case 2: return launch_tpl_bf16<2, RBF>(s, A, prm, Vhi, Vlo, Ppad, V, P, n, W, vec, nprod, S);
-    default: return launch_tpl_bf16<4, RBF>(s, A, prm, Vhi, Vlo, Ppad, V, P, n, W, vec, nprod, S);
+    case 1:
+      return launch_tpl_bf16<1, RBF>(s, A, rec, prm, Vhi, Vlo, Ppad, V, P, n, W, vec, nprod, S);
+    case 2:
+      return launch_tpl_bf16<2, RBF>(s, A, rec, prm, Vhi, Vlo, Ppad, V, P, n, W, vec, nprod, S);
+    default:
+      return launch_tpl_bf16<4, RBF>(s, A, rec, prm, Vhi, Vlo, Ppad, V, P, n, W, vec, nprod, S);
   }
 }
 
@@ -778,7 +876,8 @@ int64_t matvec_scratch_floats(int64_t n, int64_t P) {
   const int64_t bpw = 32 * nacc_for(P);
   const int64_t Ppad = (P + bpw - 1) / bpw * bpw;
   const int64_t npad = (n + BK - 1) / BK * BK;
-  return kMaxSplitK * P * n + Ppad * npad;  // partial sums + two BF16 planes
+  // partial sums + two BF16 planes + the RBF point records (4 floats each)
+  return kMaxSplitK * P * n + Ppad * npad + 4 * npad;
 }
 
 hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, int64_t P,
@@ -788,18 +887,21 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
   const bool vec = (n % 4 == 0) && aligned16(V) && aligned16(W);
   const bool fp32 = op.precision == SLQ_PREC_FP32;
   const int nprod = op.precision == SLQ_PREC_BF16X3 ? 3 : (op.precision == SLQ_PREC_BF16X2 ? 2 : 1);
-  // scratch layout: [BF16 planes (bf16 path only)] [split-K partial sums]
+  // scratch layout: [BF16 planes, RBF point records (bf16 path only)]
+  // [split-K partial sums]
   int64_t planes_floats = 0;
   __bf16 *Vhi = nullptr, *Vlo = nullptr;
+  float2* rec = nullptr;
   const int64_t npad = (n + BK - 1) / BK * BK;
   const int64_t bpw = 32 * nacc_for(P);
   const int64_t Ppad = (P + bpw - 1) / bpw * bpw;
   if (!fp32) {
     if (!scratch) return hipErrorInvalidValue;
-    planes_floats = Ppad * npad;
+    planes_floats = Ppad * npad + 4 * npad;
     if (planes_floats > scratch_floats) return hipErrorInvalidValue;
     Vhi = reinterpret_cast<__bf16*>(scratch);
     Vlo = Vhi + Ppad * npad;
+    rec = reinterpret_cast<float2*>(scratch + Ppad * npad);
     int64_t blocks = (Ppad * npad / 4 + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     if (op.kind == SLQ_OP_RBF)
@@ -819,7 +921,8 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
   if (op.kind == SLQ_OP_DENSE) {
     const bool v2 = vec && aligned16(op.matrix);
     err = fp32 ? launch_f32<false, 1>(s, op.matrix, prm, V, P, n, out, v2, S)
-               : launch_bf16<false>(s, op.matrix, prm, Vhi, Vlo, Ppad, V, P, n, out, v2, nprod, S);
+               : launch_bf16<false>(s, op.matrix, nullptr, prm, Vhi, Vlo, Ppad, V, P, n, out, v2,
+                                    nprod, S);
   } else {
     prm.X = op.points;
     prm.dim = op.dim;
@@ -827,7 +930,11 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
     prm.noise = float(op.noise);
     if (!fp32) {
       if (op.dim > 3) return hipErrorInvalidValue;  // rejected by the API before launch
-      err = launch_bf16<true>(s, nullptr, prm, Vhi, Vlo, Ppad, V, P, n, out, vec, nprod, S);
+      hipLaunchKernelGGL(k_prep_points, dim3(unsigned((npad + 255) / 256)), dim3(256), 0, s,
+                         op.points, op.dim, prm.coef, n, npad, rec);
+      err = hipGetLastError();
+      if (err != hipSuccess) return err;
+      err = launch_bf16<true>(s, nullptr, rec, prm, Vhi, Vlo, Ppad, V, P, n, out, vec, nprod, S);
     } else {
       switch (op.dim) {
         case 1: err = launch_f32<true, 1>(s, nullptr, prm, V, P, n, out, vec, S); break;
