@@ -422,53 +422,6 @@ __device__ __forceinline__ void load_a_frag16(const float* __restrict__ A, int64
   }
 }
 
-// RBF fragment for the BF16 kernels.  The base-2 exponent of a 32 x 32 tile,
-//   arg[m][i] = c |x_i|^2 + c |x_m|^2 - 2 c x_i . x_m,
-// is the rank-6 product L R with L[m] = (x_m, y_m, z_m, c |x_m|^2, 1, 0) and
-// R[:, i] = (-2c x_i, -2c y_i, -2c z_i, 1, c |x_i|^2, 0), evaluated by three
-// FP32 MFMAs (32x32x2).  The MFMA output gives lane (r, h) the exponents of
-// its own row i = r at the 16 points m = 8 (s / 4) + 4 h + (s % 4),
-// s = 0..15 -- exactly the BF16 A-fragment of that lane once the contraction
-// index is ordered that way, which the permuted probe planes provide.
-// The A operands of lane (r, h) are L[m][2t + h], t = 0..2, i.e. (x, z, 1)
-// for h = 0 and (y, c |x|^2, 0) for h = 1; the first two come from the
-// prepared point records below, the third is a lane constant.  The VALU work
-// per entry is the exp2 and the hi/lo split.  Points past n carry
-// c |x_m|^2 = -inf, so their entries are exactly 0.  The exponent's absolute
-// rounding error (~1e-6 for points in the unit cube and l = 0.5) is below
-// the 2^-17 relative error of the split BF16 representation.
-__device__ __forceinline__ floatx16 rbf_exponents_mfma(float2 xm, float a2, const float (&rb)[3]) {
-  floatx16 d;
-#pragma unroll
-  for (int e = 0; e < 16; ++e) d[e] = 0.f;
-  d = __builtin_amdgcn_mfma_f32_32x32x2f32(xm.x, rb[0], d, 0, 0, 0);
-  d = __builtin_amdgcn_mfma_f32_32x32x2f32(xm.y, rb[1], d, 0, 0, 0);
-  d = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, rb[2], d, 0, 0, 0);
-  return d;
-}
-
-// Point records of the BF16 RBF kernel: for every m < npad two float2,
-// rec[2 m] = (x, z) and rec[2 m + 1] = (y, c |x|^2) (coordinates past dim are
-// 0), and (0, 0), (0, -inf) for the padding m >= n.
-__global__ void __launch_bounds__(256)
-    k_prep_points(const float* __restrict__ X, int dim, float coef, int64_t n, int64_t npad,
-                  float2* __restrict__ rec) {
-  const int64_t m = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (m >= npad) return;
-  if (m < n) {
-    const float* src = X + m * dim;
-    const float x0 = src[0];
-    const float x1 = dim > 1 ? src[1] : 0.f;
-    const float x2 = dim > 2 ? src[2] : 0.f;
-    const float b = fmaf(x2 * x2, coef, fmaf(x1 * x1, coef, x0 * x0 * coef));
-    rec[2 * m] = make_float2(x0, x2);
-    rec[2 * m + 1] = make_float2(x1, b);
-  } else {
-    rec[2 * m] = make_float2(0.f, 0.f);
-    rec[2 * m + 1] = make_float2(0.f, -__builtin_inff());
-  }
-}
-
 template <int NPROD>
 __device__ __forceinline__ void split_frag(const float (&a)[16], u32x4 (&hi)[2], u32x4 (&lo)[2]) {
 #pragma unroll
@@ -490,6 +443,101 @@ __device__ __forceinline__ void split_frag(const float (&a)[16], u32x4 (&hi)[2],
 __device__ __forceinline__ floatx16 mfma_bf16(const u32x4& a, const u32x4& b, const floatx16& c) {
   return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
                                                  __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+// RBF exponents for the BF16 kernels.  The base-2 exponent of a 32 x 32 tile,
+//   arg[m][i] = c |x_m|^2 + c |x_i|^2 + sum_d x_m,d (-2c x_i,d),
+// is a short sum of products of FP32 numbers.  Each factor f is split into
+// three BF16 parts f = f_h + f_m + f_l (round to nearest each), and every
+// product is expanded to the parts whose size is at least 2^-16 of the
+// product: for each coordinate  hH + hM + mH + hL + lH + mM  (6 slots), and
+// the three parts of c |x_m|^2 and of c |x_i|^2 against an exact 1 -- 24 of
+// the 32 contraction slots of two BF16 MFMAs (32x32x16), the other 8 zero.
+// BF16 products are exact in FP32, so the exponent carries only the dropped
+// parts (<= 2^-24 relative per product) and the FP32 accumulation, the same
+// order as an FP32 evaluation (max abs error ~3e-6 for points in the unit
+// cube and l = 0.5), and below the 2^-17 relative error of the split BF16
+// representation of the entries.
+//
+// Slot s of the contraction (MFMA s / 16, lane half (s % 16) / 8):
+//   6 d + 0..5 (d < 3): (x_m,d ; -2c x_i,d) as hH, hM, mH, hL, lH, mM
+//   18..20: (c |x_m|^2 parts ; 1)     21..23: (1 ; c |x_i|^2 parts)
+// The point side ("L") of each point is precomputed once per matvec as 64
+// bytes; the row side ("R") is built per lane at kernel start.  The MFMA
+// output gives lane (r, h) the exponents of its own row i = r at the 16
+// points m = 8 (s / 4) + 4 h + (s % 4), s = 0..15 -- exactly the BF16
+// A-fragment of that lane once the contraction index is ordered that way,
+// which the permuted probe planes provide.  Points past n carry -inf in the
+// slot of c |x_m|^2, so their entries are exactly 0.
+__device__ __forceinline__ void split3(float x, __bf16 (&p)[3]) {
+  p[0] = __bf16(x);
+  const float r1 = x - float(p[0]);
+  p[1] = __bf16(r1);
+  p[2] = __bf16(r1 - float(p[1]));
+}
+
+// The 32 slots of one side.  side 0: point side of x (the L factors), side 1:
+// row side (the R factors, from -2c x and c |x|^2).
+__device__ __forceinline__ void exponent_slots(const float (&x)[3], float coef, int side,
+                                               bool pad, __bf16 (&slot)[32]) {
+  const __bf16 one = __bf16(1.f), zero = __bf16(0.f);
+#pragma unroll
+  for (int s = 0; s < 32; ++s) slot[s] = zero;
+  const float sq = fmaf(x[2] * x[2], coef, fmaf(x[1] * x[1], coef, x[0] * x[0] * coef));
+  __bf16 q[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    split3(side == 0 ? x[d] : -2.f * coef * x[d], q);
+    // hH, hM, mH, hL, lH, mM: point side takes (h, h, m, h, l, m), row side
+    // (H, M, H, L, H, M)
+    const int lp[6] = {0, 0, 1, 0, 2, 1}, rp[6] = {0, 1, 0, 2, 0, 1};
+#pragma unroll
+    for (int u = 0; u < 6; ++u) slot[6 * d + u] = q[side == 0 ? lp[u] : rp[u]];
+  }
+  split3(sq, q);
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    slot[18 + u] = side == 0 ? q[u] : one;
+    slot[21 + u] = side == 0 ? one : q[u];
+  }
+  if (pad) {  // a point past n: exponent -inf whatever the row
+#pragma unroll
+    for (int s = 0; s < 32; ++s) slot[s] = zero;
+    slot[18] = __bf16(-__builtin_inff());
+    slot[21] = one;
+    slot[22] = one;
+    slot[23] = one;
+  }
+}
+
+__device__ __forceinline__ floatx16 rbf_exponents_mfma(const u32x4 (&l)[2], const u32x4 (&r)[2]) {
+  floatx16 d;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) d[e] = 0.f;
+  d = mfma_bf16(l[0], r[0], d);
+  d = mfma_bf16(l[1], r[1], d);
+  return d;
+}
+
+// Point side of the exponent slots for every m < npad (64 bytes each, four
+// u32x4: slots 0-7, 8-15, 16-23, 24-31).
+__global__ void __launch_bounds__(256)
+    k_prep_points(const float* __restrict__ X, int dim, float coef, int64_t n, int64_t npad,
+                  u32x4* __restrict__ rec) {
+  const int64_t m = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (m >= npad) return;
+  float x[3] = {0.f, 0.f, 0.f};
+  if (m < n)
+    for (int d = 0; d < dim; ++d) x[d] = X[m * dim + d];
+  __bf16 slot[32];
+  exponent_slots(x, coef, 0, m >= n, slot);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    bf16x8 v;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = slot[8 * q + j];
+    rec[4 * m + q] = __builtin_bit_cast(u32x4, v);
+  }
 }
 
 // The BF16 MFMAs of one tile: acc[c] += A-fragment (ahi, alo) x probe block c
@@ -606,16 +654,16 @@ __global__ void __launch_bounds__(kThreads)
   store_out<NACC, VEC, false>(Wout, V, P, n, i0w, p0, acc, 0.f);
 }
 
-// RBF BF16 matvec.  Per tile and wave: three FP32 MFMAs give the exponents of
+// RBF BF16 matvec.  Per tile and wave: two BF16 MFMAs give the exponents of
 // the NEXT tile, the 2 NACC NPROD BF16 MFMAs of this tile follow, and the
 // exp2 / split of the next tile's fragment is interleaved between them.  The
 // probe tiles and the point records are fetched two tiles ahead into
 // alternating register sets; the loop body is one basic block (clamped tile
 // indices, unconditional staging of the other LDS buffer), so that the
-// scheduling groups below hold.
+// instruction order fixed below holds.
 template <int NACC, bool VEC, int NPROD>
 __global__ void __launch_bounds__(kThreads)
-    k_matvec_rbf_bf16(const float2* __restrict__ rec, RBFParams prm,
+    k_matvec_rbf_bf16(const u32x4* __restrict__ rec, RBFParams prm,
                       const __bf16* __restrict__ Vhi_g, const __bf16* __restrict__ Vlo_g,
                       int64_t Ppad, const float* __restrict__ V, int64_t P, int64_t n,
                       float* __restrict__ W) {
@@ -643,41 +691,47 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[c][e] = 0.f;
 
-  // lane constants: B operand R[2t + h][row] of the exponent MFMAs, and the
-  // third A operand L[m][4 + h]
-  float rb[3] = {0.f, 0.f, 0.f};
+  // lane constants: the row side of the exponent slots of this lane's row,
+  // as the B operands of the two exponent MFMAs (slots 16 t + 8 h + 0..7)
+  u32x4 rr[2];
   {
-    float u[3] = {0.f, 0.f, 0.f};
-    float ai = 0.f;
-    if (row < n) {
-      for (int d = 0; d < dim; ++d) {
-        const float x = prm.X[row * dim + d];
-        u[d] = -2.f * prm.coef * x;
-        ai = fmaf(x * x, prm.coef, ai);
-      }
+    float x[3] = {0.f, 0.f, 0.f};
+    if (row < n)
+      for (int d = 0; d < dim; ++d) x[d] = prm.X[row * dim + d];
+    __bf16 slot[32];
+    exponent_slots(x, prm.coef, 1, false, slot);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      bf16x8 v;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = slot[16 * q + 8 * h + j];
+      rr[q] = __builtin_bit_cast(u32x4, v);
     }
-    rb[0] = h ? u[1] : u[0];
-    rb[1] = h ? 1.f : u[2];
-    rb[2] = h ? 0.f : ai;
   }
-  const float a2 = h ? 0.f : 1.f;
-  // point record of this lane for tile kt (relative to kt_begin)
-  const float2* xrow = rec + 2 * (kt_begin * BK + r) + h;
-  auto load_pts = [&](int64_t kt) { return xrow[2 * BK * kt]; };
+  // point side of this lane's tile point (m0 + r) for tile kt (relative to
+  // kt_begin): the A operands of the two exponent MFMAs
+  const u32x4* prow = rec + 4 * (kt_begin * BK + r) + h;
+  auto load_pts = [&](int64_t kt, u32x4 (&l)[2]) {
+    l[0] = prow[4 * BK * kt];
+    l[1] = prow[4 * BK * kt + 2];
+  };
 
   // prologue: probe tile 0 staged, tile 1 in flight, fragment of tile 0
+  auto clamp = [&](int64_t kt) { return kt < ntiles ? kt : ntiles - 1; };
   u32x4 hA[PT], hB[PT], lA[PT], lB[PT];
-  const int64_t kt1 = ntiles > 1 ? 1 : 0;
   load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin, hA);
   if (LO) load_plane_tile<NACC>(Vlo_g, Ppad, p0, kt_begin, lA);
-  load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin + kt1, hB);
-  if (LO) load_plane_tile<NACC>(Vlo_g, Ppad, p0, kt_begin + kt1, lB);
-  float2 xA = load_pts(kt1), xB;
+  load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin + clamp(1), hB);
+  if (LO) load_plane_tile<NACC>(Vlo_g, Ppad, p0, kt_begin + clamp(1), lB);
+  u32x4 xA[2], xB[2];
+  load_pts(clamp(1), xA);
   store_plane_tile<NACC>(Vhi[0], hA);
   if (LO) store_plane_tile<NACC>(&Vlo[0][0], lA);
   u32x4 fAh[2], fAl[2], fBh[2], fBl[2];  // fragments of the current / next tile
   {
-    const floatx16 d = rbf_exponents_mfma(load_pts(0), a2, rb);
+    u32x4 x0[2];
+    load_pts(0, x0);
+    const floatx16 d = rbf_exponents_mfma(x0, rr);
     float a[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) a[e] = __builtin_amdgcn_exp2f(d[e]);
@@ -689,7 +743,8 @@ __global__ void __launch_bounds__(kThreads)
   // tile kt, staged one iteration ago -- run the exponent MFMAs of tile kt + 1
   // from xs and the BF16 MFMAs of tile kt with fragment (ahi, alo), build the
   // next fragment into (nhi, nlo), and stage tile kt + 1 from (sh, sl) into
-  // the other LDS buffer (on the last tile this copy is unused).
+  // the other LDS buffer (on the last tile this copy is unused).  Loads past
+  // the last tile are clamped.
   //
   // Instruction order is fixed with scheduling barriers: the loads, the
   // exponent MFMAs, the B-fragment reads, kLead BF16 MFMAs while the
@@ -699,17 +754,17 @@ __global__ void __launch_bounds__(kThreads)
   constexpr int kM = 2 * NPROD * NACC;  // BF16 MFMAs per tile
   constexpr int kLead = kM > 3 ? 3 : kM - 1;
   auto step = [&](int64_t kt, u32x4 (&ih)[PT], u32x4 (&il)[PT], const u32x4 (&sh)[PT],
-                  const u32x4 (&sl)[PT], float2& xi, const float2& xs, const u32x4 (&ahi)[2],
-                  const u32x4 (&alo)[2], u32x4 (&nhi)[2], u32x4 (&nlo)[2]) {
+                  const u32x4 (&sl)[PT], u32x4 (&xi)[2], const u32x4 (&xs)[2],
+                  const u32x4 (&ahi)[2], const u32x4 (&alo)[2], u32x4 (&nhi)[2],
+                  u32x4 (&nlo)[2]) {
     const int buf = int(kt & 1);
-    const int64_t kt2 = (kt + 2 < ntiles) ? kt + 2 : ntiles - 1;
     // points (needed one step later) ahead of the planes (needed at the end
     // of the next step)
-    xi = load_pts(kt2);
-    load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin + kt2, ih);
-    if (LO) load_plane_tile<NACC>(Vlo_g, Ppad, p0, kt_begin + kt2, il);
+    load_pts(clamp(kt + 2), xi);
+    load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin + clamp(kt + 2), ih);
+    if (LO) load_plane_tile<NACC>(Vlo_g, Ppad, p0, kt_begin + clamp(kt + 2), il);
     __builtin_amdgcn_sched_barrier(0);
-    const floatx16 dnext = rbf_exponents_mfma(xs, a2, rb);
+    const floatx16 dnext = rbf_exponents_mfma(xs, rr);
     const __bf16* vh = Vhi[buf] + r * LDB + 8 * h;
     const __bf16* vl = &Vlo[LO ? buf : 0][0] + r * LDB + 8 * h;
     u32x4 bh[2][NACC], bl[2][NACC];
@@ -783,7 +838,7 @@ __global__ void __launch_bounds__(256)
 }
 
 template <int NACC, bool RBF, int NPROD>
-hipError_t launch_tpl_bf16_n(hipStream_t s, const float* A, const float2* rec,
+hipError_t launch_tpl_bf16_n(hipStream_t s, const float* A, const u32x4* rec,
                              const RBFParams& prm, const __bf16* Vhi, const __bf16* Vlo,
                              int64_t Ppad, const float* V, int64_t P, int64_t n, float* W, bool vec,
                              int S) {
@@ -809,7 +864,7 @@ hipError_t launch_tpl_bf16_n(hipStream_t s, const float* A, const float2* rec,
 }
 
 template <int NACC, bool RBF>
-hipError_t launch_tpl_bf16(hipStream_t s, const float* A, const float2* rec, const RBFParams& prm,
+hipError_t launch_tpl_bf16(hipStream_t s, const float* A, const u32x4* rec, const RBFParams& prm,
                            const __bf16* Vhi, const __bf16* Vlo, int64_t Ppad, const float* V,
                            int64_t P, int64_t n, float* W, bool vec, int nprod, int S) {
   if (nprod == 3)
@@ -844,7 +899,7 @@ hipError_t launch_f32(hipStream_t s, const float* A, const RBFParams& prm, const
 int nacc_for(int64_t P) { return P <= 32 ? 1 : (P <= 64 ? 2 : 4); }
 
 template <bool RBF>
-hipError_t launch_bf16(hipStream_t s, const float* A, const float2* rec, const RBFParams& prm,
+hipError_t launch_bf16(hipStream_t s, const float* A, const u32x4* rec, const RBFParams& prm,
                        const __bf16* Vhi, const __bf16* Vlo, int64_t Ppad, const float* V,
                        int64_t P, int64_t n, float* W, bool vec, int nprod, int S) {
   switch (nacc_for(P)) {
@@ -876,8 +931,8 @@ int64_t matvec_scratch_floats(int64_t n, int64_t P) {
   const int64_t bpw = 32 * nacc_for(P);
   const int64_t Ppad = (P + bpw - 1) / bpw * bpw;
   const int64_t npad = (n + BK - 1) / BK * BK;
-  // partial sums + two BF16 planes + the RBF point records (4 floats each)
-  return kMaxSplitK * P * n + Ppad * npad + 4 * npad;
+  // partial sums + two BF16 planes + the RBF point records (16 floats each)
+  return kMaxSplitK * P * n + Ppad * npad + 16 * npad;
 }
 
 hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, int64_t P,
@@ -891,17 +946,17 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
   // [split-K partial sums]
   int64_t planes_floats = 0;
   __bf16 *Vhi = nullptr, *Vlo = nullptr;
-  float2* rec = nullptr;
+  u32x4* rec = nullptr;
   const int64_t npad = (n + BK - 1) / BK * BK;
   const int64_t bpw = 32 * nacc_for(P);
   const int64_t Ppad = (P + bpw - 1) / bpw * bpw;
   if (!fp32) {
     if (!scratch) return hipErrorInvalidValue;
-    planes_floats = Ppad * npad + 4 * npad;
+    planes_floats = Ppad * npad + 16 * npad;
     if (planes_floats > scratch_floats) return hipErrorInvalidValue;
     Vhi = reinterpret_cast<__bf16*>(scratch);
     Vlo = Vhi + Ppad * npad;
-    rec = reinterpret_cast<float2*>(scratch + Ppad * npad);
+    rec = reinterpret_cast<u32x4*>(scratch + Ppad * npad);
     int64_t blocks = (Ppad * npad / 4 + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     if (op.kind == SLQ_OP_RBF)
