@@ -118,6 +118,24 @@ int slq_hessenberg(slq_context* ctx, const slq_operator* op, const float* V0, in
 int slq_logdet(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P, int64_t k,
                double* per_probe, double* mean, float* alpha, float* beta);
 
+/* Value and parameter cotangent of the SLQ log-determinant estimate
+ *   mean = (1/P) sum_p |z_p|^2 [log T_p]_00
+ * for P probes (reverse mode through the estimator): the custom VJP of
+ * decomp.hessenberg (decomp.py:398-422), _hessenberg_adjoint with
+ * reortho="full" (decomp.py:480-600) driven by dH alone (the quadratic form
+ * discards Q, r and c, funm.py:237), the eigh VJP at [log T]_00
+ * (funm.py:331-333) and the probe mean (stochtrace.py:47-50).
+ *   RBF operators: dparams[0] = d mean / d lengthscale,
+ *                  dparams[1] = d mean / d noise (two device doubles);
+ *   stored matrices: dmatrix = d mean / d A (n x n device floats, row-major,
+ *                  the matvec VJP lambda q^T of decomp.py:587-589 summed).
+ * per_probe and mean as in slq_logdet (either may be NULL).  The forward
+ * pass is recomputed and its basis kept in context scratch; the transposed
+ * product A^T lambda uses A itself (the operator of tridiag_sym is
+ * symmetric).  Fails with SLQ_EADJOINT_K0 for k = 0 (decomp.py:483-486). */
+int slq_logdet_vjp(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P, int64_t k,
+                   double* per_probe, double* mean, double* dparams, float* dmatrix);
+
 /* ---- measurement hooks (no counterpart in the reference) -------------------
  * With profiling on, every launch of the listed kernel classes made through
  * this context is bracketed by a pair of HIP events on the context's stream.
@@ -130,7 +148,8 @@ typedef enum slq_kernel_class {
   SLQ_KC_PROJECT = 3,      /* w - Q h (+ norm partials)                  */
   SLQ_KC_NORMALIZE = 4,    /* v / |v| and basis store                    */
   SLQ_KC_QUADRATURE = 5,   /* FP64 tridiagonal eigen + quadrature        */
-  SLQ_KC_COUNT = 6
+  SLQ_KC_ADJOINT = 6,      /* whole reverse pass of slq_logdet_vjp       */
+  SLQ_KC_COUNT = 7
 } slq_kernel_class;
 int slq_profile_enable(slq_context* ctx, int on);
 int slq_profile_read(slq_context* ctx, int kernel_class, double* total_ms, int64_t* launches);

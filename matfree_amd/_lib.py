@@ -66,6 +66,11 @@ SIGNATURES = {
         [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_i64, c_void_p, c_void_p,
          c_void_p, c_void_p],
     ),
+    "slq_logdet_vjp": (
+        c_int,
+        [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_i64, c_void_p, c_void_p,
+         c_void_p, c_void_p],
+    ),
     "slq_profile_enable": (c_int, [c_void_p, c_int]),
     "slq_profile_read": (c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(c_i64)]),
@@ -73,7 +78,7 @@ SIGNATURES = {
 
 # kernel classes of the measurement hooks (slq_kernel_class)
 KERNEL_CLASSES = {"matvec": 0, "dots": 1, "project_dots": 2, "project": 3, "normalize": 4,
-                  "quadrature": 5}
+                  "quadrature": 5, "adjoint": 6}
 
 _lib = None
 _lock = threading.Lock()

@@ -1,0 +1,382 @@
+// Reverse pass of the SLQ log-determinant: the Arnoldi adjoint with full
+// reorthogonalisation (decomp.py:480-600) driven by the cotangent of
+// |z|^2 [log T]_00 (funm.py:239-241, 331-333) and the probe mean
+// (stochtrace.py:50).  The quadratic form discards Q, r and c
+// (funm.py:237), so dQ = dr = dc = 0 and the adjoint's inputs reduce to dH.
+//
+// Kernels:
+//  * k_tridiag_vjp: per probe, T = tridiag(alpha, beta) from H exactly as
+//    the forward extracts it (FP32, decomp.py:133-135), its full FP64 eigen
+//    decomposition T = U diag(theta) U^T (implicit QL, eigenvectors
+//    accumulated), and the cotangent of [log T]_00 in the Daleckii-Krein
+//    form Tbar = U (L o u u^T) U^T with u = U[0, :] and L the divided
+//    differences of log at the eigenvalues -- what the eigh VJP composed
+//    with fA[0, 0] evaluates to (funm.py:331-333).  Through
+//    alpha_i = H_ii and beta_i = (H_{i,i+1} + H_{i+1,i}) / 2 this gives
+//    dH_ii = Tbar_ii and dH_{i,i+1} = dH_{i+1,i} = Tbar_{i,i+1}, scaled by
+//    |z|^2 / P.
+//  * k_adjoint_coef / k_adjoint_combine: one step of the backward
+//    recursion (decomp.py:587-599) given the reductions of the streaming
+//    kernels (k_lanczos.hip).
+//  * k_dot_sum / k_sum_fixed: deterministic FP64 sums of elementwise
+//    products for the parameter cotangents.
+#include <cfloat>
+
+#include "slq_internal.h"
+
+namespace slq {
+namespace {
+
+constexpr int kMaxSweeps = 64;
+
+// ---------------------------------------------------------------------------
+// Tridiagonal eigen cotangent.  One wave per probe; (d, e, lg) are k doubles,
+// U is k x k row-major (U[a * k + j] = component a of eigenvector j), all in
+// LDS for k <= kVjpLdsDepth and in a global workspace beyond.  The QL
+// recurrence is evaluated redundantly by every lane (uniform control flow);
+// the rotations of U are spread over the lanes by rows.
+// ---------------------------------------------------------------------------
+constexpr int kVjpLdsDepth = 64;
+
+__device__ void tridiag_vjp_wave(const float* __restrict__ Hp, int k, double* d, double* e,
+                                 double* lg, double* U, double scale, float* __restrict__ dHp) {
+  const int lane = threadIdx.x;
+  for (int i = lane; i < k; i += 64) {
+    d[i] = double(Hp[int64_t(i) * k + i]);
+    e[i] = (i + 1 < k) ? double(0.5f * (Hp[int64_t(i + 1) * k + i] + Hp[int64_t(i) * k + i + 1]))
+                       : 0.0;
+  }
+  for (int a = lane; a < k; a += 64)
+    for (int j = 0; j < k; ++j) U[int64_t(a) * k + j] = (a == j) ? 1.0 : 0.0;
+  __syncthreads();
+
+  bool failed = false;
+  for (int l = 0; l < k && !failed; ++l) {
+    int sweeps = 0;
+    while (true) {
+      int m = l;
+      for (; m + 1 < k; ++m) {
+        const double dd = fabs(d[m]) + fabs(d[m + 1]);
+        if (fabs(e[m]) <= DBL_EPSILON * dd) break;
+      }
+      if (m == l) break;
+      if (++sweeps > kMaxSweeps) {
+        failed = true;
+        break;
+      }
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + copysign(r, g));
+      double s = 1.0, c = 1.0, shift = 0.0;
+      bool split = false;
+      for (int i = m - 1; i >= l; --i) {
+        const double f = s * e[i];
+        const double b = c * e[i];
+        r = hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {  // an off-diagonal underflowed: the matrix split
+          d[i + 1] -= shift;
+          e[m] = 0.0;
+          split = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - shift;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        shift = s * r;
+        d[i + 1] = g + shift;
+        g = c * r - b;
+        for (int a = lane; a < k; a += 64) {
+          double* ua = U + int64_t(a) * k;
+          const double ui = ua[i], ui1 = ua[i + 1];
+          ua[i + 1] = s * ui + c * ui1;
+          ua[i] = c * ui - s * ui1;
+        }
+      }
+      if (split) continue;
+      d[l] -= shift;
+      e[l] = g;
+      e[m] = 0.0;
+    }
+  }
+  // log of the eigenvalues; a non-positive eigenvalue makes the reference's
+  // log, and so its cotangent, NaN
+  __syncthreads();
+  bool nonpositive = failed;
+  for (int j = 0; j < k; ++j) nonpositive |= !(d[j] > 0.0);
+  for (int j = lane; j < k; j += 64) lg[j] = log(d[j]);
+  __syncthreads();
+
+  // Tbar = U W U^T, W_jl = u_j u_l L_jl; lane a forms the diagonal entry a
+  // and the superdiagonal entry (a, a + 1):
+  //   v_l = sum_j W_jl U_aj,  Tbar_aa = sum_l U_al v_l,
+  //   Tbar_{a,a+1} = sum_l U_{a+1,l} v_l.
+  for (int a = lane; a < k; a += 64) {
+    double tdd = 0.0, tdo = 0.0;
+    const double* ua = U + int64_t(a) * k;
+    const double* ub = U + int64_t(a + 1 < k ? a + 1 : a) * k;
+    for (int l = 0; l < k; ++l) {
+      const double ul = U[l], thl = d[l];
+      double v = 0.0;
+      for (int j = 0; j < k; ++j) {
+        const double thj = d[j], del = thl - thj;
+        double L;
+        if (j == l || del == 0.0) {
+          L = 1.0 / thj;
+        } else if (fabs(del) < 1e-3 * fabs(thj)) {
+          L = log1p(del / thj) / del;  // (log thl - log thj) / (thl - thj), no cancellation
+        } else {
+          L = (lg[l] - lg[j]) / del;
+        }
+        v += U[j] * ul * L * ua[j];
+      }
+      tdd += ua[l] * v;
+      tdo += ub[l] * v;
+    }
+    const float nan = __builtin_nanf("");
+    dHp[int64_t(a) * k + a] = nonpositive ? nan : float(scale * tdd);
+    if (a + 1 < k) {
+      const float o = nonpositive ? nan : float(scale * tdo);
+      dHp[int64_t(a + 1) * k + a] = o;  // (a, a + 1)
+      dHp[int64_t(a) * k + a + 1] = o;  // (a + 1, a)
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64)
+    k_tridiag_vjp_lds(const float* __restrict__ H, const float* __restrict__ len, int64_t P,
+                      int64_t k, float* __restrict__ dH) {
+  __shared__ double U[kVjpLdsDepth * kVjpLdsDepth];
+  __shared__ double d[kVjpLdsDepth], e[kVjpLdsDepth], lg[kVjpLdsDepth];
+  const int64_t p = blockIdx.x;
+  const double l2 = double(len[p]) * double(len[p]);
+  tridiag_vjp_wave(H + p * k * k, int(k), d, e, lg, U, l2 / double(P), dH + p * k * k);
+}
+
+__global__ void __launch_bounds__(64)
+    k_tridiag_vjp_global(const float* __restrict__ H, const float* __restrict__ len, int64_t P,
+                         int64_t k, double* __restrict__ work, float* __restrict__ dH) {
+  const int64_t p = blockIdx.x;
+  double* w = work + p * (k * k + 3 * k);
+  const double l2 = double(len[p]) * double(len[p]);
+  tridiag_vjp_wave(H + p * k * k, int(k), w + k * k, w + k * k + k, w + k * k + 2 * k, w,
+                   l2 / double(P), dH + p * k * k);
+}
+
+// ---------------------------------------------------------------------------
+// Coefficients of one backward step idx (decomp.py:592-599), one block per
+// probe.  With s_j = q_j^T (A^T lambda) (sum of the partials over chunks):
+//   Gamma[idx, j] = lower_mask[idx, j] (Pi_gamma[idx, j] - s_j), j <= idx,
+//     lower_mask = tril(1) with 1/2 on the diagonal (decomp.py:490-495),
+//     Pi_gamma = H dH^T (dQ = dc = 0, decomp.py:506), dH tridiagonal;
+//   lambda_new = (gamma_idx r + sum_j (Gamma + Gamma^T)[idx, j] q_j
+//                 - alpha_idx lambda + A^T lambda
+//                 - sum_{j > idx} H[idx, j] Lambda_j) / beta_minus_idx
+//     with gamma = dH[:, k-1] (dr = 0, decomp.py:498, 505),
+//     beta_minus = (1, H[1,0], H[2,1], ...) (decomp.py:513) and beta_plus
+//     the strict upper part of H beyond the diagonal (decomp.py:515).
+// Output per probe (stride cs = 2 k + 4): gq[0..k), gl[k..2k), and
+// (cr, cl, cv, scale) for r, lambda, A^T lambda and the final division.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+    k_adjoint_coef(const float* __restrict__ H, const float* __restrict__ dH,
+                   float* __restrict__ Gam, const float* __restrict__ part, int64_t nb,
+                   int pstride, int64_t k, int64_t idx, float* __restrict__ coef) {
+  extern __shared__ float grow[];  // Gamma[idx, 0..idx]
+  const int64_t p = blockIdx.x;
+  const float* Hp = H + p * k * k;    // column-major: H[row + col * k]
+  const float* dHp = dH + p * k * k;  // column-major
+  float* Gp = Gam + p * k * k;        // column-major
+  float* cp = coef + p * (2 * k + 4);
+  for (int64_t j = threadIdx.x; j <= idx; j += blockDim.x) {
+    float s = 0.f;
+    for (int64_t b = 0; b < nb; ++b) s += part[(p * nb + b) * pstride + j];
+    float pg = Hp[idx + j * k] * dHp[j + j * k];
+    if (j > 0) pg += Hp[idx + (j - 1) * k] * dHp[j + (j - 1) * k];
+    if (j + 1 < k) pg += Hp[idx + (j + 1) * k] * dHp[j + (j + 1) * k];
+    const float w = (j < idx) ? 1.f : 0.5f;
+    const float g = w * (pg - s);
+    grow[j] = g;
+    Gp[idx + j * k] = g;
+  }
+  __syncthreads();
+  for (int64_t j = threadIdx.x; j < k; j += blockDim.x) {
+    float gq;
+    if (j < idx)
+      gq = grow[j];
+    else if (j == idx)
+      gq = 2.f * grow[j];
+    else
+      gq = Gp[j + idx * k];  // Gamma[j, idx], set at step j
+    cp[j] = gq;
+    cp[k + j] = (j > idx) ? -Hp[idx + j * k] : 0.f;
+  }
+  if (threadIdx.x == 0) {
+    cp[2 * k + 0] = dHp[idx + (k - 1) * k];  // gamma_idx
+    cp[2 * k + 1] = -Hp[idx + idx * k];      // -alpha_idx
+    cp[2 * k + 2] = 1.f;                     // + A^T lambda
+    cp[2 * k + 3] = 1.f / (idx == 0 ? 1.f : Hp[idx + (idx - 1) * k]);
+  }
+}
+
+// y = scale (cr r + cl x + cv v + sum_{j < ncq} gq_j Q_j + sum_{jl0 <= j < k} gl_j L_j)
+// per probe, coefficients from coef (stride 2 k + 4); r, x, v may be null
+// (their terms are skipped).  With save (non-null) x is also copied there.
+template <int V>
+__global__ void __launch_bounds__(kStreamThreads)
+    k_adjoint_combine(const float* __restrict__ Q, const float* __restrict__ Lam, int64_t ldc,
+                      const float* __restrict__ r, const float* __restrict__ x,
+                      const float* __restrict__ v, float* __restrict__ y, float* __restrict__ save,
+                      int64_t n, int64_t k, int ncq, int jl0, const float* __restrict__ coef) {
+  extern __shared__ float cs[];
+  const int64_t p = blockIdx.y;
+  const int t = threadIdx.x;
+  const float* cp = coef + p * (2 * k + 4);
+  for (int64_t j = t; j < 2 * k + 4; j += blockDim.x) cs[j] = cp[j];
+  __syncthreads();
+  const float cr = cs[2 * k], cl = cs[2 * k + 1], cv = cs[2 * k + 2], sc = cs[2 * k + 3];
+  const int64_t per_block = int64_t(kStreamThreads) * V;
+  const int64_t i = int64_t(blockIdx.x) * per_block + int64_t(t) * V;
+  if (i >= n) return;
+  const int64_t off = p * n + i;
+  float acc[V];
+#pragma unroll
+  for (int u = 0; u < V; ++u) acc[u] = 0.f;
+  auto axpy = [&](float a, const float* src) {
+    if constexpr (V == 4) {
+      const float4 q = *reinterpret_cast<const float4*>(src);
+      acc[0] = fmaf(a, q.x, acc[0]);
+      acc[1] = fmaf(a, q.y, acc[1]);
+      acc[2] = fmaf(a, q.z, acc[2]);
+      acc[3] = fmaf(a, q.w, acc[3]);
+    } else {
+      acc[0] = fmaf(a, src[0], acc[0]);
+    }
+  };
+  for (int j = 0; j < ncq; ++j) axpy(cs[j], Q + int64_t(j) * ldc + off);
+  for (int64_t j = jl0; j < k; ++j) axpy(cs[k + j], Lam + j * ldc + off);
+  if (r) axpy(cr, r + off);
+  if (x) axpy(cl, x + off);
+  if (v) axpy(cv, v + off);
+  float out[V];
+#pragma unroll
+  for (int u = 0; u < V; ++u) out[u] = sc * acc[u];
+  if constexpr (V == 4) {
+    *reinterpret_cast<float4*>(y + off) = make_float4(out[0], out[1], out[2], out[3]);
+    if (save) *reinterpret_cast<float4*>(save + off) = *reinterpret_cast<const float4*>(x + off);
+  } else {
+    y[off] = out[0];
+    if (save) save[off] = x[off];
+  }
+}
+
+// Block partial sums of a[i] * b[i] in FP64 (i < count), fixed order inside
+// a block: partial[blockIdx.x].
+__global__ void __launch_bounds__(256)
+    k_dot_sum(const float* __restrict__ a, const float* __restrict__ b, int64_t count,
+              double* __restrict__ partial) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += int64_t(gridDim.x) * blockDim.x)
+    s += double(a[i]) * double(b[i]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// out[o] = f_o * sum of partial[o * count .. (o + 1) * count) in order, o < 2.
+__global__ void k_sum_fixed(const double* __restrict__ partial, int64_t count, int nout, double f0,
+                            double f1, double* __restrict__ out) {
+  const int o = threadIdx.x;
+  if (o >= nout) return;
+  double s = 0.0;
+  for (int64_t i = 0; i < count; ++i) s += partial[o * count + i];
+  out[o] = (o == 0 ? f0 : f1) * s;
+}
+
+// Coefficients that make k_adjoint_combine compute lambda = Q gamma with
+// gamma = dH[:, k - 1] (decomp.py:498-499 with dr = 0).
+__global__ void k_adjoint_init_coef(const float* __restrict__ dH, int64_t k,
+                                    float* __restrict__ coef) {
+  const int64_t p = blockIdx.x;
+  float* cp = coef + p * (2 * k + 4);
+  const float* dHp = dH + p * k * k;
+  for (int64_t j = threadIdx.x; j < 2 * k + 4; j += blockDim.x) {
+    float v = 0.f;
+    if (j < k) v = dHp[j + (k - 1) * k];
+    if (j == 2 * k + 3) v = 1.f;
+    cp[j] = v;
+  }
+}
+
+}  // namespace
+
+hipError_t launch_tridiag_vjp(hipStream_t s, const float* H, const float* len, int64_t P, int64_t k,
+                              double* work, float* dH) {
+  if (P <= 0 || k <= 0) return hipSuccess;
+  if (k <= kVjpLdsDepth)
+    hipLaunchKernelGGL(k_tridiag_vjp_lds, dim3(unsigned(P)), dim3(64), 0, s, H, len, P, k, dH);
+  else
+    hipLaunchKernelGGL(k_tridiag_vjp_global, dim3(unsigned(P)), dim3(64), 0, s, H, len, P, k, work,
+                       dH);
+  return hipGetLastError();
+}
+
+int64_t tridiag_vjp_work_doubles(int64_t P, int64_t k) {
+  return k <= kVjpLdsDepth ? 0 : P * (k * k + 3 * k);
+}
+
+hipError_t launch_adjoint_coef(hipStream_t s, const float* H, const float* dH, float* Gam,
+                               const float* part, int pstride, int64_t P, int64_t n, int64_t k,
+                               int64_t idx, float* coef) {
+  const size_t shmem = size_t(idx + 1) * sizeof(float);
+  hipLaunchKernelGGL(k_adjoint_coef, dim3(unsigned(P)), dim3(256), shmem, s, H, dH, Gam, part,
+                     num_chunks(n), pstride, k, idx, coef);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint_combine(hipStream_t s, const float* Q, const float* Lam, int64_t ldc,
+                                  const float* r, const float* x, const float* v, float* y,
+                                  float* save, int64_t P, int64_t n, int64_t k, int ncq, int jl0,
+                                  const float* coef) {
+  const auto al = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
+  const bool vec = n % 4 == 0 && ldc % 4 == 0 && al(Q) && al(y) && (!Lam || al(Lam)) &&
+                   (!r || al(r)) && (!x || al(x)) && (!v || al(v)) && (!save || al(save));
+  const int V = vec ? 4 : 1;
+  const int64_t per_block = int64_t(kStreamThreads) * V;
+  const dim3 grid(unsigned((n + per_block - 1) / per_block), unsigned(P));
+  const size_t shmem = size_t(2 * k + 4) * sizeof(float);
+  if (vec)
+    hipLaunchKernelGGL(k_adjoint_combine<4>, grid, dim3(kStreamThreads), shmem, s, Q, Lam, ldc, r,
+                       x, v, y, save, n, k, ncq, jl0, coef);
+  else
+    hipLaunchKernelGGL(k_adjoint_combine<1>, grid, dim3(kStreamThreads), shmem, s, Q, Lam, ldc, r,
+                       x, v, y, save, n, k, ncq, jl0, coef);
+  return hipGetLastError();
+}
+
+hipError_t launch_dot_sum(hipStream_t s, const float* a, const float* b, int64_t count,
+                          double* partial, int blocks) {
+  hipLaunchKernelGGL(k_dot_sum, dim3(unsigned(blocks)), dim3(256), 0, s, a, b, count, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_fixed(hipStream_t s, const double* partial, int64_t count, int nout,
+                            double f0, double f1, double* out) {
+  hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(64), 0, s, partial, count, nout, f0, f1, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint_init_coef(hipStream_t s, const float* dH, int64_t P, int64_t k,
+                                    float* coef) {
+  hipLaunchKernelGGL(k_adjoint_init_coef, dim3(unsigned(P)), dim3(256), 0, s, dH, k, coef);
+  return hipGetLastError();
+}
+
+}  // namespace slq

@@ -192,15 +192,20 @@ __global__ void __launch_bounds__(kStreamThreads)
   if (t == 0) partial[p * nb + b] = s;
 }
 
-// Reduce coefficient partials of probe p into LDS (fixed order over chunks).
+// Reduce coefficient partials of probe p into LDS (fixed order over chunks);
+// hout (if given) receives the sums, and hsub (if given) is subtracted from
+// them afterwards: hs[c] = sum_b hpart[p][b][c] - hsub[p * hsub_stride + c].
 __device__ __forceinline__ void load_coeffs(const float* __restrict__ hpart, int64_t p,
                                             int64_t nb, int ncols, int pstride, float* hs,
-                                            float* hout, int64_t hout_stride, bool write_out) {
+                                            float* hout, int64_t hout_stride, bool write_out,
+                                            const float* __restrict__ hsub = nullptr,
+                                            int64_t hsub_stride = 0) {
   for (int c = threadIdx.x; c < ncols; c += blockDim.x) {
     float s = 0.f;
     for (int64_t bb = 0; bb < nb; ++bb) s += hpart[(p * nb + bb) * pstride + c];
-    hs[c] = s;
     if (write_out && hout) hout[p * hout_stride + c] = s;
+    if (hsub) s -= hsub[p * hsub_stride + c];
+    hs[c] = s;
   }
   __syncthreads();
 }
@@ -213,14 +218,14 @@ __global__ void __launch_bounds__(kStreamThreads)
     k_project(const float* __restrict__ Q, int64_t ldc, const float* __restrict__ x,
               float* __restrict__ y, int64_t n, int ncols, const float* __restrict__ hpart,
               int pstride, float* __restrict__ norm2part, float* __restrict__ hout,
-              int64_t hout_stride) {
+              int64_t hout_stride, const float* __restrict__ hsub, int64_t hsub_stride) {
   constexpr int NE = Layout<V>::NE;
   extern __shared__ float hs[];
   __shared__ float red[kWaves];
   const int64_t p = blockIdx.y, b = blockIdx.x, nb = gridDim.x;
   const int t = threadIdx.x;
   const int64_t base = b * kChunk;
-  load_coeffs(hpart, p, nb, ncols, pstride, hs, hout, hout_stride, b == 0);
+  load_coeffs(hpart, p, nb, ncols, pstride, hs, hout, hout_stride, b == 0, hsub, hsub_stride);
 
   float s[NE][V];
 #pragma unroll
@@ -434,16 +439,17 @@ hipError_t launch_dots(hipStream_t s, const float* Q, int64_t ldc, const float* 
 
 hipError_t launch_project(hipStream_t s, const float* Q, int64_t ldc, const float* x, float* y,
                           int64_t P, int64_t n, int ncols, const float* hpart, int pstride,
-                          float* norm2part, float* hout, int64_t hout_stride) {
+                          float* norm2part, float* hout, int64_t hout_stride, const float* hsub,
+                          int64_t hsub_stride) {
   const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
   const size_t shmem = size_t(ncols > 0 ? ncols : 1) * sizeof(float);
   const bool vec = (n % 4 == 0) && aligned16(x) && aligned16(y) && aligned16(Q) && ldc % 4 == 0;
   if (vec)
     hipLaunchKernelGGL(k_project<4>, grid, dim3(kStreamThreads), shmem, s, Q, ldc, x, y, n, ncols,
-                       hpart, pstride, norm2part, hout, hout_stride);
+                       hpart, pstride, norm2part, hout, hout_stride, hsub, hsub_stride);
   else
     hipLaunchKernelGGL(k_project<1>, grid, dim3(kStreamThreads), shmem, s, Q, ldc, x, y, n, ncols,
-                       hpart, pstride, norm2part, hout, hout_stride);
+                       hpart, pstride, norm2part, hout, hout_stride, hsub, hsub_stride);
   return hipGetLastError();
 }
 

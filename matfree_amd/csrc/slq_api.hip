@@ -18,6 +18,14 @@ struct slq_context {
   // pooled scratch, grown on demand (never shrunk until destroy)
   Buf partial1, partial2, norm2, work_w, work_w2, basis, hmat, resid, vec0, scal1, scal2, eig,
       per_probe, mean, splitk;
+  // reverse pass
+  Buf adj_dh, adj_gam, adj_coef, adj_lam, adj_v1, adj_v2, adj_v3, adj_work, adj_sum;
+  std::vector<Buf*> all_bufs() {
+    return {&partial1, &partial2, &norm2,   &work_w,  &work_w2,  &basis,   &hmat,
+            &resid,    &vec0,     &scal1,   &scal2,   &eig,      &per_probe, &mean,
+            &splitk,   &adj_dh,   &adj_gam, &adj_coef, &adj_lam, &adj_v1,  &adj_v2,
+            &adj_v3,   &adj_work, &adj_sum};
+  }
   // measurement hooks: event pairs per kernel class, reused across reads
   bool profiling = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events[SLQ_KC_COUNT];
@@ -215,6 +223,121 @@ int hessenberg_core(slq_context* ctx, const slq_operator& op, const float* V0, i
   return SLQ_OK;
 }
 
+// Device state of a forward solve that the reverse pass reads.
+struct ForwardState {
+  float *Q = nullptr, *H = nullptr, *R = nullptr, *c = nullptr, *ell = nullptr;
+  int64_t ldc = 0;
+};
+
+// The SLQ forward pass (funm.py:226-241 for all probes at once), basis kept.
+int logdet_forward(slq_context* ctx, const slq_operator& op, const float* Z, int64_t P, int64_t k,
+                   double* per_probe, double* mean, float* alpha, float* beta, ForwardState& st) {
+  const int64_t n = op.n;
+  hipStream_t s = ctx->stream;
+  float *v0, *norm2;
+  double *eig, *pp;
+  SLQ_TRY(ensure_t(ctx, ctx->vec0, size_t(P * n), &v0));
+  // basis columns padded so that consecutive columns do not sit a power of
+  // two apart (all columns of a step are streamed together)
+  st.ldc = basis_stride(P, n);
+  SLQ_TRY(ensure_t(ctx, ctx->basis, size_t(k * st.ldc), &st.Q));
+  SLQ_TRY(ensure_t(ctx, ctx->hmat, size_t(P * k * k), &st.H));
+  SLQ_TRY(ensure_t(ctx, ctx->resid, size_t(P * n), &st.R));
+  SLQ_TRY(ensure_t(ctx, ctx->scal1, size_t(P), &st.c));
+  SLQ_TRY(ensure_t(ctx, ctx->scal2, size_t(P), &st.ell));
+  SLQ_TRY(ensure_t(ctx, ctx->norm2, size_t(P * num_chunks(n)), &norm2));
+  SLQ_TRY(ensure_t(ctx, ctx->eig, size_t(3 * k * P), &eig));
+  if (per_probe) {
+    pp = per_probe;
+  } else {
+    SLQ_TRY(ensure_t(ctx, ctx->per_probe, size_t(P), &pp));
+  }
+  // length = |z|; v0 = z / length (funm.py:228-229)
+  SLQ_HIP(ctx, launch_dots(s, nullptr, 0, Z, P, n, 1, norm2, 1));
+  SLQ_HIP(ctx, launch_normalize(s, Z, v0, P, n, norm2, st.ell, nullptr, 0, nullptr));
+  // tridiag_sym(matvec, v0) (funm.py:237 -> decomp.py:130-143)
+  SLQ_TRY(hessenberg_core(ctx, op, v0, P, k, st.Q, st.ldc, st.H, st.R, st.c));
+  // |z|^2 [log T]_00 (funm.py:239-241) and the probe mean (stochtrace.py:50)
+  {
+    Timed tm(ctx, SLQ_KC_QUADRATURE);
+    SLQ_HIP(ctx, launch_tridiag_quadrature(s, st.H, st.ell, P, k, eig, pp, mean, alpha, beta));
+  }
+  return SLQ_OK;
+}
+
+// The reverse pass: cotangent of the probe mean w.r.t. H (eigh VJP), the
+// Arnoldi adjoint recursion (decomp.py:480-600, reortho="full"), and the
+// operator parameters' cotangents (the matvec VJPs of decomp.py:587-589).
+int logdet_reverse(slq_context* ctx, const slq_operator& op, int64_t P, int64_t k,
+                   const ForwardState& st, double* dparams, float* dmatrix) {
+  const int64_t n = op.n, ldc = st.ldc;
+  hipStream_t s = ctx->stream;
+  const int pstride = int(k);
+  const int64_t nb = num_chunks(n);
+  constexpr int kSumBlocks = 256;
+  float *dH, *Gam, *coef, *Lam, *lam, *lam2, *vm, *part1, *part2, *mvs;
+  double *vwork = nullptr, *dsum;
+  const int64_t mv_floats = matvec_scratch_floats(n, P);
+  SLQ_TRY(ensure_t(ctx, ctx->splitk, size_t(mv_floats), &mvs));
+  SLQ_TRY(ensure_t(ctx, ctx->partial1, size_t(P * nb * pstride), &part1));
+  SLQ_TRY(ensure_t(ctx, ctx->partial2, size_t(P * nb * pstride), &part2));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_dh, size_t(P * k * k), &dH));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_gam, size_t(P * k * k), &Gam));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_coef, size_t(P * (2 * k + 4)), &coef));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_lam, size_t(k * ldc), &Lam));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_v1, size_t(P * n), &lam));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_v2, size_t(P * n), &lam2));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_v3, size_t(P * n), &vm));
+  const int64_t vw = tridiag_vjp_work_doubles(P, k);
+  if (vw > 0) SLQ_TRY(ensure_t(ctx, ctx->adj_work, size_t(vw), &vwork));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_sum, size_t(2 * k * kSumBlocks), &dsum));
+
+  // dH from the eigh VJP of |z|^2 [log T]_00 / P (funm.py:239-241, 331-333)
+  SLQ_HIP(ctx, hipMemsetAsync(dH, 0, size_t(P * k * k) * sizeof(float), s));
+  SLQ_HIP(ctx, hipMemsetAsync(Gam, 0, size_t(P * k * k) * sizeof(float), s));
+  SLQ_HIP(ctx, launch_tridiag_vjp(s, st.H, st.ell, P, k, vwork, dH));
+
+  // lambda = dr + Q gamma, gamma = dH e_K - Q^T dr with dr = 0 (decomp.py:498-499)
+  SLQ_HIP(ctx, launch_adjoint_init_coef(s, dH, P, k, coef));
+  SLQ_HIP(ctx, launch_adjoint_combine(s, st.Q, nullptr, ldc, nullptr, nullptr, nullptr, lam,
+                                      nullptr, P, n, k, int(k), int(k), coef));
+  // reverse scan (decomp.py:538, 575-599)
+  for (int64_t idx = k - 1; idx >= 0; --idx) {
+    // lambda -= Q_m (Q_m^T lambda - dH[:, idx]_m), columns j <= idx + 1 (decomp.py:576-584)
+    const int ncols = int(idx + 2 < k ? idx + 2 : k);
+    SLQ_HIP(ctx, launch_dots(s, st.Q, ldc, lam, P, n, ncols, part1, pstride));
+    SLQ_HIP(ctx, launch_project(s, st.Q, ldc, lam, lam2, P, n, ncols, part1, pstride, nullptr,
+                                nullptr, 0, dH + idx * k, k * k));
+    // A^T lambda (decomp.py:587-588; A symmetric)
+    SLQ_HIP(ctx, launch_matvec(s, op, lam2, P, vm, mvs, mv_floats));
+    // Q^T (A^T lambda) for Gamma[idx, :] (decomp.py:592-593)
+    SLQ_HIP(ctx, launch_dots(s, st.Q, ldc, vm, P, n, int(idx + 1), part2, pstride));
+    SLQ_HIP(ctx, launch_adjoint_coef(s, st.H, dH, Gam, part2, pstride, P, n, k, idx, coef));
+    // Lambda[:, idx] = lambda and the next lambda (decomp.py:596-599)
+    SLQ_HIP(ctx, launch_adjoint_combine(s, st.Q, Lam, ldc, st.R, lam2, vm, lam, Lam + idx * ldc,
+                                        P, n, k, int(k), int(idx + 1), coef));
+  }
+
+  // parameter cotangents: sum over steps and probes of the matvec VJPs
+  // lambda^T (dA) q_idx with lambda = Lambda[:, idx] (decomp.py:587-589)
+  if (op.kind == SLQ_OP_RBF) {
+    // d/d noise: lambda^T q; d/d lengthscale: lambda^T (|x_i - x_m|^2 K) q / l^3
+    for (int64_t idx = 0; idx < k; ++idx) {
+      const float* q = st.Q + idx * ldc;
+      const float* l = Lam + idx * ldc;
+      SLQ_HIP(ctx, launch_rbf_deriv_matvec(s, op, q, P, vm));
+      SLQ_HIP(ctx, launch_dot_sum(s, l, vm, P * n, dsum + idx * kSumBlocks, kSumBlocks));
+      SLQ_HIP(ctx, launch_dot_sum(s, l, q, P * n, dsum + (k + idx) * kSumBlocks, kSumBlocks));
+    }
+    const double ls = op.lengthscale;
+    SLQ_HIP(ctx, launch_sum_fixed(s, dsum, k * kSumBlocks, 2, 1.0 / (ls * ls * ls), 1.0, dparams));
+  } else {
+    (void)dmatrix;
+    return fail(ctx, SLQ_EUNSUPPORTED, "the stored-matrix cotangent is not available");
+  }
+  return SLQ_OK;
+}
+
 }  // namespace
 }  // namespace slq
 
@@ -247,11 +370,7 @@ int slq_context_destroy(slq_context* ctx) {
       (void)hipEventDestroy(ev.first);
       (void)hipEventDestroy(ev.second);
     }
-  slq_context::Buf* bufs[] = {&ctx->partial1, &ctx->partial2, &ctx->norm2,  &ctx->work_w,
-                              &ctx->work_w2,  &ctx->basis,    &ctx->hmat,   &ctx->resid,
-                              &ctx->vec0,     &ctx->scal1,    &ctx->scal2,  &ctx->eig,
-                              &ctx->per_probe, &ctx->mean, &ctx->splitk};
-  for (auto* b : bufs)
+  for (auto* b : ctx->all_bufs())
     if (b->ptr) (void)hipFree(b->ptr);
   delete ctx;
   return SLQ_OK;
@@ -267,12 +386,8 @@ const char* slq_last_error(const slq_context* ctx) { return ctx ? ctx->error.c_s
 
 int64_t slq_context_scratch_bytes(const slq_context* ctx) {
   if (!ctx) return 0;
-  const slq_context::Buf* bufs[] = {&ctx->partial1, &ctx->partial2, &ctx->norm2,  &ctx->work_w,
-                                    &ctx->work_w2,  &ctx->basis,    &ctx->hmat,   &ctx->resid,
-                                    &ctx->vec0,     &ctx->scal1,    &ctx->scal2,  &ctx->eig,
-                                    &ctx->per_probe, &ctx->mean, &ctx->splitk};
   int64_t total = 0;
-  for (auto* b : bufs) total += int64_t(b->bytes);
+  for (auto* b : const_cast<slq_context*>(ctx)->all_bufs()) total += int64_t(b->bytes);
   return total;
 }
 
@@ -322,36 +437,30 @@ int slq_logdet(slq_context* ctx, const slq_operator* op, const float* Z, int64_t
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of probes P must be positive");
   if (!Z) return fail(ctx, SLQ_EINVAL, "probe pointer is NULL");
   SLQ_TRY(set_device(ctx));
-  hipStream_t s = ctx->stream;
-  float *v0, *Q, *H, *R, *c, *ell, *norm2;
-  double *eig, *pp;
-  SLQ_TRY(ensure_t(ctx, ctx->vec0, size_t(P * n), &v0));
-  // basis columns padded so that consecutive columns do not sit a power of
-  // two apart (all columns of a step are streamed together)
-  const int64_t ldc = basis_stride(P, n);
-  SLQ_TRY(ensure_t(ctx, ctx->basis, size_t(k * ldc), &Q));
-  SLQ_TRY(ensure_t(ctx, ctx->hmat, size_t(P * k * k), &H));
-  SLQ_TRY(ensure_t(ctx, ctx->resid, size_t(P * n), &R));
-  SLQ_TRY(ensure_t(ctx, ctx->scal1, size_t(P), &c));
-  SLQ_TRY(ensure_t(ctx, ctx->scal2, size_t(P), &ell));
-  SLQ_TRY(ensure_t(ctx, ctx->norm2, size_t(P * num_chunks(n)), &norm2));
-  SLQ_TRY(ensure_t(ctx, ctx->eig, size_t(3 * k * P), &eig));
-  if (per_probe) {
-    pp = per_probe;
-  } else {
-    SLQ_TRY(ensure_t(ctx, ctx->per_probe, size_t(P), &pp));
-  }
-  // length = |z|; v0 = z / length (funm.py:228-229)
-  SLQ_HIP(ctx, launch_dots(s, nullptr, 0, Z, P, n, 1, norm2, 1));
-  SLQ_HIP(ctx, launch_normalize(s, Z, v0, P, n, norm2, ell, nullptr, 0, nullptr));
-  // tridiag_sym(matvec, v0) (funm.py:237 -> decomp.py:130-143)
-  SLQ_TRY(hessenberg_core(ctx, *op, v0, P, k, Q, ldc, H, R, c));
-  // |z|^2 [log T]_00 (funm.py:239-241) and the probe mean (stochtrace.py:50)
-  {
-    Timed tm(ctx, SLQ_KC_QUADRATURE);
-    SLQ_HIP(ctx, launch_tridiag_quadrature(s, H, ell, P, k, eig, pp, mean, alpha, beta));
-  }
-  return SLQ_OK;
+  ForwardState st;
+  return logdet_forward(ctx, *op, Z, P, k, per_probe, mean, alpha, beta, st);
+}
+
+int slq_logdet_vjp(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P, int64_t k,
+                   double* per_probe, double* mean, double* dparams, float* dmatrix) {
+  if (!ctx) return SLQ_EINVAL;
+  SLQ_TRY(check_operator(ctx, op));
+  const int64_t n = op->n;
+  if (k == 0)
+    return fail(ctx, SLQ_EADJOINT_K0,
+                "Custom Hessenberg-adjoints are not implemented for num_matvecs = 0.");
+  if (k < 0 || k > n) return fail(ctx, SLQ_EINVAL_K, exceeds_msg(k, n));
+  if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of probes P must be positive");
+  if (!Z) return fail(ctx, SLQ_EINVAL, "probe pointer is NULL");
+  if (op->kind == SLQ_OP_RBF && !dparams)
+    return fail(ctx, SLQ_EINVAL, "dparams (2 doubles) is NULL");
+  if (op->kind == SLQ_OP_DENSE && !dmatrix)
+    return fail(ctx, SLQ_EINVAL, "dmatrix (n x n floats) is NULL");
+  SLQ_TRY(set_device(ctx));
+  ForwardState st;
+  SLQ_TRY(logdet_forward(ctx, *op, Z, P, k, per_probe, mean, nullptr, nullptr, st));
+  Timed tm(ctx, SLQ_KC_ADJOINT);
+  return logdet_reverse(ctx, *op, P, k, st, dparams, dmatrix);
 }
 
 }  // extern "C"

@@ -35,12 +35,14 @@ hipError_t launch_rademacher(hipStream_t s, uint32_t k0, uint32_t k1, int64_t co
 // Q may be null with ncols = 1 to mean "sum of squares of x" (partial[p][b][0]).
 hipError_t launch_dots(hipStream_t s, const float* Q, int64_t ldc, const float* x, int64_t P,
                        int64_t n, int ncols, float* partial, int pstride);
-// y = x - sum_c Q[c] h[c] with h[c] = sum_b hpart[p][b][c]; optional
-// norm2part[p][b] = sum over chunk of y^2.  hout (if non-null) receives h for
-// probe p at hout + p * hout_stride (written by chunk 0 only).
+// y = x - sum_c Q[c] h[c] with h[c] = sum_b hpart[p][b][c] - hsub[p][c];
+// optional norm2part[p][b] = sum over chunk of y^2.  hout (if non-null)
+// receives sum_b hpart[p][b][c] at hout + p * hout_stride (written by chunk 0
+// only); hsub (if non-null) is read at hsub + p * hsub_stride.
 hipError_t launch_project(hipStream_t s, const float* Q, int64_t ldc, const float* x, float* y,
                           int64_t P, int64_t n, int ncols, const float* hpart, int pstride,
-                          float* norm2part, float* hout, int64_t hout_stride);
+                          float* norm2part, float* hout, int64_t hout_stride,
+                          const float* hsub = nullptr, int64_t hsub_stride = 0);
 // Fused: y = x - Q h1 (h1 reduced from h1part), then h2part = Q^T y per chunk.
 hipError_t launch_project_dots(hipStream_t s, const float* Q, int64_t ldc, const float* x,
                                float* y, int64_t P, int64_t n, int ncols, const float* h1part,
@@ -61,6 +63,11 @@ int64_t matvec_scratch_floats(int64_t n, int64_t P);
 hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, int64_t P,
                          float* W, float* scratch, int64_t scratch_floats);
 
+// W = (|x_i - x_m|^2 K_im) V in FP32: l^3 dK/dl applied to P vectors (no
+// nugget, no split-K).
+hipError_t launch_rbf_deriv_matvec(hipStream_t s, const slq_operator& op, const float* V,
+                                   int64_t P, float* W);
+
 // ---- tridiagonal eigen + quadrature (k_tridiag.hip) ------------------------
 // For each probe p: T = (H + H^T)/2 restricted to its tridiagonal; returns
 // per_probe[p] = scale[p]^2 * sum_j U_0j^2 log(lambda_j) in FP64, optional
@@ -68,5 +75,33 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
 hipError_t launch_tridiag_quadrature(hipStream_t s, const float* H, const float* scale,
                                      int64_t P, int64_t k, double* work, double* per_probe,
                                      double* mean, float* alpha, float* beta);
+
+// ---- reverse pass (k_adjoint.hip) ------------------------------------------
+// dH (P x k x k, column-major like H, written on the tridiagonal only; the
+// caller zeroes it) = cotangent of mean_p |z_p|^2 [log T_p]_00 w.r.t. H, with
+// len[p] = |z_p|.  work: tridiag_vjp_work_doubles(P, k) doubles (may be 0).
+hipError_t launch_tridiag_vjp(hipStream_t s, const float* H, const float* len, int64_t P, int64_t k,
+                              double* work, float* dH);
+int64_t tridiag_vjp_work_doubles(int64_t P, int64_t k);
+// One backward step idx: Gamma row idx and the combination coefficients
+// (coef, per probe 2 k + 4 floats) from the chunk partials of Q^T (A^T lambda).
+hipError_t launch_adjoint_coef(hipStream_t s, const float* H, const float* dH, float* Gam,
+                               const float* part, int pstride, int64_t P, int64_t n, int64_t k,
+                               int64_t idx, float* coef);
+// Coefficients for lambda = Q dH[:, k - 1] (the start of the recursion).
+hipError_t launch_adjoint_init_coef(hipStream_t s, const float* dH, int64_t P, int64_t k,
+                                    float* coef);
+// y = scale (cr r + cl x + cv v + sum_{j < ncq} gq_j Q_j + sum_{jl0 <= j < k} gl_j Lam_j), per
+// probe (coef layout above); r, x, v may be null; save (if given) receives x.
+hipError_t launch_adjoint_combine(hipStream_t s, const float* Q, const float* Lam, int64_t ldc,
+                                  const float* r, const float* x, const float* v, float* y,
+                                  float* save, int64_t P, int64_t n, int64_t k, int ncq, int jl0,
+                                  const float* coef);
+// partial[b] = sum over a grid-stride share of a[i] * b[i] (FP64), b < blocks.
+hipError_t launch_dot_sum(hipStream_t s, const float* a, const float* b, int64_t count,
+                          double* partial, int blocks);
+// out[o] = f_o * sum_i partial[o * count + i] (fixed order), nout <= 2.
+hipError_t launch_sum_fixed(hipStream_t s, const double* partial, int64_t count, int nout,
+                            double f0, double f1, double* out);
 
 }  // namespace slq

@@ -82,6 +82,78 @@ def logdet_per_probe(matvec, Z, num_matvecs, params, want_coeffs=False):
     return per_probe, mean
 
 
+def logdet_value_and_grad(matvec, Z, num_matvecs, params):
+    """Run ``slq_logdet_vjp`` on probes ``Z`` (P, n).
+
+    Returns ``(per_probe (P,), mean (), grads)`` where ``grads`` holds one
+    cotangent of the probe mean per parameter in ``params``: for
+    :class:`operators.RBF` the float64 scalars ``d mean / d lengthscale`` and
+    ``d mean / d noise``.  This is reverse mode through the estimator
+    (decomp.py:398-422, 480-600; funm.py:331-333; stochtrace.py:50).
+    """
+    op = operators.resolve(matvec)
+    shape = tuple(Z.shape)
+    P, n = (1, shape[0]) if len(shape) == 1 else shape
+    k = int(num_matvecs)
+    device = _runtime.target_device(Z, *params)
+    Z = _runtime.as_f32(Z, device).reshape(P, n)
+    desc, keep = op.descriptor(n, params, device)
+    per_probe = torch.empty((P,), dtype=torch.float64, device=device)
+    mean = torch.empty((), dtype=torch.float64, device=device)
+    if isinstance(op, operators.RBF):
+        dparams = torch.empty((2,), dtype=torch.float64, device=device)
+        dmatrix = None
+    else:
+        dparams = None
+        dmatrix = torch.empty((n, n), dtype=torch.float32, device=device)
+    ctx = _runtime.context(device)
+    ctx.call(
+        "slq_logdet_vjp",
+        ctypes.byref(desc),
+        _runtime.ptr(Z),
+        P,
+        k,
+        _runtime.ptr(per_probe),
+        _runtime.ptr(mean),
+        _runtime.ptr(dparams),
+        _runtime.ptr(dmatrix),
+    )
+    del keep
+    grads = (dparams[0], dparams[1]) if dparams is not None else (dmatrix,)
+    return per_probe, mean, grads
+
+
+def _needs_grad(params):
+    return torch.is_grad_enabled() and any(
+        isinstance(p, torch.Tensor) and p.requires_grad for p in params
+    )
+
+
+class _LogdetMean(torch.autograd.Function):
+    """The probe mean as a differentiable function of the operator parameters.
+
+    The forward pass runs ``slq_logdet_vjp`` (value and cotangents in one
+    call, as a custom VJP stores its residuals); the backward pass scales the
+    stored cotangents by the incoming gradient.
+    """
+
+    @staticmethod
+    def forward(ctx, matvec, samples, num_matvecs, *params):
+        _, mean, grads = logdet_value_and_grad(matvec, samples, num_matvecs, params)
+        shaped = []
+        for p, g in zip(params, grads):
+            if isinstance(p, torch.Tensor):
+                g = g.to(dtype=p.dtype, device=p.device).reshape(p.shape)
+            shaped.append(g)
+        ctx.save_for_backward(*shaped)
+        return mean
+
+    @staticmethod
+    def backward(ctx, gmean):
+        grads = ctx.saved_tensors
+        return (None, None, None, *[(gmean.to(g.dtype) * g) for g in grads])
+
+
 class _QuadFormSym:
     """``monte_carlo_funm_sym(...).quadform`` (funm.py:226-241)."""
 
@@ -101,12 +173,22 @@ class _QuadFormSym:
         return per_probe
 
     def estimate(self, matvec, samples, *parameters):
-        """Mean over probes, reduced on the device."""
+        """Mean over probes, reduced on the device.
+
+        Differentiable in the operator parameters: with any parameter a
+        tensor that requires grad, the result carries a backward that uses
+        the GPU Arnoldi adjoint (torch's counterpart of ``jax.grad`` through
+        the estimator).
+        """
+        if _needs_grad(parameters):
+            return _LogdetMean.apply(matvec, samples, self.num_matvecs, *parameters)
         _, mean = logdet_per_probe(matvec, samples, self.num_matvecs, parameters)
         return mean
 
     def __call__(self, matvec, v0, *parameters):
         flat, _ = _tree.ravel(v0, decomp._host_or_gpu(v0, *parameters))
+        if _needs_grad(parameters):
+            return _LogdetMean.apply(matvec, flat[None, :], self.num_matvecs, *parameters)
         per_probe, _ = logdet_per_probe(matvec, flat[None, :], self.num_matvecs, parameters)
         return per_probe[0]
 

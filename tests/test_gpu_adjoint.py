@@ -1,0 +1,141 @@
+"""GPU reverse mode of the SLQ log-determinant (run on an MI355X).
+
+The HIP adjoint runs in FP32 (the FP64 tridiagonal eigen-cotangent aside), so
+its cotangents are compared with the oracle's FP64 restatement of
+decomp.py:480-600 + the eigh VJP (itself pinned by central differences and a
+closed form in tests/test_oracle_slq.py), with the tolerance stated per test,
+and with central differences of the GPU forward pass.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from matfree_amd import decomp, funm, operators, problems, prng, stochtrace
+from oracle import prng as oprng
+from oracle import slq as oslq
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("n,P,k", [(256, 6, 12), (301, 5, 17)])
+def test_rbf_cotangents_match_oracle(cuda, n, P, k):
+    X = problems.rbf_points(n, seed=4)
+    theta = (0.5, 0.1)
+    Z = oprng.rademacher(oprng.prng_key(7), (P, n), np.float32)
+    per, mean, grads = funm.logdet_value_and_grad(
+        operators.RBF(X), torch.as_tensor(Z, device=cuda), k, theta
+    )
+    est64, g64 = oslq.slq_logdet_vjp(oslq.RBFOperator(X, *theta, np.float64), k, Z)
+    g = np.array([float(grads[0]), float(grads[1])])
+    # FP32 forward + FP32 adjoint against the FP64 restatement: 2e-4 normwise
+    assert rel(g, g64) < 2e-4, (g, g64)
+    assert abs(float(mean) - est64) <= 2e-5 * abs(est64)
+    vals, _ = funm.logdet_per_probe(operators.RBF(X), torch.as_tensor(Z, device=cuda), k, theta)
+    assert torch.equal(vals, per)
+
+
+def test_rbf_cotangents_match_oracle_at_depth_32(cuda):
+    # 32 probes, depth 32, (l, s2) = (0.5, 1e-2) as in the gradient
+    # configuration, on n = 2048 points where the FP64 restatement is quick
+    n, P, k = 2048, 32, 32
+    X = problems.rbf_points(n, seed=0)
+    theta = (0.5, 1e-2)
+    Z = oprng.rademacher(oprng.prng_key(0), (P, n), np.float32)
+    _, mean, grads = funm.logdet_value_and_grad(
+        operators.RBF(X), torch.as_tensor(Z, device=cuda), k, theta
+    )
+    est64, g64 = oslq.slq_logdet_vjp(oslq.RBFOperator(X, *theta, np.float64), k, Z)
+    g = np.array([float(grads[0]), float(grads[1])])
+    assert rel(g, g64) < 1e-3, (g, g64)
+    assert abs(float(mean) - est64) <= 1e-4 * abs(est64)
+
+
+def test_rbf_cotangents_match_central_differences(cuda):
+    # the gradient configuration: n = 8192 points in the unit cube, 32 probes,
+    # depth 32, (l, s2) = (0.5, 1e-2), against fourth-order central
+    # differences of the FP32 GPU forward pass.  The forward value carries
+    # FP32 Lanczos rounding (~1e-6 of |logdet| ~ 4e4), so the steps are
+    # h = 1e-2 for l and 1e-4 for s2 and the tolerances 2% and 0.5%.
+    n, P, k = 8192, 32, 32
+    X = problems.rbf_points(n, seed=0)
+    op = operators.RBF(X)
+    Z = stochtrace.sampler_signs(np.zeros(n, np.float32), num=P).flat(prng.PRNGKey(0), cuda)
+    theta = np.array([0.5, 1e-2])
+    _, mean, grads = funm.logdet_value_and_grad(op, Z, k, tuple(theta))
+    g = np.array([float(grads[0]), float(grads[1])])
+
+    def f(t):
+        _, m = funm.logdet_per_probe(op, Z, k, (float(t[0]), float(t[1])))
+        return float(m)
+
+    fd = []
+    for e, h in zip(np.eye(2), (1e-2, 1e-4)):
+        d = h * e
+        fd.append((-f(theta + 2 * d) + 8 * f(theta + d) - 8 * f(theta - d) + f(theta - 2 * d))
+                  / (12 * h))
+    assert abs(g[0] - fd[0]) <= 2e-2 * abs(fd[0]), (g, fd)
+    assert abs(g[1] - fd[1]) <= 5e-3 * abs(fd[1]), (g, fd)
+
+
+def test_full_depth_matches_dense_closed_form(cuda):
+    # at k = n the integrand is x^T log(A(theta)) x exactly (as in
+    # tests/test_oracle_slq.py); its derivative in closed form via the
+    # divided differences of log at the eigenvalues of A
+    rng = np.random.default_rng(12)
+    n = 10
+    X = rng.uniform(size=(n, 2)).astype(np.float32)
+    theta = (0.7, 0.3)
+    z = oprng.rademacher(oprng.prng_key(5), (1, n), np.float32)
+    _, _, grads = funm.logdet_value_and_grad(operators.RBF(X), torch.as_tensor(z, device=cuda),
+                                             n, theta)
+    op = oslq.RBFOperator(X.astype(np.float64), *theta, np.float64)
+    A = op.dense()
+    w, U = np.linalg.eigh(A)
+    L = np.empty((n, n))
+    for i in range(n):
+        for j in range(n):
+            L[i, j] = 1 / w[i] if i == j else (np.log(w[i]) - np.log(w[j])) / (w[i] - w[j])
+    y = U.T @ z[0].astype(np.float64)
+    d2 = np.sum((X[:, None].astype(np.float64) - X[None]) ** 2, -1)
+    dA_dl = np.exp(-d2 / (2 * theta[0] ** 2)) * d2 / theta[0] ** 3
+    for dA, gi in zip([dA_dl, np.eye(n)], grads):
+        expected = np.sum(np.outer(y, y) * L * (U.T @ dA @ U))
+        assert np.isclose(float(gi), expected, rtol=2e-4), (float(gi), expected)
+
+
+def test_estimator_autograd(cuda):
+    # the drop-in for jax.grad over the estimator: torch parameters that
+    # require grad make estimate(...) differentiable
+    n, P, k = 512, 8, 16
+    X = problems.rbf_points(n, seed=2)
+    op = operators.RBF(X)
+    tridiag = decomp.tridiag_sym(k)
+    integrand = funm.monte_carlo_funm_sym_logdet(tridiag)
+    sampler = stochtrace.sampler_signs(np.zeros(n, np.float32), num=P)
+    estimate = stochtrace.estimator_monte_carlo(integrand, sampler)
+    ls = torch.tensor(0.4, dtype=torch.float64, device=cuda, requires_grad=True)
+    s2 = torch.tensor(0.05, dtype=torch.float64, device=cuda, requires_grad=True)
+    val = estimate(op, prng.PRNGKey(3), ls, s2)
+    val.backward()
+    Z = sampler.flat(prng.PRNGKey(3), cuda)
+    _, mean, grads = funm.logdet_value_and_grad(op, Z, k, (0.4, 0.05))
+    assert float(val.detach()) == float(mean)
+    assert float(ls.grad) == float(grads[0])
+    assert float(s2.grad) == float(grads[1])
+    # without requires_grad the estimate is a plain tensor
+    plain = estimate(op, prng.PRNGKey(3), 0.4, 0.05)
+    assert not plain.requires_grad and float(plain) == float(mean)
+
+
+def test_adjoint_k0_raises(cuda):
+    X = problems.rbf_points(16, seed=1)
+    Z = torch.ones((2, 16), device=cuda)
+    with pytest.raises(ValueError, match="= 0"):
+        funm.logdet_value_and_grad(operators.RBF(X), Z, 0, (0.5, 0.1))
