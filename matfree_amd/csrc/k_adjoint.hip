@@ -315,7 +315,106 @@ __global__ void k_adjoint_init_coef(const float* __restrict__ dH, int64_t k,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Stored-matrix cotangent dA = sum_c L_c R_c^T over the K = k P columns
+// c = (idx, p) of two bases in the Krylov layout (column c at
+// base + idx * ldc + p * n): the matvec VJPs lambda q^T of decomp.py:587-589
+// summed over steps and probes.  FP32 MFMA (32x32x2): a workgroup owns a
+// 128 x 128 output tile, each wave 32 rows x 128 columns (4 accumulators);
+// both operands pass through LDS in stages of 8 columns (one float4 of each
+// per thread), double-buffered.  In an MFMA the lane half h carries the
+// contraction column 2 q + h of the stage, lane r the row / column r.
+// ---------------------------------------------------------------------------
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+constexpr int kOT = 128;         // output tile edge
+constexpr int kOK = 8;           // columns per stage
+constexpr int kOLd = kOT + 4;    // padded LDS row
+
+__global__ void __launch_bounds__(256)
+    k_outer_sum(const float* __restrict__ L, const float* __restrict__ R, int64_t ldc, int64_t P,
+                int64_t n, int64_t K, float* __restrict__ out) {
+  __shared__ __attribute__((aligned(16))) float Ls[2][kOK * kOLd];
+  __shared__ __attribute__((aligned(16))) float Rs[2][kOK * kOLd];
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6, r = lane & 31, h = lane >> 5;
+  const int64_t i0 = int64_t(blockIdx.y) * kOT, j0 = int64_t(blockIdx.x) * kOT;
+  // staging: thread t moves float4 (t & 31) of stage column t >> 5
+  const int sc = t >> 5, s4 = (t & 31) * 4;
+  auto col_ptr = [&](const float* base, int64_t c) {
+    const int64_t idx = c / P, p = c - idx * P;
+    return base + idx * ldc + p * n;
+  };
+  auto load = [&](const float* base, int64_t c0, int64_t e0) {
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int64_t c = c0 + sc, e = e0 + s4;
+    if (c < K) {
+      const float* src = col_ptr(base, c) + e;
+      if (e + 3 < n && (n & 3) == 0) {
+        v = *reinterpret_cast<const float4*>(src);
+      } else {
+        if (e + 0 < n) v.x = src[0];
+        if (e + 1 < n) v.y = src[1];
+        if (e + 2 < n) v.z = src[2];
+        if (e + 3 < n) v.w = src[3];
+      }
+    }
+    return v;
+  };
+  floatx16 acc[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[c][e] = 0.f;
+
+  const int64_t nst = (K + kOK - 1) / kOK;
+  float4 lv = load(L, 0, i0), rv = load(R, 0, j0);
+  *reinterpret_cast<float4*>(&Ls[0][sc * kOLd + s4]) = lv;
+  *reinterpret_cast<float4*>(&Rs[0][sc * kOLd + s4]) = rv;
+  __syncthreads();
+  for (int64_t st = 0; st < nst; ++st) {
+    const int b = int(st & 1);
+    const bool more = st + 1 < nst;
+    if (more) {
+      lv = load(L, (st + 1) * kOK, i0);
+      rv = load(R, (st + 1) * kOK, j0);
+    }
+#pragma unroll
+    for (int q = 0; q < kOK / 2; ++q) {
+      const float a = Ls[b][(2 * q + h) * kOLd + 32 * wave + r];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const float bb = Rs[b][(2 * q + h) * kOLd + 32 * c + r];
+        acc[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bb, acc[c], 0, 0, 0);
+      }
+    }
+    if (more) {
+      *reinterpret_cast<float4*>(&Ls[b ^ 1][sc * kOLd + s4]) = lv;
+      *reinterpret_cast<float4*>(&Rs[b ^ 1][sc * kOLd + s4]) = rv;
+    }
+    __syncthreads();
+  }
+  // lane l of accumulator c holds column j0 + 32 c + (l & 31) at rows
+  // i0 + 32 wave + 8 (s / 4) + 4 h + (s % 4)
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int64_t j = j0 + 32 * c + r;
+    if (j >= n) continue;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const int64_t i = i0 + 32 * wave + 8 * (s / 4) + 4 * h + (s % 4);
+      if (i < n) out[i * n + j] = acc[c][s];
+    }
+  }
+}
+
 }  // namespace
+
+hipError_t launch_outer_sum(hipStream_t s, const float* L, const float* R, int64_t ldc, int64_t P,
+                            int64_t n, int64_t K, float* out) {
+  const unsigned tiles = unsigned((n + kOT - 1) / kOT);
+  hipLaunchKernelGGL(k_outer_sum, dim3(tiles, tiles), dim3(256), 0, s, L, R, ldc, P, n, K, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_tridiag_vjp(hipStream_t s, const float* H, const float* len, int64_t P, int64_t k,
                               double* work, float* dH) {

@@ -332,8 +332,8 @@ int logdet_reverse(slq_context* ctx, const slq_operator& op, int64_t P, int64_t 
     const double ls = op.lengthscale;
     SLQ_HIP(ctx, launch_sum_fixed(s, dsum, k * kSumBlocks, 2, 1.0 / (ls * ls * ls), 1.0, dparams));
   } else {
-    (void)dmatrix;
-    return fail(ctx, SLQ_EUNSUPPORTED, "the stored-matrix cotangent is not available");
+    // d/dA = sum_idx sum_p lambda q^T (the VJP of A -> A q)
+    SLQ_HIP(ctx, launch_outer_sum(s, Lam, st.Q, ldc, P, n, k * P, dmatrix));
   }
   return SLQ_OK;
 }

@@ -97,6 +97,10 @@ hipError_t launch_adjoint_combine(hipStream_t s, const float* Q, const float* La
                                   const float* r, const float* x, const float* v, float* y,
                                   float* save, int64_t P, int64_t n, int64_t k, int ncq, int jl0,
                                   const float* coef);
+// out (n x n, row-major) = sum_{c < K} L_c R_c^T over columns c = (idx, p) at
+// base + idx * ldc + p * n (K = k P), FP32 MFMA.
+hipError_t launch_outer_sum(hipStream_t s, const float* L, const float* R, int64_t ldc, int64_t P,
+                            int64_t n, int64_t K, float* out);
 // partial[b] = sum over a grid-stride share of a[i] * b[i] (FP64), b < blocks.
 hipError_t launch_dot_sum(hipStream_t s, const float* a, const float* b, int64_t count,
                           double* partial, int blocks);

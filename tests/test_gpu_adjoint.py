@@ -139,3 +139,36 @@ def test_adjoint_k0_raises(cuda):
     Z = torch.ones((2, 16), device=cuda)
     with pytest.raises(ValueError, match="= 0"):
         funm.logdet_value_and_grad(operators.RBF(X), Z, 0, (0.5, 0.1))
+
+
+def test_dense_cotangent_matches_oracle(cuda):
+    # matvec(v, A) = A v with A the parameter (tests/test_decomp/
+    # test_tridiag_sym_adjoint.py differentiates w.r.t. the matrix too)
+    n, P, k = 200, 4, 12
+    A = problems.dense_spd(n, seed=5, shift=1.0)
+    Z = oprng.rademacher(oprng.prng_key(2), (P, n), np.float32)
+    _, mean, (G,) = funm.logdet_value_and_grad(
+        operators.dense_matvec, torch.as_tensor(Z, device=cuda), k, (torch.as_tensor(A, device=cuda),)
+    )
+    est64, G64 = oslq.slq_logdet_vjp(oslq.DenseOperator(A, np.float64), k, Z)
+    assert rel(G.cpu().numpy(), G64) < 2e-4
+    assert abs(float(mean) - est64) <= 2e-5 * abs(est64)
+
+
+def test_dense_cotangent_autograd_direction(cuda):
+    # <dA, D> against a central difference of the GPU forward along a
+    # symmetric direction D (FP32 forward: h = 1e-3, tolerance 1e-2)
+    n, P, k = 300, 8, 20
+    A = torch.as_tensor(problems.dense_spd(n, seed=6, shift=1.0), device=cuda)
+    D = torch.randn((n, n), device=cuda, generator=torch.Generator(device=cuda).manual_seed(0))
+    D = (D + D.T) / n
+    integrand = funm.monte_carlo_funm_sym_logdet(decomp.tridiag_sym(k))
+    sampler = stochtrace.sampler_signs(np.zeros(n, np.float32), num=P)
+    estimate = stochtrace.estimator_monte_carlo(integrand, sampler)
+    Ap = A.clone().requires_grad_(True)
+    estimate(operators.dense_matvec, prng.PRNGKey(4), Ap).backward()
+    h = 1e-3
+    fd = (float(estimate(operators.dense_matvec, prng.PRNGKey(4), A + h * D))
+          - float(estimate(operators.dense_matvec, prng.PRNGKey(4), A - h * D))) / (2 * h)
+    got = float(torch.sum(Ap.grad * D))
+    assert abs(got - fd) <= 1e-2 * abs(fd), (got, fd)
