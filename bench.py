@@ -37,8 +37,16 @@ PEAK = {  # MI355X published dense peaks
 WORKLOADS = {
     "rbf": dict(n=32768, probes=128, depth=64, lengthscale=0.5, noise=1e-2),
     "dense": dict(n=16384, probes=256, depth=64, shift=0.1),
+    "rbf-grad": dict(n=8192, probes=32, depth=32, lengthscale=0.5, noise=1e-2),
 }
-DEFAULT_PRECISION = {"rbf": "bf16x2", "dense": "fp32"}
+DEFAULT_PRECISION = {"rbf": "bf16x2", "dense": "fp32", "rbf-grad": "fp32"}
+# Serving mix: n log-uniform in [2048, 32768], probes in {16, 32, 64, 128},
+# depth uniform in [16, 64], RBF (bf16x2) or stored dense (fp32, n capped at
+# 16384 to bound the per-request matrix at 1 GB).
+SERVE_N = (2048, 32768)
+SERVE_DENSE_MAX_N = 16384
+SERVE_PROBES = (16, 32, 64, 128)
+SERVE_DEPTH = (16, 64)
 
 
 def parse():
@@ -51,6 +59,10 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-reference", action="store_true", help="skip the FP64 Cholesky check")
+    ap.add_argument("--serve-requests", type=int, default=48,
+                    help="requests of the serving measurement (0 skips it)")
+    ap.add_argument("--serve-load", type=float, default=0.5,
+                    help="offered load of the open-loop (Poisson) serving replay")
     return ap.parse_args()
 
 
@@ -167,6 +179,127 @@ def measured_traffic(kernel):
         return None
 
 
+def serve(args, device):
+    """Per-request latency of a stream of independent SLQ log-determinant solves.
+
+    Each request draws its operator (RBF on fresh points, or a stored SPD
+    matrix), dimension, probe count and depth from the serving mix; its inputs
+    are made resident in HBM first, then the request (probe generation, solve,
+    result ready on the device) is timed on its own.  Closed-loop percentiles
+    are the service times; the open-loop figures replay Poisson arrivals at
+    the given offered load through one FIFO server with those service times.
+    """
+    import numpy as np
+    import torch
+
+    from matfree_amd import funm, operators, problems, prng, stochtrace
+
+    rng = np.random.default_rng(args.seed + 1)
+    lat = []
+    meta = []
+    for r in range(args.serve_requests):
+        n = int(round(np.exp(rng.uniform(np.log(SERVE_N[0]), np.log(SERVE_N[1])))))
+        P = int(rng.choice(SERVE_PROBES))
+        k = int(rng.integers(SERVE_DEPTH[0], SERVE_DEPTH[1] + 1))
+        kind = "rbf" if (rng.random() < 0.5 or n > SERVE_DENSE_MAX_N) else "dense"
+        if kind == "rbf":
+            X = torch.as_tensor(problems.rbf_points(n, seed=1000 + r), device=device)
+            op, params = operators.RBF(X, precision="bf16x2"), (0.5, 1e-2)
+        else:
+            A = problems.dense_spd_device(n, seed=1000 + r, shift=0.1, device=device)
+            op, params = operators.DenseMatvec(precision="fp32"), (A,)
+        sampler = stochtrace.sampler_signs(np.zeros(n, np.float32), num=P)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        Z = sampler.flat(prng.PRNGKey(r), device)
+        _, mean = funm.logdet_per_probe(op, Z, k, params)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t0) * 1e3)
+        meta.append((kind, n, P, k))
+        del op, params, Z
+    lat = np.array(lat)
+    # open loop: Poisson arrivals at load rho of the mean service rate, FIFO
+    rho = args.serve_load
+    gaps = rng.exponential(lat.mean() / rho, size=len(lat))
+    arrive = np.cumsum(gaps)
+    finish = np.empty_like(arrive)
+    busy = 0.0
+    for i in range(len(lat)):
+        start = max(arrive[i], busy)
+        finish[i] = busy = start + lat[i]
+    resp = finish - arrive
+    torch.cuda.empty_cache()
+    return {
+        "requests": len(lat),
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "mean_ms": float(lat.mean()),
+        "max_ms": float(lat.max()),
+        "open_loop_load": rho,
+        "open_loop_p50_ms": float(np.percentile(resp, 50)),
+        "open_loop_p99_ms": float(np.percentile(resp, 99)),
+        "mix": (f"n log-uniform [{SERVE_N[0]}, {SERVE_N[1]}], probes {list(SERVE_PROBES)}, "
+                f"depth U[{SERVE_DEPTH[0]}, {SERVE_DEPTH[1]}], RBF bf16x2 or dense fp32 "
+                f"(dense n <= {SERVE_DENSE_MAX_N}); {sum(m[0] == 'rbf' for m in meta)} RBF"),
+    }
+
+
+def grad_step(args, device):
+    """Value and gradient (d/d lengthscale, d/d noise) of the RBF estimate."""
+    import numpy as np
+    import torch
+
+    from matfree_amd import _runtime, funm, operators, problems, prng, stochtrace
+
+    w = WORKLOADS["rbf-grad"]
+    n, P, k = w["n"], w["probes"], w["depth"]
+    prec = args.precision or DEFAULT_PRECISION["rbf-grad"]
+    X = torch.as_tensor(problems.rbf_points(n, seed=args.seed), device=device)
+    op = operators.RBF(X, precision=prec)
+    Z = stochtrace.sampler_signs(np.zeros(n, np.float32), num=P).flat(prng.PRNGKey(args.seed),
+                                                                       device)
+    params = (w["lengthscale"], w["noise"])
+    for _ in range(args.warmup):
+        funm.logdet_value_and_grad(op, Z, k, params)
+    ctx = _runtime.context(device)
+    ctx.profile(True)
+    for kc in ("matvec", "adjoint"):
+        ctx.profile_read(kc)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, mean, grads = funm.logdet_value_and_grad(op, Z, k, params)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    adj_ms, adj_n = ctx.profile_read("adjoint")
+    ctx.profile(False)
+    _, fmean = funm.logdet_per_probe(op, Z, k, params)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        funm.logdet_per_probe(op, Z, k, params)
+    torch.cuda.synchronize()
+    fwd = (time.perf_counter() - t1) / args.steps
+    return {
+        "metric": "SLQ log-det value + gradient (d/d lengthscale, d/d noise) time",
+        "value": dt * 1e3,
+        "unit": "ms",
+        "higher_is_better": False,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "dtype": prec,
+        "data": "synthetic (seeded points, Rademacher probes from PRNGKey(seed))",
+        "config": {"workload": f"rbf-grad n={n} probes={P} depth={k}", "n": n, "probes": P,
+                   "depth": k, "precision": prec, "lengthscale": w["lengthscale"],
+                   "noise": w["noise"]},
+        "forward_ms": fwd * 1e3,
+        "reverse_pass_ms": adj_ms / max(adj_n, 1),
+        "logdet": float(mean),
+        "grad": [float(grads[0]), float(grads[1])],
+    }
+
+
 def cpu_baseline(args, host, device, n, k):
     """NumPy restatement of matfree's algorithm (the oracle) on a bounded sample."""
     import numpy as np
@@ -219,6 +352,9 @@ def main():
 
     device = torch.device("cuda", 0)
     torch.cuda.set_device(device)
+    if args.workload == "rbf-grad":
+        print(json.dumps(grad_step(args, device)), flush=True)
+        return
     op, params, Z, (n, P, k), prec, host = setup(args, device)
     ctx = _runtime.context(device)
     log(f"bench: workload={args.workload} n={n} P={P} k={k} precision={prec}")
@@ -255,6 +391,11 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args, host, device, n, k)
         log(f"bench: cpu baseline {out['cpu_baseline']['value']:.1f} {UNIT}")
+    if args.serve_requests > 0:
+        del op, params, Z, host
+        torch.cuda.empty_cache()
+        out["serve"] = serve(args, device)
+        log(f"bench: serve p50 {out['serve']['p50_ms']:.2f} ms, p99 {out['serve']['p99_ms']:.2f} ms")
     print(json.dumps(out), flush=True)
 
 
