@@ -18,8 +18,9 @@
 //  * k_adjoint_coef / k_adjoint_combine: one step of the backward
 //    recursion (decomp.py:587-599) given the reductions of the streaming
 //    kernels (k_lanczos.hip).
-//  * k_dot_sum / k_sum_fixed: deterministic FP64 sums of elementwise
-//    products for the parameter cotangents.
+//  * k_outer_sum: the parameter cotangents from S = sum lambda q^T -- the
+//    stored matrix's dA = S, or the RBF contractions <S, dK/dl>, <S, I>;
+//    k_sum_fixed: their deterministic FP64 reduction.
 #include <cfloat>
 
 #include "slq_internal.h"
@@ -271,33 +272,23 @@ __global__ void __launch_bounds__(kStreamThreads)
   }
 }
 
-// Block partial sums of a[i] * b[i] in FP64 (i < count), fixed order inside
-// a block: partial[blockIdx.x].
+// out[o] = f_o * sum_i partial[i * nout + o], i < count, o < nout <= 2: one
+// block per output, a fixed strided split over the 256 threads and a fixed
+// tree (deterministic).
 __global__ void __launch_bounds__(256)
-    k_dot_sum(const float* __restrict__ a, const float* __restrict__ b, int64_t count,
-              double* __restrict__ partial) {
+    k_sum_fixed(const double* __restrict__ partial, int64_t count, int nout, double f0, double f1,
+                double* __restrict__ out) {
   __shared__ double red[256];
+  const int o = blockIdx.x;
   double s = 0.0;
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
-       i += int64_t(gridDim.x) * blockDim.x)
-    s += double(a[i]) * double(b[i]);
+  for (int64_t i = threadIdx.x; i < count; i += 256) s += partial[i * nout + o];
   red[threadIdx.x] = s;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
-}
-
-// out[o] = f_o * sum of partial[o * count .. (o + 1) * count) in order, o < 2.
-__global__ void k_sum_fixed(const double* __restrict__ partial, int64_t count, int nout, double f0,
-                            double f1, double* __restrict__ out) {
-  const int o = threadIdx.x;
-  if (o >= nout) return;
-  double s = 0.0;
-  for (int64_t i = 0; i < count; ++i) s += partial[o * count + i];
-  out[o] = (o == 0 ? f0 : f1) * s;
+  if (threadIdx.x == 0) out[o] = (o == 0 ? f0 : f1) * red[0];
 }
 
 // Coefficients that make k_adjoint_combine compute lambda = Q gamma with
@@ -330,9 +321,20 @@ constexpr int kOT = 128;         // output tile edge
 constexpr int kOK = 8;           // columns per stage
 constexpr int kOLd = kOT + 4;    // padded LDS row
 
+// RBF contraction mode: instead of storing S = sum_c L_c R_c^T, reduce
+//   sum_ij S_ij |x_i - x_j|^2 exp2(c |x_i - x_j|^2)   (l^3 times <S, dK/dl>)
+// and sum_i S_ii (<S, I>) per workgroup into red[2 * block + {0, 1}] (FP64).
+struct RBFContract {
+  const float* X;
+  int dim;
+  float coef;
+  double* red;
+};
+
+template <bool CONTRACT>
 __global__ void __launch_bounds__(256)
     k_outer_sum(const float* __restrict__ L, const float* __restrict__ R, int64_t ldc, int64_t P,
-                int64_t n, int64_t K, float* __restrict__ out) {
+                int64_t n, int64_t K, float* __restrict__ out, RBFContract rc) {
   __shared__ __attribute__((aligned(16))) float Ls[2][kOK * kOLd];
   __shared__ __attribute__((aligned(16))) float Rs[2][kOK * kOLd];
   const int t = threadIdx.x;
@@ -395,14 +397,64 @@ __global__ void __launch_bounds__(256)
   }
   // lane l of accumulator c holds column j0 + 32 c + (l & 31) at rows
   // i0 + 32 wave + 8 (s / 4) + 4 h + (s % 4)
+  if constexpr (!CONTRACT) {
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int64_t j = j0 + 32 * c + r;
-    if (j >= n) continue;
+    for (int c = 0; c < 4; ++c) {
+      const int64_t j = j0 + 32 * c + r;
+      if (j >= n) continue;
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        const int64_t i = i0 + 32 * wave + 8 * (s / 4) + 4 * h + (s % 4);
+        if (i < n) out[i * n + j] = acc[c][s];
+      }
+    }
+  } else {
+    double sd = 0.0, st2 = 0.0;
+    float xi[16][3];
 #pragma unroll
     for (int s = 0; s < 16; ++s) {
       const int64_t i = i0 + 32 * wave + 8 * (s / 4) + 4 * h + (s % 4);
-      if (i < n) out[i * n + j] = acc[c][s];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) xi[s][d] = (i < n && d < rc.dim) ? rc.X[i * rc.dim + d] : 0.f;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int64_t j = j0 + 32 * c + r;
+      if (j >= n) continue;
+      float xj[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) xj[d] = d < rc.dim ? rc.X[j * rc.dim + d] : 0.f;
+      float part = 0.f;
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        const int64_t i = i0 + 32 * wave + 8 * (s / 4) + 4 * h + (s % 4);
+        if (i >= n) continue;
+        float d2 = 0.f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const float df = xi[s][d] - xj[d];
+          d2 = fmaf(df, df, d2);
+        }
+        part = fmaf(acc[c][s], d2 * __builtin_amdgcn_exp2f(rc.coef * d2), part);
+        if (i == j) st2 += double(acc[c][s]);
+      }
+      sd += double(part);
+    }
+    __shared__ double red[2][256];
+    red[0][t] = sd;
+    red[1][t] = st2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (t < w) {
+        red[0][t] += red[0][t + w];
+        red[1][t] += red[1][t + w];
+      }
+      __syncthreads();
+    }
+    if (t == 0) {
+      const int64_t b = int64_t(blockIdx.y) * gridDim.x + blockIdx.x;
+      rc.red[2 * b] = red[0][0];
+      rc.red[2 * b + 1] = red[1][0];
     }
   }
 }
@@ -412,7 +464,23 @@ __global__ void __launch_bounds__(256)
 hipError_t launch_outer_sum(hipStream_t s, const float* L, const float* R, int64_t ldc, int64_t P,
                             int64_t n, int64_t K, float* out) {
   const unsigned tiles = unsigned((n + kOT - 1) / kOT);
-  hipLaunchKernelGGL(k_outer_sum, dim3(tiles, tiles), dim3(256), 0, s, L, R, ldc, P, n, K, out);
+  hipLaunchKernelGGL(k_outer_sum<false>, dim3(tiles, tiles), dim3(256), 0, s, L, R, ldc, P, n, K,
+                     out, RBFContract{});
+  return hipGetLastError();
+}
+
+int64_t outer_contract_blocks(int64_t n) {
+  const int64_t tiles = (n + kOT - 1) / kOT;
+  return tiles * tiles;
+}
+
+hipError_t launch_outer_rbf_contract(hipStream_t s, const float* L, const float* R, int64_t ldc,
+                                     int64_t P, int64_t n, int64_t K, const float* X, int dim,
+                                     double lengthscale, double* red) {
+  const unsigned tiles = unsigned((n + kOT - 1) / kOT);
+  RBFContract rc{X, dim, float(-1.4426950408889634 / (2.0 * lengthscale * lengthscale)), red};
+  hipLaunchKernelGGL(k_outer_sum<true>, dim3(tiles, tiles), dim3(256), 0, s, L, R, ldc, P, n, K,
+                     nullptr, rc);
   return hipGetLastError();
 }
 
@@ -460,15 +528,10 @@ hipError_t launch_adjoint_combine(hipStream_t s, const float* Q, const float* La
   return hipGetLastError();
 }
 
-hipError_t launch_dot_sum(hipStream_t s, const float* a, const float* b, int64_t count,
-                          double* partial, int blocks) {
-  hipLaunchKernelGGL(k_dot_sum, dim3(unsigned(blocks)), dim3(256), 0, s, a, b, count, partial);
-  return hipGetLastError();
-}
-
 hipError_t launch_sum_fixed(hipStream_t s, const double* partial, int64_t count, int nout,
                             double f0, double f1, double* out) {
-  hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(64), 0, s, partial, count, nout, f0, f1, out);
+  hipLaunchKernelGGL(k_sum_fixed, dim3(unsigned(nout)), dim3(256), 0, s, partial, count, nout, f0,
+                     f1, out);
   return hipGetLastError();
 }
 

@@ -107,9 +107,8 @@ __device__ __forceinline__ void load_a_frag(const float* __restrict__ A, int64_t
   }
 }
 
-// The lane's 16 generated RBF values exp2(c |x_row - x_m|^2), or with DERIV
-// the entries |x_row - x_m|^2 exp2(c |x_row - x_m|^2) of l^3 dK/dl.
-template <int DIM, bool DERIV>
+// The lane's 16 generated RBF values exp2(c |x_row - x_m|^2).
+template <int DIM>
 __device__ __forceinline__ void gen_rbf_frag(const float4* Xs, const float (&xr)[4], float coef,
                                              int64_t n, int64_t m0, int h, float (&a)[16]) {
 #pragma unroll
@@ -120,8 +119,7 @@ __device__ __forceinline__ void gen_rbf_frag(const float4* Xs, const float (&xr)
     if (DIM > 1) { d = xr[1] - xm.y; d2 = fmaf(d, d, d2); }
     if (DIM > 2) { d = xr[2] - xm.z; d2 = fmaf(d, d, d2); }
     if (DIM > 3) { d = xr[3] - xm.w; d2 = fmaf(d, d, d2); }
-    float kv = __builtin_amdgcn_exp2f(coef * d2);
-    if (DERIV) kv *= d2;
+    const float kv = __builtin_amdgcn_exp2f(coef * d2);
     a[s] = (m0 + 16 * h + s < n) ? kv : 0.f;
   }
 }
@@ -166,7 +164,7 @@ __device__ __forceinline__ void store_out(float* __restrict__ W, const float* __
   }
 }
 
-template <int NACC, bool RBF, int DIM, bool VEC, bool DERIV = false>
+template <int NACC, bool RBF, int DIM, bool VEC>
 __global__ void __launch_bounds__(kThreads)
     k_matvec_f32(const float* __restrict__ A, RBFParams prm, const float* __restrict__ V,
                  int64_t P, int64_t n, float* __restrict__ W) {
@@ -233,7 +231,7 @@ __global__ void __launch_bounds__(kThreads)
     }
     float a[16];
     if (RBF) {
-      gen_rbf_frag<DIM, DERIV>(Xs[buf], xr, prm.coef, n, m0, h, a);
+      gen_rbf_frag<DIM>(Xs[buf], xr, prm.coef, n, m0, h, a);
     } else {
 #pragma unroll
       for (int s = 0; s < 16; ++s) a[s] = areg[s];
@@ -276,7 +274,7 @@ __global__ void __launch_bounds__(kThreads)
     }
     __syncthreads();
   }
-  if (split || DERIV)
+  if (split)
     store_out<NACC, VEC, false>(Wout, V, P, n, i0w, p0, acc, 0.f);
   else
     store_out<NACC, VEC, RBF>(Wout, V, P, n, i0w, p0, acc, prm.noise);
@@ -876,26 +874,26 @@ hipError_t launch_tpl_bf16(hipStream_t s, const float* A, const u32x4* rec, cons
   return launch_tpl_bf16_n<NACC, RBF, 1>(s, A, rec, prm, Vhi, Vlo, Ppad, V, P, n, W, vec, S);
 }
 
-template <int NACC, bool RBF, int DIM, bool DERIV>
+template <int NACC, bool RBF, int DIM>
 hipError_t launch_tpl(hipStream_t s, const float* A, const RBFParams& prm, const float* V,
                       int64_t P, int64_t n, float* W, bool vec, int S) {
   const dim3 grid(unsigned((n + BI - 1) / BI), unsigned((P + 32 * NACC - 1) / (32 * NACC)),
                   unsigned(S));
   if (vec)
-    hipLaunchKernelGGL((k_matvec_f32<NACC, RBF, DIM, true, DERIV>), grid, dim3(kThreads), 0, s, A,
-                       prm, V, P, n, W);
+    hipLaunchKernelGGL((k_matvec_f32<NACC, RBF, DIM, true>), grid, dim3(kThreads), 0, s, A, prm,
+                       V, P, n, W);
   else
-    hipLaunchKernelGGL((k_matvec_f32<NACC, RBF, DIM, false, DERIV>), grid, dim3(kThreads), 0, s, A,
-                       prm, V, P, n, W);
+    hipLaunchKernelGGL((k_matvec_f32<NACC, RBF, DIM, false>), grid, dim3(kThreads), 0, s, A, prm,
+                       V, P, n, W);
   return hipGetLastError();
 }
 
-template <bool RBF, int DIM, bool DERIV = false>
+template <bool RBF, int DIM>
 hipError_t launch_f32(hipStream_t s, const float* A, const RBFParams& prm, const float* V,
                       int64_t P, int64_t n, float* W, bool vec, int S) {
-  if (P <= 32) return launch_tpl<1, RBF, DIM, DERIV>(s, A, prm, V, P, n, W, vec, S);
-  if (P <= 64) return launch_tpl<2, RBF, DIM, DERIV>(s, A, prm, V, P, n, W, vec, S);
-  return launch_tpl<4, RBF, DIM, DERIV>(s, A, prm, V, P, n, W, vec, S);
+  if (P <= 32) return launch_tpl<1, RBF, DIM>(s, A, prm, V, P, n, W, vec, S);
+  if (P <= 64) return launch_tpl<2, RBF, DIM>(s, A, prm, V, P, n, W, vec, S);
+  return launch_tpl<4, RBF, DIM>(s, A, prm, V, P, n, W, vec, S);
 }
 
 int nacc_for(int64_t P) { return P <= 32 ? 1 : (P <= 64 ? 2 : 4); }
@@ -1012,25 +1010,6 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
     hipLaunchKernelGGL(k_splitk_reduce<false>, dim3(unsigned(blocks)), dim3(256), 0, s, partial,
                        S, count, V, 0.f, W);
   return hipGetLastError();
-}
-
-hipError_t launch_rbf_deriv_matvec(hipStream_t s, const slq_operator& op, const float* V,
-                                   int64_t P, float* W) {
-  const int64_t n = op.n;
-  if (n <= 0 || P <= 0) return hipSuccess;
-  if (op.kind != SLQ_OP_RBF) return hipErrorInvalidValue;
-  const bool vec = (n % 4 == 0) && aligned16(V) && aligned16(W);
-  RBFParams prm{};
-  prm.X = op.points;
-  prm.dim = op.dim;
-  prm.coef = float(-1.4426950408889634 / (2.0 * op.lengthscale * op.lengthscale));
-  prm.noise = 0.f;
-  switch (op.dim) {
-    case 1: return launch_f32<true, 1, true>(s, nullptr, prm, V, P, n, W, vec, 1);
-    case 2: return launch_f32<true, 2, true>(s, nullptr, prm, V, P, n, W, vec, 1);
-    case 3: return launch_f32<true, 3, true>(s, nullptr, prm, V, P, n, W, vec, 1);
-    default: return launch_f32<true, 4, true>(s, nullptr, prm, V, P, n, W, vec, 1);
-  }
 }
 
 }  // namespace slq

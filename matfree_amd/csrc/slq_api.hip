@@ -274,7 +274,6 @@ int logdet_reverse(slq_context* ctx, const slq_operator& op, int64_t P, int64_t 
   hipStream_t s = ctx->stream;
   const int pstride = int(k);
   const int64_t nb = num_chunks(n);
-  constexpr int kSumBlocks = 256;
   float *dH, *Gam, *coef, *Lam, *lam, *lam2, *vm, *part1, *part2, *mvs;
   double *vwork = nullptr, *dsum;
   const int64_t mv_floats = matvec_scratch_floats(n, P);
@@ -290,7 +289,7 @@ int logdet_reverse(slq_context* ctx, const slq_operator& op, int64_t P, int64_t 
   SLQ_TRY(ensure_t(ctx, ctx->adj_v3, size_t(P * n), &vm));
   const int64_t vw = tridiag_vjp_work_doubles(P, k);
   if (vw > 0) SLQ_TRY(ensure_t(ctx, ctx->adj_work, size_t(vw), &vwork));
-  SLQ_TRY(ensure_t(ctx, ctx->adj_sum, size_t(2 * k * kSumBlocks), &dsum));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_sum, size_t(2 * outer_contract_blocks(n)), &dsum));
 
   // dH from the eigh VJP of |z|^2 [log T]_00 / P (funm.py:239-241, 331-333)
   SLQ_HIP(ctx, hipMemsetAsync(dH, 0, size_t(P * k * k) * sizeof(float), s));
@@ -321,16 +320,14 @@ int logdet_reverse(slq_context* ctx, const slq_operator& op, int64_t P, int64_t 
   // parameter cotangents: sum over steps and probes of the matvec VJPs
   // lambda^T (dA) q_idx with lambda = Lambda[:, idx] (decomp.py:587-589)
   if (op.kind == SLQ_OP_RBF) {
-    // d/d noise: lambda^T q; d/d lengthscale: lambda^T (|x_i - x_m|^2 K) q / l^3
-    for (int64_t idx = 0; idx < k; ++idx) {
-      const float* q = st.Q + idx * ldc;
-      const float* l = Lam + idx * ldc;
-      SLQ_HIP(ctx, launch_rbf_deriv_matvec(s, op, q, P, vm));
-      SLQ_HIP(ctx, launch_dot_sum(s, l, vm, P * n, dsum + idx * kSumBlocks, kSumBlocks));
-      SLQ_HIP(ctx, launch_dot_sum(s, l, q, P * n, dsum + (k + idx) * kSumBlocks, kSumBlocks));
-    }
+    // with S = sum_{idx, p} lambda q^T: d/d noise = <S, I> and
+    // d/d lengthscale = <S, dK/dl> = <S, |x_i - x_j|^2 K> / l^3, in one pass
+    // over the output tiles of S (never stored)
+    SLQ_HIP(ctx, launch_outer_rbf_contract(s, Lam, st.Q, ldc, P, n, k * P, op.points, op.dim,
+                                           op.lengthscale, dsum));
     const double ls = op.lengthscale;
-    SLQ_HIP(ctx, launch_sum_fixed(s, dsum, k * kSumBlocks, 2, 1.0 / (ls * ls * ls), 1.0, dparams));
+    SLQ_HIP(ctx, launch_sum_fixed(s, dsum, outer_contract_blocks(n), 2, 1.0 / (ls * ls * ls), 1.0,
+                                  dparams));
   } else {
     // d/dA = sum_idx sum_p lambda q^T (the VJP of A -> A q)
     SLQ_HIP(ctx, launch_outer_sum(s, Lam, st.Q, ldc, P, n, k * P, dmatrix));

@@ -63,11 +63,6 @@ int64_t matvec_scratch_floats(int64_t n, int64_t P);
 hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, int64_t P,
                          float* W, float* scratch, int64_t scratch_floats);
 
-// W = (|x_i - x_m|^2 K_im) V in FP32: l^3 dK/dl applied to P vectors (no
-// nugget, no split-K).
-hipError_t launch_rbf_deriv_matvec(hipStream_t s, const slq_operator& op, const float* V,
-                                   int64_t P, float* W);
-
 // ---- tridiagonal eigen + quadrature (k_tridiag.hip) ------------------------
 // For each probe p: T = (H + H^T)/2 restricted to its tridiagonal; returns
 // per_probe[p] = scale[p]^2 * sum_j U_0j^2 log(lambda_j) in FP64, optional
@@ -101,10 +96,14 @@ hipError_t launch_adjoint_combine(hipStream_t s, const float* Q, const float* La
 // base + idx * ldc + p * n (K = k P), FP32 MFMA.
 hipError_t launch_outer_sum(hipStream_t s, const float* L, const float* R, int64_t ldc, int64_t P,
                             int64_t n, int64_t K, float* out);
-// partial[b] = sum over a grid-stride share of a[i] * b[i] (FP64), b < blocks.
-hipError_t launch_dot_sum(hipStream_t s, const float* a, const float* b, int64_t count,
-                          double* partial, int blocks);
-// out[o] = f_o * sum_i partial[o * count + i] (fixed order), nout <= 2.
+// With S = sum_{c < K} L_c R_c^T (as above) and the RBF points X (n x dim):
+// red[2 b] = sum_ij S_ij d2_ij exp(-d2_ij / (2 l^2)), d2_ij = |x_i - x_j|^2, and
+// red[2 b + 1] = sum_i S_ii, partial over the outer_contract_blocks(n) tiles b.
+int64_t outer_contract_blocks(int64_t n);
+hipError_t launch_outer_rbf_contract(hipStream_t s, const float* L, const float* R, int64_t ldc,
+                                     int64_t P, int64_t n, int64_t K, const float* X, int dim,
+                                     double lengthscale, double* red);
+// out[o] = f_o * sum_i partial[i * nout + o] (fixed order), nout <= 2.
 hipError_t launch_sum_fixed(hipStream_t s, const double* partial, int64_t count, int nout,
                             double f0, double f1, double* out);
 
