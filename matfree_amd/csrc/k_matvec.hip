@@ -936,7 +936,8 @@ int64_t matvec_scratch_floats(int64_t n, int64_t P) {
 }
 
 hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, int64_t P,
-                         float* W, float* scratch, int64_t scratch_floats) {
+                         float* W, float* scratch, int64_t scratch_floats, hipEvent_t ev0,
+                         hipEvent_t ev1) {
   const int64_t n = op.n;
   if (n <= 0 || P <= 0) return hipSuccess;
   const bool vec = (n % 4 == 0) && aligned16(V) && aligned16(W);
@@ -973,6 +974,15 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
   float* out = (S > 1) ? partial : W;
   RBFParams prm{};
   hipError_t err;
+  if (op.kind == SLQ_OP_RBF && !fp32) {
+    hipLaunchKernelGGL(k_prep_points, dim3(unsigned((npad + 255) / 256)), dim3(256), 0, s,
+                       op.points, op.dim,
+                       float(-1.4426950408889634 / (2.0 * op.lengthscale * op.lengthscale)), n,
+                       npad, rec);
+    err = hipGetLastError();
+    if (err != hipSuccess) return err;
+  }
+  if (ev0) (void)hipEventRecord(ev0, s);
   if (op.kind == SLQ_OP_DENSE) {
     const bool v2 = vec && aligned16(op.matrix);
     err = fp32 ? launch_f32<false, 1>(s, op.matrix, prm, V, P, n, out, v2, S)
@@ -985,10 +995,6 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
     prm.noise = float(op.noise);
     if (!fp32) {
       if (op.dim > 3) return hipErrorInvalidValue;  // rejected by the API before launch
-      hipLaunchKernelGGL(k_prep_points, dim3(unsigned((npad + 255) / 256)), dim3(256), 0, s,
-                         op.points, op.dim, prm.coef, n, npad, rec);
-      err = hipGetLastError();
-      if (err != hipSuccess) return err;
       err = launch_bf16<true>(s, nullptr, rec, prm, Vhi, Vlo, Ppad, V, P, n, out, vec, nprod, S);
     } else {
       switch (op.dim) {
@@ -999,6 +1005,7 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
       }
     }
   }
+  if (ev1) (void)hipEventRecord(ev1, s);
   if (err != hipSuccess || S == 1) return err;
   const int64_t count = P * n;
   int64_t blocks = (count + 255) / 256;

@@ -119,25 +119,31 @@ int check_operator(slq_context* ctx, const slq_operator* op) {
   return SLQ_OK;
 }
 
-// Bracket one launch with an event pair when profiling is on.
+// The next (start, end) event pair of kernel class kc, or nulls when
+// profiling is off (or an event cannot be created).
+std::pair<hipEvent_t, hipEvent_t> take_events(slq_context* ctx, int kc) {
+  if (!ctx->profiling) return {nullptr, nullptr};
+  auto& vec = ctx->events[kc];
+  size_t& u = ctx->used[kc];
+  if (u == vec.size()) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+      (void)hipGetLastError();
+      return {nullptr, nullptr};
+    }
+    vec.emplace_back(a, b);
+  }
+  return vec[u++];
+}
+
+// Brackets the launches of its scope with the next event pair of class kc.
 struct Timed {
   slq_context* ctx;
   hipEvent_t end = nullptr;
   Timed(slq_context* c, int kc) : ctx(c) {
-    if (!ctx->profiling) return;
-    auto& vec = ctx->events[kc];
-    size_t& u = ctx->used[kc];
-    if (u == vec.size()) {
-      hipEvent_t a = nullptr, b = nullptr;
-      if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
-        (void)hipGetLastError();
-        return;
-      }
-      vec.emplace_back(a, b);
-    }
-    (void)hipEventRecord(vec[u].first, ctx->stream);
-    end = vec[u].second;
-    ++u;
+    const auto ev = take_events(ctx, kc);
+    if (ev.first) (void)hipEventRecord(ev.first, ctx->stream);
+    end = ev.second;
   }
   ~Timed() {
     if (end) (void)hipEventRecord(end, ctx->stream);
@@ -186,10 +192,11 @@ int hessenberg_core(slq_context* ctx, const slq_operator& op, const float* V0, i
     const float* qj = Q + j * ldc;
     const int ncols = int(j + 1);
     float* hcol = H + j * k;  // column j of every probe, stride k * k
-    // w = A v (decomp.py:460)
+    // w = A v (decomp.py:460); the events bracket the block-matvec kernel
+    // itself (not the probe split or the split-K reduction around it)
     {
-      Timed tm(ctx, SLQ_KC_MATVEC);
-      SLQ_HIP(ctx, launch_matvec(s, op, qj, P, w, mvs, mv_floats));
+      const auto ev = take_events(ctx, SLQ_KC_MATVEC);
+      SLQ_HIP(ctx, launch_matvec(s, op, qj, P, w, mvs, mv_floats, ev.first, ev.second));
     }
     // h = Q^T w (decomp.py:463)
     {

@@ -60,8 +60,11 @@ constexpr int kMaxSplitK = 4;
 // Scratch (in floats) the block matvec can use for P vectors of length n:
 // the split-K partial sums and the BF16 hi/lo planes of the vectors.
 int64_t matvec_scratch_floats(int64_t n, int64_t P);
+// ev0 / ev1 (optional) are recorded on s right before and after the block
+// matvec kernel (the launches around it excluded).
 hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, int64_t P,
-                         float* W, float* scratch, int64_t scratch_floats);
+                         float* W, float* scratch, int64_t scratch_floats,
+                         hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
 // ---- tridiagonal eigen + quadrature (k_tridiag.hip) ------------------------
 // For each probe p: T = (H + H^T)/2 restricted to its tridiagonal; returns
