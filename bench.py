@@ -309,7 +309,7 @@ def cpu_baseline(args, host, device, n, k):
     from oracle import slq as oslq
 
     w = WORKLOADS[args.workload]
-    probes = 32
+    probes = 96  # about 10-20 s of host work at the restatement's rate
     if args.workload == "rbf":
         from matfree_amd import problems
 
