@@ -136,6 +136,33 @@ int slq_logdet(slq_context* ctx, const slq_operator* op, const float* Z, int64_t
 int slq_logdet_vjp(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P, int64_t k,
                    double* per_probe, double* mean, double* dparams, float* dmatrix);
 
+/* Matrix functions of slq_funm_vectors (dense_funm_sym_eigh(matfun),
+ * funm.py:322-335, for these matfun). */
+typedef enum slq_matfun {
+  SLQ_FUNM_LOG = 0,
+  SLQ_FUNM_EXP = 1,
+  SLQ_FUNM_SIN = 2,
+  SLQ_FUNM_COS = 3,
+  SLQ_FUNM_SQRT = 4,
+  SLQ_FUNM_INV = 5 /* 1 / x */
+} slq_matfun;
+
+/* Matrix-function-vector products f(A) v_p ~ |v_p| Q_p^T f(T_p) e_1 for P
+ * vectors: funm.funm_lanczos_sym(dense_funm_sym_eigh(f), tridiag_sym(k))
+ * (funm.py:114-147) with the forward pass of slq_hessenberg and f applied to
+ * the FP64 eigendecomposition of each T_p.
+ *   V: P x n vectors; out: P x n results (may alias neither V nor scratch).
+ * Fails with SLQ_EINVAL_K unless 0 < k <= n. */
+int slq_funm_vectors(slq_context* ctx, const slq_operator* op, const float* V, int64_t P,
+                     int64_t k, int matfun, float* out);
+
+/* mean and standard error of the mean (population standard deviation over
+ * sqrt(P)) of P device doubles, in FP64 and a fixed order:
+ * estimator_monte_carlo_mean_and_sem's reductions (stochtrace.py:84-88).
+ * Either output may be NULL. */
+int slq_mean_and_sem(slq_context* ctx, const double* values, int64_t P, double* mean,
+                     double* sem);
+
 /* ---- measurement hooks (no counterpart in the reference) -------------------
  * With profiling on, every launch of the listed kernel classes made through
  * this context is bracketed by a pair of HIP events on the context's stream.

@@ -24,6 +24,8 @@ SLQ_EADJOINT_K0 = -6
 SLQ_OP_DENSE = 0
 SLQ_OP_RBF = 1
 PRECISIONS = {"fp32": 0, "bf16": 1, "bf16x3": 2, "bf16x2": 3}
+# matrix functions of slq_funm_vectors (slq_matfun)
+MATFUNS = {"log": 0, "exp": 1, "sin": 2, "cos": 3, "sqrt": 4, "reciprocal": 5}
 
 c_void_p = ctypes.c_void_p
 c_i64 = ctypes.c_int64
@@ -71,6 +73,11 @@ SIGNATURES = {
         [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_i64, c_void_p, c_void_p,
          c_void_p, c_void_p],
     ),
+    "slq_funm_vectors": (
+        c_int,
+        [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_i64, c_int, c_void_p],
+    ),
+    "slq_mean_and_sem": (c_int, [c_void_p, c_void_p, c_i64, c_void_p, c_void_p]),
     "slq_profile_enable": (c_int, [c_void_p, c_int]),
     "slq_profile_read": (c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(c_i64)]),

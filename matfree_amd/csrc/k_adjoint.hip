@@ -5,165 +5,17 @@
 // (funm.py:237), so dQ = dr = dc = 0 and the adjoint's inputs reduce to dH.
 //
 // Kernels:
-//  * k_tridiag_vjp: per probe, T = tridiag(alpha, beta) from H exactly as
-//    the forward extracts it (FP32, decomp.py:133-135), its full FP64 eigen
-//    decomposition T = U diag(theta) U^T (implicit QL, eigenvectors
-//    accumulated), and the cotangent of [log T]_00 in the Daleckii-Krein
-//    form Tbar = U (L o u u^T) U^T with u = U[0, :] and L the divided
-//    differences of log at the eigenvalues -- what the eigh VJP composed
-//    with fA[0, 0] evaluates to (funm.py:331-333).  Through
-//    alpha_i = H_ii and beta_i = (H_{i,i+1} + H_{i+1,i}) / 2 this gives
-//    dH_ii = Tbar_ii and dH_{i,i+1} = dH_{i+1,i} = Tbar_{i,i+1}, scaled by
-//    |z|^2 / P.
+//  * the eigen cotangent dH comes from k_tridiag.hip (k_tridiag_vjp);
 //  * k_adjoint_coef / k_adjoint_combine: one step of the backward
 //    recursion (decomp.py:587-599) given the reductions of the streaming
 //    kernels (k_lanczos.hip).
 //  * k_outer_sum: the parameter cotangents from S = sum lambda q^T -- the
 //    stored matrix's dA = S, or the RBF contractions <S, dK/dl>, <S, I>;
 //    k_sum_fixed: their deterministic FP64 reduction.
-#include <cfloat>
-
 #include "slq_internal.h"
 
 namespace slq {
 namespace {
-
-constexpr int kMaxSweeps = 64;
-
-// ---------------------------------------------------------------------------
-// Tridiagonal eigen cotangent.  One wave per probe; (d, e, lg) are k doubles,
-// U is k x k row-major (U[a * k + j] = component a of eigenvector j), all in
-// LDS for k <= kVjpLdsDepth and in a global workspace beyond.  The QL
-// recurrence is evaluated redundantly by every lane (uniform control flow);
-// the rotations of U are spread over the lanes by rows.
-// ---------------------------------------------------------------------------
-constexpr int kVjpLdsDepth = 64;
-
-__device__ void tridiag_vjp_wave(const float* __restrict__ Hp, int k, double* d, double* e,
-                                 double* lg, double* U, double scale, float* __restrict__ dHp) {
-  const int lane = threadIdx.x;
-  for (int i = lane; i < k; i += 64) {
-    d[i] = double(Hp[int64_t(i) * k + i]);
-    e[i] = (i + 1 < k) ? double(0.5f * (Hp[int64_t(i + 1) * k + i] + Hp[int64_t(i) * k + i + 1]))
-                       : 0.0;
-  }
-  for (int a = lane; a < k; a += 64)
-    for (int j = 0; j < k; ++j) U[int64_t(a) * k + j] = (a == j) ? 1.0 : 0.0;
-  __syncthreads();
-
-  bool failed = false;
-  for (int l = 0; l < k && !failed; ++l) {
-    int sweeps = 0;
-    while (true) {
-      int m = l;
-      for (; m + 1 < k; ++m) {
-        const double dd = fabs(d[m]) + fabs(d[m + 1]);
-        if (fabs(e[m]) <= DBL_EPSILON * dd) break;
-      }
-      if (m == l) break;
-      if (++sweeps > kMaxSweeps) {
-        failed = true;
-        break;
-      }
-      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-      double r = hypot(g, 1.0);
-      g = d[m] - d[l] + e[l] / (g + copysign(r, g));
-      double s = 1.0, c = 1.0, shift = 0.0;
-      bool split = false;
-      for (int i = m - 1; i >= l; --i) {
-        const double f = s * e[i];
-        const double b = c * e[i];
-        r = hypot(f, g);
-        e[i + 1] = r;
-        if (r == 0.0) {  // an off-diagonal underflowed: the matrix split
-          d[i + 1] -= shift;
-          e[m] = 0.0;
-          split = true;
-          break;
-        }
-        s = f / r;
-        c = g / r;
-        g = d[i + 1] - shift;
-        r = (d[i] - g) * s + 2.0 * c * b;
-        shift = s * r;
-        d[i + 1] = g + shift;
-        g = c * r - b;
-        for (int a = lane; a < k; a += 64) {
-          double* ua = U + int64_t(a) * k;
-          const double ui = ua[i], ui1 = ua[i + 1];
-          ua[i + 1] = s * ui + c * ui1;
-          ua[i] = c * ui - s * ui1;
-        }
-      }
-      if (split) continue;
-      d[l] -= shift;
-      e[l] = g;
-      e[m] = 0.0;
-    }
-  }
-  // log of the eigenvalues; a non-positive eigenvalue makes the reference's
-  // log, and so its cotangent, NaN
-  __syncthreads();
-  bool nonpositive = failed;
-  for (int j = 0; j < k; ++j) nonpositive |= !(d[j] > 0.0);
-  for (int j = lane; j < k; j += 64) lg[j] = log(d[j]);
-  __syncthreads();
-
-  // Tbar = U W U^T, W_jl = u_j u_l L_jl; lane a forms the diagonal entry a
-  // and the superdiagonal entry (a, a + 1):
-  //   v_l = sum_j W_jl U_aj,  Tbar_aa = sum_l U_al v_l,
-  //   Tbar_{a,a+1} = sum_l U_{a+1,l} v_l.
-  for (int a = lane; a < k; a += 64) {
-    double tdd = 0.0, tdo = 0.0;
-    const double* ua = U + int64_t(a) * k;
-    const double* ub = U + int64_t(a + 1 < k ? a + 1 : a) * k;
-    for (int l = 0; l < k; ++l) {
-      const double ul = U[l], thl = d[l];
-      double v = 0.0;
-      for (int j = 0; j < k; ++j) {
-        const double thj = d[j], del = thl - thj;
-        double L;
-        if (j == l || del == 0.0) {
-          L = 1.0 / thj;
-        } else if (fabs(del) < 1e-3 * fabs(thj)) {
-          L = log1p(del / thj) / del;  // (log thl - log thj) / (thl - thj), no cancellation
-        } else {
-          L = (lg[l] - lg[j]) / del;
-        }
-        v += U[j] * ul * L * ua[j];
-      }
-      tdd += ua[l] * v;
-      tdo += ub[l] * v;
-    }
-    const float nan = __builtin_nanf("");
-    dHp[int64_t(a) * k + a] = nonpositive ? nan : float(scale * tdd);
-    if (a + 1 < k) {
-      const float o = nonpositive ? nan : float(scale * tdo);
-      dHp[int64_t(a + 1) * k + a] = o;  // (a, a + 1)
-      dHp[int64_t(a) * k + a + 1] = o;  // (a + 1, a)
-    }
-  }
-}
-
-__global__ void __launch_bounds__(64)
-    k_tridiag_vjp_lds(const float* __restrict__ H, const float* __restrict__ len, int64_t P,
-                      int64_t k, float* __restrict__ dH) {
-  __shared__ double U[kVjpLdsDepth * kVjpLdsDepth];
-  __shared__ double d[kVjpLdsDepth], e[kVjpLdsDepth], lg[kVjpLdsDepth];
-  const int64_t p = blockIdx.x;
-  const double l2 = double(len[p]) * double(len[p]);
-  tridiag_vjp_wave(H + p * k * k, int(k), d, e, lg, U, l2 / double(P), dH + p * k * k);
-}
-
-__global__ void __launch_bounds__(64)
-    k_tridiag_vjp_global(const float* __restrict__ H, const float* __restrict__ len, int64_t P,
-                         int64_t k, double* __restrict__ work, float* __restrict__ dH) {
-  const int64_t p = blockIdx.x;
-  double* w = work + p * (k * k + 3 * k);
-  const double l2 = double(len[p]) * double(len[p]);
-  tridiag_vjp_wave(H + p * k * k, int(k), w + k * k, w + k * k + k, w + k * k + 2 * k, w,
-                   l2 / double(P), dH + p * k * k);
-}
 
 // ---------------------------------------------------------------------------
 // Coefficients of one backward step idx (decomp.py:592-599), one block per
@@ -482,21 +334,6 @@ hipError_t launch_outer_rbf_contract(hipStream_t s, const float* L, const float*
   hipLaunchKernelGGL(k_outer_sum<true>, dim3(tiles, tiles), dim3(256), 0, s, L, R, ldc, P, n, K,
                      nullptr, rc);
   return hipGetLastError();
-}
-
-hipError_t launch_tridiag_vjp(hipStream_t s, const float* H, const float* len, int64_t P, int64_t k,
-                              double* work, float* dH) {
-  if (P <= 0 || k <= 0) return hipSuccess;
-  if (k <= kVjpLdsDepth)
-    hipLaunchKernelGGL(k_tridiag_vjp_lds, dim3(unsigned(P)), dim3(64), 0, s, H, len, P, k, dH);
-  else
-    hipLaunchKernelGGL(k_tridiag_vjp_global, dim3(unsigned(P)), dim3(64), 0, s, H, len, P, k, work,
-                       dH);
-  return hipGetLastError();
-}
-
-int64_t tridiag_vjp_work_doubles(int64_t P, int64_t k) {
-  return k <= kVjpLdsDepth ? 0 : P * (k * k + 3 * k);
 }
 
 hipError_t launch_adjoint_coef(hipStream_t s, const float* H, const float* dH, float* Gam,

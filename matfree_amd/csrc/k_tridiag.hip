@@ -1,4 +1,6 @@
-// Per-probe tridiagonal eigen-solve and Gauss quadrature in FP64.
+// Per-probe tridiagonal eigen-solves in FP64: the Gauss quadrature of the
+// log-determinant integrand, the eigen cotangent of the reverse pass and the
+// small part of funm_lanczos_sym.
 //
 // Replaces, for every probe, funm.dense_funm_sym_eigh(log) applied to the
 // materialised tridiagonal matrix (funm.py:330-333, decomp.py:133-152) and the
@@ -26,6 +28,18 @@
 // The tridiagonal is extracted from the Arnoldi Hessenberg matrix exactly as
 // decomp.py:133-135 does it, in FP32: alpha_i = H_ii and
 // beta_i = 0.5 * (H_{i,i+1} + H_{i+1,i}); only then is it promoted to FP64.
+//
+// Reverse pass (k_tridiag_vjp): the full eigendecomposition T = U diag(theta)
+// U^T (implicit QL, eigenvectors accumulated) and the cotangent of
+// [log T]_00 in the Daleckii-Krein form Tbar = U (L o u u^T) U^T with
+// u = U[0, :] and L the divided differences of log at the eigenvalues --
+// what the eigh VJP composed with fA[0, 0] evaluates to (funm.py:331-333).
+// Through alpha_i = H_ii and beta_i = (H_{i,i+1} + H_{i+1,i}) / 2 this gives
+// dH_ii = Tbar_ii and dH_{i,i+1} = dH_{i+1,i} = Tbar_{i,i+1}, scaled by
+// |z|^2 / P.
+//
+// funm_lanczos_sym (k_tridiag_funm): f(T) e_1 = U f(theta) U^T e_1 from the
+// same eigendecomposition (funm.py:145-147, dense_funm_sym_eigh).
 #include <cfloat>
 
 #include "slq_internal.h"
@@ -36,6 +50,7 @@ namespace {
 // Depths up to this keep (a, |b|, b^2) in LDS (at most 3 * 8 * 2048 = 48 KB).
 constexpr int kLdsDepth = 2048;
 constexpr int kMaxBisect = 256;
+constexpr int kMaxSweeps = 64;
 
 __device__ __forceinline__ double wave_sum_f64(double v) {
 #pragma unroll
@@ -182,6 +197,237 @@ __global__ void __launch_bounds__(64)
                   beta ? beta + p * (k - 1) : nullptr, double(scale[p]), per_probe + p);
 }
 
+// ---------------------------------------------------------------------------
+// Full eigendecomposition of T (from H as above) by implicit QL with the
+// eigenvectors accumulated.  One wave per probe; (d, e) are k doubles, U is
+// k x k row-major (U[a * k + j] = component a of eigenvector j), in LDS for
+// k <= kEigLdsDepth and in a global workspace beyond.  The QL recurrence is
+// evaluated redundantly by every lane (uniform control flow); the rotations
+// of U are spread over the lanes by rows.  Returns false if QL did not
+// converge.  On return d holds the eigenvalues (unordered).
+// ---------------------------------------------------------------------------
+constexpr int kEigLdsDepth = 64;
+
+__device__ bool tridiag_eigvecs_wave(const float* __restrict__ Hp, int k, double* d, double* e,
+                                     double* U) {
+  const int lane = threadIdx.x;
+  for (int i = lane; i < k; i += 64) {
+    d[i] = double(Hp[int64_t(i) * k + i]);
+    e[i] = (i + 1 < k) ? double(0.5f * (Hp[int64_t(i + 1) * k + i] + Hp[int64_t(i) * k + i + 1]))
+                       : 0.0;
+  }
+  for (int a = lane; a < k; a += 64)
+    for (int j = 0; j < k; ++j) U[int64_t(a) * k + j] = (a == j) ? 1.0 : 0.0;
+  __syncthreads();
+
+  bool failed = false;
+  for (int l = 0; l < k && !failed; ++l) {
+    int sweeps = 0;
+    while (true) {
+      int m = l;
+      for (; m + 1 < k; ++m) {
+        const double dd = fabs(d[m]) + fabs(d[m + 1]);
+        if (fabs(e[m]) <= DBL_EPSILON * dd) break;
+      }
+      if (m == l) break;
+      if (++sweeps > kMaxSweeps) {
+        failed = true;
+        break;
+      }
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + copysign(r, g));
+      double s = 1.0, c = 1.0, shift = 0.0;
+      bool split = false;
+      for (int i = m - 1; i >= l; --i) {
+        const double f = s * e[i];
+        const double b = c * e[i];
+        r = hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {  // an off-diagonal underflowed: the matrix split
+          d[i + 1] -= shift;
+          e[m] = 0.0;
+          split = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - shift;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        shift = s * r;
+        d[i + 1] = g + shift;
+        g = c * r - b;
+        for (int a = lane; a < k; a += 64) {
+          double* ua = U + int64_t(a) * k;
+          const double ui = ua[i], ui1 = ua[i + 1];
+          ua[i + 1] = s * ui + c * ui1;
+          ua[i] = c * ui - s * ui1;
+        }
+      }
+      if (split) continue;
+      d[l] -= shift;
+      e[l] = g;
+      e[m] = 0.0;
+    }
+  }
+  __syncthreads();
+  return !failed;
+}
+
+// Cotangent of |z|^2 [log T]_00 / P w.r.t. H (see the file header), one wave
+// per probe; lg: k doubles of scratch.
+__device__ void tridiag_vjp_wave(const float* __restrict__ Hp, int k, double* d, double* e,
+                                 double* lg, double* U, double scale, float* __restrict__ dHp) {
+  const int lane = threadIdx.x;
+  const bool failed = !tridiag_eigvecs_wave(Hp, k, d, e, U);
+  // log of the eigenvalues; a non-positive eigenvalue makes the reference's
+  // log, and so its cotangent, NaN
+  bool nonpositive = failed;
+  for (int j = 0; j < k; ++j) nonpositive |= !(d[j] > 0.0);
+  for (int j = lane; j < k; j += 64) lg[j] = log(d[j]);
+  __syncthreads();
+
+  // Tbar = U W U^T, W_jl = u_j u_l L_jl; lane a forms the diagonal entry a
+  // and the superdiagonal entry (a, a + 1):
+  //   v_l = sum_j W_jl U_aj,  Tbar_aa = sum_l U_al v_l,
+  //   Tbar_{a,a+1} = sum_l U_{a+1,l} v_l.
+  for (int a = lane; a < k; a += 64) {
+    double tdd = 0.0, tdo = 0.0;
+    const double* ua = U + int64_t(a) * k;
+    const double* ub = U + int64_t(a + 1 < k ? a + 1 : a) * k;
+    for (int l = 0; l < k; ++l) {
+      const double ul = U[l], thl = d[l];
+      double v = 0.0;
+      for (int j = 0; j < k; ++j) {
+        const double thj = d[j], del = thl - thj;
+        double L;
+        if (j == l || del == 0.0) {
+          L = 1.0 / thj;
+        } else if (fabs(del) < 1e-3 * fabs(thj)) {
+          L = log1p(del / thj) / del;  // (log thl - log thj) / (thl - thj), no cancellation
+        } else {
+          L = (lg[l] - lg[j]) / del;
+        }
+        v += U[j] * ul * L * ua[j];
+      }
+      tdd += ua[l] * v;
+      tdo += ub[l] * v;
+    }
+    const float nan = __builtin_nanf("");
+    dHp[int64_t(a) * k + a] = nonpositive ? nan : float(scale * tdd);
+    if (a + 1 < k) {
+      const float o = nonpositive ? nan : float(scale * tdo);
+      dHp[int64_t(a + 1) * k + a] = o;  // (a, a + 1)
+      dHp[int64_t(a) * k + a + 1] = o;  // (a + 1, a)
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64)
+    k_tridiag_vjp_lds(const float* __restrict__ H, const float* __restrict__ len, int64_t P,
+                      int64_t k, float* __restrict__ dH) {
+  __shared__ double U[kEigLdsDepth * kEigLdsDepth];
+  __shared__ double d[kEigLdsDepth], e[kEigLdsDepth], lg[kEigLdsDepth];
+  const int64_t p = blockIdx.x;
+  const double l2 = double(len[p]) * double(len[p]);
+  tridiag_vjp_wave(H + p * k * k, int(k), d, e, lg, U, l2 / double(P), dH + p * k * k);
+}
+
+__global__ void __launch_bounds__(64)
+    k_tridiag_vjp_global(const float* __restrict__ H, const float* __restrict__ len, int64_t P,
+                         int64_t k, double* __restrict__ work, float* __restrict__ dH) {
+  const int64_t p = blockIdx.x;
+  double* w = work + p * (k * k + 3 * k);
+  const double l2 = double(len[p]) * double(len[p]);
+  tridiag_vjp_wave(H + p * k * k, int(k), w + k * k, w + k * k + k, w + k * k + 2 * k, w,
+                   l2 / double(P), dH + p * k * k);
+}
+
+
+// f(theta) for the matrix functions of slq_funm_vectors (slq_matfun).
+__device__ __forceinline__ double apply_matfun(int f, double x) {
+  switch (f) {
+    case SLQ_FUNM_LOG: return log(x);
+    case SLQ_FUNM_EXP: return exp(x);
+    case SLQ_FUNM_SIN: return sin(x);
+    case SLQ_FUNM_COS: return cos(x);
+    case SLQ_FUNM_SQRT: return sqrt(x);
+    default: return 1.0 / x;  // SLQ_FUNM_INV
+  }
+}
+
+// funm_lanczos_sym's small part (funm.py:145-147): y = f(T) e_1 =
+// U f(theta) U^T e_1, written as the combination coefficients of
+// |v| Q^T y for k_adjoint_combine (per probe 2 k + 4 floats: y, zeros, and
+// the scale |v| = len).  NaN coefficients if QL fails.
+__device__ void tridiag_funm_wave(const float* __restrict__ Hp, int k, double* d, double* e,
+                                  double* U, int f, float scale, float* __restrict__ cp) {
+  const int lane = threadIdx.x;
+  const bool ok = tridiag_eigvecs_wave(Hp, k, d, e, U);
+  for (int a = lane; a < k; a += 64) {
+    double y = 0.0;
+    const double* ua = U + int64_t(a) * k;
+    for (int j = 0; j < k; ++j) y += ua[j] * apply_matfun(f, d[j]) * U[j];
+    cp[a] = ok ? float(y) : __builtin_nanf("");
+    cp[k + a] = 0.f;
+  }
+  if (lane == 0) {
+    cp[2 * k + 0] = 0.f;
+    cp[2 * k + 1] = 0.f;
+    cp[2 * k + 2] = 0.f;
+    cp[2 * k + 3] = scale;
+  }
+}
+
+__global__ void __launch_bounds__(64)
+    k_tridiag_funm_lds(const float* __restrict__ H, const float* __restrict__ len, int64_t k, int f,
+                       float* __restrict__ coef) {
+  __shared__ double U[kEigLdsDepth * kEigLdsDepth];
+  __shared__ double d[kEigLdsDepth], e[kEigLdsDepth];
+  const int64_t p = blockIdx.x;
+  tridiag_funm_wave(H + p * k * k, int(k), d, e, U, f, len[p], coef + p * (2 * k + 4));
+}
+
+__global__ void __launch_bounds__(64)
+    k_tridiag_funm_global(const float* __restrict__ H, const float* __restrict__ len, int64_t k,
+                          int f, double* __restrict__ work, float* __restrict__ coef) {
+  const int64_t p = blockIdx.x;
+  double* w = work + p * (k * k + 3 * k);
+  tridiag_funm_wave(H + p * k * k, int(k), w + k * k, w + k * k + k, w, f, len[p],
+                    coef + p * (2 * k + 4));
+}
+
+// mean and standard error of the mean (population std / sqrt(P)) in FP64,
+// one block, fixed order (stochtrace.py:84-88).
+__global__ void __launch_bounds__(256)
+    k_mean_and_sem(const double* __restrict__ v, int64_t P, double* __restrict__ mean,
+                   double* __restrict__ sem) {
+  __shared__ double red[256];
+  __shared__ double mu;
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < P; i += 256) s += v[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) mu = red[0] / double(P);
+  __syncthreads();
+  double q = 0.0;
+  for (int64_t i = threadIdx.x; i < P; i += 256) q += (v[i] - mu) * (v[i] - mu);
+  red[threadIdx.x] = q;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (mean) *mean = mu;
+    if (sem) *sem = sqrt(red[0] / double(P)) / sqrt(double(P));
+  }
+}
+
 // Probe mean in a fixed (sequential) order: deterministic for a given P.
 __global__ void k_probe_mean(const double* __restrict__ per_probe, int64_t P,
                              double* __restrict__ mean) {
@@ -209,6 +455,38 @@ hipError_t launch_tridiag_quadrature(hipStream_t s, const float* H, const float*
     if (err != hipSuccess) return err;
   }
   if (mean) hipLaunchKernelGGL(k_probe_mean, dim3(1), dim3(64), 0, s, per_probe, P, mean);
+  return hipGetLastError();
+}
+
+hipError_t launch_tridiag_vjp(hipStream_t s, const float* H, const float* len, int64_t P, int64_t k,
+                              double* work, float* dH) {
+  if (P <= 0 || k <= 0) return hipSuccess;
+  if (k <= kEigLdsDepth)
+    hipLaunchKernelGGL(k_tridiag_vjp_lds, dim3(unsigned(P)), dim3(64), 0, s, H, len, P, k, dH);
+  else
+    hipLaunchKernelGGL(k_tridiag_vjp_global, dim3(unsigned(P)), dim3(64), 0, s, H, len, P, k, work,
+                       dH);
+  return hipGetLastError();
+}
+
+int64_t tridiag_vjp_work_doubles(int64_t P, int64_t k) {
+  return k <= kEigLdsDepth ? 0 : P * (k * k + 3 * k);
+}
+
+hipError_t launch_tridiag_funm(hipStream_t s, const float* H, const float* len, int64_t P,
+                               int64_t k, int f, double* work, float* coef) {
+  if (P <= 0 || k <= 0) return hipSuccess;
+  if (k <= kEigLdsDepth)
+    hipLaunchKernelGGL(k_tridiag_funm_lds, dim3(unsigned(P)), dim3(64), 0, s, H, len, k, f, coef);
+  else
+    hipLaunchKernelGGL(k_tridiag_funm_global, dim3(unsigned(P)), dim3(64), 0, s, H, len, k, f,
+                       work, coef);
+  return hipGetLastError();
+}
+
+hipError_t launch_mean_and_sem(hipStream_t s, const double* v, int64_t P, double* mean,
+                               double* sem) {
+  hipLaunchKernelGGL(k_mean_and_sem, dim3(1), dim3(256), 0, s, v, P, mean, sem);
   return hipGetLastError();
 }
 

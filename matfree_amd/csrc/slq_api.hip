@@ -445,6 +445,54 @@ int slq_logdet(slq_context* ctx, const slq_operator* op, const float* Z, int64_t
   return logdet_forward(ctx, *op, Z, P, k, per_probe, mean, alpha, beta, st);
 }
 
+int slq_funm_vectors(slq_context* ctx, const slq_operator* op, const float* V, int64_t P,
+                     int64_t k, int matfun, float* out) {
+  if (!ctx) return SLQ_EINVAL;
+  SLQ_TRY(check_operator(ctx, op));
+  const int64_t n = op->n;
+  if (k < 0 || k > n) return fail(ctx, SLQ_EINVAL_K, exceeds_msg(k, n));
+  if (k == 0) return fail(ctx, SLQ_EINVAL, "funm_lanczos_sym needs num_matvecs >= 1");
+  if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of vectors P must be positive");
+  if (!V || !out) return fail(ctx, SLQ_EINVAL, "input or output pointer is NULL");
+  if (matfun < SLQ_FUNM_LOG || matfun > SLQ_FUNM_INV)
+    return fail(ctx, SLQ_EUNSUPPORTED, "matrix function " + std::to_string(matfun) +
+                                           " is not available");
+  SLQ_TRY(set_device(ctx));
+  hipStream_t s = ctx->stream;
+  float *v0, *Q, *H, *R, *c, *ell, *norm2, *coef;
+  double* work = nullptr;
+  const int64_t ldc = basis_stride(P, n);
+  SLQ_TRY(ensure_t(ctx, ctx->vec0, size_t(P * n), &v0));
+  SLQ_TRY(ensure_t(ctx, ctx->basis, size_t(k * ldc), &Q));
+  SLQ_TRY(ensure_t(ctx, ctx->hmat, size_t(P * k * k), &H));
+  SLQ_TRY(ensure_t(ctx, ctx->resid, size_t(P * n), &R));
+  SLQ_TRY(ensure_t(ctx, ctx->scal1, size_t(P), &c));
+  SLQ_TRY(ensure_t(ctx, ctx->scal2, size_t(P), &ell));
+  SLQ_TRY(ensure_t(ctx, ctx->norm2, size_t(P * num_chunks(n)), &norm2));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_coef, size_t(P * (2 * k + 4)), &coef));
+  const int64_t vw = tridiag_vjp_work_doubles(P, k);
+  if (vw > 0) SLQ_TRY(ensure_t(ctx, ctx->adj_work, size_t(vw), &work));
+  // length = |vec|; vec / length (funm.py:138-139); tridiag_sym (:144)
+  SLQ_HIP(ctx, launch_dots(s, nullptr, 0, V, P, n, 1, norm2, 1));
+  SLQ_HIP(ctx, launch_normalize(s, V, v0, P, n, norm2, ell, nullptr, 0, nullptr));
+  SLQ_TRY(hessenberg_core(ctx, *op, v0, P, k, Q, ldc, H, R, c));
+  // length * Q^T (f(T) e_1) (:145-147)
+  SLQ_HIP(ctx, launch_tridiag_funm(s, H, ell, P, k, matfun, work, coef));
+  SLQ_HIP(ctx, launch_adjoint_combine(s, Q, nullptr, ldc, nullptr, nullptr, nullptr, out,
+                                      nullptr, P, n, k, int(k), int(k), coef));
+  return SLQ_OK;
+}
+
+int slq_mean_and_sem(slq_context* ctx, const double* values, int64_t P, double* mean,
+                     double* sem) {
+  if (!ctx) return SLQ_EINVAL;
+  if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of values P must be positive");
+  if (!values) return fail(ctx, SLQ_EINVAL, "values pointer is NULL");
+  SLQ_TRY(set_device(ctx));
+  SLQ_HIP(ctx, launch_mean_and_sem(ctx->stream, values, P, mean, sem));
+  return SLQ_OK;
+}
+
 int slq_logdet_vjp(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P, int64_t k,
                    double* per_probe, double* mean, double* dparams, float* dmatrix) {
   if (!ctx) return SLQ_EINVAL;

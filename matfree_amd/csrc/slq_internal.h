@@ -74,13 +74,21 @@ hipError_t launch_tridiag_quadrature(hipStream_t s, const float* H, const float*
                                      int64_t P, int64_t k, double* work, double* per_probe,
                                      double* mean, float* alpha, float* beta);
 
-// ---- reverse pass (k_adjoint.hip) ------------------------------------------
 // dH (P x k x k, column-major like H, written on the tridiagonal only; the
 // caller zeroes it) = cotangent of mean_p |z_p|^2 [log T_p]_00 w.r.t. H, with
 // len[p] = |z_p|.  work: tridiag_vjp_work_doubles(P, k) doubles (may be 0).
 hipError_t launch_tridiag_vjp(hipStream_t s, const float* H, const float* len, int64_t P, int64_t k,
                               double* work, float* dH);
 int64_t tridiag_vjp_work_doubles(int64_t P, int64_t k);
+// Combination coefficients (per probe 2 k + 4 floats, k_adjoint_combine's
+// layout) of |v| Q^T f(T) e_1 with |v| = len[p]; work as above.
+hipError_t launch_tridiag_funm(hipStream_t s, const float* H, const float* len, int64_t P,
+                               int64_t k, int f, double* work, float* coef);
+// mean / standard error of P doubles (one block, fixed order).
+hipError_t launch_mean_and_sem(hipStream_t s, const double* v, int64_t P, double* mean,
+                               double* sem);
+
+// ---- reverse pass (k_adjoint.hip) ------------------------------------------
 // One backward step idx: Gamma row idx and the combination coefficients
 // (coef, per probe 2 k + 4 floats) from the chunk partials of Q^T (A^T lambda).
 hipError_t launch_adjoint_coef(hipStream_t s, const float* H, const float* dH, float* Gam,

@@ -3,7 +3,9 @@
 * :func:`monte_carlo_funm_sym_logdet` (funm.py:186-202), also exported as
   :data:`integrand_funm_sym_logdet`;
 * :func:`monte_carlo_funm_sym` (funm.py:205-243) and
-  :func:`dense_funm_sym_eigh` (funm.py:322-335) for ``matfun = log``.
+  :func:`dense_funm_sym_eigh` (funm.py:322-335) for ``matfun = log``;
+* :func:`funm_lanczos_sym` (funm.py:114-147): ``f(A) v`` for log, exp, sin,
+  cos, sqrt and reciprocal.
 
 The integrand ``quadform(matvec, v0, *params) = |v0|^2 [f(T)]_00`` runs as one
 libslq call (``slq_logdet``): normalise, Arnoldi with full
@@ -16,9 +18,18 @@ import math
 import numpy as np
 import torch
 
-from . import _runtime, _tree, decomp, operators
+from . import _lib, _runtime, _tree, decomp, operators
 
-_LOG_FUNCTIONS = (np.log, math.log, torch.log)
+# The scalar functions the FP64 tridiagonal kernels apply to Ritz values
+# (NumPy, math and torch spellings of each).
+_MATFUN_CODES = [
+    ((np.log, math.log, torch.log), _lib.MATFUNS["log"]),
+    ((np.exp, math.exp, torch.exp), _lib.MATFUNS["exp"]),
+    ((np.sin, math.sin, torch.sin), _lib.MATFUNS["sin"]),
+    ((np.cos, math.cos, torch.cos), _lib.MATFUNS["cos"]),
+    ((np.sqrt, math.sqrt, torch.sqrt), _lib.MATFUNS["sqrt"]),
+    ((np.reciprocal, torch.reciprocal), _lib.MATFUNS["reciprocal"]),
+]
 
 
 class _DenseFunmSymEigh:
@@ -28,8 +39,16 @@ class _DenseFunmSymEigh:
         self.matfun = matfun
 
     @property
+    def code(self):
+        """The libslq code of ``matfun``, or None for a function it does not apply."""
+        for fns, code in _MATFUN_CODES:
+            if any(self.matfun is f for f in fns):
+                return code
+        return None
+
+    @property
     def is_log(self):
-        return any(self.matfun is f for f in _LOG_FUNCTIONS)
+        return self.code == _lib.MATFUNS["log"]
 
     def __call__(self, dense_matrix):
         raise NotImplementedError(
@@ -191,6 +210,53 @@ class _QuadFormSym:
             return _LogdetMean.apply(matvec, flat[None, :], self.num_matvecs, *parameters)
         per_probe, _ = logdet_per_probe(matvec, flat[None, :], self.num_matvecs, parameters)
         return per_probe[0]
+
+
+def funm_vectors(matvec, V, num_matvecs, params, matfun_code):
+    """``|v| Q^T f(T) e_1`` for the rows of ``V`` (P, n) via ``slq_funm_vectors``."""
+    op = operators.resolve(matvec)
+    P, n = tuple(V.shape)
+    k = int(num_matvecs)
+    if k < 0 or k > n:
+        raise ValueError(decomp._error_num_matvecs(k, maxval=n, minval=0))
+    device = _runtime.target_device(V, *params)
+    V = _runtime.as_f32(V, device).reshape(P, n)
+    desc, keep = op.descriptor(n, params, device)
+    out = torch.empty((P, n), dtype=torch.float32, device=device)
+    ctx = _runtime.context(device)
+    ctx.call("slq_funm_vectors", ctypes.byref(desc), _runtime.ptr(V), P, k, int(matfun_code),
+             _runtime.ptr(out))
+    del keep
+    return out
+
+
+class _FunmLanczosSym:
+    """``funm_lanczos_sym(dense_funm, tridiag_sym)``'s ``estimate`` (funm.py:114-147)."""
+
+    def __init__(self, dense_funm, tridiag_sym):
+        if not isinstance(dense_funm, _DenseFunmSymEigh) or dense_funm.code is None:
+            raise NotImplementedError(
+                "matfree_amd's funm_lanczos_sym applies dense_funm_sym_eigh with log, exp, sin, "
+                "cos, sqrt or reciprocal"
+            )
+        if not isinstance(tridiag_sym, decomp._TridiagSymFull):
+            raise TypeError("pass decomp.tridiag_sym(k) of matfree_amd (reortho='full')")
+        self.code = dense_funm.code
+        self.num_matvecs = tridiag_sym.num_matvecs
+
+    def batched(self, matvec, vecs, *parameters):
+        """``f(A) v`` for every row of ``vecs`` (P, n) in one call."""
+        return funm_vectors(matvec, vecs, self.num_matvecs, parameters, self.code)
+
+    def __call__(self, matvec, vec, *parameters):
+        flat, unravel = _tree.ravel(vec, decomp._host_or_gpu(vec, *parameters))
+        out = funm_vectors(matvec, flat[None, :], self.num_matvecs, parameters, self.code)
+        return unravel(out[0])
+
+
+def funm_lanczos_sym(dense_funm, tridiag_sym, /):
+    """Matrix-function-vector product via Lanczos' tridiagonalisation (funm.py:114-147)."""
+    return _FunmLanczosSym(dense_funm, tridiag_sym)
 
 
 def monte_carlo_funm_sym(dense_funm, tridiag_sym, /):

@@ -3,7 +3,9 @@
 * :func:`sampler_signs` (stochtrace.py:932-937): Rademacher probes generated
   by libslq's threefry kernel, bit-identical to ``jax.random.rademacher``;
 * :func:`estimator_monte_carlo` (stochtrace.py:7-52): probe mean of an
-  integrand, with every probe processed in one batched call.
+  integrand, with every probe processed in one batched call;
+* :func:`estimator_monte_carlo_mean_and_sem` (stochtrace.py:55-89): the mean
+  and its standard error.
 """
 
 import ctypes
@@ -55,6 +57,34 @@ class _SignSampler:
 def sampler_signs(*args_like, num):
     """Rademacher probes shaped like ``args_like`` (stochtrace.py:932-937, 957-964)."""
     return _SignSampler(args_like, num)
+
+
+def estimator_monte_carlo_mean_and_sem(integrand, /, sampler):
+    """``(mean, sem)`` of the integrand over the probes (stochtrace.py:55-89).
+
+    ``sem = std(samples) / sqrt(num_samples)`` (population standard deviation),
+    both reduced on the device in FP64.
+    """
+    if not hasattr(integrand, "batched"):
+        raise TypeError(
+            "matfree_amd's estimator runs integrands of matfree_amd.funm "
+            f"(e.g. monte_carlo_funm_sym_logdet), not {type(integrand).__name__}"
+        )
+
+    def estimate(matvecs, key, *parameters):
+        if isinstance(sampler, _SignSampler):
+            samples = sampler.flat(key, _runtime.target_device(*parameters))
+        else:
+            samples = sampler(key)
+        values = integrand.batched(matvecs, samples, *parameters).contiguous()
+        mean = torch.empty((), dtype=torch.float64, device=values.device)
+        sem = torch.empty((), dtype=torch.float64, device=values.device)
+        ctx = _runtime.context(values.device)
+        ctx.call("slq_mean_and_sem", _runtime.ptr(values), values.shape[0], _runtime.ptr(mean),
+                 _runtime.ptr(sem))
+        return mean, sem
+
+    return estimate
 
 
 def estimator_monte_carlo(integrand, /, sampler):

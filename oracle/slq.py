@@ -9,7 +9,9 @@ the reference uses ``vmap``):
   (decomp.py:125-152) -- ``T = (H + H^T) / 2`` and its two diagonals;
 * ``funm.monte_carlo_funm_sym`` / ``monte_carlo_funm_sym_logdet`` /
   ``dense_funm_sym_eigh`` (funm.py:186-243, 322-335);
-* ``stochtrace.estimator_monte_carlo`` (stochtrace.py:47-50);
+* ``stochtrace.estimator_monte_carlo`` (stochtrace.py:47-50) and
+  ``estimator_monte_carlo_mean_and_sem`` (stochtrace.py:55-89);
+* ``funm.funm_lanczos_sym`` with ``dense_funm_sym_eigh`` (funm.py:114-147);
 * ``decomp._hessenberg_adjoint`` / ``_hessenberg_adjoint_step``
   (decomp.py:480-600) and the eigh VJP that JAX applies at funm.py:331.
 
@@ -197,6 +199,34 @@ def quadform_logdet(operator, num_matvecs, Z, dtype=np.float32):
 def estimator_mean(values, dtype=np.float32):
     """``tree_map(lambda s: mean(s, axis=0), Qs)`` (stochtrace.py:50)."""
     return np.mean(np.asarray(values, dtype=dtype), axis=0, dtype=dtype)
+
+
+def estimator_mean_and_sem(values, dtype=np.float32):
+    """``estimator_monte_carlo_mean_and_sem`` reductions (stochtrace.py:84-88):
+    ``mean(s)`` and ``std(s) / sqrt(num_samples)`` (population std, ddof 0)."""
+    s = np.asarray(values, dtype=dtype)
+    return np.mean(s, axis=0, dtype=dtype), (np.std(s, axis=0, dtype=dtype)
+                                             / np.sqrt(s.shape[0])).astype(dtype)
+
+
+def dense_funm_sym_eigh(matfun, T):
+    """``dense_funm_sym_eigh(matfun)`` (funm.py:322-335) on a batch of (k, k)."""
+    w, U = np.linalg.eigh(T)  # :331
+    with np.errstate(invalid="ignore", divide="ignore"):
+        fw = matfun(w)  # :332
+    return np.matmul(U * fw[:, None, :], np.swapaxes(U, 1, 2))  # :333
+
+
+def funm_lanczos_sym(operator, num_matvecs, V, matfun, dtype=np.float32):
+    """``funm_lanczos_sym(dense_funm_sym_eigh(matfun), tridiag_sym(k))`` (funm.py:114-147)
+    for the rows of ``V`` (P, n): ``|v| Q^T f(T) e_1`` per row."""
+    V = np.asarray(V, dtype=dtype)
+    length = np.sqrt(np.sum(V * V, axis=1))  # :138
+    V0 = V / length[:, None]  # :139
+    Q, J, _, _ = tridiag_sym(operator, num_matvecs, V0, dtype=dtype)  # :144, Q (P, k, n)
+    fJ = dense_funm_sym_eigh(matfun, J)  # :145
+    y = fJ[:, :, 0]  # funm @ e1 (:146-147)
+    return length[:, None] * np.einsum("pkn,pk->pn", Q, y)  # :147 Q.T @ (funm @ e1)
 
 
 def slq_logdet(operator, num_matvecs, key, num_probes, dtype=np.float32):

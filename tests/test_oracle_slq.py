@@ -290,3 +290,47 @@ def test_planted_mistakes_are_caught(monkeypatch):
 
     monkeypatch.setattr(slq, "_hessenberg_step", shifted)
     assert relation_residual() > 1e-3
+
+
+# -- funm_lanczos_sym and mean/SEM (funm.py:114-147, stochtrace.py:55-89) -------
+
+
+@pytest.mark.parametrize("fun", [np.sin, np.exp, np.log])
+def test_funm_lanczos_full_depth_is_exact(fun):
+    # at k = n the Krylov space is everything: |v| Q^T f(T) e1 = f(A) v
+    n = 12
+    A = sym_from_eigs(np.linspace(0.2, 1.8, n), 3).astype(np.float64)
+    v = np.random.default_rng(4).standard_normal((2, n))
+    got = slq.funm_lanczos_sym(slq.DenseOperator(A, np.float64), n, v, fun, np.float64)
+    w, U = np.linalg.eigh(A)
+    expected = v @ (U * fun(w)) @ U.T
+    assert np.allclose(got, expected, atol=1e-10)
+
+
+def test_funm_lanczos_reference_configuration():
+    # tests/test_funm/test_funm_lanczos_sym.py:8-36: n = 11, eigenvalues in
+    # [0.01, 0.99], f = sin, 6 Lanczos steps, atol 1e-6 (float32 there)
+    n = 11
+    A = sym_from_eigs(np.linspace(0.01, 0.99, n), 1)
+    v = np.random.default_rng(2).standard_normal((1, n)).astype(np.float32)
+    got = slq.funm_lanczos_sym(slq.DenseOperator(A), 6, v, np.sin, np.float32)
+    w, U = np.linalg.eigh(A.astype(np.float64))
+    expected = v.astype(np.float64) @ (U * np.sin(w)) @ U.T
+    assert np.allclose(got, expected, atol=1e-6)
+
+
+def test_funm_lanczos_log_quadform_matches_integrand():
+    # v^T (|v| Q^T log(T) e1) = |v|^2 [log T]_00: the logdet integrand
+    n, k = 30, 8
+    A = sym_from_eigs(np.linspace(0.5, 2.0, n), 6).astype(np.float64)
+    Z = prng.rademacher(prng.prng_key(2), (3, n), np.float64)
+    fv = slq.funm_lanczos_sym(slq.DenseOperator(A, np.float64), k, Z, np.log, np.float64)
+    vals, _, _ = slq.quadform_logdet(slq.DenseOperator(A, np.float64), k, Z, np.float64)
+    assert np.allclose(np.sum(Z * fv, axis=1), vals, rtol=1e-12)
+
+
+def test_mean_and_sem_closed_form():
+    s = np.array([1.0, 2.0, 4.0, 7.0])
+    mean, sem = slq.estimator_mean_and_sem(s, np.float64)
+    assert mean == 3.5
+    assert np.isclose(sem, np.sqrt(np.mean((s - 3.5) ** 2)) / 2.0)
