@@ -136,6 +136,26 @@ int slq_logdet(slq_context* ctx, const slq_operator* op, const float* Z, int64_t
 int slq_logdet_vjp(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P, int64_t k,
                    double* per_probe, double* mean, double* dparams, float* dmatrix);
 
+/* Plain Lanczos (tridiag_sym(k, reortho="none"), decomp.py:155-266) for P
+ * start vectors: v0 = V0 / |V0|; per step a = q . A q, r = A q - a q - b q_prev,
+ * b = |r|, q_next = r / b -- no reorthogonalisation, one matvec per step.
+ *   Q:     k x P x n Lanczos vectors q_0 .. q_{k-1} (may be NULL: then only
+ *          three vectors per probe are kept);
+ *   alpha: P x k diagonal of T;
+ *   beta:  P x max(k, 1): the k - 1 off-diagonals of T, then the norm b of the
+ *          remainder (decomp.py:227-229);
+ *   R:     P x n residual b q_k (decomp.py:167);
+ *   c:     P values 1 / |V0| (decomp.py:171-178; unlike the "full" variant).
+ * k = 0 runs the initial step only (decomp.py:215-218).  Fails with
+ * SLQ_EINVAL_K unless 0 <= k <= n. */
+int slq_lanczos(slq_context* ctx, const slq_operator* op, const float* V0, int64_t P, int64_t k,
+                float* Q, float* alpha, float* beta, float* R, float* c);
+
+/* slq_logdet with tridiag_sym(k, reortho="none") (plain Lanczos, no basis
+ * kept: O(n) memory per probe); same outputs. */
+int slq_logdet_lanczos(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P,
+                       int64_t k, double* per_probe, double* mean, float* alpha, float* beta);
+
 /* Matrix functions of slq_funm_vectors (dense_funm_sym_eigh(matfun),
  * funm.py:322-335, for these matfun). */
 typedef enum slq_matfun {
@@ -149,12 +169,13 @@ typedef enum slq_matfun {
 
 /* Matrix-function-vector products f(A) v_p ~ |v_p| Q_p^T f(T_p) e_1 for P
  * vectors: funm.funm_lanczos_sym(dense_funm_sym_eigh(f), tridiag_sym(k))
- * (funm.py:114-147) with the forward pass of slq_hessenberg and f applied to
- * the FP64 eigendecomposition of each T_p.
+ * (funm.py:114-147) with the forward pass of slq_hessenberg (reortho_full
+ * != 0) or of slq_lanczos (reortho_full = 0) and f applied to the FP64
+ * eigendecomposition of each T_p.
  *   V: P x n vectors; out: P x n results (may alias neither V nor scratch).
  * Fails with SLQ_EINVAL_K unless 0 < k <= n. */
 int slq_funm_vectors(slq_context* ctx, const slq_operator* op, const float* V, int64_t P,
-                     int64_t k, int matfun, float* out);
+                     int64_t k, int reortho_full, int matfun, float* out);
 
 /* mean and standard error of the mean (population standard deviation over
  * sqrt(P)) of P device doubles, in FP64 and a fixed order:

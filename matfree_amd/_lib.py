@@ -73,9 +73,20 @@ SIGNATURES = {
         [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_i64, c_void_p, c_void_p,
          c_void_p, c_void_p],
     ),
+    "slq_lanczos": (
+        c_int,
+        [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_i64, c_void_p, c_void_p,
+         c_void_p, c_void_p, c_void_p],
+    ),
+    "slq_logdet_lanczos": (
+        c_int,
+        [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_i64, c_void_p, c_void_p,
+         c_void_p, c_void_p],
+    ),
     "slq_funm_vectors": (
         c_int,
-        [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_i64, c_int, c_void_p],
+        [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_i64, c_int, c_int,
+         c_void_p],
     ),
     "slq_mean_and_sem": (c_int, [c_void_p, c_void_p, c_i64, c_void_p, c_void_p]),
     "slq_profile_enable": (c_int, [c_void_p, c_int]),

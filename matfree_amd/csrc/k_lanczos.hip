@@ -18,6 +18,9 @@
 // which lane l holds the wave total of column l >> (6 - log2 NC).
 //
 // Basis layout: column c of probe p at Q + c * ldc + p * n (ldc >= P * n).
+//
+// Plain Lanczos (reortho="none", decomp.py:198-266) uses the single-column
+// dots, k_three_term and k_normalize: no basis is read.
 #include "slq_internal.h"
 
 namespace slq {
@@ -362,6 +365,55 @@ __global__ void __launch_bounds__(kStreamThreads)
 }
 
 // ---------------------------------------------------------------------------
+// three-term step of plain Lanczos (decomp.py:254-260 / 243-245):
+//   a = sum_b apart[p][b] (= q . A q), r = A q - a q - b_prev q_prev,
+// plus the norm^2 partials of r; a is written to aout[p * astride] (chunk 0).
+// qm (and b_prev) may be null for the first step.
+// ---------------------------------------------------------------------------
+template <int V>
+__global__ void __launch_bounds__(kStreamThreads)
+    k_three_term(const float* __restrict__ w, const float* __restrict__ q,
+                 const float* __restrict__ qm, int64_t n, const float* __restrict__ apart,
+                 const float* __restrict__ bprev, int64_t bstride, float* __restrict__ r,
+                 float* __restrict__ norm2part, float* __restrict__ aout, int64_t astride) {
+  constexpr int NE = Layout<V>::NE;
+  __shared__ float red[kWaves];
+  const int64_t p = blockIdx.y, b = blockIdx.x, nb = gridDim.x;
+  const int t = threadIdx.x;
+  const int64_t base = b * kChunk;
+  float a = 0.f;
+  for (int64_t bb = 0; bb < nb; ++bb) a += apart[p * nb + bb];
+  const float bm = qm ? bprev[p * bstride] : 0.f;
+  float acc = 0.f;
+#pragma unroll
+  for (int e = 0; e < NE; ++e) {
+    const int64_t i = base + (int64_t(e) * kStreamThreads + t) * V;
+    if (i < n) {
+      float wv[V], qv[V], mv[V];
+      VecIO<V>::load(w + p * n + i, wv);
+      VecIO<V>::load(q + p * n + i, qv);
+      if (qm) {
+        VecIO<V>::load(qm + p * n + i, mv);
+      } else {
+#pragma unroll
+        for (int u = 0; u < V; ++u) mv[u] = 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < V; ++u) {
+        wv[u] = (wv[u] - a * qv[u]) - bm * mv[u];
+        acc = fmaf(wv[u], wv[u], acc);
+      }
+      VecIO<V>::store(r + p * n + i, wv);
+    }
+  }
+  const float tot = block_sum(acc, red);
+  if (t == 0) {
+    norm2part[p * nb + b] = tot;
+    if (b == 0 && aout) aout[p * astride] = a;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // normalize: out = x / sqrt(sum_b norm2part), plus bookkeeping of the norm
 // ---------------------------------------------------------------------------
 constexpr int kNormVec = 2;  // float4 groups per thread
@@ -370,7 +422,8 @@ template <int V>
 __global__ void __launch_bounds__(kStreamThreads)
     k_normalize(const float* __restrict__ x, float* __restrict__ out, int64_t n, int64_t nbn,
                 const float* __restrict__ norm2part, float* __restrict__ norm_out,
-                float* __restrict__ hslot, int64_t hslot_stride, float* __restrict__ inv_out) {
+                float* __restrict__ hslot, int64_t hslot_stride, float* __restrict__ hslot2,
+                float* __restrict__ inv_out) {
   __shared__ float nrm;
   const int64_t p = blockIdx.y;
   const int t = threadIdx.x;
@@ -382,6 +435,7 @@ __global__ void __launch_bounds__(kStreamThreads)
     if (blockIdx.x == 0) {
       if (norm_out) norm_out[p] = len;
       if (hslot) hslot[p * hslot_stride] = len;
+      if (hslot2) hslot2[p * hslot_stride] = len;
       if (inv_out) inv_out[p] = 1.0f / len;
     }
   }
@@ -478,9 +532,56 @@ hipError_t launch_project_dots(hipStream_t s, const float* Q, int64_t ldc, const
   return hipGetLastError();
 }
 
+hipError_t launch_three_term(hipStream_t s, const float* w, const float* q, const float* qm,
+                             int64_t P, int64_t n, const float* apart, const float* bprev,
+                             int64_t bstride, float* r, float* norm2part, float* aout,
+                             int64_t astride) {
+  const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
+  const bool vec = (n % 4 == 0) && aligned16(w) && aligned16(q) && aligned16(r) &&
+                   (!qm || aligned16(qm));
+  if (vec)
+    hipLaunchKernelGGL(k_three_term<4>, grid, dim3(kStreamThreads), 0, s, w, q, qm, n, apart,
+                       bprev, bstride, r, norm2part, aout, astride);
+  else
+    hipLaunchKernelGGL(k_three_term<1>, grid, dim3(kStreamThreads), 0, s, w, q, qm, n, apart,
+                       bprev, bstride, r, norm2part, aout, astride);
+  return hipGetLastError();
+}
+
+// out[p][i] = scale[p] * x[p][i]
+template <int V>
+__global__ void __launch_bounds__(kStreamThreads)
+    k_scale_rows(const float* __restrict__ x, const float* __restrict__ scale, int64_t sstride,
+                 int64_t n, float* __restrict__ out) {
+  const int64_t p = blockIdx.y;
+  const int64_t i = (int64_t(blockIdx.x) * kStreamThreads + threadIdx.x) * V;
+  if (i >= n) return;
+  const float sc = scale[p * sstride];
+  float v[V];
+  VecIO<V>::load(x + p * n + i, v);
+#pragma unroll
+  for (int u = 0; u < V; ++u) v[u] = sc * v[u];
+  VecIO<V>::store(out + p * n + i, v);
+}
+
+hipError_t launch_scale_rows(hipStream_t s, const float* x, const float* scale, int64_t sstride,
+                             int64_t P, int64_t n, float* out) {
+  const bool vec = (n % 4 == 0) && aligned16(x) && aligned16(out);
+  const int V = vec ? 4 : 1;
+  const int64_t per_block = int64_t(kStreamThreads) * V;
+  const dim3 grid(unsigned((n + per_block - 1) / per_block), unsigned(P));
+  if (vec)
+    hipLaunchKernelGGL(k_scale_rows<4>, grid, dim3(kStreamThreads), 0, s, x, scale, sstride, n,
+                       out);
+  else
+    hipLaunchKernelGGL(k_scale_rows<1>, grid, dim3(kStreamThreads), 0, s, x, scale, sstride, n,
+                       out);
+  return hipGetLastError();
+}
+
 hipError_t launch_normalize(hipStream_t s, const float* x, float* out, int64_t P, int64_t n,
                             const float* norm2part, float* norm_out, float* hslot,
-                            int64_t hslot_stride, float* inv_out) {
+                            int64_t hslot_stride, float* inv_out, float* hslot2) {
   const bool vec = (n % 4 == 0) && aligned16(x) && aligned16(out);
   const int V = vec ? 4 : 1;
   const int64_t per_block = int64_t(kStreamThreads) * V * kNormVec;
@@ -488,10 +589,10 @@ hipError_t launch_normalize(hipStream_t s, const float* x, float* out, int64_t P
   const int64_t nbn = num_chunks(n);
   if (vec)
     hipLaunchKernelGGL(k_normalize<4>, grid, dim3(kStreamThreads), 0, s, x, out, n, nbn, norm2part,
-                       norm_out, hslot, hslot_stride, inv_out);
+                       norm_out, hslot, hslot_stride, hslot2, inv_out);
   else
     hipLaunchKernelGGL(k_normalize<1>, grid, dim3(kStreamThreads), 0, s, x, out, n, nbn, norm2part,
-                       norm_out, hslot, hslot_stride, inv_out);
+                       norm_out, hslot, hslot_stride, hslot2, inv_out);
   return hipGetLastError();
 }
 

@@ -11,14 +11,23 @@
 // Algorithm (one wave per probe, one eigenvalue per lane):
 //  * theta_j by bisection on the Sturm count of T - x I (the number of
 //    negative pivots of its LDL^T factorisation), to full FP64 resolution;
-//  * the weight U_0j^2 from the residue of the resolvent entry
-//    g(x) = [(T - x I)^{-1}]_00 = sum_j U_0j^2 / (theta_j - x).  Its
-//    reciprocal r_0(x) is the bottom-up continued fraction
-//      r_{m-1} = a_{m-1} - x,   r_i = a_i - x - b_i^2 / r_{i+1},
-//    and U_0j^2 = -1 / r_0'(theta_j) with
-//      r_i' = -1 + b_i^2 r_{i+1}' / r_{i+1}^2.
-//    Every term of r_i' has the same sign (r_i' <= -1), so the recurrence
-//    has no cancellation.
+//  * the weight U_0j^2 from the twisted factorisation of T - theta_j I
+//    (Parlett and Dhillon, "Fernando's solution to Wilkinson's problem",
+//    1997): the stationary pivots D+_i (top-down LDL^T) and D-_i (bottom-up
+//    UDU^T), the twist r = argmin_i |gamma_i| with
+//    gamma_i = D+_i + D-_i - (a_i - theta), and the eigenvector z with
+//    z_r = 1, z_i = -(b_i / D+_i) z_{i+1} (i < r),
+//    z_{i+1} = -(b_i / D-_{i+1}) z_i (i >= r); then U_0j^2 = z_0^2 / |z|^2,
+//    accumulated as ratios so that no component is formed.  The plain residue
+//    of the resolvent entry [(T - x I)^{-1}]_00 is the special case r = 0 and
+//    is useless when U_0j is tiny, which is what the copies of converged
+//    Ritz values in a Lanczos tridiagonal without reorthogonalisation have;
+//    at r = argmin |gamma| the vector is accurate to O(eps |T| / gap).
+//  * If the weights do not sum to one to kWeightSumTol (eigenvalues that
+//    coincide to working precision, where the twisted vectors of a cluster
+//    are not orthogonal), the wave discards them and takes U_0j^2 from
+//    implicit QL with the first row of U accumulated (orthogonal by
+//    construction).
 // Eigenvalues outside the leading unreduced block (below the first
 // negligible off-diagonal, |b_m| <= eps (|a_m| + |a_{m+1}|)) have U_0j = 0 and
 // do not contribute; the solver works on that block.  A non-positive Ritz
@@ -47,10 +56,13 @@
 namespace slq {
 namespace {
 
-// Depths up to this keep (a, |b|, b^2) in LDS (at most 3 * 8 * 2048 = 48 KB).
-constexpr int kLdsDepth = 2048;
+// Depths up to this keep (a, |b|, b^2) and the 64 lanes' bottom-up pivots in
+// LDS (8 * (3 + 64) * 64 = 33.5 KB).
+constexpr int kQuadLdsDepth = 64;
 constexpr int kMaxBisect = 256;
 constexpr int kMaxSweeps = 64;
+// Twisted weights summing to one within this are accepted.
+constexpr double kWeightSumTol = 1e-9;
 
 __device__ __forceinline__ double wave_sum_f64(double v) {
 #pragma unroll
@@ -91,26 +103,136 @@ __device__ __forceinline__ int sturm_count(const double* __restrict__ a,
   return cnt;
 }
 
-// U_0j^2 = -1 / r_0'(theta) for the leading m x m block.
-__device__ __forceinline__ double first_component_sq(const double* __restrict__ a,
-                                                     const double* __restrict__ b2, int m,
-                                                     double theta, double pivmin) {
-  double r = a[m - 1] - theta;
-  double rp = -1.0;
+// U_0j^2 of the eigenvalue theta of the leading m x m block from the twisted
+// factorisation (see the file header).  dm: this lane's column of m doubles
+// at stride 64 (the bottom-up pivots, kept for the twist search).  Running
+// ratios: sp_i = sum_{l<=i} (z_l / z_i)^2, qq_i = (z_0 / z_i)^2 / sp_i (in
+// [0, 1], so that it cannot overflow) and sm_r = sum_{l>=r} (z_l / z_r)^2;
+// then z_0^2 / |z|^2 = qq_r / (1 + (sm_r - 1) / sp_r).
+__device__ __forceinline__ double twisted_weight(const double* __restrict__ a,
+                                                 const double* __restrict__ b2, int m,
+                                                 double theta, double pivmin, double* dm) {
+  double d = a[m - 1] - theta;
+  if (fabs(d) <= pivmin) d = -pivmin;
+  dm[(m - 1) * 64] = d;
   for (int i = m - 2; i >= 0; --i) {
-    if (fabs(r) <= pivmin) r = -pivmin;
-    const double t = 1.0 / r;
-    const double bt = b2[i] * t;
-    rp = -1.0 + bt * t * rp;
-    r = (a[i] - theta) - bt;
+    d = (a[i] - theta) - b2[i] / d;
+    if (fabs(d) <= pivmin) d = -pivmin;
+    dm[i * 64] = d;
   }
-  return -1.0 / rp;
+  double dp = a[0] - theta;
+  if (fabs(dp) <= pivmin) dp = -pivmin;
+  double sp = 1.0, qq = 1.0;
+  double best = fabs(dp + dm[0] - (a[0] - theta));
+  int r = 0;
+  double sp_r = 1.0, qq_r = 1.0;
+  for (int i = 1; i < m; ++i) {
+    const double inv = 1.0 / dp;
+    const double bt = b2[i - 1] * inv;  // b_{i-1}^2 / D+_{i-1}
+    const double x = bt * inv * sp;     // (z_{i-1} / z_i)^2 sp_{i-1}
+    qq *= x / (1.0 + x);
+    sp = 1.0 + x;
+    dp = (a[i] - theta) - bt;
+    if (fabs(dp) <= pivmin) dp = -pivmin;
+    const double g = fabs(dp + dm[i * 64] - (a[i] - theta));
+    if (g < best) {
+      best = g;
+      r = i;
+      sp_r = sp;
+      qq_r = qq;
+    }
+  }
+  double sm = 1.0;
+  for (int i = m - 2; i >= r; --i) {
+    const double inv = 1.0 / dm[(i + 1) * 64];
+    sm = 1.0 + b2[i] * inv * inv * sm;
+  }
+  return qq_r / (1.0 + (sm - 1.0) / sp_r);
 }
 
-// One wave per probe.  (a, ab, b2) point at 3 arrays of k doubles (LDS or
-// global workspace).
+// ---------------------------------------------------------------------------
+// Full eigendecomposition of T (from H as above) by implicit QL with the
+// eigenvectors accumulated.  One wave per probe; (d, e) are k doubles, U is
+// rows x k row-major (U[a * k + j] = component a of eigenvector j; rows = k
+// for the whole matrix, 1 for its first row only), in LDS for
+// k <= kEigLdsDepth and in a global workspace beyond.  The QL recurrence is
+// evaluated redundantly by every lane (uniform control flow); the rotations
+// of U are spread over the lanes by rows.  Returns false if QL did not
+// converge.  On return d holds the eigenvalues (unordered).
+// ---------------------------------------------------------------------------
+constexpr int kEigLdsDepth = 64;
+
+__device__ bool tridiag_eigvecs_wave(const float* __restrict__ Hp, int k, double* d, double* e,
+                                     double* U, int rows) {
+  const int lane = threadIdx.x;
+  for (int i = lane; i < k; i += 64) {
+    d[i] = double(Hp[int64_t(i) * k + i]);
+    e[i] = (i + 1 < k) ? double(0.5f * (Hp[int64_t(i + 1) * k + i] + Hp[int64_t(i) * k + i + 1]))
+                       : 0.0;
+  }
+  for (int a = lane; a < rows; a += 64)
+    for (int j = 0; j < k; ++j) U[int64_t(a) * k + j] = (a == j) ? 1.0 : 0.0;
+  __syncthreads();
+
+  bool failed = false;
+  for (int l = 0; l < k && !failed; ++l) {
+    int sweeps = 0;
+    while (true) {
+      int m = l;
+      for (; m + 1 < k; ++m) {
+        const double dd = fabs(d[m]) + fabs(d[m + 1]);
+        if (fabs(e[m]) <= DBL_EPSILON * dd) break;
+      }
+      if (m == l) break;
+      if (++sweeps > kMaxSweeps) {
+        failed = true;
+        break;
+      }
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + copysign(r, g));
+      double s = 1.0, c = 1.0, shift = 0.0;
+      bool split = false;
+      for (int i = m - 1; i >= l; --i) {
+        const double f = s * e[i];
+        const double b = c * e[i];
+        r = hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {  // an off-diagonal underflowed: the matrix split
+          d[i + 1] -= shift;
+          e[m] = 0.0;
+          split = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - shift;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        shift = s * r;
+        d[i + 1] = g + shift;
+        g = c * r - b;
+        for (int a = lane; a < rows; a += 64) {
+          double* ua = U + int64_t(a) * k;
+          const double ui = ua[i], ui1 = ua[i + 1];
+          ua[i + 1] = s * ui + c * ui1;
+          ua[i] = c * ui - s * ui1;
+        }
+      }
+      if (split) continue;
+      d[l] -= shift;
+      e[l] = g;
+      e[m] = 0.0;
+    }
+  }
+  __syncthreads();
+  return !failed;
+}
+
+// One wave per probe.  (a, ab, b2) point at 3 arrays of k doubles and dm at
+// 64 columns of k doubles, column l at dm + l with stride 64 (LDS or global
+// workspace).
 __device__ void quadrature_wave(const float* __restrict__ Hp, int k, double* a, double* ab,
-                                double* b2, float* alpha, float* beta, double len,
+                                double* b2, double* dm, float* alpha, float* beta, double len,
                                 double* out) {
   const int lane = threadIdx.x;
   // tridiagonal of T = (H + H^T)/2 in FP32 (decomp.py:133-135); H column-major
@@ -153,7 +275,7 @@ __device__ void quadrature_wave(const float* __restrict__ Hp, int k, double* a, 
   // Any eigenvalue of T at or below zero makes the reference's log NaN.
   const bool nonpositive = sturm_count(a, b2, k, 0.0, pivmin) > 0;
 
-  double acc = 0.0;
+  double acc = 0.0, wsum = 0.0;
   const int m = split;
   for (int j = lane; j < m; j += 64) {
     double lo = gl, hi = gu;
@@ -167,10 +289,21 @@ __device__ void quadrature_wave(const float* __restrict__ Hp, int k, double* a, 
         lo = mid;
     }
     const double theta = 0.5 * (lo + hi);
-    const double w = first_component_sq(a, b2, m, theta, pivmin);
+    const double w = twisted_weight(a, b2, m, theta, pivmin, dm + lane);
+    wsum += w;
     acc += w * log(theta);
   }
   acc = wave_sum_f64(acc);
+  wsum = wave_sum_f64(wsum);
+  if (!(fabs(wsum - 1.0) <= kWeightSumTol)) {
+    // eigenvalues that coincide to working precision: first row of U by QL,
+    // with (a, ab, b2) reused as (d, e, u)
+    __syncthreads();
+    const bool ok = tridiag_eigvecs_wave(Hp, k, a, ab, b2, 1);
+    acc = 0.0;
+    for (int j = lane; j < k; j += 64) acc += b2[j] * b2[j] * log(a[j]);
+    acc = ok ? wave_sum_f64(acc) : __builtin_nan("");
+  }
   if (lane == 0) *out = nonpositive ? __builtin_nan("") : len * len * acc;
 }
 
@@ -178,100 +311,25 @@ __global__ void __launch_bounds__(64)
     k_tridiag_quadrature_lds(const float* __restrict__ H, const float* __restrict__ scale,
                              int64_t k, double* __restrict__ per_probe,
                              float* __restrict__ alpha, float* __restrict__ beta) {
-  extern __shared__ double lds[];  // 3 k doubles
+  __shared__ double tri[3 * kQuadLdsDepth];
+  __shared__ double dm[64 * kQuadLdsDepth];
   const int64_t p = blockIdx.x;
-  quadrature_wave(H + p * k * k, int(k), lds, lds + k, lds + 2 * k,
+  quadrature_wave(H + p * k * k, int(k), tri, tri + k, tri + 2 * k, dm,
                   alpha ? alpha + p * k : nullptr, beta ? beta + p * (k - 1) : nullptr,
                   double(scale[p]), per_probe + p);
 }
 
-// Arrays in a global workspace (3 k P doubles) for depths beyond kLdsDepth.
+// Arrays in a global workspace (67 k doubles per probe) beyond kQuadLdsDepth.
 __global__ void __launch_bounds__(64)
     k_tridiag_quadrature_global(const float* __restrict__ H, const float* __restrict__ scale,
                                 int64_t k, double* __restrict__ work,
                                 double* __restrict__ per_probe, float* __restrict__ alpha,
                                 float* __restrict__ beta) {
   const int64_t p = blockIdx.x;
-  double* w = work + p * 3 * k;
-  quadrature_wave(H + p * k * k, int(k), w, w + k, w + 2 * k, alpha ? alpha + p * k : nullptr,
-                  beta ? beta + p * (k - 1) : nullptr, double(scale[p]), per_probe + p);
-}
-
-// ---------------------------------------------------------------------------
-// Full eigendecomposition of T (from H as above) by implicit QL with the
-// eigenvectors accumulated.  One wave per probe; (d, e) are k doubles, U is
-// k x k row-major (U[a * k + j] = component a of eigenvector j), in LDS for
-// k <= kEigLdsDepth and in a global workspace beyond.  The QL recurrence is
-// evaluated redundantly by every lane (uniform control flow); the rotations
-// of U are spread over the lanes by rows.  Returns false if QL did not
-// converge.  On return d holds the eigenvalues (unordered).
-// ---------------------------------------------------------------------------
-constexpr int kEigLdsDepth = 64;
-
-__device__ bool tridiag_eigvecs_wave(const float* __restrict__ Hp, int k, double* d, double* e,
-                                     double* U) {
-  const int lane = threadIdx.x;
-  for (int i = lane; i < k; i += 64) {
-    d[i] = double(Hp[int64_t(i) * k + i]);
-    e[i] = (i + 1 < k) ? double(0.5f * (Hp[int64_t(i + 1) * k + i] + Hp[int64_t(i) * k + i + 1]))
-                       : 0.0;
-  }
-  for (int a = lane; a < k; a += 64)
-    for (int j = 0; j < k; ++j) U[int64_t(a) * k + j] = (a == j) ? 1.0 : 0.0;
-  __syncthreads();
-
-  bool failed = false;
-  for (int l = 0; l < k && !failed; ++l) {
-    int sweeps = 0;
-    while (true) {
-      int m = l;
-      for (; m + 1 < k; ++m) {
-        const double dd = fabs(d[m]) + fabs(d[m + 1]);
-        if (fabs(e[m]) <= DBL_EPSILON * dd) break;
-      }
-      if (m == l) break;
-      if (++sweeps > kMaxSweeps) {
-        failed = true;
-        break;
-      }
-      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-      double r = hypot(g, 1.0);
-      g = d[m] - d[l] + e[l] / (g + copysign(r, g));
-      double s = 1.0, c = 1.0, shift = 0.0;
-      bool split = false;
-      for (int i = m - 1; i >= l; --i) {
-        const double f = s * e[i];
-        const double b = c * e[i];
-        r = hypot(f, g);
-        e[i + 1] = r;
-        if (r == 0.0) {  // an off-diagonal underflowed: the matrix split
-          d[i + 1] -= shift;
-          e[m] = 0.0;
-          split = true;
-          break;
-        }
-        s = f / r;
-        c = g / r;
-        g = d[i + 1] - shift;
-        r = (d[i] - g) * s + 2.0 * c * b;
-        shift = s * r;
-        d[i + 1] = g + shift;
-        g = c * r - b;
-        for (int a = lane; a < k; a += 64) {
-          double* ua = U + int64_t(a) * k;
-          const double ui = ua[i], ui1 = ua[i + 1];
-          ua[i + 1] = s * ui + c * ui1;
-          ua[i] = c * ui - s * ui1;
-        }
-      }
-      if (split) continue;
-      d[l] -= shift;
-      e[l] = g;
-      e[m] = 0.0;
-    }
-  }
-  __syncthreads();
-  return !failed;
+  double* w = work + p * 67 * k;
+  quadrature_wave(H + p * k * k, int(k), w, w + k, w + 2 * k, w + 3 * k,
+                  alpha ? alpha + p * k : nullptr, beta ? beta + p * (k - 1) : nullptr,
+                  double(scale[p]), per_probe + p);
 }
 
 // Cotangent of |z|^2 [log T]_00 / P w.r.t. H (see the file header), one wave
@@ -279,7 +337,7 @@ __device__ bool tridiag_eigvecs_wave(const float* __restrict__ Hp, int k, double
 __device__ void tridiag_vjp_wave(const float* __restrict__ Hp, int k, double* d, double* e,
                                  double* lg, double* U, double scale, float* __restrict__ dHp) {
   const int lane = threadIdx.x;
-  const bool failed = !tridiag_eigvecs_wave(Hp, k, d, e, U);
+  const bool failed = !tridiag_eigvecs_wave(Hp, k, d, e, U, k);
   // log of the eigenvalues; a non-positive eigenvalue makes the reference's
   // log, and so its cotangent, NaN
   bool nonpositive = failed;
@@ -363,7 +421,7 @@ __device__ __forceinline__ double apply_matfun(int f, double x) {
 __device__ void tridiag_funm_wave(const float* __restrict__ Hp, int k, double* d, double* e,
                                   double* U, int f, float scale, float* __restrict__ cp) {
   const int lane = threadIdx.x;
-  const bool ok = tridiag_eigvecs_wave(Hp, k, d, e, U);
+  const bool ok = tridiag_eigvecs_wave(Hp, k, d, e, U, k);
   for (int a = lane; a < k; a += 64) {
     double y = 0.0;
     const double* ua = U + int64_t(a) * k;
@@ -428,6 +486,21 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Plain Lanczos coefficients from T (column-major P x k x k) and the
+// remainder norms: alpha[p][i] = T_ii, beta[p][i] = T_{i+1,i} (i < k - 1) and
+// beta[p][kk - 1] = blast[p], kk = max(k, 1).
+__global__ void k_lanczos_coeffs(const float* __restrict__ T, const float* __restrict__ blast,
+                                 int64_t P, int64_t k, float* __restrict__ alpha,
+                                 float* __restrict__ beta) {
+  const int64_t p = blockIdx.x;
+  const int64_t kk = k > 0 ? k : 1;
+  for (int64_t i = threadIdx.x; i < k; i += blockDim.x) {
+    alpha[p * k + i] = T[p * k * k + i * k + i];
+    if (i + 1 < k) beta[p * kk + i] = T[p * k * k + i * k + i + 1];
+  }
+  if (threadIdx.x == 0) beta[p * kk + kk - 1] = blast[p];
+}
+
 // Probe mean in a fixed (sequential) order: deterministic for a given P.
 __global__ void k_probe_mean(const double* __restrict__ per_probe, int64_t P,
                              double* __restrict__ mean) {
@@ -445,9 +518,9 @@ hipError_t launch_tridiag_quadrature(hipStream_t s, const float* H, const float*
                                      float* alpha, float* beta) {
   if (P <= 0) return hipSuccess;
   if (k > 0) {
-    if (k <= kLdsDepth)
-      hipLaunchKernelGGL(k_tridiag_quadrature_lds, dim3(unsigned(P)), dim3(64),
-                         size_t(3 * k) * sizeof(double), s, H, scale, k, per_probe, alpha, beta);
+    if (k <= kQuadLdsDepth)
+      hipLaunchKernelGGL(k_tridiag_quadrature_lds, dim3(unsigned(P)), dim3(64), 0, s, H, scale, k,
+                         per_probe, alpha, beta);
     else
       hipLaunchKernelGGL(k_tridiag_quadrature_global, dim3(unsigned(P)), dim3(64), 0, s, H, scale,
                          k, work, per_probe, alpha, beta);
@@ -469,6 +542,10 @@ hipError_t launch_tridiag_vjp(hipStream_t s, const float* H, const float* len, i
   return hipGetLastError();
 }
 
+int64_t tridiag_quadrature_work_doubles(int64_t P, int64_t k) {
+  return k <= kQuadLdsDepth ? 0 : P * 67 * k;
+}
+
 int64_t tridiag_vjp_work_doubles(int64_t P, int64_t k) {
   return k <= kEigLdsDepth ? 0 : P * (k * k + 3 * k);
 }
@@ -481,6 +558,13 @@ hipError_t launch_tridiag_funm(hipStream_t s, const float* H, const float* len, 
   else
     hipLaunchKernelGGL(k_tridiag_funm_global, dim3(unsigned(P)), dim3(64), 0, s, H, len, k, f,
                        work, coef);
+  return hipGetLastError();
+}
+
+hipError_t launch_lanczos_coeffs(hipStream_t s, const float* T, const float* blast, int64_t P,
+                                 int64_t k, float* alpha, float* beta) {
+  hipLaunchKernelGGL(k_lanczos_coeffs, dim3(unsigned(P)), dim3(64), 0, s, T, blast, P, k, alpha,
+                     beta);
   return hipGetLastError();
 }
 

@@ -20,11 +20,13 @@ struct slq_context {
       per_probe, mean, splitk;
   // reverse pass
   Buf adj_dh, adj_gam, adj_coef, adj_lam, adj_v1, adj_v2, adj_v3, adj_work, adj_sum;
+  // plain Lanczos: three rotating vectors and the last off-diagonal
+  Buf ring, blast;
   std::vector<Buf*> all_bufs() {
     return {&partial1, &partial2, &norm2,   &work_w,  &work_w2,  &basis,   &hmat,
             &resid,    &vec0,     &scal1,   &scal2,   &eig,      &per_probe, &mean,
             &splitk,   &adj_dh,   &adj_gam, &adj_coef, &adj_lam, &adj_v1,  &adj_v2,
-            &adj_v3,   &adj_work, &adj_sum};
+            &adj_v3,   &adj_work, &adj_sum, &ring,    &blast};
   }
   // measurement hooks: event pairs per kernel class, reused across reads
   bool profiling = false;
@@ -230,6 +232,68 @@ int hessenberg_core(slq_context* ctx, const slq_operator& op, const float* V0, i
   return SLQ_OK;
 }
 
+// Plain Lanczos (tridiag_sym(reortho="none"), decomp.py:198-266) for P start
+// vectors.  T_p goes into H (P x k x k, column-major: alpha on the diagonal,
+// beta on both off-diagonals), the last off-diagonal (the remainder's norm)
+// into blast[p], R = blast * q_k, c = 1 / |v0|.  With Q non-null the k
+// vectors q_0 .. q_{k-1} are kept at Q + j * ldc; otherwise three rotating
+// vectors are used (no basis).  k = 0 runs the initial step only.
+int lanczos_core(slq_context* ctx, const slq_operator& op, const float* V0, int64_t P, int64_t k,
+                 float* Q, int64_t ldc, float* H, float* blast, float* R, float* c) {
+  const int64_t n = op.n;
+  hipStream_t s = ctx->stream;
+  const int64_t nb = num_chunks(n);
+  float *apart, *norm2, *w, *r, *ring, *mvs;
+  const int64_t mv_floats = matvec_scratch_floats(n, P);
+  SLQ_TRY(ensure_t(ctx, ctx->splitk, size_t(mv_floats), &mvs));
+  SLQ_TRY(ensure_t(ctx, ctx->partial1, size_t(P * nb), &apart));
+  SLQ_TRY(ensure_t(ctx, ctx->norm2, size_t(P * nb), &norm2));
+  SLQ_TRY(ensure_t(ctx, ctx->work_w, size_t(P * n), &w));
+  SLQ_TRY(ensure_t(ctx, ctx->work_w2, size_t(P * n), &r));
+  SLQ_TRY(ensure_t(ctx, ctx->ring, size_t(3 * P * n), &ring));
+  if (k > 0) SLQ_HIP(ctx, hipMemsetAsync(H, 0, size_t(P * k * k) * sizeof(float), s));
+  // vector j: basis column j when kept (j < k), else ring slot j % 3
+  auto vec = [&](int64_t j) -> float* {
+    if (Q && j < k) return Q + j * ldc;
+    return ring + (j % 3) * P * n;
+  };
+  // v0 = vec / |vec| (decomp.py:208-209); c = 1 / |vec| (:178)
+  SLQ_HIP(ctx, launch_dots(s, nullptr, 0, V0, P, n, 1, norm2, 1));
+  SLQ_HIP(ctx, launch_normalize(s, V0, vec(0), P, n, norm2, nullptr, nullptr, 0, c));
+  const int64_t steps = k > 0 ? k : 1;
+  for (int64_t j = 0; j < steps; ++j) {
+    const float* q = vec(j);
+    const float* qm = j > 0 ? vec(j - 1) : nullptr;
+    {
+      const auto ev = take_events(ctx, SLQ_KC_MATVEC);
+      SLQ_HIP(ctx, launch_matvec(s, op, q, P, w, mvs, mv_floats, ev.first, ev.second));
+    }
+    // a = q . A q; r = A q - a q - b q_prev (decomp.py:243-245, 254-260)
+    {
+      Timed tm(ctx, SLQ_KC_DOTS);
+      SLQ_HIP(ctx, launch_dots(s, q, 0, w, P, n, 1, apart, 1));
+    }
+    {
+      Timed tm(ctx, SLQ_KC_PROJECT);
+      SLQ_HIP(ctx, launch_three_term(s, w, q, qm, P, n, apart,
+                                     j > 0 ? H + (j - 1) * k + j : nullptr, k * k, r, norm2,
+                                     j < k ? H + j * k + j : nullptr, k * k));
+    }
+    // b = |r|; x = r / b (:246-247, 261-262); b is T's off-diagonal except
+    // after the last step, where it is the remainder's norm
+    {
+      Timed tm(ctx, SLQ_KC_NORMALIZE);
+      const bool inner = j + 1 < k;
+      SLQ_HIP(ctx, launch_normalize(s, r, vec(j + 1), P, n, norm2, inner ? nullptr : blast,
+                                    inner ? H + j * k + j + 1 : nullptr, k * k, nullptr,
+                                    inner ? H + (j + 1) * k + j : nullptr));
+    }
+  }
+  // residual = b * x (decomp.py:167)
+  SLQ_HIP(ctx, launch_scale_rows(s, vec(steps), blast, 1, P, n, R));
+  return SLQ_OK;
+}
+
 // Device state of a forward solve that the reverse pass reads.
 struct ForwardState {
   float *Q = nullptr, *H = nullptr, *R = nullptr, *c = nullptr, *ell = nullptr;
@@ -253,7 +317,7 @@ int logdet_forward(slq_context* ctx, const slq_operator& op, const float* Z, int
   SLQ_TRY(ensure_t(ctx, ctx->scal1, size_t(P), &st.c));
   SLQ_TRY(ensure_t(ctx, ctx->scal2, size_t(P), &st.ell));
   SLQ_TRY(ensure_t(ctx, ctx->norm2, size_t(P * num_chunks(n)), &norm2));
-  SLQ_TRY(ensure_t(ctx, ctx->eig, size_t(3 * k * P), &eig));
+  SLQ_TRY(ensure_t(ctx, ctx->eig, size_t(tridiag_quadrature_work_doubles(P, k)), &eig));
   if (per_probe) {
     pp = per_probe;
   } else {
@@ -445,8 +509,67 @@ int slq_logdet(slq_context* ctx, const slq_operator* op, const float* Z, int64_t
   return logdet_forward(ctx, *op, Z, P, k, per_probe, mean, alpha, beta, st);
 }
 
+int slq_lanczos(slq_context* ctx, const slq_operator* op, const float* V0, int64_t P, int64_t k,
+                float* Q, float* alpha, float* beta, float* R, float* c) {
+  if (!ctx) return SLQ_EINVAL;
+  SLQ_TRY(check_operator(ctx, op));
+  const int64_t n = op->n;
+  if (k < 0 || k > n) return fail(ctx, SLQ_EINVAL_K, exceeds_msg(k, n));
+  if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of vectors P must be positive");
+  if (!V0 || !R || !c || !beta || (k > 0 && !alpha))
+    return fail(ctx, SLQ_EINVAL, "input or output pointer is NULL");
+  SLQ_TRY(set_device(ctx));
+  hipStream_t s = ctx->stream;
+  float* H = nullptr;
+  if (k > 0) SLQ_TRY(ensure_t(ctx, ctx->hmat, size_t(P * k * k), &H));
+  float* bl;
+  SLQ_TRY(ensure_t(ctx, ctx->blast, size_t(P), &bl));
+  SLQ_TRY(lanczos_core(ctx, *op, V0, P, k, Q, P * n, H, bl, R, c));
+  // (alpha, beta[0 .. k-2]) from T, beta[k-1] = the remainder's norm
+  SLQ_HIP(ctx, launch_lanczos_coeffs(s, H, bl, P, k, alpha, beta));
+  return SLQ_OK;
+}
+
+int slq_logdet_lanczos(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P,
+                       int64_t k, double* per_probe, double* mean, float* alpha, float* beta) {
+  if (!ctx) return SLQ_EINVAL;
+  SLQ_TRY(check_operator(ctx, op));
+  const int64_t n = op->n;
+  if (k < 0 || k > n) return fail(ctx, SLQ_EINVAL_K, exceeds_msg(k, n));
+  if (k == 0)
+    return fail(ctx, SLQ_EINVAL, "the log-determinant integrand needs num_matvecs >= 1");
+  if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of probes P must be positive");
+  if (!Z) return fail(ctx, SLQ_EINVAL, "probe pointer is NULL");
+  SLQ_TRY(set_device(ctx));
+  hipStream_t s = ctx->stream;
+  float *v0, *H, *R, *c, *ell, *norm2, *bl;
+  double *eig, *pp;
+  SLQ_TRY(ensure_t(ctx, ctx->vec0, size_t(P * n), &v0));
+  SLQ_TRY(ensure_t(ctx, ctx->hmat, size_t(P * k * k), &H));
+  SLQ_TRY(ensure_t(ctx, ctx->resid, size_t(P * n), &R));
+  SLQ_TRY(ensure_t(ctx, ctx->scal1, size_t(P), &c));
+  SLQ_TRY(ensure_t(ctx, ctx->scal2, size_t(P), &ell));
+  SLQ_TRY(ensure_t(ctx, ctx->blast, size_t(P), &bl));
+  SLQ_TRY(ensure_t(ctx, ctx->norm2, size_t(P * num_chunks(n)), &norm2));
+  SLQ_TRY(ensure_t(ctx, ctx->eig, size_t(tridiag_quadrature_work_doubles(P, k)), &eig));
+  if (per_probe) {
+    pp = per_probe;
+  } else {
+    SLQ_TRY(ensure_t(ctx, ctx->per_probe, size_t(P), &pp));
+  }
+  // length = |z|; v0 = z / length (funm.py:228-229); plain Lanczos without a basis
+  SLQ_HIP(ctx, launch_dots(s, nullptr, 0, Z, P, n, 1, norm2, 1));
+  SLQ_HIP(ctx, launch_normalize(s, Z, v0, P, n, norm2, ell, nullptr, 0, nullptr));
+  SLQ_TRY(lanczos_core(ctx, *op, v0, P, k, nullptr, 0, H, bl, R, c));
+  {
+    Timed tm(ctx, SLQ_KC_QUADRATURE);
+    SLQ_HIP(ctx, launch_tridiag_quadrature(s, H, ell, P, k, eig, pp, mean, alpha, beta));
+  }
+  return SLQ_OK;
+}
+
 int slq_funm_vectors(slq_context* ctx, const slq_operator* op, const float* V, int64_t P,
-                     int64_t k, int matfun, float* out) {
+                     int64_t k, int reortho_full, int matfun, float* out) {
   if (!ctx) return SLQ_EINVAL;
   SLQ_TRY(check_operator(ctx, op));
   const int64_t n = op->n;
@@ -475,7 +598,13 @@ int slq_funm_vectors(slq_context* ctx, const slq_operator* op, const float* V, i
   // length = |vec|; vec / length (funm.py:138-139); tridiag_sym (:144)
   SLQ_HIP(ctx, launch_dots(s, nullptr, 0, V, P, n, 1, norm2, 1));
   SLQ_HIP(ctx, launch_normalize(s, V, v0, P, n, norm2, ell, nullptr, 0, nullptr));
-  SLQ_TRY(hessenberg_core(ctx, *op, v0, P, k, Q, ldc, H, R, c));
+  if (reortho_full) {
+    SLQ_TRY(hessenberg_core(ctx, *op, v0, P, k, Q, ldc, H, R, c));
+  } else {
+    float* bl;
+    SLQ_TRY(ensure_t(ctx, ctx->blast, size_t(P), &bl));
+    SLQ_TRY(lanczos_core(ctx, *op, v0, P, k, Q, ldc, H, bl, R, c));
+  }
   // length * Q^T (f(T) e_1) (:145-147)
   SLQ_HIP(ctx, launch_tridiag_funm(s, H, ell, P, k, matfun, work, coef));
   SLQ_HIP(ctx, launch_adjoint_combine(s, Q, nullptr, ldc, nullptr, nullptr, nullptr, out,

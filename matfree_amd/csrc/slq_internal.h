@@ -48,10 +48,21 @@ hipError_t launch_project_dots(hipStream_t s, const float* Q, int64_t ldc, const
                                float* y, int64_t P, int64_t n, int ncols, const float* h1part,
                                float* h2part, int pstride, float* hout, int64_t hout_stride);
 // out[p][i] = x[p][i] / sqrt(sum_b norm2part[p][b]); optionally writes the
-// norm to norm_out[p] (float) and to hslot[p * hslot_stride].
+// norm to norm_out[p] (float), to hslot[p * hslot_stride] and
+// hslot2[p * hslot_stride], and its reciprocal to inv_out[p].
 hipError_t launch_normalize(hipStream_t s, const float* x, float* out, int64_t P, int64_t n,
                             const float* norm2part, float* norm_out, float* hslot,
-                            int64_t hslot_stride, float* inv_out);
+                            int64_t hslot_stride, float* inv_out, float* hslot2 = nullptr);
+// Plain Lanczos step: r = w - a q - b_prev qm with a = sum_b apart[p][b]
+// (single-column dots of q and w), b_prev = bprev[p * bstride] (qm may be
+// null), norm^2 partials of r, a written to aout[p * astride].
+hipError_t launch_three_term(hipStream_t s, const float* w, const float* q, const float* qm,
+                             int64_t P, int64_t n, const float* apart, const float* bprev,
+                             int64_t bstride, float* r, float* norm2part, float* aout,
+                             int64_t astride);
+// out[p][i] = scale[p * sstride] * x[p][i]
+hipError_t launch_scale_rows(hipStream_t s, const float* x, const float* scale, int64_t sstride,
+                             int64_t P, int64_t n, float* out);
 
 // ---- block matvec (k_matvec.hip) -------------------------------------------
 // Largest split-K factor of the block matvec; its partial sums need
@@ -69,10 +80,12 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
 // ---- tridiagonal eigen + quadrature (k_tridiag.hip) ------------------------
 // For each probe p: T = (H + H^T)/2 restricted to its tridiagonal; returns
 // per_probe[p] = scale[p]^2 * sum_j U_0j^2 log(lambda_j) in FP64, optional
-// alpha/beta extraction, and the mean over probes.
+// alpha/beta extraction, and the mean over probes.  work:
+// tridiag_quadrature_work_doubles(P, k) doubles (may be 0).
 hipError_t launch_tridiag_quadrature(hipStream_t s, const float* H, const float* scale,
                                      int64_t P, int64_t k, double* work, double* per_probe,
                                      double* mean, float* alpha, float* beta);
+int64_t tridiag_quadrature_work_doubles(int64_t P, int64_t k);
 
 // dH (P x k x k, column-major like H, written on the tridiagonal only; the
 // caller zeroes it) = cotangent of mean_p |z_p|^2 [log T_p]_00 w.r.t. H, with
@@ -84,6 +97,10 @@ int64_t tridiag_vjp_work_doubles(int64_t P, int64_t k);
 // layout) of |v| Q^T f(T) e_1 with |v| = len[p]; work as above.
 hipError_t launch_tridiag_funm(hipStream_t s, const float* H, const float* len, int64_t P,
                                int64_t k, int f, double* work, float* coef);
+// alpha (P x k) and beta (P x max(k, 1)) of plain Lanczos from its T (in an
+// H-shaped buffer) and the remainder norms blast (P).
+hipError_t launch_lanczos_coeffs(hipStream_t s, const float* T, const float* blast, int64_t P,
+                                 int64_t k, float* alpha, float* beta);
 // mean / standard error of P doubles (one block, fixed order).
 hipError_t launch_mean_and_sem(hipStream_t s, const double* v, int64_t P, double* mean,
                                double* sem);

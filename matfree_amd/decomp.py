@@ -123,22 +123,83 @@ class _TridiagSymFull:
         )
 
 
+def lanczos(matvec, V, k, params):
+    """Run ``slq_lanczos`` (plain three-term Lanczos) on the rows of ``V`` (P, n).
+
+    Returns ``Q`` (k, P, n), ``alpha`` (P, k), ``beta`` (P, max(k, 1)) -- the
+    off-diagonals of T, then the remainder's norm --, ``R`` (P, n) and
+    ``c = 1 / |v|`` (P,).
+    """
+    op = operators.resolve(matvec)
+    P, n = tuple(V.shape)
+    if k < 0 or k > n:  # decomp.py:160-162
+        raise ValueError(_error_num_matvecs(k, maxval=n, minval=0))
+    device = _runtime.target_device(V, *params)
+    V = _runtime.as_f32(V, device)
+    desc, keep = op.descriptor(n, params, device)
+    Q = torch.empty((k, P, n), dtype=torch.float32, device=device)
+    alpha = torch.empty((P, k), dtype=torch.float32, device=device)
+    beta = torch.empty((P, max(k, 1)), dtype=torch.float32, device=device)
+    R = torch.empty((P, n), dtype=torch.float32, device=device)
+    c = torch.empty((P,), dtype=torch.float32, device=device)
+    ctx = _runtime.context(device)
+    ctx.call(
+        "slq_lanczos",
+        ctypes.byref(desc),
+        _runtime.ptr(V),
+        P,
+        k,
+        _runtime.ptr(Q) if k > 0 else None,
+        _runtime.ptr(alpha) if k > 0 else None,
+        _runtime.ptr(beta),
+        _runtime.ptr(R),
+        _runtime.ptr(c),
+    )
+    del keep
+    return Q, alpha, beta, R, c
+
+
+class _TridiagSymNone:
+    """``decompose`` of ``tridiag_sym(k, reortho="none")`` (decomp.py:155-193)."""
+
+    def __init__(self, num_matvecs, materialize, custom_vjp):
+        self.num_matvecs = int(num_matvecs)
+        self.materialize = bool(materialize)
+        self.custom_vjp = bool(custom_vjp)
+
+    def batched(self, matvec, vecs, *params):
+        k = self.num_matvecs
+        Q, alpha, beta, R, c = lanczos(matvec, vecs, k, params)
+        offdiags = beta[:, : max(k - 1, 0)]
+        matrix = _todense_tridiag_sym(alpha, offdiags) if self.materialize else (alpha, offdiags)
+        # residual = b q (decomp.py:167); init_length_inv = 1 / |vec| (:171-178)
+        return _DecompResult(Q_tall=Q.permute(1, 0, 2), J_small=matrix, residual=R,
+                             init_length_inv=c)
+
+    def __call__(self, matvec, vec, *params):
+        flat, unravel = _tree.ravel(vec, _host_or_gpu(vec, *params))
+        out = self.batched(matvec, flat[None, :], *params)
+        matrix = out.J_small[0] if self.materialize else (out.J_small[0][0], out.J_small[1][0])
+        return _DecompResult(
+            Q_tall=unravel(out.Q_tall[0]),
+            J_small=matrix,
+            residual=unravel(out.residual[0]),
+            init_length_inv=out.init_length_inv[0],
+        )
+
+
 def tridiag_sym(num_matvecs: int, /, *, materialize: bool = True, reortho: str = "full",
                 custom_vjp: bool = True):
     """Tridiagonalisation of a symmetric operator (decomp.py:30-122).
 
     ``reortho="full"`` runs Arnoldi with classical Gram-Schmidt applied twice
-    and symmetrises ``T = (H + H^T)/2`` (decomp.py:125-145), on the GPU.
-    ``reortho="none"`` (plain three-term Lanczos, decomp.py:155-348) is not
-    part of this build and raises ``NotImplementedError``.
+    and symmetrises ``T = (H + H^T)/2`` (decomp.py:125-145); ``reortho="none"``
+    runs plain three-term Lanczos (decomp.py:155-266).  Both on the GPU.
     """
     if reortho == "full":
         return _TridiagSymFull(num_matvecs, materialize=materialize, custom_vjp=custom_vjp)
     if reortho == "none":
-        raise NotImplementedError(
-            "tridiag_sym(reortho='none') (plain Lanczos) is not provided by matfree_amd; "
-            "use reortho='full'"
-        )
+        return _TridiagSymNone(num_matvecs, materialize=materialize, custom_vjp=custom_vjp)
     msg = f"reortho={reortho} unsupported. Choose eiter {'full', 'none'}."
     raise ValueError(msg)
 

@@ -62,8 +62,9 @@ def dense_funm_sym_eigh(matfun):
     return _DenseFunmSymEigh(matfun)
 
 
-def logdet_per_probe(matvec, Z, num_matvecs, params, want_coeffs=False):
-    """Run ``slq_logdet`` on probes ``Z`` (P, n).
+def logdet_per_probe(matvec, Z, num_matvecs, params, want_coeffs=False, reortho="full"):
+    """Run ``slq_logdet`` (``reortho="full"``) or ``slq_logdet_lanczos``
+    (``reortho="none"``, plain Lanczos) on probes ``Z`` (P, n).
 
     Returns ``(per_probe (P,) float64, mean () float64)`` and, with
     ``want_coeffs``, the tridiagonal ``(alpha (P, k), beta (P, k-1))``.
@@ -85,7 +86,7 @@ def logdet_per_probe(matvec, Z, num_matvecs, params, want_coeffs=False):
         beta = torch.empty((P, max(k - 1, 0)), dtype=torch.float32, device=device)
     ctx = _runtime.context(device)
     ctx.call(
-        "slq_logdet",
+        "slq_logdet" if reortho == "full" else "slq_logdet_lanczos",
         ctypes.byref(desc),
         _runtime.ptr(Z),
         P,
@@ -181,14 +182,14 @@ class _QuadFormSym:
             raise NotImplementedError(
                 "matfree_amd's GPU integrand implements dense_funm_sym_eigh(log) only"
             )
-        if not isinstance(tridiag_sym, decomp._TridiagSymFull):
-            raise TypeError("pass decomp.tridiag_sym(k) of matfree_amd (reortho='full')")
+        self.reortho = _reortho_of(tridiag_sym)
         self.tridiag_sym = tridiag_sym
         self.num_matvecs = tridiag_sym.num_matvecs
 
     def batched(self, matvec, samples, *parameters):
         """Per-probe values ``|z_p|^2 [log T_p]_00`` for the rows of ``samples``."""
-        per_probe, _ = logdet_per_probe(matvec, samples, self.num_matvecs, parameters)
+        per_probe, _ = logdet_per_probe(matvec, samples, self.num_matvecs, parameters,
+                                        reortho=self.reortho)
         return per_probe
 
     def estimate(self, matvec, samples, *parameters):
@@ -200,19 +201,38 @@ class _QuadFormSym:
         the estimator).
         """
         if _needs_grad(parameters):
+            self._require_full_for_grad()
             return _LogdetMean.apply(matvec, samples, self.num_matvecs, *parameters)
-        _, mean = logdet_per_probe(matvec, samples, self.num_matvecs, parameters)
+        _, mean = logdet_per_probe(matvec, samples, self.num_matvecs, parameters,
+                                   reortho=self.reortho)
         return mean
+
+    def _require_full_for_grad(self):
+        if self.reortho != "full":
+            raise NotImplementedError(
+                "gradients run the Arnoldi adjoint of tridiag_sym(reortho='full'); the plain "
+                "Lanczos adjoint (decomp.py:268-348) is not provided"
+            )
 
     def __call__(self, matvec, v0, *parameters):
         flat, _ = _tree.ravel(v0, decomp._host_or_gpu(v0, *parameters))
         if _needs_grad(parameters):
+            self._require_full_for_grad()
             return _LogdetMean.apply(matvec, flat[None, :], self.num_matvecs, *parameters)
-        per_probe, _ = logdet_per_probe(matvec, flat[None, :], self.num_matvecs, parameters)
+        per_probe, _ = logdet_per_probe(matvec, flat[None, :], self.num_matvecs, parameters,
+                                        reortho=self.reortho)
         return per_probe[0]
 
 
-def funm_vectors(matvec, V, num_matvecs, params, matfun_code):
+def _reortho_of(tridiag_sym):
+    if isinstance(tridiag_sym, decomp._TridiagSymFull):
+        return "full"
+    if isinstance(tridiag_sym, decomp._TridiagSymNone):
+        return "none"
+    raise TypeError("pass decomp.tridiag_sym(k, reortho=...) of matfree_amd")
+
+
+def funm_vectors(matvec, V, num_matvecs, params, matfun_code, reortho="full"):
     """``|v| Q^T f(T) e_1`` for the rows of ``V`` (P, n) via ``slq_funm_vectors``."""
     op = operators.resolve(matvec)
     P, n = tuple(V.shape)
@@ -224,8 +244,8 @@ def funm_vectors(matvec, V, num_matvecs, params, matfun_code):
     desc, keep = op.descriptor(n, params, device)
     out = torch.empty((P, n), dtype=torch.float32, device=device)
     ctx = _runtime.context(device)
-    ctx.call("slq_funm_vectors", ctypes.byref(desc), _runtime.ptr(V), P, k, int(matfun_code),
-             _runtime.ptr(out))
+    ctx.call("slq_funm_vectors", ctypes.byref(desc), _runtime.ptr(V), P, k,
+             1 if reortho == "full" else 0, int(matfun_code), _runtime.ptr(out))
     del keep
     return out
 
@@ -239,18 +259,18 @@ class _FunmLanczosSym:
                 "matfree_amd's funm_lanczos_sym applies dense_funm_sym_eigh with log, exp, sin, "
                 "cos, sqrt or reciprocal"
             )
-        if not isinstance(tridiag_sym, decomp._TridiagSymFull):
-            raise TypeError("pass decomp.tridiag_sym(k) of matfree_amd (reortho='full')")
+        self.reortho = _reortho_of(tridiag_sym)
         self.code = dense_funm.code
         self.num_matvecs = tridiag_sym.num_matvecs
 
     def batched(self, matvec, vecs, *parameters):
         """``f(A) v`` for every row of ``vecs`` (P, n) in one call."""
-        return funm_vectors(matvec, vecs, self.num_matvecs, parameters, self.code)
+        return funm_vectors(matvec, vecs, self.num_matvecs, parameters, self.code, self.reortho)
 
     def __call__(self, matvec, vec, *parameters):
         flat, unravel = _tree.ravel(vec, decomp._host_or_gpu(vec, *parameters))
-        out = funm_vectors(matvec, flat[None, :], self.num_matvecs, parameters, self.code)
+        out = funm_vectors(matvec, flat[None, :], self.num_matvecs, parameters, self.code,
+                           self.reortho)
         return unravel(out[0])
 
 

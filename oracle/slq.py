@@ -7,6 +7,8 @@ the reference uses ``vmap``):
   (decomp.py:426-477) -- Arnoldi with classical Gram-Schmidt applied twice;
 * ``decomp._tridiag_reortho_full`` + ``_todense_tridiag_sym``
   (decomp.py:125-152) -- ``T = (H + H^T) / 2`` and its two diagonals;
+* ``decomp._tridiag_forward`` (decomp.py:198-266) -- plain three-term Lanczos
+  (``reortho="none"``);
 * ``funm.monte_carlo_funm_sym`` / ``monte_carlo_funm_sym_logdet`` /
   ``dense_funm_sym_eigh`` (funm.py:186-243, 322-335);
 * ``stochtrace.estimator_monte_carlo`` (stochtrace.py:47-50) and
@@ -170,6 +172,65 @@ def tridiag_sym(operator, num_matvecs, V, dtype=np.float32, materialize=True):
 # ---------------------------------------------------------------------------
 
 
+def lanczos_forward(operator, num_matvecs, V, dtype=np.float32):
+    """``_tridiag_forward`` (decomp.py:198-266): plain three-term Lanczos, batched.
+
+    Returns ``(Q (P, k, n), diags (P, k), offdiags (P, k-1), (q_k (P, n),
+    b_k (P,)), c)`` with ``c = 1 / |v|`` (decomp.py:231).  Per step
+    ``a = v . A v``, ``r = A v - a v - b v_prev``, ``b = |r|``, ``x = r / b``
+    (decomp.py:243-247, 254-262).
+    """
+    V = np.asarray(V, dtype=dtype)
+    P, n = V.shape
+    k = num_matvecs
+    if k < 0 or k > n:
+        raise ValueError(_error_num_matvecs(k, maxval=n, minval=0))
+    length = np.sqrt(np.sum(V * V, axis=1))
+    vectors = np.zeros((k + 1, P, n), dtype=dtype)  # :201
+    offdiags = np.zeros((P, k), dtype=dtype)
+    diags = np.zeros((P, k), dtype=dtype)
+    v0 = V / length[:, None]  # :208
+    vectors[0] = v0
+
+    def step(vec, b, vec_prev):
+        w = operator.apply(vec).astype(dtype)
+        a = np.sum(vec * w, axis=1).astype(dtype)
+        r = w - a[:, None] * vec
+        if vec_prev is not None:
+            r = r - b[:, None] * vec_prev
+        bn = np.sqrt(np.sum(r * r, axis=1)).astype(dtype)
+        return r / bn[:, None], bn, a
+
+    x, b, a = step(v0, None, None)  # _tridiag_fwd_init (:237-247)
+    if k == 0:
+        return vectors[:0].swapaxes(0, 1), diags, offdiags[:, :0], (x, b), (1 / length).astype(dtype)
+    vectors[1], offdiags[:, 0], diags[:, 0] = x, b, a
+    prev = v0
+    for i in range(1, k):  # :225-228, _tridiag_fwd_step
+        cur = vectors[i]
+        x, b, a = step(cur, offdiags[:, i - 1], prev)
+        vectors[i + 1], offdiags[:, i], diags[:, i] = x, b, a
+        prev = cur
+    return (vectors[:-1].swapaxes(0, 1), diags, offdiags[:, :-1], (vectors[-1], offdiags[:, -1]),
+            (1 / length).astype(dtype))
+
+
+def quadform_logdet_lanczos(operator, num_matvecs, Z, dtype=np.float32):
+    """``quadform`` with ``tridiag_sym(k, reortho="none")`` (funm.py:226-241)."""
+    Z = np.asarray(Z, dtype=dtype)
+    length = np.sqrt(np.sum(Z * Z, axis=1))
+    V0 = Z / length[:, None]
+    _, d, e, _, _ = lanczos_forward(operator, num_matvecs, V0, dtype=dtype)
+    k = d.shape[1]
+    J = np.zeros((Z.shape[0], k, k), dtype=dtype)
+    idx = np.arange(k)
+    J[:, idx, idx] = d
+    J[:, idx[:-1], idx[:-1] + 1] = e
+    J[:, idx[:-1] + 1, idx[:-1]] = e
+    fA = dense_funm_sym_eigh_log(J)
+    return length**2 * fA[:, 0, 0], d, e
+
+
 def dense_funm_sym_eigh_log(T):
     """``dense_funm_sym_eigh(np.log)`` (funm.py:322-335) on a batch of (k, k)."""
     w, U = np.linalg.eigh(T)  # :331 (LAPACK syevd; float32 stays float32)
@@ -217,13 +278,22 @@ def dense_funm_sym_eigh(matfun, T):
     return np.matmul(U * fw[:, None, :], np.swapaxes(U, 1, 2))  # :333
 
 
-def funm_lanczos_sym(operator, num_matvecs, V, matfun, dtype=np.float32):
-    """``funm_lanczos_sym(dense_funm_sym_eigh(matfun), tridiag_sym(k))`` (funm.py:114-147)
-    for the rows of ``V`` (P, n): ``|v| Q^T f(T) e_1`` per row."""
+def funm_lanczos_sym(operator, num_matvecs, V, matfun, dtype=np.float32, reortho="full"):
+    """``funm_lanczos_sym(dense_funm_sym_eigh(matfun), tridiag_sym(k, reortho))``
+    (funm.py:114-147) for the rows of ``V`` (P, n): ``|v| Q^T f(T) e_1`` per row."""
     V = np.asarray(V, dtype=dtype)
     length = np.sqrt(np.sum(V * V, axis=1))  # :138
     V0 = V / length[:, None]  # :139
-    Q, J, _, _ = tridiag_sym(operator, num_matvecs, V0, dtype=dtype)  # :144, Q (P, k, n)
+    if reortho == "full":
+        Q, J, _, _ = tridiag_sym(operator, num_matvecs, V0, dtype=dtype)  # :144, Q (P, k, n)
+    else:
+        Q, d, e, _, _ = lanczos_forward(operator, num_matvecs, V0, dtype=dtype)
+        k = d.shape[1]
+        J = np.zeros((V.shape[0], k, k), dtype=dtype)
+        idx = np.arange(k)
+        J[:, idx, idx] = d
+        J[:, idx[:-1], idx[:-1] + 1] = e
+        J[:, idx[:-1] + 1, idx[:-1]] = e
     fJ = dense_funm_sym_eigh(matfun, J)  # :145
     y = fJ[:, :, 0]  # funm @ e1 (:146-147)
     return length[:, None] * np.einsum("pkn,pk->pn", Q, y)  # :147 Q.T @ (funm @ e1)

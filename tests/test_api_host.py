@@ -13,9 +13,12 @@ def test_tridiag_sym_bad_reortho_raises_valueerror():
         decomp.tridiag_sym(3, reortho="partial")
 
 
-def test_tridiag_sym_reortho_none_is_out_of_scope():
-    with pytest.raises(NotImplementedError):
-        decomp.tridiag_sym(3, reortho="none")
+def test_tridiag_sym_reortho_flags():
+    # decomp.py:112-122: "full" and "none" are accepted, anything else raises
+    assert isinstance(decomp.tridiag_sym(3, reortho="none"), decomp._TridiagSymNone)
+    assert isinstance(decomp.tridiag_sym(3, reortho="full"), decomp._TridiagSymFull)
+    with pytest.raises(ValueError, match="unsupported"):
+        decomp.tridiag_sym(3, reortho="partial")
 
 
 @pytest.mark.parametrize("reortho_wrong", [True, "full_with_sparsity", "None"])

@@ -1,0 +1,168 @@
+"""GPU plain Lanczos: tridiag_sym(k, reortho="none") (decomp.py:155-266).
+
+The reference's own checks for this variant (tests/test_decomp/
+test_tridiag_sym.py, parametrised over reortho) run through the drop-in API,
+plus parity with the oracle's restatement of decomp.py:198-266 at depths
+where the FP32 vectors are still orthogonal (beyond that, rounding differences
+of the three-term recurrence grow by design).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from matfree_amd import decomp, funm, operators, problems, prng, stochtrace
+from oracle import prng as oprng
+from oracle import slq as oslq
+
+pytestmark = pytest.mark.gpu
+
+
+def sym_from_eigs(eigvals, seed):
+    n = len(eigvals)
+    Q, _ = np.linalg.qr(np.random.default_rng(seed).standard_normal((n, n)))
+    return ((Q * eigvals) @ Q.T).astype(np.float32)
+
+
+@pytest.mark.parametrize("num_matvecs", [1, 5, 11])
+def test_mid_rank_decomposition(cuda, num_matvecs):
+    # tests/test_decomp/test_tridiag_sym.py:43-68 (reortho="none")
+    n = 12
+    A = sym_from_eigs(np.arange(1.0, 2.0, 1 / n), 1)
+    vector = np.flip(np.arange(1.0, 1.0 + n)).astype(np.float32)
+    algorithm = decomp.tridiag_sym(num_matvecs, reortho="none", materialize=True)
+    Q_pytree, T, q_pytree, _ = algorithm(operators.dense_matvec, [(vector,)], A)
+    [(Q,)] = Q_pytree
+    [(q,)] = q_pytree
+    Q, T, q = Q.cpu().numpy(), T.cpu().numpy(), q.cpu().numpy()
+    e_K = np.eye(num_matvecs)[-1]
+    np.testing.assert_allclose(A @ Q.T - Q.T @ T - np.outer(q, e_K), 0.0, atol=1e-5)
+
+
+def test_full_rank_reconstruction(cuda):
+    # tests/test_decomp/test_tridiag_sym.py:7-40 (reortho="none": tolerance 1e-1)
+    n = 12
+    A = sym_from_eigs(np.arange(1.0, 2.0, 1 / n), 1)
+    vector = np.flip(np.arange(1.0, 1.0 + n)).astype(np.float32)
+    Q_pytree, T, *_ = decomp.tridiag_sym(n, reortho="none")(operators.dense_matvec,
+                                                             [(vector,)], A)
+    [(Q,)] = Q_pytree
+    Q, T = Q.cpu().numpy(), T.cpu().numpy()
+    assert np.allclose(Q.T @ T @ Q, A, atol=1e-1, rtol=1e-1)
+
+
+def test_coefficients_match_oracle(cuda):
+    n, P, k = 1000, 8, 10
+    A = problems.dense_spd(n, seed=3, shift=1.0)
+    V = np.random.default_rng(1).standard_normal((P, n)).astype(np.float32)
+    out = decomp.tridiag_sym(k, reortho="none", materialize=False).batched(
+        operators.dense_matvec, torch.as_tensor(V, device=cuda), torch.as_tensor(A, device=cuda))
+    d, e = out.J_small
+    _, od, oe, (oq, ob), oc = oslq.lanczos_forward(oslq.DenseOperator(A), k, V, np.float32)
+    scale = np.abs(od).max()
+    assert np.abs(d.cpu().numpy() - od).max() <= 1e-5 * scale
+    assert np.abs(e.cpu().numpy() - oe).max() <= 1e-5 * scale
+    np.testing.assert_allclose(out.init_length_inv.cpu().numpy(), oc, rtol=1e-6)
+    # residual b q_k
+    np.testing.assert_allclose(out.residual.cpu().numpy(), ob[:, None] * oq, atol=2e-4)
+
+
+def test_k0_runs_the_initial_step(cuda):
+    # decomp.py:215-218: zero steps still return the initial residual
+    n = 6
+    A = sym_from_eigs(np.linspace(1, 2, n), 2)
+    v = np.arange(1.0, 1.0 + n).astype(np.float32)
+    Q, T, r, c = decomp.tridiag_sym(0, reortho="none")(operators.dense_matvec, v, A)
+    _, _, _, (oq, ob), oc = oslq.lanczos_forward(oslq.DenseOperator(A), 0, v[None], np.float32)
+    assert Q.shape == (0, n) and T.shape == (0, 0)
+    np.testing.assert_allclose(r.cpu().numpy(), ob[0] * oq[0], atol=1e-5)
+    assert np.isclose(float(c), oc[0])
+
+
+def test_logdet_matches_oracle(cuda):
+    n, P, k = 1024, 16, 30
+    A = problems.dense_spd(n, seed=0, shift=1.0)
+    Z = oprng.rademacher(oprng.prng_key(4), (P, n), np.float32)
+    integrand = funm.monte_carlo_funm_sym_logdet(decomp.tridiag_sym(k, reortho="none"))
+    per = integrand.batched(operators.dense_matvec, torch.as_tensor(Z, device=cuda),
+                            torch.as_tensor(A, device=cuda)).cpu().numpy()
+    o32, _, _ = oslq.quadform_logdet_lanczos(oslq.DenseOperator(A), k, Z, np.float32)
+    o64, _, _ = oslq.quadform_logdet_lanczos(oslq.DenseOperator(A, np.float64), k, Z, np.float64)
+    err = np.abs(per - o64) / np.abs(o64)
+    err32 = np.abs(o32 - o64) / np.abs(o64)
+    assert err.max() <= 2 * err32.max() + 1e-6, (err.max(), err32.max())
+
+
+def test_funm_reference_configuration_none(cuda):
+    # tests/test_funm/test_funm_lanczos_sym.py:8-36 with reortho="none"
+    n = 11
+    A = sym_from_eigs(np.linspace(0.01, 0.99, n), 1)
+    v = np.random.default_rng(2).standard_normal(n).astype(np.float32)
+    f = funm.funm_lanczos_sym(funm.dense_funm_sym_eigh(np.sin),
+                              decomp.tridiag_sym(6, materialize=True, reortho="none"))
+    [(received,)] = f(operators.dense_matvec, [(v,)], A)
+    w, U = np.linalg.eigh(A.astype(np.float64))
+    expected = (U * np.sin(w)) @ U.T @ v.astype(np.float64)
+    assert np.allclose(received.cpu().numpy(), expected, atol=1e-6)
+
+
+def test_rbf_estimator_none_close_to_full(cuda):
+    # the estimator with plain Lanczos on the RBF operator (no basis kept)
+    n, P, k = 4096, 16, 32
+    X = problems.rbf_points(n, seed=3)
+    sampler = stochtrace.sampler_signs(np.zeros(n, np.float32), num=P)
+    est = {}
+    for reortho in ("full", "none"):
+        integrand = funm.monte_carlo_funm_sym_logdet(decomp.tridiag_sym(k, reortho=reortho))
+        est[reortho] = float(stochtrace.estimator_monte_carlo(integrand, sampler)(
+            operators.RBF(X), prng.PRNGKey(0), 0.5, 0.05))
+    assert abs(est["none"] - est["full"]) <= 1e-3 * abs(est["full"])
+
+
+def test_quadrature_with_ghost_ritz_values(cuda):
+    # Without reorthogonalisation, converged Ritz values reappear as copies
+    # whose eigenvectors have (numerically) zero first component; the FP64
+    # quadrature of the GPU's own tridiagonal must still match numpy's eigh
+    n, P, k = 2048, 8, 48
+    X = problems.rbf_points(n, seed=0)
+    Z = oprng.rademacher(oprng.prng_key(1), (P, n), np.float32)
+    per, _, alpha, beta = funm.logdet_per_probe(
+        operators.RBF(X), torch.as_tensor(Z, device=cuda), k, (1.0, 1e-3), want_coeffs=True,
+        reortho="none")
+    # |z| = sqrt(n) is carried in FP32, as in the GPU's scale
+    q = oslq.tridiag_eig_quadrature(alpha.cpu().numpy(), beta.cpu().numpy(),
+                                    np.sqrt(np.full(P, n, np.float32)))
+    np.testing.assert_allclose(per.cpu().numpy(), q, rtol=1e-9)
+
+
+@pytest.mark.parametrize("reortho", ["full", "none"])
+def test_quadrature_with_coincident_eigenvalues(cuda, reortho):
+    # Wilkinson's W21+ (shifted by 2): its largest eigenvalues come in pairs
+    # that agree to ~1e-14, below what bisection separates.  From e_1 (and
+    # e_21) the Krylov tridiagonal is the matrix itself, exactly in FP32, so
+    # the integrand is [log A]_00 (resp. [log A]_{20,20}) in closed form.
+    n = 21
+    a = np.abs(np.arange(n) - 10).astype(np.float64) + 2.0
+    A = np.diag(a) + np.diag(np.ones(n - 1), 1) + np.diag(np.ones(n - 1), -1)
+    Z = np.zeros((2, n), np.float32)
+    Z[0, 0] = 1.0
+    Z[1, -1] = 1.0
+    per, _, alpha, beta = funm.logdet_per_probe(
+        operators.dense_matvec, torch.as_tensor(Z, device=cuda), n,
+        (torch.as_tensor(A.astype(np.float32), device=cuda),), want_coeffs=True, reortho=reortho)
+    w, U = np.linalg.eigh(A)
+    logA = (U * np.log(w)) @ U.T
+    np.testing.assert_allclose(per.cpu().numpy(), [logA[0, 0], logA[-1, -1]], rtol=1e-12)
+    q = oslq.tridiag_eig_quadrature(alpha.cpu().numpy(), beta.cpu().numpy(), np.ones(2))
+    np.testing.assert_allclose(per.cpu().numpy(), q, rtol=1e-12)
+
+
+def test_grad_with_none_raises(cuda):
+    n = 64
+    X = problems.rbf_points(n, seed=1)
+    integrand = funm.monte_carlo_funm_sym_logdet(decomp.tridiag_sym(8, reortho="none"))
+    est = stochtrace.estimator_monte_carlo(integrand,
+                                           stochtrace.sampler_signs(np.zeros(n), num=2))
+    ls = torch.tensor(0.5, device=cuda, requires_grad=True)
+    with pytest.raises(NotImplementedError):
+        est(operators.RBF(X), prng.PRNGKey(0), ls, 0.1)

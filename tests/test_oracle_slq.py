@@ -334,3 +334,46 @@ def test_mean_and_sem_closed_form():
     mean, sem = slq.estimator_mean_and_sem(s, np.float64)
     assert mean == 3.5
     assert np.isclose(sem, np.sqrt(np.mean((s - 3.5) ** 2)) / 2.0)
+
+
+# -- plain Lanczos, reortho="none" (decomp.py:155-266) -------------------------
+
+
+@pytest.mark.parametrize("k", [1, 5, 11])
+def test_lanczos_decomposition_relation(k):
+    # tests/test_decomp/test_tridiag_sym.py:43-68 with reortho="none":
+    # A Q^T = Q^T T + q e_K^T, q the residual b q_k
+    n = 12
+    A = sym_from_eigs(np.arange(1.0, 2.0, 1 / n), 1).astype(np.float64)
+    v = np.flip(np.arange(1.0, 1.0 + n))[None, :]
+    Q, d, e, (qk, bk), c = slq.lanczos_forward(slq.DenseOperator(A, np.float64), k, v, np.float64)
+    T = np.diag(d[0]) + np.diag(e[0], 1) + np.diag(e[0], -1)
+    Qt = Q[0].T
+    resid = bk[0] * qk[0]
+    assert np.allclose(A @ Qt - Qt @ T - np.outer(resid, np.eye(k)[-1]), 0.0, atol=1e-10)
+    assert np.isclose(c[0], 1 / np.linalg.norm(v))
+
+
+def test_lanczos_matches_full_reorthogonalisation_in_fp64():
+    # in exact arithmetic the three-term recurrence and Arnoldi with full
+    # reorthogonalisation produce the same T; in FP64 at small depth they
+    # agree to rounding
+    n, k = 60, 8
+    A = sym_from_eigs(np.linspace(0.5, 3.0, n), 4).astype(np.float64)
+    V = np.random.default_rng(5).standard_normal((3, n))
+    _, d, e, _, _ = slq.lanczos_forward(slq.DenseOperator(A, np.float64), k, V, np.float64)
+    _, (fd, fe), _, _ = slq.tridiag_sym(slq.DenseOperator(A, np.float64), k,
+                                        V / np.linalg.norm(V, axis=1, keepdims=True),
+                                        np.float64, materialize=False)
+    assert np.allclose(d, fd, atol=1e-12)
+    assert np.allclose(e, fe, atol=1e-12)
+
+
+def test_lanczos_full_rank_reconstruction():
+    # tests/test_decomp/test_tridiag_sym.py:7-40, reortho="none" tolerance 1e-1
+    n = 12
+    A = sym_from_eigs(np.arange(1.0, 2.0, 1 / n), 1).astype(np.float64)
+    v = np.flip(np.arange(1.0, 1.0 + n))[None, :]
+    Q, d, e, _, _ = slq.lanczos_forward(slq.DenseOperator(A, np.float64), n, v, np.float64)
+    T = np.diag(d[0]) + np.diag(e[0], 1) + np.diag(e[0], -1)
+    assert np.allclose(Q[0].T @ T @ Q[0], A, atol=1e-1, rtol=1e-1)
