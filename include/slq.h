@@ -156,6 +156,20 @@ int slq_lanczos(slq_context* ctx, const slq_operator* op, const float* V0, int64
 int slq_logdet_lanczos(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P,
                        int64_t k, double* per_probe, double* mean, float* alpha, float* beta);
 
+/* slq_logdet_vjp with tridiag_sym(k, reortho="none"): the custom VJP of the
+ * plain Lanczos decomposition (decomp.py:181-197), _tridiag_adjoint and
+ * _tridiag_adjoint_step (decomp.py:268-348) driven by the cotangents of the
+ * diagonals alone (zero for the vectors, the residual and the remainder's
+ * norm: funm.py:237 keeps only T), the eigh VJP at [log T]_00
+ * (funm.py:331-333) and the probe mean (stochtrace.py:47-50).  dparams and
+ * dmatrix as in slq_logdet_vjp (dmatrix: sum of the matvec VJPs x_i lambda_i^T
+ * of decomp.py:337-338).  The forward pass is recomputed with its k vectors
+ * kept in context scratch.  Fails with SLQ_EINVAL for k = 0 (the integrand
+ * has no tridiagonal) and SLQ_EINVAL_K unless k <= n. */
+int slq_logdet_lanczos_vjp(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P,
+                           int64_t k, double* per_probe, double* mean, double* dparams,
+                           float* dmatrix);
+
 /* Matrix functions of slq_funm_vectors (dense_funm_sym_eigh(matfun),
  * funm.py:322-335, for these matfun). */
 typedef enum slq_matfun {
@@ -196,7 +210,7 @@ typedef enum slq_kernel_class {
   SLQ_KC_PROJECT = 3,      /* w - Q h (+ norm partials)                  */
   SLQ_KC_NORMALIZE = 4,    /* v / |v| and basis store                    */
   SLQ_KC_QUADRATURE = 5,   /* FP64 tridiagonal eigen + quadrature        */
-  SLQ_KC_ADJOINT = 6,      /* whole reverse pass of slq_logdet_vjp       */
+  SLQ_KC_ADJOINT = 6,      /* whole reverse pass of slq_logdet(_lanczos)_vjp */
   SLQ_KC_COUNT = 7
 } slq_kernel_class;
 int slq_profile_enable(slq_context* ctx, int on);

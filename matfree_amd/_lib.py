@@ -83,6 +83,11 @@ SIGNATURES = {
         [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_i64, c_void_p, c_void_p,
          c_void_p, c_void_p],
     ),
+    "slq_logdet_lanczos_vjp": (
+        c_int,
+        [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_i64, c_void_p, c_void_p,
+         c_void_p, c_void_p],
+    ),
     "slq_funm_vectors": (
         c_int,
         [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_i64, c_int, c_int,

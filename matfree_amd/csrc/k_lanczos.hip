@@ -596,4 +596,156 @@ hipError_t launch_normalize(hipStream_t s, const float* x, float* out, int64_t P
   return hipGetLastError();
 }
 
+
+// ---------------------------------------------------------------------------
+// Plain Lanczos adjoint (decomp.py:268-348): reverse step i in two passes.
+// k_ladj_lambda: lambda = -xi / b + mu x_{i+1} + nu x_i with
+//   mu = db - lambda_plus . x_i + x_{i+1} . xi / b,  nu = da + x_i . xi / b
+// from the step's partial dots [lambda_plus . x_i, x_{i+1} . xi, x_i . xi].
+// k_ladj_xi: xi' = -A lambda + a lambda + b lambda_plus - b nu x_{i+1}
+//   (b x_{i+1} is the residual R at the last step), fused with the partial
+//   dots of step i - 1: [lambda . x_{i-1}, x_i . xi', x_{i-1} . xi'].
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ void ladj_coeffs(const LanczosAdjStep& st, int64_t p, int64_t nb,
+                                            float& b, float& mu, float& nu) {
+  b = st.b[p * st.bstride];
+  float d0 = 0.f, d1 = 0.f, d2 = 0.f;
+  if (st.part) {
+    const float* pp = st.part + p * nb * 3;
+    for (int64_t bb = 0; bb < nb; ++bb) {
+      d0 += pp[3 * bb + 0];
+      d1 += pp[3 * bb + 1];
+      d2 += pp[3 * bb + 2];
+    }
+  }
+  const float db = st.db1 ? st.db1[p * st.sstride] + st.db2[p * st.sstride] : 0.f;
+  mu = db - d0 + d1 / b;
+  nu = st.da[p * st.sstride] + d2 / b;
+}
+
+template <int V>
+__global__ void __launch_bounds__(kStreamThreads)
+    k_ladj_lambda(const float* __restrict__ xi, const float* __restrict__ x,
+                  const float* __restrict__ xplus, int64_t n, LanczosAdjStep st,
+                  float* __restrict__ lam) {
+  constexpr int NE = Layout<V>::NE;
+  const int64_t p = blockIdx.y, nb = gridDim.x;
+  const int t = threadIdx.x;
+  const int64_t base = int64_t(blockIdx.x) * kChunk;
+  float b, mu, nu;
+  ladj_coeffs(st, p, nb, b, mu, nu);
+#pragma unroll
+  for (int e = 0; e < NE; ++e) {
+    const int64_t i = base + (int64_t(e) * kStreamThreads + t) * V;
+    if (i < n) {
+      float xv[V], out[V];
+      VecIO<V>::load(x + p * n + i, xv);
+#pragma unroll
+      for (int u = 0; u < V; ++u) out[u] = nu * xv[u];
+      if (xi) {
+        float zv[V];
+        VecIO<V>::load(xi + p * n + i, zv);
+#pragma unroll
+        for (int u = 0; u < V; ++u) out[u] -= zv[u] / b;
+      }
+      if (xplus) {
+        float yv[V];
+        VecIO<V>::load(xplus + p * n + i, yv);
+#pragma unroll
+        for (int u = 0; u < V; ++u) out[u] = fmaf(mu, yv[u], out[u]);
+      }
+      VecIO<V>::store(lam + p * n + i, out);
+    }
+  }
+}
+
+template <int V>
+__global__ void __launch_bounds__(kStreamThreads)
+    k_ladj_xi(const float* __restrict__ W, const float* __restrict__ lam,
+              const float* __restrict__ lam_plus, const float* __restrict__ xplus,
+              int xplus_is_residual, const float* __restrict__ x_prev,
+              const float* __restrict__ x, int64_t n, LanczosAdjStep st,
+              float* __restrict__ xi_out, float* __restrict__ part_out) {
+  constexpr int NE = Layout<V>::NE;
+  __shared__ float red[kWaves * 4];
+  const int64_t p = blockIdx.y, nb = gridDim.x;
+  const int t = threadIdx.x;
+  const int64_t base = int64_t(blockIdx.x) * kChunk;
+  float b, mu, nu;
+  ladj_coeffs(st, p, nb, b, mu, nu);
+  const float a = st.a[p * st.sstride];
+  const float cx = xplus_is_residual ? nu : b * nu;
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int e = 0; e < NE; ++e) {
+    const int64_t i = base + (int64_t(e) * kStreamThreads + t) * V;
+    if (i < n) {
+      float wv[V], lv[V], yv[V], out[V];
+      VecIO<V>::load(W + p * n + i, wv);
+      VecIO<V>::load(lam + p * n + i, lv);
+      VecIO<V>::load(xplus + p * n + i, yv);
+#pragma unroll
+      for (int u = 0; u < V; ++u) out[u] = fmaf(a, lv[u], -wv[u]);
+      if (lam_plus) {
+        float mv[V];
+        VecIO<V>::load(lam_plus + p * n + i, mv);
+#pragma unroll
+        for (int u = 0; u < V; ++u) out[u] = fmaf(b, mv[u], out[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < V; ++u) out[u] = fmaf(-cx, yv[u], out[u]);
+      VecIO<V>::store(xi_out + p * n + i, out);
+      if (part_out) {
+        float xv[V], pv[V];
+        VecIO<V>::load(x + p * n + i, xv);
+        VecIO<V>::load(x_prev + p * n + i, pv);
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+          acc[0] = fmaf(lv[u], pv[u], acc[0]);
+          acc[1] = fmaf(xv[u], out[u], acc[1]);
+          acc[2] = fmaf(pv[u], out[u], acc[2]);
+        }
+      }
+    }
+  }
+  if (part_out) block_column_sums<4>(acc, 3, red, part_out + (p * nb + blockIdx.x) * 3);
+}
+
+}  // namespace
+
+hipError_t launch_lanczos_adj_lambda(hipStream_t s, const float* xi, const float* x,
+                                     const float* xplus, int64_t P, int64_t n,
+                                     const LanczosAdjStep& st, float* lam) {
+  const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
+  const bool vec = (n % 4 == 0) && aligned16(x) && aligned16(lam) && (!xi || aligned16(xi)) &&
+                   (!xplus || aligned16(xplus));
+  if (vec)
+    hipLaunchKernelGGL(k_ladj_lambda<4>, grid, dim3(kStreamThreads), 0, s, xi, x, xplus, n, st,
+                       lam);
+  else
+    hipLaunchKernelGGL(k_ladj_lambda<1>, grid, dim3(kStreamThreads), 0, s, xi, x, xplus, n, st,
+                       lam);
+  return hipGetLastError();
+}
+
+hipError_t launch_lanczos_adj_xi(hipStream_t s, const float* W, const float* lam,
+                                 const float* lam_plus, const float* xplus, bool xplus_is_residual,
+                                 const float* x_prev, const float* x, int64_t P, int64_t n,
+                                 const LanczosAdjStep& st, float* xi, float* part_out) {
+  const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
+  const bool vec = (n % 4 == 0) && aligned16(W) && aligned16(lam) && aligned16(xplus) &&
+                   aligned16(xi) && (!lam_plus || aligned16(lam_plus)) &&
+                   (!part_out || (aligned16(x) && aligned16(x_prev)));
+  const int res = xplus_is_residual ? 1 : 0;
+  if (vec)
+    hipLaunchKernelGGL(k_ladj_xi<4>, grid, dim3(kStreamThreads), 0, s, W, lam, lam_plus, xplus,
+                       res, x_prev, x, n, st, xi, part_out);
+  else
+    hipLaunchKernelGGL(k_ladj_xi<1>, grid, dim3(kStreamThreads), 0, s, W, lam, lam_plus, xplus,
+                       res, x_prev, x, n, st, xi, part_out);
+  return hipGetLastError();
+}
+
 }  // namespace slq

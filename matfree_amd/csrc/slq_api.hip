@@ -297,6 +297,7 @@ int lanczos_core(slq_context* ctx, const slq_operator& op, const float* V0, int6
 // Device state of a forward solve that the reverse pass reads.
 struct ForwardState {
   float *Q = nullptr, *H = nullptr, *R = nullptr, *c = nullptr, *ell = nullptr;
+  float* blast = nullptr;  // plain Lanczos: the remainder's norm per probe
   int64_t ldc = 0;
 };
 
@@ -329,6 +330,46 @@ int logdet_forward(slq_context* ctx, const slq_operator& op, const float* Z, int
   // tridiag_sym(matvec, v0) (funm.py:237 -> decomp.py:130-143)
   SLQ_TRY(hessenberg_core(ctx, op, v0, P, k, st.Q, st.ldc, st.H, st.R, st.c));
   // |z|^2 [log T]_00 (funm.py:239-241) and the probe mean (stochtrace.py:50)
+  {
+    Timed tm(ctx, SLQ_KC_QUADRATURE);
+    SLQ_HIP(ctx, launch_tridiag_quadrature(s, st.H, st.ell, P, k, eig, pp, mean, alpha, beta));
+  }
+  return SLQ_OK;
+}
+
+// The SLQ forward pass on plain Lanczos (tridiag_sym(reortho="none"),
+// funm.py:226-241 for all probes); with keep_basis the vectors
+// q_0 .. q_{k-1} stay in st.Q (basis layout) for the reverse pass, otherwise
+// only three rotating vectors are used.
+int logdet_lanczos_forward(slq_context* ctx, const slq_operator& op, const float* Z, int64_t P,
+                           int64_t k, double* per_probe, double* mean, float* alpha, float* beta,
+                           bool keep_basis, ForwardState& st) {
+  const int64_t n = op.n;
+  hipStream_t s = ctx->stream;
+  float *v0, *norm2;
+  double *eig, *pp;
+  SLQ_TRY(ensure_t(ctx, ctx->vec0, size_t(P * n), &v0));
+  if (keep_basis) {
+    st.ldc = basis_stride(P, n);
+    SLQ_TRY(ensure_t(ctx, ctx->basis, size_t(k * st.ldc), &st.Q));
+  }
+  SLQ_TRY(ensure_t(ctx, ctx->hmat, size_t(P * k * k), &st.H));
+  SLQ_TRY(ensure_t(ctx, ctx->resid, size_t(P * n), &st.R));
+  SLQ_TRY(ensure_t(ctx, ctx->scal1, size_t(P), &st.c));
+  SLQ_TRY(ensure_t(ctx, ctx->scal2, size_t(P), &st.ell));
+  SLQ_TRY(ensure_t(ctx, ctx->blast, size_t(P), &st.blast));
+  SLQ_TRY(ensure_t(ctx, ctx->norm2, size_t(P * num_chunks(n)), &norm2));
+  SLQ_TRY(ensure_t(ctx, ctx->eig, size_t(tridiag_quadrature_work_doubles(P, k)), &eig));
+  if (per_probe) {
+    pp = per_probe;
+  } else {
+    SLQ_TRY(ensure_t(ctx, ctx->per_probe, size_t(P), &pp));
+  }
+  // length = |z|; v0 = z / length (funm.py:228-229)
+  SLQ_HIP(ctx, launch_dots(s, nullptr, 0, Z, P, n, 1, norm2, 1));
+  SLQ_HIP(ctx, launch_normalize(s, Z, v0, P, n, norm2, st.ell, nullptr, 0, nullptr));
+  SLQ_TRY(lanczos_core(ctx, op, v0, P, k, keep_basis ? st.Q : nullptr, st.ldc, st.H, st.blast,
+                       st.R, st.c));
   {
     Timed tm(ctx, SLQ_KC_QUADRATURE);
     SLQ_HIP(ctx, launch_tridiag_quadrature(s, st.H, st.ell, P, k, eig, pp, mean, alpha, beta));
@@ -402,6 +443,86 @@ int logdet_reverse(slq_context* ctx, const slq_operator& op, int64_t P, int64_t 
   } else {
     // d/dA = sum_idx sum_p lambda q^T (the VJP of A -> A q)
     SLQ_HIP(ctx, launch_outer_sum(s, Lam, st.Q, ldc, P, n, k * P, dmatrix));
+  }
+  return SLQ_OK;
+}
+
+// The plain-Lanczos reverse pass: dT from the eigh VJP (as above; with
+// T = diag(a) + offdiag(b, +-1) the chain gives da_i = dT_ii and
+// db_i = dT_{i,i+1} + dT_{i+1,i}), the adjoint recursion of
+// decomp.py:268-348 with zero cotangents for the vectors and the remainder
+// (quadform discards them, funm.py:237), and the parameter cotangents
+// sum_i d/dp <x_i, A(p) lambda_i> (the matvec VJPs of decomp.py:337-338).
+// The recursion's last xi feeds only the start vector's cotangent
+// (decomp.py:305), which the estimator does not need, so the final matvec is
+// skipped.
+int logdet_lanczos_reverse(slq_context* ctx, const slq_operator& op, int64_t P, int64_t k,
+                           const ForwardState& st, double* dparams, float* dmatrix) {
+  const int64_t n = op.n, ldc = st.ldc, kk = k * k;
+  hipStream_t s = ctx->stream;
+  const int64_t nb = num_chunks(n);
+  float *dH, *Lam, *xi, *W, *part1, *part2, *mvs;
+  double *vwork = nullptr, *dsum = nullptr;
+  const int64_t mv_floats = matvec_scratch_floats(n, P);
+  SLQ_TRY(ensure_t(ctx, ctx->splitk, size_t(mv_floats), &mvs));
+  SLQ_TRY(ensure_t(ctx, ctx->partial1, size_t(P * nb * 3), &part1));
+  SLQ_TRY(ensure_t(ctx, ctx->partial2, size_t(P * nb * 3), &part2));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_dh, size_t(P * kk), &dH));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_lam, size_t(k * ldc), &Lam));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_v1, size_t(P * n), &xi));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_v2, size_t(P * n), &W));
+  const int64_t vw = tridiag_vjp_work_doubles(P, k);
+  if (vw > 0) SLQ_TRY(ensure_t(ctx, ctx->adj_work, size_t(vw), &vwork));
+  if (op.kind == SLQ_OP_RBF)
+    SLQ_TRY(ensure_t(ctx, ctx->adj_sum, size_t(2 * outer_contract_blocks(n)), &dsum));
+
+  // dT from the eigh VJP of |z|^2 [log T]_00 / P (funm.py:239-241, 331-333)
+  SLQ_HIP(ctx, hipMemsetAsync(dH, 0, size_t(P * kk) * sizeof(float), s));
+  SLQ_HIP(ctx, launch_tridiag_vjp(s, st.H, st.ell, P, k, vwork, dH));
+
+  // reverse scan i = k-1 .. 0 (decomp.py:290-300) from xi = 0, lambda_plus = 0
+  float* part[2] = {part1, part2};
+  for (int64_t i = k - 1; i >= 0; --i) {
+    const bool last = (i + 1 == k);
+    LanczosAdjStep sc;
+    sc.a = st.H + i * k + i;
+    sc.da = dH + i * k + i;
+    sc.sstride = kk;
+    if (last) {
+      // b_{k-1} is the remainder's norm; its cotangent is zero, and so are
+      // xi and lambda_plus: mu = 0 and lambda = da x_{k-1}
+      sc.b = st.blast;
+      sc.bstride = 1;
+    } else {
+      sc.b = st.H + i * k + i + 1;  // T_{i+1,i}
+      sc.bstride = kk;
+      sc.db1 = dH + i * k + i + 1;
+      sc.db2 = dH + (i + 1) * k + i;
+      sc.part = part[i & 1];
+    }
+    const float* x = st.Q + i * ldc;
+    float* lam = Lam + i * ldc;
+    SLQ_HIP(ctx, launch_lanczos_adj_lambda(s, last ? nullptr : xi, x,
+                                           last ? nullptr : st.Q + (i + 1) * ldc, P, n, sc, lam));
+    if (i == 0) break;
+    // A lambda (decomp.py:337; the operator is symmetric)
+    SLQ_HIP(ctx, launch_matvec(s, op, lam, P, W, mvs, mv_floats));
+    // xi for step i - 1 (decomp.py:341) and that step's dots
+    SLQ_HIP(ctx, launch_lanczos_adj_xi(s, W, lam, last ? nullptr : Lam + (i + 1) * ldc,
+                                       last ? st.R : st.Q + (i + 1) * ldc, last,
+                                       st.Q + (i - 1) * ldc, x, P, n, sc, xi, part[(i - 1) & 1]));
+  }
+
+  // parameter cotangents: S = sum_{i, p} x_i lambda_i^T
+  if (op.kind == SLQ_OP_RBF) {
+    SLQ_HIP(ctx, launch_outer_rbf_contract(s, st.Q, Lam, ldc, P, n, k * P, op.points, op.dim,
+                                           op.lengthscale, dsum));
+    const double ls = op.lengthscale;
+    SLQ_HIP(ctx, launch_sum_fixed(s, dsum, outer_contract_blocks(n), 2, 1.0 / (ls * ls * ls), 1.0,
+                                  dparams));
+  } else {
+    // d/dA of <x, A lambda> is x lambda^T
+    SLQ_HIP(ctx, launch_outer_sum(s, st.Q, Lam, ldc, P, n, k * P, dmatrix));
   }
   return SLQ_OK;
 }
@@ -541,31 +662,8 @@ int slq_logdet_lanczos(slq_context* ctx, const slq_operator* op, const float* Z,
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of probes P must be positive");
   if (!Z) return fail(ctx, SLQ_EINVAL, "probe pointer is NULL");
   SLQ_TRY(set_device(ctx));
-  hipStream_t s = ctx->stream;
-  float *v0, *H, *R, *c, *ell, *norm2, *bl;
-  double *eig, *pp;
-  SLQ_TRY(ensure_t(ctx, ctx->vec0, size_t(P * n), &v0));
-  SLQ_TRY(ensure_t(ctx, ctx->hmat, size_t(P * k * k), &H));
-  SLQ_TRY(ensure_t(ctx, ctx->resid, size_t(P * n), &R));
-  SLQ_TRY(ensure_t(ctx, ctx->scal1, size_t(P), &c));
-  SLQ_TRY(ensure_t(ctx, ctx->scal2, size_t(P), &ell));
-  SLQ_TRY(ensure_t(ctx, ctx->blast, size_t(P), &bl));
-  SLQ_TRY(ensure_t(ctx, ctx->norm2, size_t(P * num_chunks(n)), &norm2));
-  SLQ_TRY(ensure_t(ctx, ctx->eig, size_t(tridiag_quadrature_work_doubles(P, k)), &eig));
-  if (per_probe) {
-    pp = per_probe;
-  } else {
-    SLQ_TRY(ensure_t(ctx, ctx->per_probe, size_t(P), &pp));
-  }
-  // length = |z|; v0 = z / length (funm.py:228-229); plain Lanczos without a basis
-  SLQ_HIP(ctx, launch_dots(s, nullptr, 0, Z, P, n, 1, norm2, 1));
-  SLQ_HIP(ctx, launch_normalize(s, Z, v0, P, n, norm2, ell, nullptr, 0, nullptr));
-  SLQ_TRY(lanczos_core(ctx, *op, v0, P, k, nullptr, 0, H, bl, R, c));
-  {
-    Timed tm(ctx, SLQ_KC_QUADRATURE);
-    SLQ_HIP(ctx, launch_tridiag_quadrature(s, H, ell, P, k, eig, pp, mean, alpha, beta));
-  }
-  return SLQ_OK;
+  ForwardState st;
+  return logdet_lanczos_forward(ctx, *op, Z, P, k, per_probe, mean, alpha, beta, false, st);
 }
 
 int slq_funm_vectors(slq_context* ctx, const slq_operator* op, const float* V, int64_t P,
@@ -642,6 +740,28 @@ int slq_logdet_vjp(slq_context* ctx, const slq_operator* op, const float* Z, int
   SLQ_TRY(logdet_forward(ctx, *op, Z, P, k, per_probe, mean, nullptr, nullptr, st));
   Timed tm(ctx, SLQ_KC_ADJOINT);
   return logdet_reverse(ctx, *op, P, k, st, dparams, dmatrix);
+}
+
+int slq_logdet_lanczos_vjp(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P,
+                           int64_t k, double* per_probe, double* mean, double* dparams,
+                           float* dmatrix) {
+  if (!ctx) return SLQ_EINVAL;
+  SLQ_TRY(check_operator(ctx, op));
+  const int64_t n = op->n;
+  if (k < 0 || k > n) return fail(ctx, SLQ_EINVAL_K, exceeds_msg(k, n));
+  if (k == 0)
+    return fail(ctx, SLQ_EINVAL, "the log-determinant integrand needs num_matvecs >= 1");
+  if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of probes P must be positive");
+  if (!Z) return fail(ctx, SLQ_EINVAL, "probe pointer is NULL");
+  if (op->kind == SLQ_OP_RBF && !dparams)
+    return fail(ctx, SLQ_EINVAL, "dparams (2 doubles) is NULL");
+  if (op->kind == SLQ_OP_DENSE && !dmatrix)
+    return fail(ctx, SLQ_EINVAL, "dmatrix (n x n floats) is NULL");
+  SLQ_TRY(set_device(ctx));
+  ForwardState st;
+  SLQ_TRY(logdet_lanczos_forward(ctx, *op, Z, P, k, per_probe, mean, nullptr, nullptr, true, st));
+  Timed tm(ctx, SLQ_KC_ADJOINT);
+  return logdet_lanczos_reverse(ctx, *op, P, k, st, dparams, dmatrix);
 }
 
 }  // extern "C"

@@ -64,6 +64,34 @@ hipError_t launch_three_term(hipStream_t s, const float* w, const float* q, cons
 hipError_t launch_scale_rows(hipStream_t s, const float* x, const float* scale, int64_t sstride,
                              int64_t P, int64_t n, float* out);
 
+// Plain Lanczos adjoint (decomp.py:268-348), reverse step i.  Per-probe
+// scalars: a_i = a[p * sstride], b_i = b[p * bstride], da_i = da[p * sstride],
+// db_i = db1[p * sstride] + db2[p * sstride] (db1 null: 0); part: the step's
+// partial dots (P x num_chunks(n) x 3: [lambda_plus . x_i, x_{i+1} . xi,
+// x_i . xi], null: all zero).
+struct LanczosAdjStep {
+  const float* a = nullptr;
+  const float* b = nullptr;
+  int64_t bstride = 0;
+  const float* da = nullptr;
+  const float* db1 = nullptr;
+  const float* db2 = nullptr;
+  int64_t sstride = 0;
+  const float* part = nullptr;
+};
+// lambda = -xi / b + mu x_{i+1} + nu x_i (xi null: 0; xplus null: the mu term
+// is dropped, valid where mu = 0)
+hipError_t launch_lanczos_adj_lambda(hipStream_t s, const float* xi, const float* x,
+                                     const float* xplus, int64_t P, int64_t n,
+                                     const LanczosAdjStep& st, float* lam);
+// xi = -W + a lambda + b lambda_plus - b nu xplus (xplus_is_residual: xplus
+// is b x_{i+1} already); with part_out, the partial dots of step i - 1
+// [lambda . x_prev, x . xi, x_prev . xi]
+hipError_t launch_lanczos_adj_xi(hipStream_t s, const float* W, const float* lam,
+                                 const float* lam_plus, const float* xplus, bool xplus_is_residual,
+                                 const float* x_prev, const float* x, int64_t P, int64_t n,
+                                 const LanczosAdjStep& st, float* xi, float* part_out);
+
 // ---- block matvec (k_matvec.hip) -------------------------------------------
 // Largest split-K factor of the block matvec; its partial sums need
 // kMaxSplitK * P * n floats of scratch (split-K is off when scratch is null).

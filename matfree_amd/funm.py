@@ -102,14 +102,17 @@ def logdet_per_probe(matvec, Z, num_matvecs, params, want_coeffs=False, reortho=
     return per_probe, mean
 
 
-def logdet_value_and_grad(matvec, Z, num_matvecs, params):
-    """Run ``slq_logdet_vjp`` on probes ``Z`` (P, n).
+def logdet_value_and_grad(matvec, Z, num_matvecs, params, reortho="full"):
+    """Run ``slq_logdet_vjp`` (``reortho="full"``: the Arnoldi adjoint) or
+    ``slq_logdet_lanczos_vjp`` (``reortho="none"``: the Lanczos adjoint) on
+    probes ``Z`` (P, n).
 
     Returns ``(per_probe (P,), mean (), grads)`` where ``grads`` holds one
     cotangent of the probe mean per parameter in ``params``: for
     :class:`operators.RBF` the float64 scalars ``d mean / d lengthscale`` and
     ``d mean / d noise``.  This is reverse mode through the estimator
-    (decomp.py:398-422, 480-600; funm.py:331-333; stochtrace.py:50).
+    (decomp.py:398-422, 480-600 or 181-197, 268-348; funm.py:331-333;
+    stochtrace.py:50).
     """
     op = operators.resolve(matvec)
     shape = tuple(Z.shape)
@@ -128,7 +131,7 @@ def logdet_value_and_grad(matvec, Z, num_matvecs, params):
         dmatrix = torch.empty((n, n), dtype=torch.float32, device=device)
     ctx = _runtime.context(device)
     ctx.call(
-        "slq_logdet_vjp",
+        "slq_logdet_vjp" if reortho == "full" else "slq_logdet_lanczos_vjp",
         ctypes.byref(desc),
         _runtime.ptr(Z),
         P,
@@ -158,8 +161,8 @@ class _LogdetMean(torch.autograd.Function):
     """
 
     @staticmethod
-    def forward(ctx, matvec, samples, num_matvecs, *params):
-        _, mean, grads = logdet_value_and_grad(matvec, samples, num_matvecs, params)
+    def forward(ctx, matvec, samples, num_matvecs, reortho, *params):
+        _, mean, grads = logdet_value_and_grad(matvec, samples, num_matvecs, params, reortho)
         shaped = []
         for p, g in zip(params, grads):
             if isinstance(p, torch.Tensor):
@@ -171,7 +174,7 @@ class _LogdetMean(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gmean):
         grads = ctx.saved_tensors
-        return (None, None, None, *[(gmean.to(g.dtype) * g) for g in grads])
+        return (None, None, None, None, *[(gmean.to(g.dtype) * g) for g in grads])
 
 
 class _QuadFormSym:
@@ -197,28 +200,21 @@ class _QuadFormSym:
 
         Differentiable in the operator parameters: with any parameter a
         tensor that requires grad, the result carries a backward that uses
-        the GPU Arnoldi adjoint (torch's counterpart of ``jax.grad`` through
+        the GPU adjoint of the decomposition (Arnoldi for ``reortho="full"``,
+        Lanczos for ``"none"``; torch's counterpart of ``jax.grad`` through
         the estimator).
         """
         if _needs_grad(parameters):
-            self._require_full_for_grad()
-            return _LogdetMean.apply(matvec, samples, self.num_matvecs, *parameters)
+            return _LogdetMean.apply(matvec, samples, self.num_matvecs, self.reortho, *parameters)
         _, mean = logdet_per_probe(matvec, samples, self.num_matvecs, parameters,
                                    reortho=self.reortho)
         return mean
 
-    def _require_full_for_grad(self):
-        if self.reortho != "full":
-            raise NotImplementedError(
-                "gradients run the Arnoldi adjoint of tridiag_sym(reortho='full'); the plain "
-                "Lanczos adjoint (decomp.py:268-348) is not provided"
-            )
-
     def __call__(self, matvec, v0, *parameters):
         flat, _ = _tree.ravel(v0, decomp._host_or_gpu(v0, *parameters))
         if _needs_grad(parameters):
-            self._require_full_for_grad()
-            return _LogdetMean.apply(matvec, flat[None, :], self.num_matvecs, *parameters)
+            return _LogdetMean.apply(matvec, flat[None, :], self.num_matvecs, self.reortho,
+                                     *parameters)
         per_probe, _ = logdet_per_probe(matvec, flat[None, :], self.num_matvecs, parameters,
                                         reortho=self.reortho)
         return per_probe[0]

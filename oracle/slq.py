@@ -15,7 +15,9 @@ the reference uses ``vmap``):
   ``estimator_monte_carlo_mean_and_sem`` (stochtrace.py:55-89);
 * ``funm.funm_lanczos_sym`` with ``dense_funm_sym_eigh`` (funm.py:114-147);
 * ``decomp._hessenberg_adjoint`` / ``_hessenberg_adjoint_step``
-  (decomp.py:480-600) and the eigh VJP that JAX applies at funm.py:331.
+  (decomp.py:480-600) and the eigh VJP that JAX applies at funm.py:331;
+* ``decomp._tridiag_adjoint`` / ``_tridiag_adjoint_step`` (decomp.py:268-348),
+  the reverse pass of ``reortho="none"``.
 
 ``dtype`` selects float32 (the reference's default precision) or float64.
 All arrays are NumPy; nothing here touches a GPU.
@@ -463,4 +465,91 @@ def slq_logdet_vjp(operator, num_matvecs, Z, dtype=np.float64):
             dc=0.0,
         )
         grad = dp if grad is None else grad + dp
+    return np.mean(values), grad
+
+
+# ---------------------------------------------------------------------------
+# Reverse: plain Lanczos adjoint (decomp.py:268-348, reortho="none")
+# ---------------------------------------------------------------------------
+
+
+def _lanczos_adjoint_step(operator, xi, lam_plus, x, xplus, a, b, da, db, dx):
+    """One reverse step of ``_tridiag_adjoint_step`` (decomp.py:321-348)."""
+    xi = xi / b
+    mu = db - lam_plus @ x + xplus @ xi
+    nu = da + x @ xi
+    lam = -xi + mu * xplus + nu * x
+    Alam = operator.apply(lam[None])[0]
+    g = operator.param_vjp(x, lam)  # vjp of matvec(lambda) at cotangent x (:337-338)
+    xi = -dx - Alam + a * lam + b * lam_plus - b * nu * xplus
+    return xi, lam, g
+
+
+def lanczos_adjoint(operator, alphas, betas, xs, dalphas, dbetas, dxs, initvec_norm):
+    """``_tridiag_adjoint`` + ``_tridiag_adjoint_step`` (decomp.py:268-348) for one probe.
+
+    ``xs`` (k + 1, n) are the Lanczos vectors including the remainder's
+    direction, ``betas`` (k,) the off-diagonals including the remainder's
+    norm, ``dalphas``/``dbetas`` (k,) and ``dxs`` (k + 1, n) their cotangents
+    (decomp.py:190-197 stacks them the same way).  The reverse scan runs
+    i = k-1 .. 0 from ``xi = -dxs[-1]``, ``lambda_plus = 0`` (:290-297):
+
+        xi /= b_i;  mu = db_i - lambda_plus . x_i + x_{i+1} . xi
+        nu = da_i + x_i . xi;  lambda = -xi + mu x_{i+1} + nu x_i
+        grad += d/dp <x_i, A(p) lambda>      (vjp of matvec(lambda) at x_i)
+        xi = -dx_i - A lambda + a_i lambda + b_i lambda_plus - b_i nu x_{i+1}
+
+    Returns ``(grad_initvec, grad_params, lambdas)`` with
+    ``grad_initvec = ((xi . x_0) x_0 - xi) / |v|`` for the final ``xi``
+    (:298-305: the scan's carry is ``(xs, xi, lambda)``, so its second entry,
+    named ``lambda_1`` there, is ``xi``).
+    """
+    k = len(alphas)
+    xi = -np.asarray(dxs[-1], np.float64)
+    lam_plus = np.zeros_like(xi)
+    grad = None
+    lambdas = np.empty((k, xi.shape[0]))
+    for i in range(k - 1, -1, -1):
+        xi, lam, g = _lanczos_adjoint_step(operator, xi, lam_plus, xs[i], xs[i + 1], alphas[i],
+                                           betas[i], dalphas[i], dbetas[i], dxs[i])
+        grad = g if grad is None else grad + g
+        lam_plus = lam
+        lambdas[i] = lam
+    grad_initvec = ((xi @ xs[0]) * xs[0] - xi) / initvec_norm
+    return grad_initvec, grad, lambdas
+
+
+def slq_logdet_lanczos_vjp(operator, num_matvecs, Z, dtype=np.float64):
+    """Gradient of ``mean_p |z_p|^2 [log T_p]_00`` with ``tridiag_sym(reortho="none")``.
+
+    Chain: eigh VJP (funm.py:331) -> dense tridiagonal assembly
+    (decomp.py:148-152: ``d_bar = diag(Jbar)``, ``e_bar = 2 diag(Jbar, 1)``)
+    -> Lanczos adjoint with ``dxs = 0`` and a zero cotangent for the
+    remainder's norm (quadform discards Q and the residual, funm.py:237)
+    -> mean over probes (stochtrace.py:50).  Returns ``(estimate, dparams)``.
+    """
+    Z = np.asarray(Z, dtype=dtype)
+    P, n = Z.shape
+    k = num_matvecs
+    if k < 1:
+        raise ValueError("the log-determinant integrand needs num_matvecs >= 1")
+    length = np.sqrt(np.sum(Z * Z, axis=1))
+    V0 = Z / length[:, None]
+    Q, d, e, (qk, bk), _ = lanczos_forward(operator, k, V0, dtype=dtype)
+    values = np.empty(P)
+    grad = None
+    for p in range(P):
+        dp_, ep = np.asarray(d[p], np.float64), np.asarray(e[p], np.float64)
+        J = np.diag(dp_) + np.diag(ep, 1) + np.diag(ep, -1)
+        w, U = np.linalg.eigh(J)
+        values[p] = length[p] ** 2 * np.sum(U[0] ** 2 * np.log(w))
+        Jbar = logdet_quadform_cotangent_T(J) * (length[p] ** 2 / P)
+        dalphas = np.diag(Jbar).copy()
+        dbetas = np.zeros(k)
+        dbetas[: k - 1] = np.diag(Jbar, 1) + np.diag(Jbar, -1)
+        xs = np.concatenate([np.asarray(Q[p], np.float64), np.asarray(qk[p], np.float64)[None]])
+        betas = np.concatenate([ep, [float(bk[p])]])
+        _, g, _ = lanczos_adjoint(operator, dp_, betas, xs, dalphas, dbetas,
+                                  np.zeros((k + 1, n)), float(length[p]))
+        grad = g if grad is None else grad + g
     return np.mean(values), grad

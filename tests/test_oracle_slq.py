@@ -377,3 +377,117 @@ def test_lanczos_full_rank_reconstruction():
     Q, d, e, _, _ = slq.lanczos_forward(slq.DenseOperator(A, np.float64), n, v, np.float64)
     T = np.diag(d[0]) + np.diag(e[0], 1) + np.diag(e[0], -1)
     assert np.allclose(Q[0].T @ T @ Q[0], A, atol=1e-1, rtol=1e-1)
+
+
+# -- plain Lanczos adjoint (decomp.py:268-348) ----------------------------------
+
+
+def _lanczos_outputs(A, v, k):
+    Q, d, e, (qk, bk), _ = slq.lanczos_forward(slq.DenseOperator(A, np.float64), k, v[None],
+                                              np.float64)
+    xs = np.concatenate([Q[0], qk])
+    betas = np.concatenate([e[0], bk])
+    return xs, d[0], betas
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_lanczos_adjoint_vjp_random_cotangents(seed):
+    # tests/test_decomp/test_tridiag_sym_adjoint.py:7-50 (reortho="none", n = 10,
+    # k = 4): the custom VJP against differentiation of the forward pass, here
+    # central differences of the FP64 restatement along random directions of
+    # (vector, symmetric matrix), for random cotangents on every output
+    n, k = 10, 4
+    rng = np.random.default_rng(seed)
+    A0 = sym_from_eigs(rng.uniform(size=n) + 1.0, seed).astype(np.float64)
+    v0 = rng.standard_normal(n)
+    dxs = rng.standard_normal((k + 1, n))
+    da = rng.standard_normal(k)
+    db = rng.standard_normal(k)
+
+    def f(v, A):
+        xs, d, betas = _lanczos_outputs(A, v, k)
+        return np.sum(dxs * xs) + da @ d + db @ betas
+
+    xs, d, betas = _lanczos_outputs(A0, v0, k)
+    gv, gA, _ = slq.lanczos_adjoint(slq.DenseOperator(A0, np.float64), d, betas, xs, da, db, dxs,
+                                    np.linalg.norm(v0))
+    dv = rng.standard_normal(n)
+    D = rng.standard_normal((n, n))
+    D = D + D.T
+    h = 1e-6
+    fd = (f(v0 + h * dv, A0 + h * D) - f(v0 - h * dv, A0 - h * D)) / (2 * h)
+    assert np.isclose(gv @ dv + np.sum(gA * D), fd, rtol=1e-6), (gv @ dv + np.sum(gA * D), fd)
+
+
+@pytest.mark.parametrize("k", [4, 8])
+def test_lanczos_adjoint_matches_arnoldi_adjoint(k):
+    # both reverse passes differentiate the same T(theta) in exact arithmetic;
+    # in FP64 at depths where Lanczos is still orthogonal they agree to rounding
+    X = np.random.default_rng(1).uniform(size=(200, 3))
+    Z = prng.rademacher(prng.prng_key(3), (3, 200), np.float64)
+    op = slq.RBFOperator(X, 0.5, 0.1, np.float64)
+    e1, g1 = slq.slq_logdet_lanczos_vjp(op, k, Z)
+    e2, g2 = slq.slq_logdet_vjp(op, k, Z)
+    assert np.isclose(e1, e2, rtol=1e-12)
+    assert np.allclose(g1, g2, rtol=1e-10)
+
+
+def test_lanczos_adjoint_rbf_central_differences():
+    # depth 16 on 40 points: past the loss of orthogonality, so the Lanczos
+    # value differs from Arnoldi's and only its own derivative matches
+    rng = np.random.default_rng(11)
+    X = rng.uniform(size=(40, 3))
+    Z = prng.rademacher(prng.prng_key(3), (4, 40), np.float64)
+    k = 16
+
+    def f(theta):
+        op = slq.RBFOperator(X, theta[0], theta[1], np.float64)
+        vals, _, _ = slq.quadform_logdet_lanczos(op, k, Z, np.float64)
+        return vals.mean()
+
+    theta = np.array([0.5, 0.1])
+    _, g = slq.slq_logdet_lanczos_vjp(slq.RBFOperator(X, *theta, np.float64), k, Z)
+    assert np.allclose(g, _fd_grad(f, theta, 1e-5), rtol=1e-5)
+
+
+def test_lanczos_adjoint_planted_mistakes_are_caught(monkeypatch):
+    n, k = 10, 4
+    rng = np.random.default_rng(1)
+    A0 = sym_from_eigs(rng.uniform(size=n) + 1.0, 1).astype(np.float64)
+    v0 = rng.standard_normal(n)
+    dxs = rng.standard_normal((k + 1, n))
+    da = rng.standard_normal(k)
+    db = rng.standard_normal(k)
+    dv = rng.standard_normal(n)
+    D = rng.standard_normal((n, n))
+    D = D + D.T
+
+    def mismatch():
+        xs, d, betas = _lanczos_outputs(A0, v0, k)
+        gv, gA, _ = slq.lanczos_adjoint(slq.DenseOperator(A0, np.float64), d, betas, xs, da, db,
+                                        dxs, np.linalg.norm(v0))
+        h = 1e-6
+
+        def f(v, A):
+            xs, d, betas = _lanczos_outputs(A, v, k)
+            return np.sum(dxs * xs) + da @ d + db @ betas
+
+        fd = (f(v0 + h * dv, A0 + h * D) - f(v0 - h * dv, A0 - h * D)) / (2 * h)
+        return abs(gv @ dv + np.sum(gA * D) - fd) / abs(fd)
+
+    assert mismatch() < 1e-6
+    original = slq._lanczos_adjoint_step
+
+    # (1) wrong sign of the b * lambda_plus term
+    def wrong_sign(operator, xi, lam_plus, x, xplus, a, b, da, db, dx):
+        return original(operator, xi, -lam_plus, x, xplus, a, b, da, db, dx)
+
+    monkeypatch.setattr(slq, "_lanczos_adjoint_step", wrong_sign)
+    assert mismatch() > 1e-3
+
+    # (2) neighbouring vectors swapped (index error)
+    def swapped(operator, xi, lam_plus, x, xplus, a, b, da, db, dx):
+        return original(operator, xi, lam_plus, xplus, x, a, b, da, db, dx)
+
+    monkeypatch.setattr(slq, "_lanczos_adjoint_step", swapped)
+    assert mismatch() > 1e-3
