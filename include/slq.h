@@ -198,6 +198,48 @@ int slq_funm_vectors(slq_context* ctx, const slq_operator* op, const float* V, i
 int slq_mean_and_sem(slq_context* ctx, const double* values, int64_t P, double* mean,
                      double* sem);
 
+/* ---- the GP log-density caller (tutorials/8_gaussian_logpdf.py:38-52) ------ */
+
+/* Partial Cholesky factorisation of the operator's matrix, the element
+ * function mat_el(i, j, *params) of low_rank.cholesky_partial
+ * (low_rank.py:56-99, pivot = 0) and low_rank.cholesky_partial_pivot
+ * (low_rank.py:102-198, pivot = 1): A[i][j] for a stored matrix, the RBF
+ * kernel exp(-|x_i - x_j|^2 / (2 l^2)) plus noise on the diagonal.
+ *   L:       n x rank device floats, row-major (the reference's (nrows, rank)
+ *            result, rows in the original order);
+ *   perm:    n device int64 (may be NULL): the pivot order, position -> row;
+ *   success: one device int, 1 if every l_ii^2 was positive (the pivoted
+ *            variant's info["success"]).
+ * Fails with SLQ_EINVAL ("Rank exceeds nrows" / "Rank must be positive",
+ * low_rank.py:112-117) unless 1 <= rank <= n, SLQ_EUNSUPPORTED above 4096. */
+int slq_cholesky_partial(slq_context* ctx, const slq_operator* op, int64_t rank, int pivot,
+                         float* L, int64_t* perm, int* success);
+
+/* The preconditioner's solve (low_rank.py:33-51):
+ * out = (s I + L L^T)^{-1} V for the P rows of V (P x n), by the Woodbury
+ * identity with the capacitance I + L^T L / s factored in FP64 (factored on
+ * every call, as the reference does). */
+int slq_lowrank_solve(slq_context* ctx, const float* L, int64_t n, int64_t rank, double shift,
+                      const float* V, int64_t P, float* out);
+
+/* Conjugate gradients for A X = B over the P rows of B (backend/linalg.py:
+ * 107-108, jax.scipy.sparse.linalg.cg with x0 = 0 under vmap): stop a row
+ * when rs <= max(tol^2 |b|^2, atol^2) (rs = r . z, or r . r with a
+ * preconditioner) or after maxiter steps; converged rows keep their state.
+ * With L non-NULL the preconditioner is (s I + L L^T)^{-1} (rank columns of
+ * L, n x rank row-major).  iterations (P device int64, may be NULL) receives
+ * the steps each row took.  The host reads one int per iteration. */
+int slq_cg(slq_context* ctx, const slq_operator* op, const float* B, int64_t P, double tol,
+           double atol, int64_t maxiter, const float* L, int64_t rank, double shift, float* X,
+           int64_t* iterations);
+
+/* Cotangents of sum_p x_p^T A(theta) x_p w.r.t. the operator parameters (the
+ * matvec VJP with cotangent x at x; what the derivative of the quadratic form
+ * y^T A^{-1} y needs, as -(these) at x = A^{-1} y): RBF: dparams[0] =
+ * sum x^T dK/dl x, dparams[1] = sum x^T x; stored matrix: dmatrix = sum x x^T. */
+int slq_quadform_vjp(slq_context* ctx, const slq_operator* op, const float* X, int64_t P,
+                     double* dparams, float* dmatrix);
+
 /* ---- measurement hooks (no counterpart in the reference) -------------------
  * With profiling on, every launch of the listed kernel classes made through
  * this context is bracketed by a pair of HIP events on the context's stream.

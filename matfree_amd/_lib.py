@@ -94,6 +94,20 @@ SIGNATURES = {
          c_void_p],
     ),
     "slq_mean_and_sem": (c_int, [c_void_p, c_void_p, c_i64, c_void_p, c_void_p]),
+    "slq_cholesky_partial": (
+        c_int, [c_void_p, ctypes.POINTER(OperatorStruct), c_i64, c_int, c_void_p, c_void_p, c_void_p],
+    ),
+    "slq_lowrank_solve": (
+        c_int, [c_void_p, c_void_p, c_i64, c_i64, ctypes.c_double, c_void_p, c_i64, c_void_p],
+    ),
+    "slq_cg": (
+        c_int,
+        [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, ctypes.c_double,
+         ctypes.c_double, c_i64, c_void_p, c_i64, ctypes.c_double, c_void_p, c_void_p],
+    ),
+    "slq_quadform_vjp": (
+        c_int, [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_void_p, c_void_p],
+    ),
     "slq_profile_enable": (c_int, [c_void_p, c_int]),
     "slq_profile_read": (c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(c_i64)]),

@@ -22,12 +22,19 @@ struct slq_context {
   Buf adj_dh, adj_gam, adj_coef, adj_lam, adj_v1, adj_v2, adj_v3, adj_work, adj_sum;
   // plain Lanczos: three rotating vectors and the last off-diagonal
   Buf ring, blast;
+  // partial Cholesky, Woodbury solve and CG
+  Buf lr_lc, lr_perm, lr_bval, lr_bidx, lr_rowi, lr_small, lr_gp, lr_c, lr_wp, lr_z, cg_r, cg_z,
+      cg_d, cg_q, cg_parts, cg_scal;
   std::vector<Buf*> all_bufs() {
     return {&partial1, &partial2, &norm2,   &work_w,  &work_w2,  &basis,   &hmat,
             &resid,    &vec0,     &scal1,   &scal2,   &eig,      &per_probe, &mean,
             &splitk,   &adj_dh,   &adj_gam, &adj_coef, &adj_lam, &adj_v1,  &adj_v2,
-            &adj_v3,   &adj_work, &adj_sum, &ring,    &blast};
+            &adj_v3,   &adj_work, &adj_sum, &ring,    &blast,    &lr_lc,   &lr_perm,
+            &lr_bval,  &lr_bidx,  &lr_rowi, &lr_small, &lr_gp,   &lr_c,    &lr_wp,
+            &lr_z,     &cg_r,     &cg_z,    &cg_d,    &cg_q,     &cg_parts, &cg_scal};
   }
+  // pinned host word for the CG loop's count of active columns
+  int* host_count = nullptr;
   // measurement hooks: event pairs per kernel class, reused across reads
   bool profiling = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events[SLQ_KC_COUNT];
@@ -447,6 +454,34 @@ int logdet_reverse(slq_context* ctx, const slq_operator& op, int64_t P, int64_t 
   return SLQ_OK;
 }
 
+// Largest rank of the partial Cholesky / Woodbury kernels (LDS staging).
+constexpr int64_t kMaxRank = 4096;
+
+int check_rank(slq_context* ctx, int64_t rank, int64_t n) {
+  if (rank > n)
+    return fail(ctx, SLQ_EINVAL,
+                "Rank exceeds nrows: " + std::to_string(rank) + " >= " + std::to_string(n) + ".");
+  if (rank < 1)
+    return fail(ctx, SLQ_EINVAL, "Rank must be positive, but " + std::to_string(rank) + " < 1.");
+  if (rank > kMaxRank)
+    return fail(ctx, SLQ_EUNSUPPORTED,
+                "rank " + std::to_string(rank) + " exceeds the supported maximum " +
+                    std::to_string(kMaxRank));
+  return SLQ_OK;
+}
+
+// C = chol(I + L^T L / s) in context scratch (Woodbury factor of s I + L L^T).
+int woodbury_factor(slq_context* ctx, const float* L, int64_t n, int64_t r, double shift,
+                    double** C) {
+  float* gp;
+  int* ok;
+  SLQ_TRY(ensure_t(ctx, ctx->lr_gp, size_t(gram_splits(n) * r * r), &gp));
+  SLQ_TRY(ensure_t(ctx, ctx->lr_c, size_t(r * r), C));
+  SLQ_TRY(ensure_t(ctx, ctx->lr_small, size_t(16), &ok));
+  SLQ_HIP(ctx, launch_woodbury_setup(ctx->stream, L, n, int(r), shift, gp, *C, ok));
+  return SLQ_OK;
+}
+
 // The plain-Lanczos reverse pass: dT from the eigh VJP (as above; with
 // T = diag(a) + offdiag(b, +-1) the chain gives da_i = dT_ii and
 // db_i = dT_{i,i+1} + dT_{i+1,i}), the adjoint recursion of
@@ -561,6 +596,7 @@ int slq_context_destroy(slq_context* ctx) {
     }
   for (auto* b : ctx->all_bufs())
     if (b->ptr) (void)hipFree(b->ptr);
+  if (ctx->host_count) (void)hipHostFree(ctx->host_count);
   delete ctx;
   return SLQ_OK;
 }
@@ -740,6 +776,162 @@ int slq_logdet_vjp(slq_context* ctx, const slq_operator* op, const float* Z, int
   SLQ_TRY(logdet_forward(ctx, *op, Z, P, k, per_probe, mean, nullptr, nullptr, st));
   Timed tm(ctx, SLQ_KC_ADJOINT);
   return logdet_reverse(ctx, *op, P, k, st, dparams, dmatrix);
+}
+
+int slq_cholesky_partial(slq_context* ctx, const slq_operator* op, int64_t rank, int pivot,
+                         float* L, int64_t* perm, int* success) {
+  if (!ctx) return SLQ_EINVAL;
+  SLQ_TRY(check_operator(ctx, op));
+  const int64_t n = op->n;
+  SLQ_TRY(check_rank(ctx, rank, n));
+  if (!L || !success) return fail(ctx, SLQ_EINVAL, "output pointer is NULL");
+  SLQ_TRY(set_device(ctx));
+  const int64_t nblk = (n + 255) / 256;
+  float *lc, *bval, *rowi, *lii;
+  int64_t *pm, *bidx;
+  SLQ_TRY(ensure_t(ctx, ctx->lr_lc, size_t(rank * n), &lc));
+  SLQ_TRY(ensure_t(ctx, ctx->lr_perm, size_t(n), &pm));
+  SLQ_TRY(ensure_t(ctx, ctx->lr_bval, size_t(nblk), &bval));
+  SLQ_TRY(ensure_t(ctx, ctx->lr_bidx, size_t(nblk), &bidx));
+  SLQ_TRY(ensure_t(ctx, ctx->lr_rowi, size_t(rank + 1), &rowi));
+  lii = rowi + rank;
+  SLQ_HIP(ctx, launch_cholesky_partial(ctx->stream, *op, int(rank), pivot != 0, lc, pm, bval,
+                                       bidx, rowi, lii, success, L, perm));
+  return SLQ_OK;
+}
+
+int slq_lowrank_solve(slq_context* ctx, const float* L, int64_t n, int64_t rank, double shift,
+                      const float* V, int64_t P, float* out) {
+  if (!ctx) return SLQ_EINVAL;
+  SLQ_TRY(check_rank(ctx, rank, n));
+  if (!(shift > 0.0)) return fail(ctx, SLQ_EINVAL, "the shift s must be positive");
+  if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of vectors P must be positive");
+  if (!L || !V || !out) return fail(ctx, SLQ_EINVAL, "input or output pointer is NULL");
+  SLQ_TRY(set_device(ctx));
+  double *C, *z;
+  float* wp;
+  SLQ_TRY(woodbury_factor(ctx, L, n, rank, shift, &C));
+  SLQ_TRY(ensure_t(ctx, ctx->lr_wp, size_t(woodbury_chunks(n) * P * rank), &wp));
+  SLQ_TRY(ensure_t(ctx, ctx->lr_z, size_t(P * rank), &z));
+  SLQ_HIP(ctx, launch_woodbury_apply(ctx->stream, L, n, int(rank), shift, C, V, P, wp, z, out));
+  return SLQ_OK;
+}
+
+int slq_cg(slq_context* ctx, const slq_operator* op, const float* B, int64_t P, double tol,
+           double atol, int64_t maxiter, const float* L, int64_t rank, double shift, float* X,
+           int64_t* iterations) {
+  if (!ctx) return SLQ_EINVAL;
+  SLQ_TRY(check_operator(ctx, op));
+  const int64_t n = op->n;
+  if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of right-hand sides P must be positive");
+  if (!B || !X) return fail(ctx, SLQ_EINVAL, "input or output pointer is NULL");
+  if (!(tol >= 0.0) || !(atol >= 0.0) || maxiter < 0)
+    return fail(ctx, SLQ_EINVAL, "tol, atol and maxiter must be non-negative");
+  const bool precond = L != nullptr;
+  if (precond) {
+    SLQ_TRY(check_rank(ctx, rank, n));
+    if (!(shift > 0.0)) return fail(ctx, SLQ_EINVAL, "the shift s must be positive");
+  }
+  SLQ_TRY(set_device(ctx));
+  hipStream_t s = ctx->stream;
+  if (!ctx->host_count) {
+    void* h = nullptr;
+    if (hipHostMalloc(&h, 64, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(ctx, SLQ_ENOMEM, "pinned host allocation failed");
+    }
+    ctx->host_count = static_cast<int*>(h);
+  }
+  const int64_t nb = cg_chunks(n);
+  float *r, *z = nullptr, *d, *q, *parts, *scal, *mvs;
+  SLQ_TRY(ensure_t(ctx, ctx->cg_r, size_t(P * n), &r));
+  SLQ_TRY(ensure_t(ctx, ctx->cg_d, size_t(P * n), &d));
+  SLQ_TRY(ensure_t(ctx, ctx->cg_q, size_t(P * n), &q));
+  if (precond) SLQ_TRY(ensure_t(ctx, ctx->cg_z, size_t(P * n), &z));
+  SLQ_TRY(ensure_t(ctx, ctx->cg_parts, size_t(4 * P * nb), &parts));
+  // scalars: gamma, atol2, alpha, beta (floats), iters (int64), active, upd,
+  // count (ints), 16 words per column plus the count
+  SLQ_TRY(ensure_t(ctx, ctx->cg_scal, size_t(16 * P + 64), &scal));
+  const int64_t mv_floats = matvec_scratch_floats(n, P);
+  SLQ_TRY(ensure_t(ctx, ctx->splitk, size_t(mv_floats), &mvs));
+  float *pq = parts, *rr = parts + P * nb, *rz = parts + 2 * P * nb, *bb = parts + 3 * P * nb;
+  CgScalars sc;
+  sc.gamma = scal;
+  sc.atol2 = scal + P;
+  sc.alpha = scal + 2 * P;
+  sc.beta = scal + 3 * P;
+  sc.iters = reinterpret_cast<int64_t*>(scal + 4 * P);  // 16 P bytes in: 8-byte aligned
+  sc.active = reinterpret_cast<int*>(scal + 8 * P + 2);
+  sc.upd = reinterpret_cast<int*>(scal + 9 * P + 2);
+  int* nactive = reinterpret_cast<int*>(scal + 10 * P + 2);
+  CgPrecond M;
+  if (precond) {
+    double* C;
+    SLQ_TRY(woodbury_factor(ctx, L, n, rank, shift, &C));
+    M.L = L;
+    M.rank = int(rank);
+    M.shift = shift;
+    M.C = C;
+    SLQ_TRY(ensure_t(ctx, ctx->lr_wp, size_t(woodbury_chunks(n) * P * rank), &M.Wp));
+    SLQ_TRY(ensure_t(ctx, ctx->lr_z, size_t(P * rank), &M.Z));
+  }
+  // x0 = 0; r0 = b - A x0 = b; p0 = z0 = M r0; gamma0 = r0 . z0
+  SLQ_HIP(ctx, hipMemsetAsync(X, 0, size_t(P * n) * sizeof(float), s));
+  SLQ_HIP(ctx, hipMemcpyAsync(r, B, size_t(P * n) * sizeof(float), hipMemcpyDeviceToDevice, s));
+  if (precond) {
+    SLQ_HIP(ctx, launch_woodbury_apply(s, L, n, int(rank), shift, M.C, r, P, M.Wp, M.Z, z));
+    SLQ_HIP(ctx, launch_cg_dot(s, r, z, P, n, rz));
+    SLQ_HIP(ctx, launch_cg_dot(s, r, r, P, n, rr));
+  } else {
+    SLQ_HIP(ctx, launch_cg_dot(s, r, r, P, n, rz));
+  }
+  SLQ_HIP(ctx, hipMemcpyAsync(d, precond ? z : r, size_t(P * n) * sizeof(float),
+                              hipMemcpyDeviceToDevice, s));
+  SLQ_HIP(ctx, launch_cg_dot(s, B, B, P, n, bb));
+  SLQ_HIP(ctx, launch_cg_start(s, rz, bb, precond ? rr : nullptr, P, n, float(tol), float(atol),
+                               maxiter, sc, nactive));
+  while (true) {
+    SLQ_HIP(ctx, hipMemcpyAsync(ctx->host_count, nactive, sizeof(int), hipMemcpyDeviceToHost, s));
+    SLQ_HIP(ctx, hipStreamSynchronize(s));
+    if (*ctx->host_count == 0) break;
+    {
+      const auto ev = take_events(ctx, SLQ_KC_MATVEC);
+      SLQ_HIP(ctx, launch_matvec(s, *op, d, P, q, mvs, mv_floats, ev.first, ev.second));
+    }
+    SLQ_HIP(ctx, launch_cg_step(s, X, r, d, q, z, precond, P, n, maxiter, sc, pq, rr, rz, nactive,
+                                precond ? &M : nullptr));
+  }
+  if (iterations)
+    SLQ_HIP(ctx, hipMemcpyAsync(iterations, sc.iters, size_t(P) * sizeof(int64_t),
+                                hipMemcpyDeviceToDevice, s));
+  return SLQ_OK;
+}
+
+int slq_quadform_vjp(slq_context* ctx, const slq_operator* op, const float* X, int64_t P,
+                     double* dparams, float* dmatrix) {
+  if (!ctx) return SLQ_EINVAL;
+  SLQ_TRY(check_operator(ctx, op));
+  const int64_t n = op->n;
+  if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of vectors P must be positive");
+  if (!X) return fail(ctx, SLQ_EINVAL, "input pointer is NULL");
+  if (op->kind == SLQ_OP_RBF && !dparams)
+    return fail(ctx, SLQ_EINVAL, "dparams (2 doubles) is NULL");
+  if (op->kind == SLQ_OP_DENSE && !dmatrix)
+    return fail(ctx, SLQ_EINVAL, "dmatrix (n x n floats) is NULL");
+  SLQ_TRY(set_device(ctx));
+  hipStream_t s = ctx->stream;
+  if (op->kind == SLQ_OP_RBF) {
+    double* dsum;
+    SLQ_TRY(ensure_t(ctx, ctx->adj_sum, size_t(2 * outer_contract_blocks(n)), &dsum));
+    SLQ_HIP(ctx, launch_outer_rbf_contract(s, X, X, P * n, P, n, P, op->points, op->dim,
+                                           op->lengthscale, dsum));
+    const double ls = op->lengthscale;
+    SLQ_HIP(ctx, launch_sum_fixed(s, dsum, outer_contract_blocks(n), 2, 1.0 / (ls * ls * ls), 1.0,
+                                  dparams));
+  } else {
+    SLQ_HIP(ctx, launch_outer_sum(s, X, X, P * n, P, n, P, dmatrix));
+  }
+  return SLQ_OK;
 }
 
 int slq_logdet_lanczos_vjp(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P,

@@ -163,4 +163,60 @@ hipError_t launch_outer_rbf_contract(hipStream_t s, const float* L, const float*
 hipError_t launch_sum_fixed(hipStream_t s, const double* partial, int64_t count, int nout,
                             double f0, double f1, double* out);
 
+// ---- GP caller: partial Cholesky, Woodbury solve, CG (k_lowrank.hip) ------
+// Partial Cholesky of the operator's matrix (elements A[r][c], or the RBF
+// kernel plus noise on the diagonal) to rank `rank`, with or without pivoting.
+// Scratch: Lc rank x n floats, perm n int64, bval/bidx ceil(n/256) each,
+// rowi rank floats, lii 1 float.  Output: L (n x rank, row-major, original
+// row order), perm_out (optional, n int64: position -> row), success (1 if
+// every l_ii^2 > 0).
+hipError_t launch_cholesky_partial(hipStream_t s, const slq_operator& op, int rank, bool pivot,
+                                   float* Lc, int64_t* perm, float* bval, int64_t* bidx,
+                                   float* rowi, float* lii, int* success, float* L,
+                                   int64_t* perm_out);
+// Woodbury factor of s I + L L^T: C = chol(I + L^T L / s) (r x r FP64, lower,
+// row-major).  Gp: gram_splits(n) * r * r floats of scratch; ok = 0 if C is
+// not positive definite.
+int64_t gram_splits(int64_t n);
+hipError_t launch_woodbury_setup(hipStream_t s, const float* L, int64_t n, int r, double shift,
+                                 float* Gp, double* C, int* ok);
+// out = (s I + L L^T)^{-1} V for the P rows of V.  Wp: woodbury_chunks(n) *
+// P * r floats, Z: P * r doubles of scratch.
+int64_t woodbury_chunks(int64_t n);
+hipError_t launch_woodbury_apply(hipStream_t s, const float* L, int64_t n, int r, double shift,
+                                 const double* C, const float* V, int64_t P, float* Wp, double* Z,
+                                 float* out);
+// CG over P columns: per-column scalars and the preconditioner's buffers.
+struct CgScalars {
+  float* gamma = nullptr;
+  float* atol2 = nullptr;
+  float* alpha = nullptr;
+  float* beta = nullptr;
+  int64_t* iters = nullptr;
+  int* active = nullptr;
+  int* upd = nullptr;
+};
+struct CgPrecond {
+  const float* L = nullptr;
+  int rank = 0;
+  double shift = 1.0;
+  const double* C = nullptr;
+  float* Wp = nullptr;
+  double* Z = nullptr;
+};
+// partial dots of the CG kernels: P x cg_chunks(n) floats
+int64_t cg_chunks(int64_t n);
+hipError_t launch_cg_dot(hipStream_t s, const float* a, const float* b, int64_t P, int64_t n,
+                         float* part);
+// gamma = sum rz, atol2 = max(tol^2 sum bb, atol^2), rs = sum rr (or gamma
+// when rr is null), active = rs > atol2 && maxiter > 0; *nactive = count
+hipError_t launch_cg_start(hipStream_t s, const float* rz, const float* bb, const float* rr,
+                           int64_t P, int64_t n, float tol, float atol, int64_t maxiter,
+                           const CgScalars& sc, int* nactive);
+// One CG iteration after q = A d: alpha, x and r, z = M r (precond), gamma,
+// beta, the stopping rule, d = z + beta d; *nactive = columns still active.
+hipError_t launch_cg_step(hipStream_t s, float* x, float* r, float* d, const float* q, float* z,
+                          bool precond, int64_t P, int64_t n, int64_t maxiter, const CgScalars& sc,
+                          float* pq, float* rr, float* rz, int* nactive, const CgPrecond* M);
+
 }  // namespace slq

@@ -27,6 +27,15 @@ Pinning (what keeps the restatement honest; tests live in
   the dense closed form ``d/dtheta x^T log(A) x`` at full depth.
 * ``slq.tridiag_eig_quadrature`` (the FP64 eigen/quadrature checker used for the
   GPU kernel): ``numpy.linalg.eigh`` of the dense tridiagonal matrix.
+* ``slq.lanczos_forward`` / ``slq.lanczos_adjoint`` (``reortho="none"``): the
+  reference's decomposition relation and reconstruction, agreement with
+  Arnoldi in FP64 while orthogonal, central differences for random
+  cotangents on every output, planted mistakes.
+* ``lowrank`` (partial Cholesky with and without pivoting, the Woodbury
+  preconditioner, CG with JAX's stopping rule, the Gaussian log-density of the
+  GP tutorial): the reference's low-rank tests, ``numpy.linalg.cholesky``,
+  dense solves, CG's termination in at most d steps for d distinct
+  eigenvalues, ``scipy.stats.multivariate_normal.logpdf``, planted mistakes.
 
 No function is left "parity unpinned" except the bitwise XLA summation order
 and LAPACK ``ssyevd`` last bits, which only an import of JAX could fix (JAX is

@@ -83,3 +83,34 @@ def test_rbf_operator_validation():
         operators.RBF(np.zeros((4, 5)))
     with pytest.raises(ValueError):
         operators.RBF(np.zeros((4, 3)), precision="fp64")
+
+
+# -- low_rank / linalg / gp (low_rank.py:56-198, tutorials/8_gaussian_logpdf.py) --
+
+
+@pytest.mark.parametrize("factory,word", [("cholesky_partial", "n"),
+                                          ("cholesky_partial_pivot", "nrows")])
+def test_cholesky_rank_out_of_range_raises(factory, word):
+    # low_rank.py:74-79 and 118-123: raised before any device use
+    from matfree_amd import low_rank
+
+    f = getattr(low_rank, factory)
+    with pytest.raises(ValueError, match=f"Rank exceeds {word}: 6 >= 5."):
+        f(operators.dense_matvec, nrows=5, rank=6)(np.eye(5))
+    with pytest.raises(ValueError, match="Rank must be positive, but 0 < 1."):
+        f(operators.dense_matvec, nrows=5, rank=0)(np.eye(5))
+
+
+def test_cholesky_rejects_python_element_functions():
+    from matfree_amd import low_rank
+
+    cov = np.eye(4)
+    with pytest.raises(TypeError, match="operators"):
+        low_rank.cholesky_partial_pivot(lambda i, j: cov[i, j], nrows=4, rank=2)()
+
+
+def test_cg_rejects_foreign_preconditioners():
+    from matfree_amd import linalg
+
+    with pytest.raises(TypeError):
+        linalg.cg(lambda v: v, np.ones(3))
