@@ -9,7 +9,11 @@ Arnoldi step), so ``value = probes * depth * steps / seconds``.
 Default workload: the matrix-free RBF log-determinant, n = 32768 points
 uniform in [0, 1]^3 (seeded), K_ij = exp(-|x_i - x_j|^2 / (2 * 0.5^2)) +
 1e-2 delta_ij, 128 probes, depth 64.  ``--workload dense`` runs the stored
-dense SPD case (n = 16384, 256 probes, depth 64, A = B B^T / n + 0.1 I).
+dense SPD case (n = 16384, 256 probes, depth 64, A = B B^T / n + 0.1 I);
+``rbf-none`` the RBF case with ``tridiag_sym(64, reortho="none")`` (plain
+Lanczos, no basis); ``rbf-grad`` value and gradient at n = 8192, 32 probes,
+depth 32; ``gp-logpdf`` the Gaussian log-density of the GP tutorial (SLQ
+log-determinant + CG, value and gradient) at n = 8192.
 
 Printed: one JSON line with the throughput, the relative error against an
 FP64 dense-Cholesky log-determinant of the same operator, the roofline of the
@@ -38,8 +42,11 @@ WORKLOADS = {
     "rbf": dict(n=32768, probes=128, depth=64, lengthscale=0.5, noise=1e-2),
     "dense": dict(n=16384, probes=256, depth=64, shift=0.1),
     "rbf-grad": dict(n=8192, probes=32, depth=32, lengthscale=0.5, noise=1e-2),
+    "rbf-none": dict(n=32768, probes=128, depth=64, lengthscale=0.5, noise=1e-2),
+    "gp-logpdf": dict(n=8192, probes=32, depth=32, lengthscale=0.5, noise=1e-2, rank=128),
 }
-DEFAULT_PRECISION = {"rbf": "bf16x2", "dense": "fp32", "rbf-grad": "fp32"}
+DEFAULT_PRECISION = {"rbf": "bf16x2", "dense": "fp32", "rbf-grad": "fp32", "rbf-none": "bf16x2",
+                     "gp-logpdf": "fp32"}
 # Serving mix: n log-uniform in [2048, 32768], probes in {16, 32, 64, 128},
 # depth uniform in [16, 64], RBF (bf16x2) or stored dense (fp32, n capped at
 # 16384 to bound the per-request matrix at 1 GB).
@@ -79,7 +86,7 @@ def setup(args, device):
     w = WORKLOADS[args.workload]
     n, P, k = w["n"], w["probes"], w["depth"]
     prec = args.precision or DEFAULT_PRECISION[args.workload]
-    if args.workload == "rbf":
+    if args.workload in ("rbf", "rbf-none"):
         X = problems.rbf_points(n, seed=args.seed)
         op = operators.RBF(X, precision=prec)
         params = (w["lengthscale"], w["noise"])
@@ -101,7 +108,7 @@ def fp64_reference(args, host, device):
     from matfree_amd import problems
 
     w = WORKLOADS[args.workload]
-    if args.workload == "rbf":
+    if args.workload in ("rbf", "rbf-none"):
         K = problems.rbf_dense64_device(host["points"], w["lengthscale"], w["noise"], device)
     else:
         K = host["matrix"].to(torch.float64)
@@ -111,13 +118,13 @@ def fp64_reference(args, host, device):
     return val
 
 
-def timed_solves(op, params, Z, k, steps, warmup, ctx):
+def timed_solves(op, params, Z, k, steps, warmup, ctx, reortho="full"):
     import torch
 
     from matfree_amd import funm
 
     for _ in range(warmup):
-        funm.logdet_per_probe(op, Z, k, params)
+        funm.logdet_per_probe(op, Z, k, params, reortho=reortho)
     torch.cuda.synchronize()
     ctx.profile(True)
     for kc in ("matvec", "dots", "project_dots", "project", "normalize", "quadrature"):
@@ -126,7 +133,7 @@ def timed_solves(op, params, Z, k, steps, warmup, ctx):
     t0 = time.perf_counter()
     mean = None
     for _ in range(steps):
-        _, mean = funm.logdet_per_probe(op, Z, k, params)
+        _, mean = funm.logdet_per_probe(op, Z, k, params, reortho=reortho)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     kernels = {}
@@ -300,6 +307,105 @@ def grad_step(args, device):
     }
 
 
+def gp_step(args, device):
+    """The GP tutorial's log-density, value and gradient (tutorials/8_gaussian_logpdf.py).
+
+    logpdf(y) = -(y . A^{-1} y + logdet A + n log(2 pi)) / 2 with the SLQ
+    estimate of logdet A (Arnoldi, FP32) and CG for A^{-1} y, differentiated
+    in (lengthscale, noise) by torch autograd; beside it CG with the rank-r
+    pivoted partial-Cholesky preconditioner of K, and the dense FP64 value.
+    """
+    import math
+
+    import numpy as np
+    import torch
+
+    from matfree_amd import _runtime, gp, linalg, low_rank, operators, problems, prng
+
+    w = WORKLOADS["gp-logpdf"]
+    n, P, k, rank = w["n"], w["probes"], w["depth"], w["rank"]
+    prec = args.precision or DEFAULT_PRECISION["gp-logpdf"]
+    Xh = problems.rbf_points(n, seed=args.seed)
+    X = torch.as_tensor(Xh, device=device)
+    op = operators.RBF(X, precision=prec)
+    y = torch.as_tensor(np.random.default_rng(args.seed).standard_normal(n).astype(np.float32),
+                        device=device)
+    logpdf = gp.create_logpdf(op, np.zeros(n, np.float32), num_matvecs=k, num_samples=P)
+    key = prng.PRNGKey(args.seed)
+
+    def value_and_grad():
+        ls = torch.tensor(w["lengthscale"], dtype=torch.float64, device=device, requires_grad=True)
+        s2 = torch.tensor(w["noise"], dtype=torch.float64, device=device, requires_grad=True)
+        val = logpdf(y, key, ls, s2)
+        val.backward()
+        return val.detach(), ls.grad, s2.grad
+
+    for _ in range(args.warmup):
+        value_and_grad()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        val, gl, gs = value_and_grad()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    params = (w["lengthscale"], w["noise"])
+    ctx = _runtime.context(device)
+    ctx.profile(True)
+    ctx.profile_read("matvec")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    x0, it0 = linalg.cg_batched(op, y, params)
+    torch.cuda.synchronize()
+    t_cg = time.perf_counter() - t1
+    mv_ms, mv_n = ctx.profile_read("matvec")
+    ctx.profile(False)
+    t2 = time.perf_counter()
+    M = low_rank.preconditioner(low_rank.cholesky_partial_pivot(op, nrows=n, rank=rank)).bind(
+        w["noise"], w["lengthscale"], 0.0)
+    torch.cuda.synchronize()
+    t_chol = time.perf_counter() - t2
+    t3 = time.perf_counter()
+    x1, it1 = linalg.cg_batched(op, y, params, M=M)
+    torch.cuda.synchronize()
+    t_pcg = time.perf_counter() - t3
+    out = {
+        "metric": "GP log-density value + gradient (d/d lengthscale, d/d noise) time",
+        "value": dt * 1e3,
+        "unit": "ms",
+        "higher_is_better": False,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "dtype": prec,
+        "data": "synthetic (seeded points and targets, Rademacher probes from PRNGKey(seed))",
+        "config": {"workload": f"gp-logpdf n={n} probes={P} depth={k}", "n": n, "probes": P,
+                   "depth": k, "precision": prec, "lengthscale": w["lengthscale"],
+                   "noise": w["noise"], "cg_tol": 1e-5},
+        "logpdf": float(val),
+        "grad": [float(gl), float(gs)],
+        "cg": {"iterations": int(it0), "ms": t_cg * 1e3,
+               "matvec_ms_per_iteration": mv_ms / max(mv_n, 1)},
+        "pcg": {"rank": rank, "iterations": int(it1), "ms": t_pcg * 1e3,
+                "cholesky_ms": t_chol * 1e3},
+    }
+    if not args.no_reference:
+        K = problems.rbf_dense64_device(Xh, w["lengthscale"], w["noise"], device)
+        L = torch.linalg.cholesky(K)
+        y64 = y.to(torch.float64)
+        alpha = torch.cholesky_solve(y64[:, None], L)[:, 0]
+        logdet = 2.0 * float(torch.log(torch.diagonal(L)).sum())
+        ref = -0.5 * (float(y64 @ alpha) + logdet + n * math.log(2 * math.pi))
+        out["logpdf_fp64"] = ref
+        out["rel_err_vs_fp64"] = abs(float(val) - ref) / abs(ref)
+        out["cg_rel_err_vs_fp64"] = float(torch.linalg.norm(x0.to(torch.float64) - alpha)
+                                          / torch.linalg.norm(alpha))
+        out["pcg_rel_err_vs_fp64"] = float(torch.linalg.norm(x1.to(torch.float64) - alpha)
+                                           / torch.linalg.norm(alpha))
+        del K, L
+        torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(args, host, device, n, k):
     """NumPy restatement of matfree's algorithm (the oracle) on a bounded sample."""
     import numpy as np
@@ -310,7 +416,7 @@ def cpu_baseline(args, host, device, n, k):
 
     w = WORKLOADS[args.workload]
     probes = 96  # about 10-20 s of host work at the restatement's rate
-    if args.workload == "rbf":
+    if args.workload in ("rbf", "rbf-none"):
         from matfree_amd import problems
 
         K = problems.rbf_dense64_device(host["points"], w["lengthscale"], w["noise"], device)
@@ -328,11 +434,16 @@ def cpu_baseline(args, host, device, n, k):
     except Exception:
         cores = len(os.sched_getaffinity(0))
     t0 = time.perf_counter()
-    oslq.quadform_logdet(op, k, Z, np.float32)
+    if args.workload == "rbf-none":
+        oslq.quadform_logdet_lanczos(op, k, Z, np.float32)
+        form = "batched A@V, three-term recurrence, eigh"
+    else:
+        oslq.quadform_logdet(op, k, Z, np.float32)
+        form = "batched A@V, full-width CGS2, eigh"
     dt = time.perf_counter() - t0
     sample = (f"{probes} probes x {k} steps of the same n={n} {args.workload} operator, "
-              "FP32 NumPy restatement of matfree's formulation (batched A@V, full-width CGS2, "
-              "eigh); operator materialised on the host before timing")
+              f"FP32 NumPy restatement of matfree's formulation ({form}); operator "
+              "materialised on the host before timing")
     return {"value": probes * k / dt, "unit": UNIT, "cores": cores, "kind": "port",
             "sample": sample, "seconds": dt}
 
@@ -355,10 +466,14 @@ def main():
     if args.workload == "rbf-grad":
         print(json.dumps(grad_step(args, device)), flush=True)
         return
+    if args.workload == "gp-logpdf":
+        print(json.dumps(gp_step(args, device)), flush=True)
+        return
     op, params, Z, (n, P, k), prec, host = setup(args, device)
     ctx = _runtime.context(device)
     log(f"bench: workload={args.workload} n={n} P={P} k={k} precision={prec}")
-    dt, est, kernels = timed_solves(op, params, Z, k, args.steps, args.warmup, ctx)
+    reortho = "none" if args.workload == "rbf-none" else "full"
+    dt, est, kernels = timed_solves(op, params, Z, k, args.steps, args.warmup, ctx, reortho)
     value = P * k * args.steps / dt
     log(f"bench: {dt / args.steps * 1e3:.2f} ms/solve, {value:.0f} {UNIT}, logdet {est:.8e}")
 

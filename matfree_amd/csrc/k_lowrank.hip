@@ -30,7 +30,8 @@
 // (x0 = 0; rs = gamma without a preconditioner, r . r with one; a converged
 // column keeps its state while the others iterate).  Per-column scalars live
 // in small device arrays updated by one-block kernels; the host reads the
-// number of active columns once per iteration.
+// number of active columns after batches of 1, 2, 4, 8, 8, ... iterations
+// (an iteration with no active column changes nothing).
 #include <cfloat>
 
 #include "slq_internal.h"
@@ -408,22 +409,17 @@ __global__ void __launch_bounds__(kT)
   if (threadIdx.x == 0) *nactive = int(tot + 0.5f);
 }
 
-// alpha = gamma / (p . A p) for the active columns
-__global__ void __launch_bounds__(kT)
-    k_cg_alpha(const float* __restrict__ pq, int64_t P, int64_t nb, CgScalars sc) {
-  for (int64_t p = threadIdx.x; p < P; p += kT)
-    sc.alpha[p] = sc.active[p] ? sc.gamma[p] / sum_parts(pq, p, nb) : 0.f;
-}
-
-// x += alpha p, r -= alpha A p (active columns) and the partial r . r
+// alpha = gamma / (p . A p) (every block sums the partials itself), then
+// x += alpha p, r -= alpha A p for the active columns and the partial r . r
 __global__ void __launch_bounds__(kT)
     k_cg_update(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ d,
-                const float* __restrict__ q, int64_t n, CgScalars sc, float* __restrict__ rr) {
+                const float* __restrict__ q, int64_t n, const float* __restrict__ pq,
+                CgScalars sc, float* __restrict__ rr) {
   __shared__ float red[kT / 64];
   const int64_t p = blockIdx.y, c = blockIdx.x, nb = gridDim.x;
   const int64_t base = c * kCgChunk;
   const bool act = sc.active[p] != 0;
-  const float al = sc.alpha[p];
+  const float al = act ? sc.gamma[p] / sum_parts(pq, p, nb) : 0.f;
   float acc = 0.f;
 #pragma unroll
   for (int e = 0; e < kCgPer; ++e) {
@@ -570,8 +566,7 @@ hipError_t launch_cg_step(hipStream_t s, float* x, float* r, float* d, const flo
   const int64_t nb = cg_chunks(n);
   const dim3 grid{unsigned(nb), unsigned(P)};
   hipLaunchKernelGGL(k_cg_dot, grid, dim3(kT), 0, s, d, q, n, pq);
-  hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kT), 0, s, pq, P, nb, sc);
-  hipLaunchKernelGGL(k_cg_update, grid, dim3(kT), 0, s, x, r, d, q, n, sc, rr);
+  hipLaunchKernelGGL(k_cg_update, grid, dim3(kT), 0, s, x, r, d, q, n, pq, sc, rr);
   if (precond) {
     hipError_t e = launch_woodbury_apply(s, M->L, n, M->rank, M->shift, M->C, r, P, M->Wp, M->Z, z);
     if (e != hipSuccess) return e;

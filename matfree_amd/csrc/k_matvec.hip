@@ -821,6 +821,126 @@ __global__ void __launch_bounds__(kThreads)
     store_out<NACC, VEC, true>(Wout, V, P, n, i0w, p0, acc, prm.noise);
 }
 
+// RBF matvec for a few vectors (P <= kRowsMaxP), as CG's single right-hand
+// side needs it: the MFMA kernels pad the probe dimension to 32, so here the
+// kernel is evaluated on the vector units instead, exactly in FP32
+// (d2 = sum (x_i - x_j)^2 from the coordinates, exp2(coef d2)).  A workgroup
+// owns kRowsRB rows (kRowsPerLane per lane) and a column slice; chunks of 256
+// points (coordinates padded to four, one 16-byte LDS read per point, and the
+// vector entries) are staged in LDS and each wave takes a quarter of every
+// chunk; the four waves' sums are added in LDS in a fixed order.  Column
+// slices > 1 write split-K partial sums.
+constexpr int kRowsMaxP = 4;
+constexpr int kRowsPerLane = 2;
+constexpr int kRowsRB = 64 * kRowsPerLane;
+constexpr int kRowsChunk = 256;
+constexpr int kRowsMaxSplit = 16;
+
+template <int NP>
+__global__ void __launch_bounds__(256)
+    k_matvec_rbf_rows(RBFParams prm, const float* __restrict__ V, int64_t n,
+                      float* __restrict__ W) {
+  __shared__ float4 xs[kRowsChunk];
+  __shared__ float vs[kRowsChunk * NP];
+  __shared__ float red[3][NP][kRowsRB];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int dim = prm.dim;
+  const int64_t j0 = n * blockIdx.y / gridDim.y, j1 = n * (blockIdx.y + 1) / gridDim.y;
+  float4 xi[kRowsPerLane];
+#pragma unroll
+  for (int u = 0; u < kRowsPerLane; ++u) {
+    const int64_t row = int64_t(blockIdx.x) * kRowsRB + 64 * u + lane;
+    float c[4] = {0.f, 0.f, 0.f, 0.f};
+    if (row < n)
+      for (int d = 0; d < dim; ++d) c[d] = prm.X[row * dim + d];
+    xi[u] = make_float4(c[0], c[1], c[2], c[3]);
+  }
+  float acc[kRowsPerLane][NP];
+#pragma unroll
+  for (int u = 0; u < kRowsPerLane; ++u)
+#pragma unroll
+    for (int q = 0; q < NP; ++q) acc[u][q] = 0.f;
+  for (int64_t c0 = j0; c0 < j1; c0 += kRowsChunk) {
+    const int64_t cnt = j1 - c0 < kRowsChunk ? j1 - c0 : kRowsChunk;
+    {
+      const int64_t j = c0 + t;
+      const bool ok = t < cnt;
+      float c[4] = {0.f, 0.f, 0.f, 0.f};
+      if (ok)
+        for (int d = 0; d < dim; ++d) c[d] = prm.X[j * dim + d];
+      xs[t] = make_float4(c[0], c[1], c[2], c[3]);
+#pragma unroll
+      for (int q = 0; q < NP; ++q) vs[t * NP + q] = ok ? V[q * n + j] : 0.f;
+    }
+    __syncthreads();
+    const int jb = wave * (kRowsChunk / 4);
+    const int je = jb + kRowsChunk / 4 < cnt ? jb + kRowsChunk / 4 : int(cnt);
+    for (int jj = jb; jj < je; ++jj) {
+      const float4 xj = xs[jj];
+      float vj[NP];
+#pragma unroll
+      for (int q = 0; q < NP; ++q) vj[q] = vs[jj * NP + q];
+#pragma unroll
+      for (int u = 0; u < kRowsPerLane; ++u) {
+        const float dx = xi[u].x - xj.x, dy = xi[u].y - xj.y;
+        const float dz = xi[u].z - xj.z, dw = xi[u].w - xj.w;
+        const float d2 = fmaf(dw, dw, fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+        const float k = __builtin_amdgcn_exp2f(prm.coef * d2);
+#pragma unroll
+        for (int q = 0; q < NP; ++q) acc[u][q] = fmaf(k, vj[q], acc[u][q]);
+      }
+    }
+    __syncthreads();
+  }
+  if (wave > 0)
+#pragma unroll
+    for (int u = 0; u < kRowsPerLane; ++u)
+#pragma unroll
+      for (int q = 0; q < NP; ++q) red[wave - 1][q][64 * u + lane] = acc[u][q];
+  __syncthreads();
+  if (wave == 0) {
+    const bool split = gridDim.y > 1;
+#pragma unroll
+    for (int u = 0; u < kRowsPerLane; ++u) {
+      const int64_t row = int64_t(blockIdx.x) * kRowsRB + 64 * u + lane;
+      if (row >= n) continue;
+      const int li = 64 * u + lane;
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        const float sum = ((acc[u][q] + red[0][q][li]) + red[1][q][li]) + red[2][q][li];
+        if (split)
+          W[int64_t(blockIdx.y) * NP * n + q * n + row] = sum;
+        else
+          W[q * n + row] = fmaf(prm.noise, V[q * n + row], sum);
+      }
+    }
+  }
+}
+
+hipError_t launch_rows(hipStream_t s, const RBFParams& prm, const float* V, int64_t P, int64_t n,
+                       float* W, int S) {
+  const dim3 grid(unsigned((n + kRowsRB - 1) / kRowsRB), unsigned(S));
+  switch (P) {
+    case 1: hipLaunchKernelGGL(k_matvec_rbf_rows<1>, grid, dim3(256), 0, s, prm, V, n, W); break;
+    case 2: hipLaunchKernelGGL(k_matvec_rbf_rows<2>, grid, dim3(256), 0, s, prm, V, n, W); break;
+    case 3: hipLaunchKernelGGL(k_matvec_rbf_rows<3>, grid, dim3(256), 0, s, prm, V, n, W); break;
+    default: hipLaunchKernelGGL(k_matvec_rbf_rows<4>, grid, dim3(256), 0, s, prm, V, n, W); break;
+  }
+  return hipGetLastError();
+}
+
+// Column slices of the row kernel: up to kRowsMaxSplit, while there are fewer
+// than 1024 workgroups, every slice keeps two chunks and the partial sums
+// fit the scratch.
+int choose_rows_splitk(int64_t n, int64_t P, int64_t avail_floats) {
+  const int64_t wgs = (n + kRowsRB - 1) / kRowsRB;
+  int S = 1;
+  while (S < kRowsMaxSplit && wgs * S < 1024 && n / (2 * S) >= 2 * kRowsChunk &&
+         int64_t(2 * S) * P * n <= avail_floats)
+    S *= 2;
+  return S;
+}
+
 inline bool aligned16(const void* ptr) { return (reinterpret_cast<uintptr_t>(ptr) & 15) == 0; }
 
 // W = sum_z partial[z] (+ s2 V), summed in the fixed order z = 0..S-1.
@@ -940,6 +1060,26 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
                          hipEvent_t ev1) {
   const int64_t n = op.n;
   if (n <= 0 || P <= 0) return hipSuccess;
+  if (op.kind == SLQ_OP_RBF && P <= kRowsMaxP && op.dim >= 1 && op.dim <= 4) {
+    // a few vectors: the row kernel in FP32 (every precision gets FP32 here)
+    RBFParams prm{};
+    prm.X = op.points;
+    prm.dim = op.dim;
+    prm.coef = float(-1.4426950408889634 / (2.0 * op.lengthscale * op.lengthscale));
+    prm.noise = float(op.noise);
+    const int S = scratch ? choose_rows_splitk(n, P, scratch_floats) : 1;
+    float* out = S > 1 ? scratch : W;
+    if (ev0) (void)hipEventRecord(ev0, s);
+    hipError_t err = launch_rows(s, prm, V, P, n, out, S);
+    if (ev1) (void)hipEventRecord(ev1, s);
+    if (err != hipSuccess || S == 1) return err;
+    const int64_t count = P * n;
+    int64_t blocks = (count + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_splitk_reduce<true>, dim3(unsigned(blocks)), dim3(256), 0, s, scratch,
+                       S, count, V, prm.noise, W);
+    return hipGetLastError();
+  }
   const bool vec = (n % 4 == 0) && aligned16(V) && aligned16(W);
   const bool fp32 = op.precision == SLQ_PREC_FP32;
   const int nprod = op.precision == SLQ_PREC_BF16X3 ? 3 : (op.precision == SLQ_PREC_BF16X2 ? 2 : 1);

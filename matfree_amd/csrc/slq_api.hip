@@ -858,7 +858,6 @@ int slq_cg(slq_context* ctx, const slq_operator* op, const float* B, int64_t P, 
   CgScalars sc;
   sc.gamma = scal;
   sc.atol2 = scal + P;
-  sc.alpha = scal + 2 * P;
   sc.beta = scal + 3 * P;
   sc.iters = reinterpret_cast<int64_t*>(scal + 4 * P);  // 16 P bytes in: 8-byte aligned
   sc.active = reinterpret_cast<int*>(scal + 8 * P + 2);
@@ -890,16 +889,20 @@ int slq_cg(slq_context* ctx, const slq_operator* op, const float* B, int64_t P, 
   SLQ_HIP(ctx, launch_cg_dot(s, B, B, P, n, bb));
   SLQ_HIP(ctx, launch_cg_start(s, rz, bb, precond ? rr : nullptr, P, n, float(tol), float(atol),
                                maxiter, sc, nactive));
-  while (true) {
+  // The host reads the count of active columns after batches of 1, 2, 4, 8,
+  // 8, ... iterations; an iteration with no active column changes nothing.
+  for (int64_t batch = 1;; batch = batch < 8 ? 2 * batch : 8) {
     SLQ_HIP(ctx, hipMemcpyAsync(ctx->host_count, nactive, sizeof(int), hipMemcpyDeviceToHost, s));
     SLQ_HIP(ctx, hipStreamSynchronize(s));
     if (*ctx->host_count == 0) break;
-    {
-      const auto ev = take_events(ctx, SLQ_KC_MATVEC);
-      SLQ_HIP(ctx, launch_matvec(s, *op, d, P, q, mvs, mv_floats, ev.first, ev.second));
+    for (int64_t b = 0; b < batch; ++b) {
+      {
+        const auto ev = take_events(ctx, SLQ_KC_MATVEC);
+        SLQ_HIP(ctx, launch_matvec(s, *op, d, P, q, mvs, mv_floats, ev.first, ev.second));
+      }
+      SLQ_HIP(ctx, launch_cg_step(s, X, r, d, q, z, precond, P, n, maxiter, sc, pq, rr, rz,
+                                  nactive, precond ? &M : nullptr));
     }
-    SLQ_HIP(ctx, launch_cg_step(s, X, r, d, q, z, precond, P, n, maxiter, sc, pq, rr, rz, nactive,
-                                precond ? &M : nullptr));
   }
   if (iterations)
     SLQ_HIP(ctx, hipMemcpyAsync(iterations, sc.iters, size_t(P) * sizeof(int64_t),

@@ -190,7 +190,6 @@ hipError_t launch_woodbury_apply(hipStream_t s, const float* L, int64_t n, int r
 struct CgScalars {
   float* gamma = nullptr;
   float* atol2 = nullptr;
-  float* alpha = nullptr;
   float* beta = nullptr;
   int64_t* iters = nullptr;
   int* active = nullptr;

@@ -88,6 +88,27 @@ def test_rbf_matvec(cuda, n, P, dim):
     assert rel(W.cpu().numpy(), V.astype(np.float64) @ K) < 2e-6
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16x2"])
+@pytest.mark.parametrize("n,P,dim", [(3000, 1, 3), (5000, 2, 1), (777, 3, 2), (20000, 4, 3),
+                                     (50, 1, 3)])
+def test_rbf_matvec_few_vectors(cuda, precision, n, P, dim):
+    # up to four vectors (CG's right-hand sides) run the FP32 row kernel,
+    # whatever the requested precision; several column slices at n >= 2048
+    X = problems.rbf_points(n, seed=n, dim=dim)
+    V = np.random.default_rng(3).standard_normal((P, n)).astype(np.float32)
+    W = operators.RBF(X, precision=precision)(torch.as_tensor(V, device=cuda), 0.4, 0.1)
+    K = problems.rbf_dense64(X, 0.4, 0.1)
+    assert rel(W.cpu().numpy(), V.astype(np.float64) @ K) < 2e-6
+
+
+def test_rbf_matvec_few_vectors_dim4(cuda):
+    X = problems.rbf_points(2500, seed=1, dim=4)
+    V = np.random.default_rng(4).standard_normal((2, 2500)).astype(np.float32)
+    W = operators.RBF(X)(torch.as_tensor(V, device=cuda), 0.6, 0.05)
+    K = problems.rbf_dense64(X, 0.6, 0.05)
+    assert rel(W.cpu().numpy(), V.astype(np.float64) @ K) < 2e-6
+
+
 @pytest.mark.parametrize("precision,tol", [("bf16x3", 3e-5), ("bf16x2", 4e-3), ("bf16", 6e-3)])
 @pytest.mark.parametrize("n,P,dim", [(1000, 16, 3), (256, 128, 3), (130, 70, 3), (333, 40, 1),
                                      (96, 200, 2)])
