@@ -413,6 +413,40 @@ __global__ void __launch_bounds__(kStreamThreads)
   }
 }
 
+// BF16 hi / lo planes of V elements i .. i + V - 1 of probe p, in the layout
+// of the block matvec's split pass (k_matvec.hip: tile-major, 32 columns per
+// tile and probe; RBF tiles with column groups of four ordered as the MFMA
+// fragments expect): hi = bf16(x), lo = bf16(x - hi), round to nearest.
+__device__ __forceinline__ int plane_group(int c4, int perm) {
+  return perm ? ((c4 & 4) | ((c4 & 1) << 1) | ((c4 & 2) >> 1)) : c4;
+}
+
+template <int V>
+__device__ __forceinline__ void store_planes(const PlaneOut& pl, int64_t p, int64_t i,
+                                             const float (&v)[V]) {
+  if constexpr (V == 4) {  // i % 4 == 0: one group of four, one 8-byte store per plane
+    const int64_t dst = ((i / 32) * pl.Ppad + p) * 32 + 4 * plane_group(int((i % 32) / 4), pl.perm);
+    __bf16 h[4], l[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      h[u] = __bf16(v[u]);
+      l[u] = __bf16(v[u] - float(h[u]));
+    }
+    *reinterpret_cast<uint2*>(pl.hi + dst) = __builtin_bit_cast(uint2, h);
+    if (pl.lo) *reinterpret_cast<uint2*>(pl.lo + dst) = __builtin_bit_cast(uint2, l);
+  } else {
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      const int64_t m = i + u;
+      const int64_t dst =
+          ((m / 32) * pl.Ppad + p) * 32 + 4 * plane_group(int((m % 32) / 4), pl.perm) + (m % 4);
+      const __bf16 h = __bf16(v[u]);
+      pl.hi[dst] = h;
+      if (pl.lo) pl.lo[dst] = __bf16(v[u] - float(h));
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // normalize: out = x / sqrt(sum_b norm2part), plus bookkeeping of the norm
 // ---------------------------------------------------------------------------
@@ -423,7 +457,7 @@ __global__ void __launch_bounds__(kStreamThreads)
     k_normalize(const float* __restrict__ x, float* __restrict__ out, int64_t n, int64_t nbn,
                 const float* __restrict__ norm2part, float* __restrict__ norm_out,
                 float* __restrict__ hslot, int64_t hslot_stride, float* __restrict__ hslot2,
-                float* __restrict__ inv_out) {
+                float* __restrict__ inv_out, PlaneOut pl) {
   __shared__ float nrm;
   const int64_t p = blockIdx.y;
   const int t = threadIdx.x;
@@ -452,6 +486,7 @@ __global__ void __launch_bounds__(kStreamThreads)
 #pragma unroll
       for (int u = 0; u < V; ++u) v[u] = v[u] / len;
       VecIO<V>::store(out + p * n + i, v);
+      if (pl.hi) store_planes<V>(pl, p, i, v);
     }
   }
 }
@@ -581,7 +616,9 @@ hipError_t launch_scale_rows(hipStream_t s, const float* x, const float* scale, 
 
 hipError_t launch_normalize(hipStream_t s, const float* x, float* out, int64_t P, int64_t n,
                             const float* norm2part, float* norm_out, float* hslot,
-                            int64_t hslot_stride, float* inv_out, float* hslot2) {
+                            int64_t hslot_stride, float* inv_out, float* hslot2,
+                            const PlaneOut* planes) {
+  const PlaneOut pl = planes ? *planes : PlaneOut{};
   const bool vec = (n % 4 == 0) && aligned16(x) && aligned16(out);
   const int V = vec ? 4 : 1;
   const int64_t per_block = int64_t(kStreamThreads) * V * kNormVec;
@@ -589,10 +626,10 @@ hipError_t launch_normalize(hipStream_t s, const float* x, float* out, int64_t P
   const int64_t nbn = num_chunks(n);
   if (vec)
     hipLaunchKernelGGL(k_normalize<4>, grid, dim3(kStreamThreads), 0, s, x, out, n, nbn, norm2part,
-                       norm_out, hslot, hslot_stride, hslot2, inv_out);
+                       norm_out, hslot, hslot_stride, hslot2, inv_out, pl);
   else
     hipLaunchKernelGGL(k_normalize<1>, grid, dim3(kStreamThreads), 0, s, x, out, n, nbn, norm2part,
-                       norm_out, hslot, hslot_stride, hslot2, inv_out);
+                       norm_out, hslot, hslot_stride, hslot2, inv_out, pl);
   return hipGetLastError();
 }
 

@@ -327,7 +327,7 @@ __device__ __forceinline__ int perm_group(int c4) {
 
 // Split BF16 planes of V, zero-padded, tile-major: the 32 columns of tile kt
 // of probe p sit at (kt * Ppad + p) * 32, so one workgroup's probe tile is a
-// contiguous block of 64 * BPW bytes.
+// contiguous block of 64 * BPW bytes.  Vlo may be null (plane not read).
 template <bool PERM>
 __global__ void __launch_bounds__(256)
     k_split_probes(const float* __restrict__ V, int64_t P, int64_t n, int64_t Ppad, int64_t npad,
@@ -350,7 +350,7 @@ __global__ void __launch_bounds__(256)
     const int c4 = int((m % BK) / 4);
     const int64_t dst = (tile * Ppad + p) * BK + 4 * (PERM ? perm_group(c4) : c4);
     *reinterpret_cast<uint2*>(Vhi + dst) = make_uint2(h0, h1);
-    *reinterpret_cast<uint2*>(Vlo + dst) = make_uint2(l0, l1);
+    if (Vlo) *reinterpret_cast<uint2*>(Vlo + dst) = make_uint2(l0, l1);
   }
 }
 
@@ -1055,9 +1055,22 @@ int64_t matvec_scratch_floats(int64_t n, int64_t P) {
   return kMaxSplitK * P * n + Ppad * npad + 16 * npad;
 }
 
+PlaneOut matvec_planes(const slq_operator& op, int64_t P, float* scratch) {
+  PlaneOut out;
+  const bool rows = op.kind == SLQ_OP_RBF && P <= kRowsMaxP && op.dim >= 1 && op.dim <= 4;
+  if (rows || op.precision == SLQ_PREC_FP32 || !scratch) return out;
+  const int64_t npad = (op.n + BK - 1) / BK * BK;
+  const int64_t bpw = 32 * nacc_for(P);
+  out.Ppad = (P + bpw - 1) / bpw * bpw;
+  out.hi = reinterpret_cast<__bf16*>(scratch);
+  out.lo = op.precision == SLQ_PREC_BF16X3 ? out.hi + out.Ppad * npad : nullptr;
+  out.perm = op.kind == SLQ_OP_RBF ? 1 : 0;
+  return out;
+}
+
 hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, int64_t P,
                          float* W, float* scratch, int64_t scratch_floats, hipEvent_t ev0,
-                         hipEvent_t ev1) {
+                         hipEvent_t ev1, int flags) {
   const int64_t n = op.n;
   if (n <= 0 || P <= 0) return hipSuccess;
   if (op.kind == SLQ_OP_RBF && P <= kRowsMaxP && op.dim >= 1 && op.dim <= 4) {
@@ -1098,23 +1111,26 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
     Vhi = reinterpret_cast<__bf16*>(scratch);
     Vlo = Vhi + Ppad * npad;
     rec = reinterpret_cast<u32x4*>(scratch + Ppad * npad);
-    int64_t blocks = (Ppad * npad / 4 + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
-    if (op.kind == SLQ_OP_RBF)
-      hipLaunchKernelGGL(k_split_probes<true>, dim3(unsigned(blocks)), dim3(256), 0, s, V, P, n,
-                         Ppad, npad, Vhi, Vlo);
-    else
-      hipLaunchKernelGGL(k_split_probes<false>, dim3(unsigned(blocks)), dim3(256), 0, s, V, P, n,
-                         Ppad, npad, Vhi, Vlo);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (!(flags & kMvPlanesReady)) {
+      int64_t blocks = (Ppad * npad / 4 + 255) / 256;
+      if (blocks > 8192) blocks = 8192;
+      __bf16* lo = nprod == 3 ? Vlo : nullptr;  // the low plane is read by bf16x3 only
+      if (op.kind == SLQ_OP_RBF)
+        hipLaunchKernelGGL(k_split_probes<true>, dim3(unsigned(blocks)), dim3(256), 0, s, V, P, n,
+                           Ppad, npad, Vhi, lo);
+      else
+        hipLaunchKernelGGL(k_split_probes<false>, dim3(unsigned(blocks)), dim3(256), 0, s, V, P,
+                           n, Ppad, npad, Vhi, lo);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
   }
   float* partial = scratch ? scratch + planes_floats : nullptr;
   const int S = partial ? choose_splitk(n, P, scratch_floats - planes_floats) : 1;
   float* out = (S > 1) ? partial : W;
   RBFParams prm{};
   hipError_t err;
-  if (op.kind == SLQ_OP_RBF && !fp32) {
+  if (op.kind == SLQ_OP_RBF && !fp32 && !(flags & kMvPointsReady)) {
     hipLaunchKernelGGL(k_prep_points, dim3(unsigned((npad + 255) / 256)), dim3(256), 0, s,
                        op.points, op.dim,
                        float(-1.4426950408889634 / (2.0 * op.lengthscale * op.lengthscale)), n,

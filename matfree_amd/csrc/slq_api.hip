@@ -197,6 +197,9 @@ int hessenberg_core(slq_context* ctx, const slq_operator& op, const float* V0, i
   }
   // step 0: v /= length; Q[:, 0] = v (decomp.py:456-457)
   SLQ_HIP(ctx, launch_normalize(s, V0, Q, P, n, norm2, nullptr, nullptr, 0, c));
+  // after the first matvec (which splits its input and prepares the point
+  // records) the normalisation writes the next input's BF16 planes itself
+  const PlaneOut planes = matvec_planes(op, P, mvs);
   for (int64_t j = 0; j < k; ++j) {
     const float* qj = Q + j * ldc;
     const int ncols = int(j + 1);
@@ -205,7 +208,8 @@ int hessenberg_core(slq_context* ctx, const slq_operator& op, const float* V0, i
     // itself (not the probe split or the split-K reduction around it)
     {
       const auto ev = take_events(ctx, SLQ_KC_MATVEC);
-      SLQ_HIP(ctx, launch_matvec(s, op, qj, P, w, mvs, mv_floats, ev.first, ev.second));
+      const int flags = j == 0 ? 0 : (kMvPointsReady | (planes.hi ? kMvPlanesReady : 0));
+      SLQ_HIP(ctx, launch_matvec(s, op, qj, P, w, mvs, mv_floats, ev.first, ev.second, flags));
     }
     // h = Q^T w (decomp.py:463)
     {
@@ -233,7 +237,7 @@ int hessenberg_core(slq_context* ctx, const slq_operator& op, const float* V0, i
       // step's v /= length; Q[:, j+1] = v (decomp.py:456-457)
       Timed tm(ctx, SLQ_KC_NORMALIZE);
       SLQ_HIP(ctx, launch_normalize(s, w, Q + (j + 1) * ldc, P, n, norm2, nullptr, hcol + j + 1,
-                                    k * k, nullptr));
+                                    k * k, nullptr, nullptr, planes.hi ? &planes : nullptr));
     }
   }
   return SLQ_OK;
@@ -268,12 +272,14 @@ int lanczos_core(slq_context* ctx, const slq_operator& op, const float* V0, int6
   SLQ_HIP(ctx, launch_dots(s, nullptr, 0, V0, P, n, 1, norm2, 1));
   SLQ_HIP(ctx, launch_normalize(s, V0, vec(0), P, n, norm2, nullptr, nullptr, 0, c));
   const int64_t steps = k > 0 ? k : 1;
+  const PlaneOut planes = matvec_planes(op, P, mvs);
   for (int64_t j = 0; j < steps; ++j) {
     const float* q = vec(j);
     const float* qm = j > 0 ? vec(j - 1) : nullptr;
     {
       const auto ev = take_events(ctx, SLQ_KC_MATVEC);
-      SLQ_HIP(ctx, launch_matvec(s, op, q, P, w, mvs, mv_floats, ev.first, ev.second));
+      const int flags = j == 0 ? 0 : (kMvPointsReady | (planes.hi ? kMvPlanesReady : 0));
+      SLQ_HIP(ctx, launch_matvec(s, op, q, P, w, mvs, mv_floats, ev.first, ev.second, flags));
     }
     // a = q . A q; r = A q - a q - b q_prev (decomp.py:243-245, 254-260)
     {
@@ -293,7 +299,8 @@ int lanczos_core(slq_context* ctx, const slq_operator& op, const float* V0, int6
       const bool inner = j + 1 < k;
       SLQ_HIP(ctx, launch_normalize(s, r, vec(j + 1), P, n, norm2, inner ? nullptr : blast,
                                     inner ? H + j * k + j + 1 : nullptr, k * k, nullptr,
-                                    inner ? H + (j + 1) * k + j : nullptr));
+                                    inner ? H + (j + 1) * k + j : nullptr,
+                                    planes.hi ? &planes : nullptr));
     }
   }
   // residual = b * x (decomp.py:167)

@@ -29,6 +29,17 @@ inline int64_t num_chunks(int64_t n) { return (n + kChunk - 1) / kChunk; }
 // ---- probes (k_probes.hip) -------------------------------------------------
 hipError_t launch_rademacher(hipStream_t s, uint32_t k0, uint32_t k1, int64_t count, float* out);
 
+// BF16 planes of the block matvec's input (tile-major, zero-padded; see
+// k_matvec.hip), written by the kernel that produces the input so that the
+// matvec can skip its own split pass (kMvPlanesReady).  hi == nullptr: the
+// matvec reads no planes for this operator and probe count.
+struct PlaneOut {
+  __bf16* hi = nullptr;
+  __bf16* lo = nullptr;  // null when the matvec does not read the low plane
+  int64_t Ppad = 0;
+  int perm = 0;  // RBF fragment order of the columns within a tile
+};
+
 // ---- streaming Gram-Schmidt kernels (k_lanczos.hip) ------------------------
 // Basis column c of probe p starts at Q + c * ldc + p * n (ldc >= P * n).
 // partial[p][b][c] = sum_{i in chunk b} Q[c][p][i] * x[p][i], c < ncols.
@@ -52,7 +63,8 @@ hipError_t launch_project_dots(hipStream_t s, const float* Q, int64_t ldc, const
 // hslot2[p * hslot_stride], and its reciprocal to inv_out[p].
 hipError_t launch_normalize(hipStream_t s, const float* x, float* out, int64_t P, int64_t n,
                             const float* norm2part, float* norm_out, float* hslot,
-                            int64_t hslot_stride, float* inv_out, float* hslot2 = nullptr);
+                            int64_t hslot_stride, float* inv_out, float* hslot2 = nullptr,
+                            const PlaneOut* planes = nullptr);
 // Plain Lanczos step: r = w - a q - b_prev qm with a = sum_b apart[p][b]
 // (single-column dots of q and w), b_prev = bprev[p * bstride] (qm may be
 // null), norm^2 partials of r, a written to aout[p * astride].
@@ -101,9 +113,17 @@ constexpr int kMaxSplitK = 4;
 int64_t matvec_scratch_floats(int64_t n, int64_t P);
 // ev0 / ev1 (optional) are recorded on s right before and after the block
 // matvec kernel (the launches around it excluded).
+// flags: kMvPlanesReady -- the scratch planes already hold V (written by a
+// PlaneOut producer after a call that split them, so the padding is zero);
+// kMvPointsReady -- the RBF point records in the scratch are those of op
+// (a previous call with the same operator and scratch).
+constexpr int kMvPlanesReady = 1;
+constexpr int kMvPointsReady = 2;
 hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, int64_t P,
                          float* W, float* scratch, int64_t scratch_floats,
-                         hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+                         hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, int flags = 0);
+// The planes launch_matvec reads for (op, P) in scratch.
+PlaneOut matvec_planes(const slq_operator& op, int64_t P, float* scratch);
 
 // ---- tridiagonal eigen + quadrature (k_tridiag.hip) ------------------------
 // For each probe p: T = (H + H^T)/2 restricted to its tridiagonal; returns

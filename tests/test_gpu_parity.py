@@ -247,6 +247,42 @@ def test_hessenberg_matches_oracle(cuda, n, P, k):
 # ---------------------------------------------------------------------------
 
 
+@pytest.mark.parametrize("precision,tol_full,tol_none",
+                         [("bf16x3", 1e-4, 3e-3), ("bf16x2", 1e-3, 0.1), ("bf16", 0.2, 0.2)])
+@pytest.mark.parametrize("reortho", ["full", "none"])
+def test_bf16_rbf_logdet_tracks_fp32(cuda, precision, tol_full, tol_none, reortho):
+    # multi-step paths in BF16: after the first matvec the normalisation
+    # writes the next Krylov vector's BF16 planes itself (40 probes pad to 64,
+    # n = 4000 pads to whole 32-column tiles).  Plain BF16 rounds K itself to
+    # 8 bits, which moves the small eigenvalues at s2 = 0.1 by tens of percent
+    # (measured: 9% per probe); plain Lanczos in FP32 moves by ~3e-4 per probe
+    # under any change of rounding at this depth (measured with bf16x3), and by
+    # ~4% when every Krylov vector is rounded to BF16 (bf16x2: it loses
+    # orthogonality sooner); a misplaced plane would be off by far more.  The
+    # fused planes are bit-identical to the split pass (same log-determinant
+    # to the last bit at n = 32768, 128 probes, both forms).
+    tol = tol_full if reortho == "full" else tol_none
+    n, P, k = 4000, 40, 24
+    X = problems.rbf_points(n, seed=5)
+    Z = torch.as_tensor(oprng.rademacher(oprng.prng_key(3), (P, n), np.float32), device=cuda)
+    ref, _ = funm.logdet_per_probe(operators.RBF(X), Z, k, (0.5, 0.1), reortho=reortho)
+    got, _ = funm.logdet_per_probe(operators.RBF(X, precision=precision), Z, k, (0.5, 0.1),
+                                   reortho=reortho)
+    ref, got = ref.cpu().numpy(), got.cpu().numpy()
+    assert np.max(np.abs(got - ref) / np.abs(ref)) <= tol
+
+
+@pytest.mark.parametrize("precision,tol", [("bf16x3", 1e-4), ("bf16", 3e-2)])
+def test_bf16_dense_logdet_tracks_fp32(cuda, precision, tol):
+    n, P, k = 2000, 24, 20
+    A = torch.as_tensor(problems.dense_spd(n, seed=2, shift=1.0), device=cuda)
+    Z = torch.as_tensor(oprng.rademacher(oprng.prng_key(4), (P, n), np.float32), device=cuda)
+    ref, _ = funm.logdet_per_probe(operators.DenseMatvec(), Z, k, (A,))
+    got, _ = funm.logdet_per_probe(operators.DenseMatvec(precision=precision), Z, k, (A,))
+    ref, got = ref.cpu().numpy(), got.cpu().numpy()
+    assert np.max(np.abs(got - ref) / np.abs(ref)) <= tol
+
+
 def test_logdet_c1_matches_oracle_and_cholesky(cuda):
     # Plain SLQ logdet: A = B B^T / n + I, n = 1024, 64 probes, depth 30
     n, P, k = 1024, 64, 30
