@@ -333,9 +333,10 @@ __global__ void __launch_bounds__(64)
 }
 
 // Cotangent of |z|^2 [log T]_00 / P w.r.t. H (see the file header), one wave
-// per probe; lg: k doubles of scratch.
+// per probe; lg: k doubles and Wm: k x k doubles of scratch.
 __device__ void tridiag_vjp_wave(const float* __restrict__ Hp, int k, double* d, double* e,
-                                 double* lg, double* U, double scale, float* __restrict__ dHp) {
+                                 double* lg, double* U, double* Wm, double scale,
+                                 float* __restrict__ dHp) {
   const int lane = threadIdx.x;
   const bool failed = !tridiag_eigvecs_wave(Hp, k, d, e, U, k);
   // log of the eigenvalues; a non-positive eigenvalue makes the reference's
@@ -345,8 +346,25 @@ __device__ void tridiag_vjp_wave(const float* __restrict__ Hp, int k, double* d,
   for (int j = lane; j < k; j += 64) lg[j] = log(d[j]);
   __syncthreads();
 
-  // Tbar = U W U^T, W_jl = u_j u_l L_jl; lane a forms the diagonal entry a
-  // and the superdiagonal entry (a, a + 1):
+  // W_jl = u_j u_l L_jl (u = U[0, :], L the divided differences of log), all
+  // k^2 entries once, spread over the lanes
+  for (int idx = lane; idx < k * k; idx += 64) {
+    const int j = idx / k, l = idx - j * k;
+    const double thj = d[j], thl = d[l], del = thl - thj;
+    double L;
+    if (j == l || del == 0.0) {
+      L = 1.0 / thj;
+    } else if (fabs(del) < 1e-3 * fabs(thj)) {
+      L = log1p(del / thj) / del;  // (log thl - log thj) / (thl - thj), no cancellation
+    } else {
+      L = (lg[l] - lg[j]) / del;
+    }
+    Wm[idx] = U[j] * U[l] * L;
+  }
+  __syncthreads();
+
+  // Tbar = U W U^T; lane a forms the diagonal entry a and the superdiagonal
+  // entry (a, a + 1):
   //   v_l = sum_j W_jl U_aj,  Tbar_aa = sum_l U_al v_l,
   //   Tbar_{a,a+1} = sum_l U_{a+1,l} v_l.
   for (int a = lane; a < k; a += 64) {
@@ -354,20 +372,8 @@ __device__ void tridiag_vjp_wave(const float* __restrict__ Hp, int k, double* d,
     const double* ua = U + int64_t(a) * k;
     const double* ub = U + int64_t(a + 1 < k ? a + 1 : a) * k;
     for (int l = 0; l < k; ++l) {
-      const double ul = U[l], thl = d[l];
       double v = 0.0;
-      for (int j = 0; j < k; ++j) {
-        const double thj = d[j], del = thl - thj;
-        double L;
-        if (j == l || del == 0.0) {
-          L = 1.0 / thj;
-        } else if (fabs(del) < 1e-3 * fabs(thj)) {
-          L = log1p(del / thj) / del;  // (log thl - log thj) / (thl - thj), no cancellation
-        } else {
-          L = (lg[l] - lg[j]) / del;
-        }
-        v += U[j] * ul * L * ua[j];
-      }
+      for (int j = 0; j < k; ++j) v += Wm[j * k + l] * ua[j];
       tdd += ua[l] * v;
       tdo += ub[l] * v;
     }
@@ -385,20 +391,21 @@ __global__ void __launch_bounds__(64)
     k_tridiag_vjp_lds(const float* __restrict__ H, const float* __restrict__ len, int64_t P,
                       int64_t k, float* __restrict__ dH) {
   __shared__ double U[kEigLdsDepth * kEigLdsDepth];
+  __shared__ double Wm[kEigLdsDepth * kEigLdsDepth];
   __shared__ double d[kEigLdsDepth], e[kEigLdsDepth], lg[kEigLdsDepth];
   const int64_t p = blockIdx.x;
   const double l2 = double(len[p]) * double(len[p]);
-  tridiag_vjp_wave(H + p * k * k, int(k), d, e, lg, U, l2 / double(P), dH + p * k * k);
+  tridiag_vjp_wave(H + p * k * k, int(k), d, e, lg, U, Wm, l2 / double(P), dH + p * k * k);
 }
 
 __global__ void __launch_bounds__(64)
     k_tridiag_vjp_global(const float* __restrict__ H, const float* __restrict__ len, int64_t P,
                          int64_t k, double* __restrict__ work, float* __restrict__ dH) {
   const int64_t p = blockIdx.x;
-  double* w = work + p * (k * k + 3 * k);
+  double* w = work + p * (2 * k * k + 3 * k);
   const double l2 = double(len[p]) * double(len[p]);
   tridiag_vjp_wave(H + p * k * k, int(k), w + k * k, w + k * k + k, w + k * k + 2 * k, w,
-                   l2 / double(P), dH + p * k * k);
+                   w + k * k + 3 * k, l2 / double(P), dH + p * k * k);
 }
 
 
@@ -547,7 +554,7 @@ int64_t tridiag_quadrature_work_doubles(int64_t P, int64_t k) {
 }
 
 int64_t tridiag_vjp_work_doubles(int64_t P, int64_t k) {
-  return k <= kEigLdsDepth ? 0 : P * (k * k + 3 * k);
+  return k <= kEigLdsDepth ? 0 : P * (2 * k * k + 3 * k);
 }
 
 hipError_t launch_tridiag_funm(hipStream_t s, const float* H, const float* len, int64_t P,
