@@ -122,10 +122,33 @@ __device__ __forceinline__ float block_sum(float v, float* red) {
 // dots: partial[p][b][c] = sum over chunk b of Q[c][p][:] . x[p][:]
 // Columns in batches of NC (all held as per-lane partial sums).
 // ---------------------------------------------------------------------------
-template <int V, int NC>
+// x of probe p, elements i .. i + V - 1: loaded, or (sm.partial) formed
+// from a deferred split-K sum exactly as the matvec's reduction does and
+// stored to x.
+template <int V>
+__device__ __forceinline__ void load_x(float* x, const MvSplitSum& sm, int64_t P, int64_t n,
+                                       int64_t p, int64_t i, float (&xv)[V]) {
+  const int64_t count = P * n, off = p * n + i;
+  VecIO<V>::load(sm.partial + off, xv);
+  for (int z = 1; z < sm.S; ++z) {
+    float pz[V];
+    VecIO<V>::load(sm.partial + int64_t(z) * count + off, pz);
+#pragma unroll
+    for (int v = 0; v < V; ++v) xv[v] += pz[v];
+  }
+  if (sm.nugget) {
+    float vv[V];
+    VecIO<V>::load(sm.V + off, vv);
+#pragma unroll
+    for (int v = 0; v < V; ++v) xv[v] = fmaf(sm.noise, vv[v], xv[v]);
+  }
+  VecIO<V>::store(x + p * n + i, xv);
+}
+
+template <int V, int NC, bool SUM>
 __global__ void __launch_bounds__(kStreamThreads)
-    k_dots(const float* __restrict__ Q, int64_t ldc, const float* __restrict__ x, int64_t n,
-           int ncols, float* __restrict__ partial, int pstride) {
+    k_dots(const float* __restrict__ Q, int64_t ldc, const float* __restrict__ x, float* xw,
+           int64_t n, int ncols, float* __restrict__ partial, int pstride, MvSplitSum sm) {
   constexpr int NE = Layout<V>::NE;
   __shared__ float red[kWaves * NC];
   const int64_t p = blockIdx.y;
@@ -139,7 +162,10 @@ __global__ void __launch_bounds__(kStreamThreads)
   for (int e = 0; e < NE; ++e) {
     const int64_t i = base + (int64_t(e) * kStreamThreads + t) * V;
     if (i < n) {
-      VecIO<V>::load(x + p * n + i, xv[e]);
+      if constexpr (SUM)
+        load_x<V>(xw, sm, int64_t(gridDim.y), n, p, i, xv[e]);
+      else
+        VecIO<V>::load(x + p * n + i, xv[e]);
     } else {
 #pragma unroll
       for (int v = 0; v < V; ++v) xv[e][v] = 0.f;
@@ -498,14 +524,19 @@ hipError_t launch_dots_v(hipStream_t s, const float* Q, int64_t ldc, const float
                          int64_t n, int ncols, float* partial, int pstride) {
   const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
   const dim3 block(kStreamThreads);
+  const MvSplitSum sm;
   if (ncols <= 8)
-    hipLaunchKernelGGL((k_dots<V, 8>), grid, block, 0, s, Q, ldc, x, n, ncols, partial, pstride);
+    hipLaunchKernelGGL((k_dots<V, 8, false>), grid, block, 0, s, Q, ldc, x, nullptr, n, ncols,
+                       partial, pstride, sm);
   else if (ncols <= 16)
-    hipLaunchKernelGGL((k_dots<V, 16>), grid, block, 0, s, Q, ldc, x, n, ncols, partial, pstride);
+    hipLaunchKernelGGL((k_dots<V, 16, false>), grid, block, 0, s, Q, ldc, x, nullptr, n, ncols,
+                       partial, pstride, sm);
   else if (ncols <= 32)
-    hipLaunchKernelGGL((k_dots<V, 32>), grid, block, 0, s, Q, ldc, x, n, ncols, partial, pstride);
+    hipLaunchKernelGGL((k_dots<V, 32, false>), grid, block, 0, s, Q, ldc, x, nullptr, n, ncols,
+                       partial, pstride, sm);
   else
-    hipLaunchKernelGGL((k_dots<V, 64>), grid, block, 0, s, Q, ldc, x, n, ncols, partial, pstride);
+    hipLaunchKernelGGL((k_dots<V, 64, false>), grid, block, 0, s, Q, ldc, x, nullptr, n, ncols,
+                       partial, pstride, sm);
   return hipGetLastError();
 }
 
@@ -524,6 +555,23 @@ hipError_t launch_dots(hipStream_t s, const float* Q, int64_t ldc, const float* 
   }
   return vec ? launch_dots_v<4>(s, Q, ldc, x, P, n, ncols, partial, pstride)
              : launch_dots_v<1>(s, Q, ldc, x, P, n, ncols, partial, pstride);
+}
+
+hipError_t launch_dots_sum(hipStream_t s, const float* Q, int64_t ldc, const MvSplitSum& sum,
+                           float* x, int64_t P, int64_t n, int ncols, float* partial, int pstride) {
+  // single column (the Lanczos coefficient q . A q); several columns would
+  // need the wider instantiations, which the Arnoldi pass does not use
+  if (ncols != 1) return hipErrorInvalidValue;
+  const bool vec = (n % 4 == 0) && aligned16(x) && aligned16(Q) && ldc % 4 == 0 &&
+                   aligned16(sum.partial) && (!sum.nugget || aligned16(sum.V));
+  const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
+  if (vec)
+    hipLaunchKernelGGL((k_dots<4, 8, true>), grid, dim3(kStreamThreads), 0, s, Q, ldc, nullptr, x,
+                       n, ncols, partial, pstride, sum);
+  else
+    hipLaunchKernelGGL((k_dots<1, 8, true>), grid, dim3(kStreamThreads), 0, s, Q, ldc, nullptr, x,
+                       n, ncols, partial, pstride, sum);
+  return hipGetLastError();
 }
 
 hipError_t launch_project(hipStream_t s, const float* Q, int64_t ldc, const float* x, float* y,

@@ -1070,7 +1070,8 @@ PlaneOut matvec_planes(const slq_operator& op, int64_t P, float* scratch) {
 
 hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, int64_t P,
                          float* W, float* scratch, int64_t scratch_floats, hipEvent_t ev0,
-                         hipEvent_t ev1, int flags) {
+                         hipEvent_t ev1, int flags, MvSplitSum* defer) {
+  if (defer) *defer = MvSplitSum{};
   const int64_t n = op.n;
   if (n <= 0 || P <= 0) return hipSuccess;
   if (op.kind == SLQ_OP_RBF && P <= kRowsMaxP && op.dim >= 1 && op.dim <= 4) {
@@ -1086,6 +1087,10 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
     hipError_t err = launch_rows(s, prm, V, P, n, out, S);
     if (ev1) (void)hipEventRecord(ev1, s);
     if (err != hipSuccess || S == 1) return err;
+    if (defer) {
+      *defer = MvSplitSum{scratch, S, V, prm.noise, 1};
+      return hipSuccess;
+    }
     const int64_t count = P * n;
     int64_t blocks = (count + 255) / 256;
     if (blocks > 4096) blocks = 4096;
@@ -1163,6 +1168,10 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
   }
   if (ev1) (void)hipEventRecord(ev1, s);
   if (err != hipSuccess || S == 1) return err;
+  if (defer) {
+    *defer = MvSplitSum{partial, S, V, prm.noise, op.kind == SLQ_OP_RBF ? 1 : 0};
+    return hipSuccess;
+  }
   const int64_t count = P * n;
   int64_t blocks = (count + 255) / 256;
   if (blocks > 4096) blocks = 4096;

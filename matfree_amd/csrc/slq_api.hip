@@ -273,18 +273,24 @@ int lanczos_core(slq_context* ctx, const slq_operator& op, const float* V0, int6
   SLQ_HIP(ctx, launch_normalize(s, V0, vec(0), P, n, norm2, nullptr, nullptr, 0, c));
   const int64_t steps = k > 0 ? k : 1;
   const PlaneOut planes = matvec_planes(op, P, mvs);
+  MvSplitSum pend;
   for (int64_t j = 0; j < steps; ++j) {
     const float* q = vec(j);
     const float* qm = j > 0 ? vec(j - 1) : nullptr;
     {
       const auto ev = take_events(ctx, SLQ_KC_MATVEC);
       const int flags = j == 0 ? 0 : (kMvPointsReady | (planes.hi ? kMvPlanesReady : 0));
-      SLQ_HIP(ctx, launch_matvec(s, op, q, P, w, mvs, mv_floats, ev.first, ev.second, flags));
+      SLQ_HIP(ctx, launch_matvec(s, op, q, P, w, mvs, mv_floats, ev.first, ev.second, flags,
+                                 &pend));
     }
-    // a = q . A q; r = A q - a q - b q_prev (decomp.py:243-245, 254-260)
+    // a = q . A q; r = A q - a q - b q_prev (decomp.py:243-245, 254-260); a
+    // split-K matvec result is summed into w by the dot kernel
     {
       Timed tm(ctx, SLQ_KC_DOTS);
-      SLQ_HIP(ctx, launch_dots(s, q, 0, w, P, n, 1, apart, 1));
+      if (pend.partial)
+        SLQ_HIP(ctx, launch_dots_sum(s, q, 0, pend, w, P, n, 1, apart, 1));
+      else
+        SLQ_HIP(ctx, launch_dots(s, q, 0, w, P, n, 1, apart, 1));
     }
     {
       Timed tm(ctx, SLQ_KC_PROJECT);

@@ -40,12 +40,27 @@ struct PlaneOut {
   int perm = 0;  // RBF fragment order of the columns within a tile
 };
 
+// A split-K sum left to the consumer: W = sum_{z < S} partial[z * P * n + .]
+// (added in the order z = 0 .. S - 1) + noise * V when nugget (the reduction
+// of launch_matvec, bit for bit).  partial == nullptr: W was written.
+struct MvSplitSum {
+  const float* partial = nullptr;
+  int S = 1;
+  const float* V = nullptr;
+  float noise = 0.f;
+  int nugget = 0;
+};
+
 // ---- streaming Gram-Schmidt kernels (k_lanczos.hip) ------------------------
 // Basis column c of probe p starts at Q + c * ldc + p * n (ldc >= P * n).
 // partial[p][b][c] = sum_{i in chunk b} Q[c][p][i] * x[p][i], c < ncols.
 // Q may be null with ncols = 1 to mean "sum of squares of x" (partial[p][b][0]).
 hipError_t launch_dots(hipStream_t s, const float* Q, int64_t ldc, const float* x, int64_t P,
                        int64_t n, int ncols, float* partial, int pstride);
+// launch_dots of x = the split-K sum `sum` (a deferred launch_matvec result),
+// which is also written to x (Q non-null).
+hipError_t launch_dots_sum(hipStream_t s, const float* Q, int64_t ldc, const MvSplitSum& sum,
+                           float* x, int64_t P, int64_t n, int ncols, float* partial, int pstride);
 // y = x - sum_c Q[c] h[c] with h[c] = sum_b hpart[p][b][c] - hsub[p][c];
 // optional norm2part[p][b] = sum over chunk of y^2.  hout (if non-null)
 // receives sum_b hpart[p][b][c] at hout + p * hout_stride (written by chunk 0
@@ -119,9 +134,12 @@ int64_t matvec_scratch_floats(int64_t n, int64_t P);
 // (a previous call with the same operator and scratch).
 constexpr int kMvPlanesReady = 1;
 constexpr int kMvPointsReady = 2;
+// With defer non-null a split-K result is not reduced into W; *defer
+// describes it instead (W is then not written).
 hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, int64_t P,
                          float* W, float* scratch, int64_t scratch_floats,
-                         hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, int flags = 0);
+                         hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, int flags = 0,
+                         MvSplitSum* defer = nullptr);
 // The planes launch_matvec reads for (op, P) in scratch.
 PlaneOut matvec_planes(const slq_operator& op, int64_t P, float* scratch);
 
