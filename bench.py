@@ -452,8 +452,8 @@ def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are out of scope "
-                                   "(see DESIGN.md for how the path would shard)",
+        print(json.dumps({"error": "runs on one GPU only: probes would shard across GPUs with one "
+                                   "all-reduce of the per-probe values, which is not implemented",
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
 
