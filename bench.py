@@ -11,7 +11,8 @@ uniform in [0, 1]^3 (seeded), K_ij = exp(-|x_i - x_j|^2 / (2 * 0.5^2)) +
 1e-2 delta_ij, 128 probes, depth 64.  ``--workload dense`` runs the stored
 dense SPD case (n = 16384, 256 probes, depth 64, A = B B^T / n + 0.1 I);
 ``rbf-none`` the RBF case with ``tridiag_sym(64, reortho="none")`` (plain
-Lanczos, no basis); ``rbf-grad`` value and gradient at n = 8192, 32 probes,
+Lanczos, no basis); ``rbf-funm`` ``funm_lanczos_sym`` (log(A) v for the 128
+probe vectors, depth 64); ``rbf-grad`` value and gradient at n = 8192, 32 probes,
 depth 32; ``gp-logpdf`` the Gaussian log-density of the GP tutorial (SLQ
 log-determinant + CG, value and gradient) at n = 8192.
 
@@ -43,10 +44,11 @@ WORKLOADS = {
     "dense": dict(n=16384, probes=256, depth=64, shift=0.1),
     "rbf-grad": dict(n=8192, probes=32, depth=32, lengthscale=0.5, noise=1e-2),
     "rbf-none": dict(n=32768, probes=128, depth=64, lengthscale=0.5, noise=1e-2),
+    "rbf-funm": dict(n=32768, probes=128, depth=64, lengthscale=0.5, noise=1e-2),
     "gp-logpdf": dict(n=8192, probes=32, depth=32, lengthscale=0.5, noise=1e-2, rank=128),
 }
 DEFAULT_PRECISION = {"rbf": "bf16x2", "dense": "fp32", "rbf-grad": "fp32", "rbf-none": "bf16x2",
-                     "gp-logpdf": "fp32"}
+                     "rbf-funm": "bf16x2", "gp-logpdf": "fp32"}
 # Serving mix: n log-uniform in [2048, 32768], probes in {16, 32, 64, 128},
 # depth uniform in [16, 64], RBF (bf16x2) or stored dense (fp32, n capped at
 # 16384 to bound the per-request matrix at 1 GB).
@@ -86,7 +88,7 @@ def setup(args, device):
     w = WORKLOADS[args.workload]
     n, P, k = w["n"], w["probes"], w["depth"]
     prec = args.precision or DEFAULT_PRECISION[args.workload]
-    if args.workload in ("rbf", "rbf-none"):
+    if args.workload in ("rbf", "rbf-none", "rbf-funm"):
         X = problems.rbf_points(n, seed=args.seed)
         op = operators.RBF(X, precision=prec)
         params = (w["lengthscale"], w["noise"])
@@ -307,6 +309,60 @@ def grad_step(args, device):
     }
 
 
+def funm_step(args, device):
+    """funm_lanczos_sym(dense_funm_sym_eigh(log), tridiag_sym(64)): log(A) v for the
+    128 probe vectors of the RBF problem, with the FP32 form beside it."""
+    import numpy as np
+    import torch
+
+    from matfree_amd import _lib, _runtime, funm, operators
+
+    op, params, Z, (n, P, k), prec, _ = setup(args, device)
+    code = _lib.MATFUNS["log"]
+    ctx = _runtime.context(device)
+    for _ in range(args.warmup):
+        funm.funm_vectors(op, Z, k, params, code)
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    for kc in ("matvec", "dots", "project_dots", "project", "normalize"):
+        ctx.profile_read(kc)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        Y = funm.funm_vectors(op, Z, k, params, code)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    kernels = {}
+    for kc in ("matvec", "dots", "project_dots", "project", "normalize"):
+        ms, cnt = ctx.profile_read(kc)
+        kernels[kc] = {"total_ms": ms, "launches": cnt}
+    ctx.profile(False)
+    out = {
+        "metric": "funm_lanczos_sym probe-Lanczos-steps/s (log(A) v)",
+        "value": P * k / dt,
+        "unit": UNIT,
+        "higher_is_better": True,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "dtype": prec + "+fp32",
+        "data": "synthetic (seeded points, Rademacher vectors from PRNGKey(seed))",
+        "config": {"workload": f"rbf-funm n={n} vectors={P} depth={k}", "n": n, "probes": P,
+                   "depth": k, "precision": prec, "matfun": "log",
+                   "lengthscale": WORKLOADS["rbf-funm"]["lengthscale"],
+                   "noise": WORKLOADS["rbf-funm"]["noise"]},
+        "roofline": roofline(kernels, n, P, k, prec, "rbf"),
+        "kernels_ms_per_solve": {kc: v["total_ms"] / args.steps for kc, v in kernels.items()},
+    }
+    if not args.no_reference:
+        # the same f(A) V with the FP32 matvec: the BF16 form's deviation
+        op32 = operators.RBF(op.points, precision="fp32")
+        Y32 = funm.funm_vectors(op32, Z, k, params, code)
+        out["rel_diff_vs_fp32_form"] = float(torch.linalg.norm(Y - Y32) / torch.linalg.norm(Y32))
+    return out
+
+
 def gp_step(args, device):
     """The GP tutorial's log-density, value and gradient (tutorials/8_gaussian_logpdf.py).
 
@@ -468,6 +524,9 @@ def main():
         return
     if args.workload == "gp-logpdf":
         print(json.dumps(gp_step(args, device)), flush=True)
+        return
+    if args.workload == "rbf-funm":
+        print(json.dumps(funm_step(args, device)), flush=True)
         return
     op, params, Z, (n, P, k), prec, host = setup(args, device)
     ctx = _runtime.context(device)
