@@ -426,13 +426,16 @@ __device__ __forceinline__ double apply_matfun(int f, double x) {
 // |v| Q^T y for k_adjoint_combine (per probe 2 k + 4 floats: y, zeros, and
 // the scale |v| = len).  NaN coefficients if QL fails.
 __device__ void tridiag_funm_wave(const float* __restrict__ Hp, int k, double* d, double* e,
-                                  double* U, int f, float scale, float* __restrict__ cp) {
+                                  double* fd, double* U, int f, float scale,
+                                  float* __restrict__ cp) {
   const int lane = threadIdx.x;
   const bool ok = tridiag_eigvecs_wave(Hp, k, d, e, U, k);
+  for (int j = lane; j < k; j += 64) fd[j] = apply_matfun(f, d[j]);  // f(theta_j) once
+  __syncthreads();
   for (int a = lane; a < k; a += 64) {
     double y = 0.0;
     const double* ua = U + int64_t(a) * k;
-    for (int j = 0; j < k; ++j) y += ua[j] * apply_matfun(f, d[j]) * U[j];
+    for (int j = 0; j < k; ++j) y += ua[j] * fd[j] * U[j];
     cp[a] = ok ? float(y) : __builtin_nanf("");
     cp[k + a] = 0.f;
   }
@@ -448,9 +451,9 @@ __global__ void __launch_bounds__(64)
     k_tridiag_funm_lds(const float* __restrict__ H, const float* __restrict__ len, int64_t k, int f,
                        float* __restrict__ coef) {
   __shared__ double U[kEigLdsDepth * kEigLdsDepth];
-  __shared__ double d[kEigLdsDepth], e[kEigLdsDepth];
+  __shared__ double d[kEigLdsDepth], e[kEigLdsDepth], fd[kEigLdsDepth];
   const int64_t p = blockIdx.x;
-  tridiag_funm_wave(H + p * k * k, int(k), d, e, U, f, len[p], coef + p * (2 * k + 4));
+  tridiag_funm_wave(H + p * k * k, int(k), d, e, fd, U, f, len[p], coef + p * (2 * k + 4));
 }
 
 __global__ void __launch_bounds__(64)
@@ -458,8 +461,8 @@ __global__ void __launch_bounds__(64)
                           int f, double* __restrict__ work, float* __restrict__ coef) {
   const int64_t p = blockIdx.x;
   double* w = work + p * (k * k + 3 * k);
-  tridiag_funm_wave(H + p * k * k, int(k), w + k * k, w + k * k + k, w, f, len[p],
-                    coef + p * (2 * k + 4));
+  tridiag_funm_wave(H + p * k * k, int(k), w + k * k, w + k * k + k, w + k * k + 2 * k, w, f,
+                    len[p], coef + p * (2 * k + 4));
 }
 
 // mean and standard error of the mean (population std / sqrt(P)) in FP64,
