@@ -29,9 +29,13 @@
 // CG: the per-column state machine of JAX's preconditioned CG under vmap
 // (x0 = 0; rs = gamma without a preconditioner, r . r with one; a converged
 // column keeps its state while the others iterate).  Per-column scalars live
-// in small device arrays updated by one-block kernels; the host reads the
-// number of active columns after batches of 1, 2, 4, 8, 8, ... iterations
-// (an iteration with no active column changes nothing).
+// in small device arrays; every block of an iteration's kernels sums the
+// partial dots it needs itself, and the scalars that change (gamma, active)
+// are double-buffered by iteration parity, so an iteration is four launches:
+// the matvec, d . A d (adding up the matvec's split-K partials), the x / r
+// update, and beta with the new direction.  The host reads the number of
+// active columns after batches of 1, 2, 4, 8, 8, ... iterations (an iteration
+// with no active column changes nothing).
 #include <cfloat>
 
 #include "slq_internal.h"
@@ -40,8 +44,13 @@ namespace slq {
 namespace {
 
 constexpr int kT = 256;
-constexpr int kCgPer = 8;  // elements per thread of the CG streaming kernels
+constexpr int kCgPer = 8;  // elements per thread of the CG dot kernel
 constexpr int kCgChunk = kT * kCgPer;
+// The CG kernels of an iteration run kTq threads per chunk of kCgChunk
+// elements (kQPer each): a column of a few thousand elements has only a few
+// chunks, so more threads per chunk shorten these latency-bound launches.
+constexpr int kTq = 1024;
+constexpr int kQPer = kCgChunk / kTq;
 
 __device__ __forceinline__ float wsum_f(float v) {
 #pragma unroll
@@ -55,7 +64,8 @@ __device__ __forceinline__ double wsum_d(double v) {
   return v;
 }
 
-// Block sum (kT threads); the result is valid in thread 0.
+// Block sum (NT threads, red[NT / 64]); the result is valid in thread 0.
+template <int NT = kT>
 __device__ __forceinline__ float block_sum_f(float v, float* red) {
   v = wsum_f(v);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -63,7 +73,7 @@ __device__ __forceinline__ float block_sum_f(float v, float* red) {
   __syncthreads();
   float s = 0.f;
   if (threadIdx.x == 0)
-    for (int w = 0; w < kT / 64; ++w) s += red[w];
+    for (int w = 0; w < NT / 64; ++w) s += red[w];
   __syncthreads();
   return s;
 }
@@ -379,13 +389,51 @@ __global__ void __launch_bounds__(kT)
   if (threadIdx.x == 0) part[p * nb + c] = s;
 }
 
+// q = sum_z qsum.partial[z] (+ noise d), the deferred split-K sum of the
+// matvec of d added in the order of launch_matvec's own reduction (so q is
+// the same, bit for bit), written to q, and the per-column partial dots d . q
+// over the same chunks as k_cg_dot; the partials' loads are issued 16 at a
+// time.
+__global__ void __launch_bounds__(kTq)
+    k_cg_dot_q(const float* __restrict__ d, MvSplitSum qsum, int64_t n, float* __restrict__ q,
+               float* __restrict__ part) {
+  __shared__ float red[kTq / 64];
+  const int64_t p = blockIdx.y, c = blockIdx.x, nb = gridDim.x;
+  const int64_t count = int64_t(gridDim.y) * n;
+  const int64_t base = c * kCgChunk;
+  float acc = 0.f;
+#pragma unroll
+  for (int e = 0; e < kQPer; ++e) {
+    const int64_t i = base + int64_t(e) * kTq + threadIdx.x;
+    if (i < n) {
+      const int64_t at = p * n + i;
+      float v = qsum.partial[at];
+      for (int z0 = 1; z0 < qsum.S; z0 += 16) {
+        float pz[16];
+#pragma unroll
+        for (int z = 0; z < 16; ++z)
+          pz[z] = z0 + z < qsum.S ? qsum.partial[int64_t(z0 + z) * count + at] : 0.f;
+#pragma unroll
+        for (int z = 0; z < 16; ++z)
+          if (z0 + z < qsum.S) v += pz[z];
+      }
+      const float dv = d[at];
+      if (qsum.nugget) v = fmaf(qsum.noise, dv, v);
+      q[at] = v;
+      acc = fmaf(dv, v, acc);
+    }
+  }
+  const float s = block_sum_f<kTq>(acc, red);
+  if (threadIdx.x == 0) part[p * nb + c] = s;
+}
+
 __device__ __forceinline__ float sum_parts(const float* part, int64_t p, int64_t nb) {
   float s = 0.f;
   for (int64_t c = 0; c < nb; ++c) s += part[p * nb + c];
   return s;
 }
 
-// initial scalars: gamma = r . z, atol2 = max(tol^2 b.b, atol^2), active
+// generation 0: gamma = r . z, atol2 = max(tol^2 b.b, atol^2), active
 __global__ void __launch_bounds__(kT)
     k_cg_start(const float* __restrict__ rz, const float* __restrict__ bb,
                const float* __restrict__ rr, int64_t P, int64_t nb, float tol, float atol,
@@ -402,7 +450,6 @@ __global__ void __launch_bounds__(kT)
     sc.iters[p] = 0;
     const int act = (rs > a2) && (maxiter > 0);
     sc.active[p] = act;
-    sc.upd[p] = 0;
     cnt += float(act);
   }
   const float tot = block_sum_f(cnt, red);
@@ -410,20 +457,21 @@ __global__ void __launch_bounds__(kT)
 }
 
 // alpha = gamma / (p . A p) (every block sums the partials itself), then
-// x += alpha p, r -= alpha A p for the active columns and the partial r . r
-__global__ void __launch_bounds__(kT)
+// x += alpha p, r -= alpha A p for the columns active in generation gen and
+// the partial r . r
+__global__ void __launch_bounds__(kTq)
     k_cg_update(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ d,
                 const float* __restrict__ q, int64_t n, const float* __restrict__ pq,
-                CgScalars sc, float* __restrict__ rr) {
-  __shared__ float red[kT / 64];
-  const int64_t p = blockIdx.y, c = blockIdx.x, nb = gridDim.x;
+                CgScalars sc, int gen, float* __restrict__ rr) {
+  __shared__ float red[kTq / 64];
+  const int64_t p = blockIdx.y, c = blockIdx.x, nb = gridDim.x, P = gridDim.y;
   const int64_t base = c * kCgChunk;
-  const bool act = sc.active[p] != 0;
-  const float al = act ? sc.gamma[p] / sum_parts(pq, p, nb) : 0.f;
+  const bool act = sc.active[gen * P + p] != 0;
+  const float al = act ? sc.gamma[gen * P + p] / sum_parts(pq, p, nb) : 0.f;
   float acc = 0.f;
 #pragma unroll
-  for (int e = 0; e < kCgPer; ++e) {
-    const int64_t i = base + int64_t(e) * kT + threadIdx.x;
+  for (int e = 0; e < kQPer; ++e) {
+    const int64_t i = base + int64_t(e) * kTq + threadIdx.x;
     if (i < n) {
       float rv = r[p * n + i];
       if (act) {
@@ -434,48 +482,51 @@ __global__ void __launch_bounds__(kT)
       acc = fmaf(rv, rv, acc);
     }
   }
-  const float s = block_sum_f(acc, red);
+  const float s = block_sum_f<kTq>(acc, red);
   if (threadIdx.x == 0) rr[p * nb + c] = s;
 }
 
-// gamma' = r . z, beta = gamma' / gamma, the stopping rule and the count of
-// columns still active
-__global__ void __launch_bounds__(kT)
-    k_cg_beta(const float* __restrict__ rz, const float* __restrict__ rr, int precond, int64_t P,
-              int64_t nb, int64_t maxiter, CgScalars sc, int* __restrict__ nactive) {
-  __shared__ float red[kT / 64];
-  float cnt = 0.f;
-  for (int64_t p = threadIdx.x; p < P; p += kT) {
-    const int act = sc.active[p];
-    sc.upd[p] = act;
+// For the columns active in generation gen: gamma' = r . z, beta = gamma' /
+// gamma and p = z + beta p (every block sums the partials itself); block 0 of
+// each column writes generation gen ^ 1 of gamma and active (the stopping
+// rule) and the iteration count.
+__global__ void __launch_bounds__(kTq)
+    k_cg_beta_direction(float* __restrict__ d, const float* __restrict__ z, int64_t n,
+                        const float* __restrict__ rz, const float* __restrict__ rr, int precond,
+                        int64_t maxiter, CgScalars sc, int gen) {
+  const int64_t p = blockIdx.y, c = blockIdx.x, nb = gridDim.x, P = gridDim.y;
+  const int64_t cur = gen * P + p, nxt = (gen ^ 1) * P + p;
+  const int act = sc.active[cur];
+  const float g = act ? sum_parts(precond ? rz : rr, p, nb) : 0.f;
+  if (c == 0 && threadIdx.x == 0) {
+    int next = 0;
     if (act) {
-      const float g = sum_parts(precond ? rz : rr, p, nb);
-      sc.beta[p] = g / sc.gamma[p];
-      sc.gamma[p] = g;
       const int64_t k = sc.iters[p] + 1;
       sc.iters[p] = k;
       const float rs = precond ? sum_parts(rr, p, nb) : g;
-      const int next = (rs > sc.atol2[p]) && (k < maxiter);
-      sc.active[p] = next;
-      cnt += float(next);
+      next = (rs > sc.atol2[p]) && (k < maxiter);
     }
+    sc.gamma[nxt] = act ? g : sc.gamma[cur];
+    sc.active[nxt] = next;
   }
-  const float tot = block_sum_f(cnt, red);
-  if (threadIdx.x == 0) *nactive = int(tot + 0.5f);
-}
-
-// p = z + beta p for the columns updated in this iteration
-__global__ void __launch_bounds__(kT)
-    k_cg_direction(float* __restrict__ d, const float* __restrict__ z, int64_t n, CgScalars sc) {
-  const int64_t p = blockIdx.y, c = blockIdx.x;
-  if (!sc.upd[p]) return;
-  const float be = sc.beta[p];
+  if (!act) return;
+  const float be = g / sc.gamma[cur];
   const int64_t base = c * kCgChunk;
 #pragma unroll
-  for (int e = 0; e < kCgPer; ++e) {
-    const int64_t i = base + int64_t(e) * kT + threadIdx.x;
+  for (int e = 0; e < kQPer; ++e) {
+    const int64_t i = base + int64_t(e) * kTq + threadIdx.x;
     if (i < n) d[p * n + i] = fmaf(be, d[p * n + i], z[p * n + i]);
   }
+}
+
+// *nactive = number of columns active in generation gen
+__global__ void __launch_bounds__(kT)
+    k_cg_count(CgScalars sc, int gen, int64_t P, int* __restrict__ nactive) {
+  __shared__ float red[kT / 64];
+  float cnt = 0.f;
+  for (int64_t p = threadIdx.x; p < P; p += kT) cnt += float(sc.active[gen * P + p] != 0);
+  const float tot = block_sum_f(cnt, red);
+  if (threadIdx.x == 0) *nactive = int(tot + 0.5f);
 }
 
 inline unsigned blocks_for(int64_t count) { return unsigned((count + kT - 1) / kT); }
@@ -559,22 +610,29 @@ hipError_t launch_cg_start(hipStream_t s, const float* rz, const float* bb, cons
   return hipGetLastError();
 }
 
-hipError_t launch_cg_step(hipStream_t s, float* x, float* r, float* d, const float* q, float* z,
-                          bool precond, int64_t P, int64_t n, int64_t maxiter, const CgScalars& sc,
-                          float* pq, float* rr, float* rz, int* nactive,
-                          const CgPrecond* M) {
+hipError_t launch_cg_step(hipStream_t s, float* x, float* r, float* d, float* q,
+                          const MvSplitSum& qsum, float* z, bool precond, int64_t P, int64_t n,
+                          int64_t maxiter, const CgScalars& sc, int gen, float* pq, float* rr,
+                          float* rz, const CgPrecond* M) {
   const int64_t nb = cg_chunks(n);
   const dim3 grid{unsigned(nb), unsigned(P)};
-  hipLaunchKernelGGL(k_cg_dot, grid, dim3(kT), 0, s, d, q, n, pq);
-  hipLaunchKernelGGL(k_cg_update, grid, dim3(kT), 0, s, x, r, d, q, n, pq, sc, rr);
+  if (qsum.partial)
+    hipLaunchKernelGGL(k_cg_dot_q, grid, dim3(kTq), 0, s, d, qsum, n, q, pq);
+  else
+    hipLaunchKernelGGL(k_cg_dot, grid, dim3(kT), 0, s, d, q, n, pq);
+  hipLaunchKernelGGL(k_cg_update, grid, dim3(kTq), 0, s, x, r, d, q, n, pq, sc, gen, rr);
   if (precond) {
     hipError_t e = launch_woodbury_apply(s, M->L, n, M->rank, M->shift, M->C, r, P, M->Wp, M->Z, z);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_cg_dot, grid, dim3(kT), 0, s, r, z, n, rz);
   }
-  hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kT), 0, s, rz, rr, precond ? 1 : 0, P, nb, maxiter,
-                     sc, nactive);
-  hipLaunchKernelGGL(k_cg_direction, grid, dim3(kT), 0, s, d, precond ? z : r, n, sc);
+  hipLaunchKernelGGL(k_cg_beta_direction, grid, dim3(kTq), 0, s, d, precond ? z : r, n, rz, rr,
+                     precond ? 1 : 0, maxiter, sc, gen);
+  return hipGetLastError();
+}
+
+hipError_t launch_cg_count(hipStream_t s, const CgScalars& sc, int gen, int64_t P, int* nactive) {
+  hipLaunchKernelGGL(k_cg_count, dim3(1), dim3(kT), 0, s, sc, gen, P, nactive);
   return hipGetLastError();
 }
 

@@ -823,37 +823,36 @@ __global__ void __launch_bounds__(kThreads)
 
 // RBF matvec for a few vectors (P <= kRowsMaxP), as CG's single right-hand
 // side needs it: the MFMA kernels pad the probe dimension to 32, so here the
-// kernel is evaluated on the vector units instead, exactly in FP32
-// (d2 = sum (x_i - x_j)^2 from the coordinates, exp2(coef d2)).  A workgroup
-// owns kRowsRB rows (kRowsPerLane per lane) and a column slice; chunks of 256
-// points (coordinates padded to four, one 16-byte LDS read per point, and the
-// vector entries) are staged in LDS and each wave takes a quarter of every
-// chunk; the four waves' sums are added in LDS in a fixed order.  Column
-// slices > 1 write split-K partial sums.
+// kernel is evaluated on the vector units instead, in FP32 from the points:
+// with the coordinates scaled by s = sqrt(log2(e) / (2 l^2)) when they are
+// loaded, k = exp2(-sum_d (s x_i - s x_j)^2).  A workgroup owns kRowsRB rows
+// (kRowsPerLane per lane) and a column slice; chunks of 256 points (scaled
+// coordinates, with the vector entry in the fourth slot for one vector of
+// dimension <= 3: one 16-byte LDS read per point) are staged in LDS and each
+// wave takes a quarter of every chunk; the four waves' sums are added in LDS
+// in a fixed order.  Column slices > 1 write split-K partial sums.
 constexpr int kRowsMaxP = 4;
 constexpr int kRowsPerLane = 2;
 constexpr int kRowsRB = 64 * kRowsPerLane;
 constexpr int kRowsChunk = 256;
 constexpr int kRowsMaxSplit = 16;
 
-template <int NP>
+template <int NP, int DIM>
 __global__ void __launch_bounds__(256)
-    k_matvec_rbf_rows(RBFParams prm, const float* __restrict__ V, int64_t n,
-                      float* __restrict__ W) {
+    k_matvec_rbf_rows(const float* __restrict__ X, float scale, float noise,
+                      const float* __restrict__ V, int64_t n, float* __restrict__ W) {
+  constexpr bool PACKV = DIM <= 3 && NP == 1;  // vector entry in the fourth slot
   __shared__ float4 xs[kRowsChunk];
-  __shared__ float vs[kRowsChunk * NP];
+  __shared__ float vs[PACKV ? 1 : kRowsChunk * NP];
   __shared__ float red[3][NP][kRowsRB];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int dim = prm.dim;
   const int64_t j0 = n * blockIdx.y / gridDim.y, j1 = n * (blockIdx.y + 1) / gridDim.y;
-  float4 xi[kRowsPerLane];
+  float xi[kRowsPerLane][DIM];
 #pragma unroll
   for (int u = 0; u < kRowsPerLane; ++u) {
     const int64_t row = int64_t(blockIdx.x) * kRowsRB + 64 * u + lane;
-    float c[4] = {0.f, 0.f, 0.f, 0.f};
-    if (row < n)
-      for (int d = 0; d < dim; ++d) c[d] = prm.X[row * dim + d];
-    xi[u] = make_float4(c[0], c[1], c[2], c[3]);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) xi[u][d] = row < n ? X[row * DIM + d] * scale : 0.f;
   }
   float acc[kRowsPerLane][NP];
 #pragma unroll
@@ -866,26 +865,39 @@ __global__ void __launch_bounds__(256)
       const int64_t j = c0 + t;
       const bool ok = t < cnt;
       float c[4] = {0.f, 0.f, 0.f, 0.f};
-      if (ok)
-        for (int d = 0; d < dim; ++d) c[d] = prm.X[j * dim + d];
-      xs[t] = make_float4(c[0], c[1], c[2], c[3]);
+      if (ok) {
 #pragma unroll
-      for (int q = 0; q < NP; ++q) vs[t * NP + q] = ok ? V[q * n + j] : 0.f;
+        for (int d = 0; d < DIM; ++d) c[d] = X[j * DIM + d] * scale;
+        if (PACKV) c[3] = V[j];
+      }
+      xs[t] = make_float4(c[0], c[1], c[2], c[3]);
+      if (!PACKV)
+#pragma unroll
+        for (int q = 0; q < NP; ++q) vs[t * NP + q] = ok ? V[q * n + j] : 0.f;
     }
     __syncthreads();
     const int jb = wave * (kRowsChunk / 4);
     const int je = jb + kRowsChunk / 4 < cnt ? jb + kRowsChunk / 4 : int(cnt);
     for (int jj = jb; jj < je; ++jj) {
       const float4 xj = xs[jj];
+      const float xc[4] = {xj.x, xj.y, xj.z, xj.w};
       float vj[NP];
+      if (PACKV) {
+        vj[0] = xj.w;
+      } else {
 #pragma unroll
-      for (int q = 0; q < NP; ++q) vj[q] = vs[jj * NP + q];
+        for (int q = 0; q < NP; ++q) vj[q] = vs[jj * NP + q];
+      }
 #pragma unroll
       for (int u = 0; u < kRowsPerLane; ++u) {
-        const float dx = xi[u].x - xj.x, dy = xi[u].y - xj.y;
-        const float dz = xi[u].z - xj.z, dw = xi[u].w - xj.w;
-        const float d2 = fmaf(dw, dw, fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
-        const float k = __builtin_amdgcn_exp2f(prm.coef * d2);
+        const float d0 = xi[u][0] - xc[0];
+        float d2 = d0 * d0;
+#pragma unroll
+        for (int d = 1; d < DIM; ++d) {
+          const float dd = xi[u][d] - xc[d];
+          d2 = fmaf(dd, dd, d2);
+        }
+        const float k = __builtin_amdgcn_exp2f(-d2);
 #pragma unroll
         for (int q = 0; q < NP; ++q) acc[u][q] = fmaf(k, vj[q], acc[u][q]);
       }
@@ -911,20 +923,44 @@ __global__ void __launch_bounds__(256)
         if (split)
           W[int64_t(blockIdx.y) * NP * n + q * n + row] = sum;
         else
-          W[q * n + row] = fmaf(prm.noise, V[q * n + row], sum);
+          W[q * n + row] = fmaf(noise, V[q * n + row], sum);
       }
     }
   }
 }
 
-hipError_t launch_rows(hipStream_t s, const RBFParams& prm, const float* V, int64_t P, int64_t n,
-                       float* W, int S) {
-  const dim3 grid(unsigned((n + kRowsRB - 1) / kRowsRB), unsigned(S));
+template <int DIM>
+void launch_rows_dim(hipStream_t s, const float* X, float scale, float noise, const float* V,
+                     int64_t P, int64_t n, float* W, dim3 grid) {
   switch (P) {
-    case 1: hipLaunchKernelGGL(k_matvec_rbf_rows<1>, grid, dim3(256), 0, s, prm, V, n, W); break;
-    case 2: hipLaunchKernelGGL(k_matvec_rbf_rows<2>, grid, dim3(256), 0, s, prm, V, n, W); break;
-    case 3: hipLaunchKernelGGL(k_matvec_rbf_rows<3>, grid, dim3(256), 0, s, prm, V, n, W); break;
-    default: hipLaunchKernelGGL(k_matvec_rbf_rows<4>, grid, dim3(256), 0, s, prm, V, n, W); break;
+    case 1:
+      hipLaunchKernelGGL((k_matvec_rbf_rows<1, DIM>), grid, dim3(256), 0, s, X, scale, noise, V,
+                         n, W);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_matvec_rbf_rows<2, DIM>), grid, dim3(256), 0, s, X, scale, noise, V,
+                         n, W);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_matvec_rbf_rows<3, DIM>), grid, dim3(256), 0, s, X, scale, noise, V,
+                         n, W);
+      break;
+    default:
+      hipLaunchKernelGGL((k_matvec_rbf_rows<4, DIM>), grid, dim3(256), 0, s, X, scale, noise, V,
+                         n, W);
+      break;
+  }
+}
+
+// scale = sqrt(log2(e) / (2 l^2)); dim in 1..4
+hipError_t launch_rows(hipStream_t s, const float* X, int dim, float scale, float noise,
+                       const float* V, int64_t P, int64_t n, float* W, int S) {
+  const dim3 grid(unsigned((n + kRowsRB - 1) / kRowsRB), unsigned(S));
+  switch (dim) {
+    case 1: launch_rows_dim<1>(s, X, scale, noise, V, P, n, W, grid); break;
+    case 2: launch_rows_dim<2>(s, X, scale, noise, V, P, n, W, grid); break;
+    case 3: launch_rows_dim<3>(s, X, scale, noise, V, P, n, W, grid); break;
+    default: launch_rows_dim<4>(s, X, scale, noise, V, P, n, W, grid); break;
   }
   return hipGetLastError();
 }
@@ -1076,26 +1112,23 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
   if (n <= 0 || P <= 0) return hipSuccess;
   if (op.kind == SLQ_OP_RBF && P <= kRowsMaxP && op.dim >= 1 && op.dim <= 4) {
     // a few vectors: the row kernel in FP32 (every precision gets FP32 here)
-    RBFParams prm{};
-    prm.X = op.points;
-    prm.dim = op.dim;
-    prm.coef = float(-1.4426950408889634 / (2.0 * op.lengthscale * op.lengthscale));
-    prm.noise = float(op.noise);
+    const float scale = float(sqrt(1.4426950408889634 / (2.0 * op.lengthscale * op.lengthscale)));
+    const float noise = float(op.noise);
     const int S = scratch ? choose_rows_splitk(n, P, scratch_floats) : 1;
     float* out = S > 1 ? scratch : W;
     if (ev0) (void)hipEventRecord(ev0, s);
-    hipError_t err = launch_rows(s, prm, V, P, n, out, S);
+    hipError_t err = launch_rows(s, op.points, op.dim, scale, noise, V, P, n, out, S);
     if (ev1) (void)hipEventRecord(ev1, s);
     if (err != hipSuccess || S == 1) return err;
     if (defer) {
-      *defer = MvSplitSum{scratch, S, V, prm.noise, 1};
+      *defer = MvSplitSum{scratch, S, V, noise, 1};
       return hipSuccess;
     }
     const int64_t count = P * n;
     int64_t blocks = (count + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_splitk_reduce<true>, dim3(unsigned(blocks)), dim3(256), 0, s, scratch,
-                       S, count, V, prm.noise, W);
+                       S, count, V, noise, W);
     return hipGetLastError();
   }
   const bool vec = (n % 4 == 0) && aligned16(V) && aligned16(W);

@@ -862,19 +862,17 @@ int slq_cg(slq_context* ctx, const slq_operator* op, const float* B, int64_t P, 
   SLQ_TRY(ensure_t(ctx, ctx->cg_q, size_t(P * n), &q));
   if (precond) SLQ_TRY(ensure_t(ctx, ctx->cg_z, size_t(P * n), &z));
   SLQ_TRY(ensure_t(ctx, ctx->cg_parts, size_t(4 * P * nb), &parts));
-  // scalars: gamma, atol2, alpha, beta (floats), iters (int64), active, upd,
-  // count (ints), 16 words per column plus the count
+  // scalars: gamma (2 generations), atol2 (floats), iters (int64), active
+  // (2 generations), the count (ints); 16 words per column plus the count
   SLQ_TRY(ensure_t(ctx, ctx->cg_scal, size_t(16 * P + 64), &scal));
   const int64_t mv_floats = matvec_scratch_floats(n, P);
   SLQ_TRY(ensure_t(ctx, ctx->splitk, size_t(mv_floats), &mvs));
   float *pq = parts, *rr = parts + P * nb, *rz = parts + 2 * P * nb, *bb = parts + 3 * P * nb;
   CgScalars sc;
   sc.gamma = scal;
-  sc.atol2 = scal + P;
-  sc.beta = scal + 3 * P;
+  sc.atol2 = scal + 2 * P;
   sc.iters = reinterpret_cast<int64_t*>(scal + 4 * P);  // 16 P bytes in: 8-byte aligned
   sc.active = reinterpret_cast<int*>(scal + 8 * P + 2);
-  sc.upd = reinterpret_cast<int*>(scal + 9 * P + 2);
   int* nactive = reinterpret_cast<int*>(scal + 10 * P + 2);
   CgPrecond M;
   if (precond) {
@@ -904,18 +902,23 @@ int slq_cg(slq_context* ctx, const slq_operator* op, const float* B, int64_t P, 
                                maxiter, sc, nactive));
   // The host reads the count of active columns after batches of 1, 2, 4, 8,
   // 8, ... iterations; an iteration with no active column changes nothing.
+  // Iteration t reads scalar generation t % 2 and writes the other.
+  int64_t t = 0;
   for (int64_t batch = 1;; batch = batch < 8 ? 2 * batch : 8) {
     SLQ_HIP(ctx, hipMemcpyAsync(ctx->host_count, nactive, sizeof(int), hipMemcpyDeviceToHost, s));
     SLQ_HIP(ctx, hipStreamSynchronize(s));
     if (*ctx->host_count == 0) break;
-    for (int64_t b = 0; b < batch; ++b) {
+    for (int64_t b = 0; b < batch; ++b, ++t) {
+      MvSplitSum qsum;
       {
         const auto ev = take_events(ctx, SLQ_KC_MATVEC);
-        SLQ_HIP(ctx, launch_matvec(s, *op, d, P, q, mvs, mv_floats, ev.first, ev.second));
+        SLQ_HIP(ctx, launch_matvec(s, *op, d, P, q, mvs, mv_floats, ev.first, ev.second, 0,
+                                   &qsum));
       }
-      SLQ_HIP(ctx, launch_cg_step(s, X, r, d, q, z, precond, P, n, maxiter, sc, pq, rr, rz,
-                                  nactive, precond ? &M : nullptr));
+      SLQ_HIP(ctx, launch_cg_step(s, X, r, d, q, qsum, z, precond, P, n, maxiter, sc,
+                                  int(t & 1), pq, rr, rz, precond ? &M : nullptr));
     }
+    SLQ_HIP(ctx, launch_cg_count(s, sc, int(t & 1), P, nactive));
   }
   if (iterations)
     SLQ_HIP(ctx, hipMemcpyAsync(iterations, sc.iters, size_t(P) * sizeof(int64_t),

@@ -225,13 +225,14 @@ hipError_t launch_woodbury_apply(hipStream_t s, const float* L, int64_t n, int r
                                  const double* C, const float* V, int64_t P, float* Wp, double* Z,
                                  float* out);
 // CG over P columns: per-column scalars and the preconditioner's buffers.
+// gamma and active hold two generations of P entries each: iteration t reads
+// generation t % 2 and writes generation (t + 1) % 2, so that every block of
+// a kernel can read the old values while one block writes the new ones.
 struct CgScalars {
-  float* gamma = nullptr;
-  float* atol2 = nullptr;
-  float* beta = nullptr;
-  int64_t* iters = nullptr;
-  int* active = nullptr;
-  int* upd = nullptr;
+  float* gamma = nullptr;   // 2 P
+  float* atol2 = nullptr;   // P
+  int64_t* iters = nullptr; // P
+  int* active = nullptr;    // 2 P
 };
 struct CgPrecond {
   const float* L = nullptr;
@@ -245,15 +246,21 @@ struct CgPrecond {
 int64_t cg_chunks(int64_t n);
 hipError_t launch_cg_dot(hipStream_t s, const float* a, const float* b, int64_t P, int64_t n,
                          float* part);
-// gamma = sum rz, atol2 = max(tol^2 sum bb, atol^2), rs = sum rr (or gamma
-// when rr is null), active = rs > atol2 && maxiter > 0; *nactive = count
+// generation 0: gamma = sum rz, atol2 = max(tol^2 sum bb, atol^2), rs = sum
+// rr (or gamma when rr is null), active = rs > atol2 && maxiter > 0;
+// *nactive = count
 hipError_t launch_cg_start(hipStream_t s, const float* rz, const float* bb, const float* rr,
                            int64_t P, int64_t n, float tol, float atol, int64_t maxiter,
                            const CgScalars& sc, int* nactive);
-// One CG iteration after q = A d: alpha, x and r, z = M r (precond), gamma,
-// beta, the stopping rule, d = z + beta d; *nactive = columns still active.
-hipError_t launch_cg_step(hipStream_t s, float* x, float* r, float* d, const float* q, float* z,
-                          bool precond, int64_t P, int64_t n, int64_t maxiter, const CgScalars& sc,
-                          float* pq, float* rr, float* rz, int* nactive, const CgPrecond* M);
+// Iteration t (gen = t % 2) after the matvec of d: q = A d (the deferred
+// split-K sum `qsum` is added up and written to q when its partial is
+// non-null), alpha, x and r, z = M r (precond), gamma, beta, the stopping
+// rule and d = z + beta d for the columns active in generation gen.
+hipError_t launch_cg_step(hipStream_t s, float* x, float* r, float* d, float* q,
+                          const MvSplitSum& qsum, float* z, bool precond, int64_t P, int64_t n,
+                          int64_t maxiter, const CgScalars& sc, int gen, float* pq, float* rr,
+                          float* rz, const CgPrecond* M);
+// *nactive = number of columns active in generation gen
+hipError_t launch_cg_count(hipStream_t s, const CgScalars& sc, int gen, int64_t P, int* nactive);
 
 }  // namespace slq
