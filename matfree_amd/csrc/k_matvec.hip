@@ -363,18 +363,24 @@ struct PlaneTile {
   static constexpr int kPerThread = (kPieces + kThreads - 1) / kThreads;
 };
 
+// The contiguous BPW x 32 block of a plane tile starting at base.
 template <int NACC>
-__device__ __forceinline__ void load_plane_tile(const __bf16* __restrict__ plane, int64_t Ppad,
-                                                int64_t p0, int64_t kt,
-                                                u32x4 (&reg)[PlaneTile<NACC>::kPerThread]) {
+__device__ __forceinline__ void load_plane_tile_at(const __bf16* __restrict__ base,
+                                                   u32x4 (&reg)[PlaneTile<NACC>::kPerThread]) {
   const int t = threadIdx.x;
-  const __bf16* base = plane + (kt * Ppad + p0) * BK;  // contiguous BPW x 32 block
 #pragma unroll
   for (int u = 0; u < PlaneTile<NACC>::kPerThread; ++u) {
     const int f = t + u * kThreads;
     if (PlaneTile<NACC>::kPieces % kThreads == 0 || f < PlaneTile<NACC>::kPieces)
       reg[u] = *reinterpret_cast<const u32x4*>(base + 8 * f);
   }
+}
+
+template <int NACC>
+__device__ __forceinline__ void load_plane_tile(const __bf16* __restrict__ plane, int64_t Ppad,
+                                                int64_t p0, int64_t kt,
+                                                u32x4 (&reg)[PlaneTile<NACC>::kPerThread]) {
+  load_plane_tile_at<NACC>(plane + (kt * Ppad + p0) * BK, reg);
 }
 
 template <int NACC>
@@ -753,6 +759,13 @@ __global__ void __launch_bounds__(kThreads)
   // work issues while the matrix core runs), and the LDS stores last.
   constexpr int kM = 2 * NPROD * NACC;  // BF16 MFMAs per tile
   constexpr int kLead = kM > 3 ? 3 : kM - 1;
+  // The loads of step kt read tile min(kt + 2, ntiles - 1): three pointers
+  // advanced by one tile per step while a further tile exists (cheaper than
+  // recomputing 64-bit tile offsets each step).
+  const u32x4* pnext = prow + 4 * BK * clamp(2);
+  const int64_t pstep = Ppad * BK;
+  const __bf16* hnext = Vhi_g + ((kt_begin + clamp(2)) * Ppad + p0) * BK;
+  const __bf16* lnext = LO ? Vlo_g + ((kt_begin + clamp(2)) * Ppad + p0) * BK : nullptr;
   auto step = [&](int64_t kt, u32x4 (&ih)[PT], u32x4 (&il)[PT], const u32x4 (&sh)[PT],
                   const u32x4 (&sl)[PT], u32x4 (&xi)[2], const u32x4 (&xs)[2],
                   const u32x4 (&ahi)[2], const u32x4 (&alo)[2], u32x4 (&nhi)[2],
@@ -760,9 +773,15 @@ __global__ void __launch_bounds__(kThreads)
     const int buf = int(kt & 1);
     // points (needed one step later) ahead of the planes (needed at the end
     // of the next step)
-    load_pts(clamp(kt + 2), xi);
-    load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin + clamp(kt + 2), ih);
-    if (LO) load_plane_tile<NACC>(Vlo_g, Ppad, p0, kt_begin + clamp(kt + 2), il);
+    xi[0] = pnext[0];
+    xi[1] = pnext[2];
+    load_plane_tile_at<NACC>(hnext, ih);
+    if (LO) load_plane_tile_at<NACC>(lnext, il);
+    if (kt + 3 < ntiles) {
+      pnext += 4 * BK;
+      hnext += pstep;
+      if (LO) lnext += pstep;
+    }
     __builtin_amdgcn_sched_barrier(0);
     const floatx16 dnext = rbf_exponents_mfma(xs, rr);
     const __bf16* vh = Vhi[buf] + r * LDB + 8 * h;
