@@ -36,7 +36,10 @@ def test_num_matvecs_out_of_range_raises_exceeds(k):
     with pytest.raises(ValueError, match="exceeds"):
         decomp.tridiag_sym(k)(operators.dense_matvec, v0, A)
     with pytest.raises(ValueError, match="exceeds"):
-        decomp.hessenberg(k, reortho="full")(operators.dense_matvec, v0, A)
+        decomp.tridiag_sym(k, reortho="none")(operators.dense_matvec, v0, A)
+    for reortho in ("full", "none"):
+        with pytest.raises(ValueError, match="exceeds"):
+            decomp.hessenberg(k, reortho=reortho)(operators.dense_matvec, v0, A)
     integrand = funm.monte_carlo_funm_sym_logdet(decomp.tridiag_sym(k))
     with pytest.raises(ValueError, match="exceeds"):
         integrand(operators.dense_matvec, v0, A)

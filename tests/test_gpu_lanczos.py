@@ -288,3 +288,26 @@ def test_lanczos_adjoint_k0_raises(cuda):
     Z = torch.ones((2, 16), device=cuda)
     with pytest.raises(ValueError, match="num_matvecs >= 1"):
         funm.logdet_value_and_grad(operators.RBF(X), Z, 0, (0.5, 0.1), reortho="none")
+
+
+@pytest.mark.parametrize("num_matvecs", [6, 13, 0])
+@pytest.mark.parametrize("method", ["tridiag_none", "tridiag_full", "hessenberg_none",
+                                    "hessenberg_full"])
+def test_output_shapes_consistent(cuda, num_matvecs, method):
+    # tests/test_decomp/test_consistency.py:26-45: every factorisation returns
+    # (k, n) vectors, a (k, k) matrix, an (n,) residual and a scalar
+    nrows = 13
+    rng = np.random.default_rng(1)
+    A = torch.as_tensor(rng.standard_normal((nrows, nrows)).astype(np.float32), device=cuda)
+    v0 = rng.standard_normal(nrows).astype(np.float32)
+    kind, reortho = method.split("_")
+    if kind == "tridiag":
+        algorithm = decomp.tridiag_sym(num_matvecs, reortho=reortho)
+    else:
+        algorithm = decomp.hessenberg(num_matvecs, reortho=reortho)
+    Us, B, res, ln = algorithm(operators.dense_matvec, v0, A)
+    assert tuple(Us.shape) == (num_matvecs, nrows)
+    assert tuple(B.shape) == (num_matvecs, num_matvecs)
+    assert tuple(res.shape) == (nrows,)
+    assert tuple(ln.shape) == ()
+    assert np.all(np.isfinite(res.cpu().numpy()))
