@@ -228,7 +228,9 @@ int slq_lowrank_solve(slq_context* ctx, const float* L, int64_t n, int64_t rank,
  * preconditioner) or after maxiter steps; converged rows keep their state.
  * With L non-NULL the preconditioner is (s I + L L^T)^{-1} (rank columns of
  * L, n x rank row-major).  iterations (P device int64, may be NULL) receives
- * the steps each row took.  The host reads one int per iteration. */
+ * the steps each row took.  The host reads one int (the number of rows still
+ * active) after batches of 1, 2, 4, 8, 8, ... iterations; iterations run
+ * after the last row stopped change nothing. */
 int slq_cg(slq_context* ctx, const slq_operator* op, const float* B, int64_t P, double tol,
            double atol, int64_t maxiter, const float* L, int64_t rank, double shift, float* X,
            int64_t* iterations);
