@@ -88,6 +88,17 @@ __device__ __forceinline__ double wave_max_f64(double v) {
   return v;
 }
 
+// 1 / q for the Sturm recurrence: the hardware reciprocal refined by two
+// Newton steps (q is finite and |q| >= pivmin there, so the special cases a
+// full division handles do not arise).  The recurrence is a serial chain of
+// these, so the shorter sequence is what bounds the bisection's speed.
+__device__ __forceinline__ double sturm_recip(double q) {
+  double r = __builtin_amdgcn_rcp(q);
+  r = fma(r, fma(-q, r, 1.0), r);
+  r = fma(r, fma(-q, r, 1.0), r);
+  return r;
+}
+
 // Number of eigenvalues of the leading m x m block that are < x.
 __device__ __forceinline__ int sturm_count(const double* __restrict__ a,
                                            const double* __restrict__ b2, int m, double x,
@@ -96,7 +107,7 @@ __device__ __forceinline__ int sturm_count(const double* __restrict__ a,
   if (fabs(q) <= pivmin) q = -pivmin;
   int cnt = q < 0.0;
   for (int i = 1; i < m; ++i) {
-    q = (a[i] - x) - b2[i - 1] / q;
+    q = fma(-b2[i - 1], sturm_recip(q), a[i] - x);
     if (fabs(q) <= pivmin) q = -pivmin;
     cnt += q < 0.0;
   }
@@ -153,17 +164,26 @@ __device__ __forceinline__ double twisted_weight(const double* __restrict__ a,
 // ---------------------------------------------------------------------------
 // Full eigendecomposition of T (from H as above) by implicit QL with the
 // eigenvectors accumulated.  One wave per probe; (d, e) are k doubles, U is
-// rows x k row-major (U[a * k + j] = component a of eigenvector j; rows = k
-// for the whole matrix, 1 for its first row only), in LDS for
-// k <= kEigLdsDepth and in a global workspace beyond.  The QL recurrence is
-// evaluated redundantly by every lane (uniform control flow); the rotations
-// of U are spread over the lanes by rows.  Returns false if QL did not
-// converge.  On return d holds the eigenvalues (unordered).
+// rows x k row-major with row stride ldu (U[a * ldu + j] = component a of
+// eigenvector j; rows = k for the whole matrix, 1 for its first row only), in
+// LDS for k <= kEigLdsDepth and in a global workspace beyond.  The QL
+// recurrence is evaluated redundantly by every lane (uniform control flow);
+// the rotations of U are spread over the lanes by rows.  Returns false if QL
+// did not converge.  On return d holds the eigenvalues (unordered).
+//
+// The lanes of a rotation touch U[a * ldu + i] for a = lane: with an odd
+// stride in doubles (eig_ld) these fall into distinct LDS bank pairs, where a
+// stride of k (a multiple of 32 doubles for k = 32, 64) put all 64 lanes on
+// one bank.  The wave routines of this file are inlined into their kernels:
+// a called function sees generic pointers and reaches LDS arrays through
+// flat instructions, with longer latency on these serial recurrences.
 // ---------------------------------------------------------------------------
 constexpr int kEigLdsDepth = 64;
+__host__ __device__ constexpr int eig_ld(int k) { return k | 1; }
 
-__device__ bool tridiag_eigvecs_wave(const float* __restrict__ Hp, int k, double* d, double* e,
-                                     double* U, int rows) {
+__device__ __forceinline__ bool tridiag_eigvecs_wave(const float* __restrict__ Hp, int k,
+                                                     double* d, double* e, double* U, int rows,
+                                                     int ldu) {
   const int lane = threadIdx.x;
   for (int i = lane; i < k; i += 64) {
     d[i] = double(Hp[int64_t(i) * k + i]);
@@ -171,17 +191,25 @@ __device__ bool tridiag_eigvecs_wave(const float* __restrict__ Hp, int k, double
                        : 0.0;
   }
   for (int a = lane; a < rows; a += 64)
-    for (int j = 0; j < k; ++j) U[int64_t(a) * k + j] = (a == j) ? 1.0 : 0.0;
+    for (int j = 0; j < k; ++j) U[int64_t(a) * ldu + j] = (a == j) ? 1.0 : 0.0;
   __syncthreads();
 
   bool failed = false;
   for (int l = 0; l < k && !failed; ++l) {
     int sweeps = 0;
     while (true) {
-      int m = l;
-      for (; m + 1 < k; ++m) {
-        const double dd = fabs(d[m]) + fabs(d[m + 1]);
-        if (fabs(e[m]) <= DBL_EPSILON * dd) break;
+      // m = first index >= l with a negligible e[m] (or k - 1): every lane
+      // tests one index, a ballot picks the first
+      int m = k - 1;
+      for (int base = l; base < k - 1; base += 64) {
+        const int mm = base + lane;
+        bool stop = true;
+        if (mm + 1 < k) stop = fabs(e[mm]) <= DBL_EPSILON * (fabs(d[mm]) + fabs(d[mm + 1]));
+        const unsigned long long hit = __ballot(stop);
+        if (hit) {
+          m = base + __builtin_ctzll(hit);
+          break;
+        }
       }
       if (m == l) break;
       if (++sweeps > kMaxSweeps) {
@@ -193,30 +221,49 @@ __device__ bool tridiag_eigvecs_wave(const float* __restrict__ Hp, int k, double
       g = d[m] - d[l] + e[l] / (g + copysign(r, g));
       double s = 1.0, c = 1.0, shift = 0.0;
       bool split = false;
+      // e[i], d[i], d[i + 1] of the current step in registers; step i stores
+      // only index i + 1, so the next step's inputs are read ahead
+      double ei = e[m - 1], di = d[m - 1], di1 = d[m];
       for (int i = m - 1; i >= l; --i) {
-        const double f = s * e[i];
-        const double b = c * e[i];
-        r = hypot(f, g);
+        const double ei_next = i > l ? e[i - 1] : 0.0;
+        const double di_next = i > l ? d[i - 1] : 0.0;
+        const double f = s * ei;
+        const double b = c * ei;
+        // r = |(f, g)| and its reciprocal: one square root and one division
+        // while f^2 + g^2 stays far from under- and overflow (the rotation
+        // chain is the serial part of QL), hypot otherwise
+        const double ss = fma(f, f, g * g);
+        double ir;
+        if (ss > 1e-280 && ss < 1e280) {
+          r = sqrt(ss);
+          ir = 1.0 / r;
+        } else {
+          r = hypot(f, g);
+          ir = r == 0.0 ? 0.0 : 1.0 / r;
+        }
         e[i + 1] = r;
         if (r == 0.0) {  // an off-diagonal underflowed: the matrix split
-          d[i + 1] -= shift;
+          d[i + 1] = di1 - shift;
           e[m] = 0.0;
           split = true;
           break;
         }
-        s = f / r;
-        c = g / r;
-        g = d[i + 1] - shift;
-        r = (d[i] - g) * s + 2.0 * c * b;
+        s = f * ir;
+        c = g * ir;
+        g = di1 - shift;
+        r = (di - g) * s + 2.0 * c * b;
         shift = s * r;
         d[i + 1] = g + shift;
         g = c * r - b;
         for (int a = lane; a < rows; a += 64) {
-          double* ua = U + int64_t(a) * k;
+          double* ua = U + int64_t(a) * ldu;
           const double ui = ua[i], ui1 = ua[i + 1];
           ua[i + 1] = s * ui + c * ui1;
           ua[i] = c * ui - s * ui1;
         }
+        ei = ei_next;
+        di1 = di;
+        di = di_next;
       }
       if (split) continue;
       d[l] -= shift;
@@ -231,9 +278,9 @@ __device__ bool tridiag_eigvecs_wave(const float* __restrict__ Hp, int k, double
 // One wave per probe.  (a, ab, b2) point at 3 arrays of k doubles and dm at
 // 64 columns of k doubles, column l at dm + l with stride 64 (LDS or global
 // workspace).
-__device__ void quadrature_wave(const float* __restrict__ Hp, int k, double* a, double* ab,
-                                double* b2, double* dm, float* alpha, float* beta, double len,
-                                double* out) {
+__device__ __forceinline__ void quadrature_wave(const float* __restrict__ Hp, int k, double* a,
+                                                double* ab, double* b2, double* dm, float* alpha,
+                                                float* beta, double len, double* out) {
   const int lane = threadIdx.x;
   // tridiagonal of T = (H + H^T)/2 in FP32 (decomp.py:133-135); H column-major
   for (int i = lane; i < k; i += 64) {
@@ -299,7 +346,7 @@ __device__ void quadrature_wave(const float* __restrict__ Hp, int k, double* a, 
     // eigenvalues that coincide to working precision: first row of U by QL,
     // with (a, ab, b2) reused as (d, e, u)
     __syncthreads();
-    const bool ok = tridiag_eigvecs_wave(Hp, k, a, ab, b2, 1);
+    const bool ok = tridiag_eigvecs_wave(Hp, k, a, ab, b2, 1, k);
     acc = 0.0;
     for (int j = lane; j < k; j += 64) acc += b2[j] * b2[j] * log(a[j]);
     acc = ok ? wave_sum_f64(acc) : __builtin_nan("");
@@ -333,12 +380,13 @@ __global__ void __launch_bounds__(64)
 }
 
 // Cotangent of |z|^2 [log T]_00 / P w.r.t. H (see the file header), one wave
-// per probe; lg: k doubles and Wm: k x k doubles of scratch.
-__device__ void tridiag_vjp_wave(const float* __restrict__ Hp, int k, double* d, double* e,
-                                 double* lg, double* U, double* Wm, double scale,
-                                 float* __restrict__ dHp) {
+// per probe; U: k x eig_ld(k), lg: k and Wm: k x k doubles of scratch.
+__device__ __forceinline__ void tridiag_vjp_wave(const float* __restrict__ Hp, int k, double* d,
+                                                 double* e, double* lg, double* U, double* Wm,
+                                                 double scale, float* __restrict__ dHp) {
   const int lane = threadIdx.x;
-  const bool failed = !tridiag_eigvecs_wave(Hp, k, d, e, U, k);
+  const int ldu = eig_ld(k);
+  const bool failed = !tridiag_eigvecs_wave(Hp, k, d, e, U, k, ldu);
   // log of the eigenvalues; a non-positive eigenvalue makes the reference's
   // log, and so its cotangent, NaN
   bool nonpositive = failed;
@@ -369,8 +417,8 @@ __device__ void tridiag_vjp_wave(const float* __restrict__ Hp, int k, double* d,
   //   Tbar_{a,a+1} = sum_l U_{a+1,l} v_l.
   for (int a = lane; a < k; a += 64) {
     double tdd = 0.0, tdo = 0.0;
-    const double* ua = U + int64_t(a) * k;
-    const double* ub = U + int64_t(a + 1 < k ? a + 1 : a) * k;
+    const double* ua = U + int64_t(a) * ldu;
+    const double* ub = U + int64_t(a + 1 < k ? a + 1 : a) * ldu;
     for (int l = 0; l < k; ++l) {
       double v = 0.0;
       for (int j = 0; j < k; ++j) v += Wm[j * k + l] * ua[j];
@@ -390,7 +438,7 @@ __device__ void tridiag_vjp_wave(const float* __restrict__ Hp, int k, double* d,
 __global__ void __launch_bounds__(64)
     k_tridiag_vjp_lds(const float* __restrict__ H, const float* __restrict__ len, int64_t P,
                       int64_t k, float* __restrict__ dH) {
-  __shared__ double U[kEigLdsDepth * kEigLdsDepth];
+  __shared__ double U[kEigLdsDepth * eig_ld(kEigLdsDepth)];
   __shared__ double Wm[kEigLdsDepth * kEigLdsDepth];
   __shared__ double d[kEigLdsDepth], e[kEigLdsDepth], lg[kEigLdsDepth];
   const int64_t p = blockIdx.x;
@@ -402,10 +450,11 @@ __global__ void __launch_bounds__(64)
     k_tridiag_vjp_global(const float* __restrict__ H, const float* __restrict__ len, int64_t P,
                          int64_t k, double* __restrict__ work, float* __restrict__ dH) {
   const int64_t p = blockIdx.x;
-  double* w = work + p * (2 * k * k + 3 * k);
+  const int64_t ku = k * eig_ld(int(k));  // U
+  double* w = work + p * (ku + k * k + 3 * k);
   const double l2 = double(len[p]) * double(len[p]);
-  tridiag_vjp_wave(H + p * k * k, int(k), w + k * k, w + k * k + k, w + k * k + 2 * k, w,
-                   w + k * k + 3 * k, l2 / double(P), dH + p * k * k);
+  tridiag_vjp_wave(H + p * k * k, int(k), w + ku, w + ku + k, w + ku + 2 * k, w, w + ku + 3 * k,
+                   l2 / double(P), dH + p * k * k);
 }
 
 
@@ -425,16 +474,17 @@ __device__ __forceinline__ double apply_matfun(int f, double x) {
 // U f(theta) U^T e_1, written as the combination coefficients of
 // |v| Q^T y for k_adjoint_combine (per probe 2 k + 4 floats: y, zeros, and
 // the scale |v| = len).  NaN coefficients if QL fails.
-__device__ void tridiag_funm_wave(const float* __restrict__ Hp, int k, double* d, double* e,
-                                  double* fd, double* U, int f, float scale,
-                                  float* __restrict__ cp) {
+__device__ __forceinline__ void tridiag_funm_wave(const float* __restrict__ Hp, int k, double* d,
+                                                  double* e, double* fd, double* U, int f,
+                                                  float scale, float* __restrict__ cp) {
   const int lane = threadIdx.x;
-  const bool ok = tridiag_eigvecs_wave(Hp, k, d, e, U, k);
+  const int ldu = eig_ld(k);
+  const bool ok = tridiag_eigvecs_wave(Hp, k, d, e, U, k, ldu);
   for (int j = lane; j < k; j += 64) fd[j] = apply_matfun(f, d[j]);  // f(theta_j) once
   __syncthreads();
   for (int a = lane; a < k; a += 64) {
     double y = 0.0;
-    const double* ua = U + int64_t(a) * k;
+    const double* ua = U + int64_t(a) * ldu;
     for (int j = 0; j < k; ++j) y += ua[j] * fd[j] * U[j];
     cp[a] = ok ? float(y) : __builtin_nanf("");
     cp[k + a] = 0.f;
@@ -450,7 +500,7 @@ __device__ void tridiag_funm_wave(const float* __restrict__ Hp, int k, double* d
 __global__ void __launch_bounds__(64)
     k_tridiag_funm_lds(const float* __restrict__ H, const float* __restrict__ len, int64_t k, int f,
                        float* __restrict__ coef) {
-  __shared__ double U[kEigLdsDepth * kEigLdsDepth];
+  __shared__ double U[kEigLdsDepth * eig_ld(kEigLdsDepth)];
   __shared__ double d[kEigLdsDepth], e[kEigLdsDepth], fd[kEigLdsDepth];
   const int64_t p = blockIdx.x;
   tridiag_funm_wave(H + p * k * k, int(k), d, e, fd, U, f, len[p], coef + p * (2 * k + 4));
@@ -460,9 +510,10 @@ __global__ void __launch_bounds__(64)
     k_tridiag_funm_global(const float* __restrict__ H, const float* __restrict__ len, int64_t k,
                           int f, double* __restrict__ work, float* __restrict__ coef) {
   const int64_t p = blockIdx.x;
-  double* w = work + p * (k * k + 3 * k);
-  tridiag_funm_wave(H + p * k * k, int(k), w + k * k, w + k * k + k, w + k * k + 2 * k, w, f,
-                    len[p], coef + p * (2 * k + 4));
+  const int64_t ku = k * eig_ld(int(k));  // U
+  double* w = work + p * (ku + 3 * k);
+  tridiag_funm_wave(H + p * k * k, int(k), w + ku, w + ku + k, w + ku + 2 * k, w, f, len[p],
+                    coef + p * (2 * k + 4));
 }
 
 // mean and standard error of the mean (population std / sqrt(P)) in FP64,
@@ -557,7 +608,7 @@ int64_t tridiag_quadrature_work_doubles(int64_t P, int64_t k) {
 }
 
 int64_t tridiag_vjp_work_doubles(int64_t P, int64_t k) {
-  return k <= kEigLdsDepth ? 0 : P * (2 * k * k + 3 * k);
+  return k <= kEigLdsDepth ? 0 : P * (k * eig_ld(int(k)) + k * k + 3 * k);
 }
 
 hipError_t launch_tridiag_funm(hipStream_t s, const float* H, const float* len, int64_t P,
