@@ -125,11 +125,14 @@ def timed_solves(op, params, Z, k, steps, warmup, ctx, reortho="full"):
 
     from matfree_amd import funm
 
+    classes = ("matvec", "dots", "project_dots", "project", "normalize", "quadrature")
     for _ in range(warmup):
         funm.logdet_per_probe(op, Z, k, params, reortho=reortho)
     torch.cuda.synchronize()
-    ctx.profile(True)
-    for kc in ("matvec", "dots", "project_dots", "project", "normalize", "quadrature"):
+    # timed region: events around the block matvec only (the roofline's
+    # kernel); every event record costs device time
+    ctx.profile(True, ("matvec",))
+    for kc in classes:
         ctx.profile_read(kc)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -138,8 +141,12 @@ def timed_solves(op, params, Z, k, steps, warmup, ctx, reortho="full"):
         _, mean = funm.logdet_per_probe(op, Z, k, params, reortho=reortho)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    kernels = {}
-    for kc in ("matvec", "dots", "project_dots", "project", "normalize", "quadrature"):
+    kernels = {"matvec": dict(zip(("total_ms", "launches"), ctx.profile_read("matvec")))}
+    # per-class breakdown from as many further solves with every class bracketed
+    ctx.profile(True)
+    for _ in range(steps):
+        funm.logdet_per_probe(op, Z, k, params, reortho=reortho)
+    for kc in classes[1:]:
         ms, cnt = ctx.profile_read(kc)
         kernels[kc] = {"total_ms": ms, "launches": cnt}
     ctx.profile(False)
@@ -271,7 +278,7 @@ def grad_step(args, device):
     for _ in range(args.warmup):
         funm.logdet_value_and_grad(op, Z, k, params)
     ctx = _runtime.context(device)
-    ctx.profile(True)
+    ctx.profile(True, ("adjoint",))
     for kc in ("matvec", "adjoint"):
         ctx.profile_read(kc)
     torch.cuda.synchronize()
@@ -323,8 +330,9 @@ def funm_step(args, device):
     for _ in range(args.warmup):
         funm.funm_vectors(op, Z, k, params, code)
     torch.cuda.synchronize()
-    ctx.profile(True)
-    for kc in ("matvec", "dots", "project_dots", "project", "normalize"):
+    classes = ("matvec", "dots", "project_dots", "project", "normalize")
+    ctx.profile(True, ("matvec",))  # timed region: the roofline kernel's events only
+    for kc in classes:
         ctx.profile_read(kc)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -332,8 +340,11 @@ def funm_step(args, device):
         Y = funm.funm_vectors(op, Z, k, params, code)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
-    kernels = {}
-    for kc in ("matvec", "dots", "project_dots", "project", "normalize"):
+    kernels = {"matvec": dict(zip(("total_ms", "launches"), ctx.profile_read("matvec")))}
+    ctx.profile(True)  # per-class breakdown from further calls
+    for _ in range(args.steps):
+        funm.funm_vectors(op, Z, k, params, code)
+    for kc in classes[1:]:
         ms, cnt = ctx.profile_read(kc)
         kernels[kc] = {"total_ms": ms, "launches": cnt}
     ctx.profile(False)
@@ -406,7 +417,7 @@ def gp_step(args, device):
     dt = (time.perf_counter() - t0) / args.steps
     params = (w["lengthscale"], w["noise"])
     ctx = _runtime.context(device)
-    ctx.profile(True)
+    ctx.profile(True, ("matvec",))
     ctx.profile_read("matvec")
     torch.cuda.synchronize()
     t1 = time.perf_counter()

@@ -243,10 +243,14 @@ int slq_quadform_vjp(slq_context* ctx, const slq_operator* op, const float* X, i
                      double* dparams, float* dmatrix);
 
 /* ---- measurement hooks (no counterpart in the reference) -------------------
- * With profiling on, every launch of the listed kernel classes made through
- * this context is bracketed by a pair of HIP events on the context's stream.
- * slq_profile_read synchronises the stream, returns the summed device time
- * (ms) and launch count of one class since the last read, and resets it. */
+ * With profiling on, every launch of the selected kernel classes made through
+ * this context is bracketed by a pair of HIP events on the context's stream
+ * (each event record costs device time too: select only the classes that are
+ * read).  slq_profile_enable(ctx, on) selects all classes (on != 0) or none;
+ * slq_profile_select(ctx, mask) selects the classes kc with bit (1 << kc) of
+ * mask set.  slq_profile_read synchronises the stream, returns the summed
+ * device time (ms) and launch count of one class since the last read, and
+ * resets it. */
 typedef enum slq_kernel_class {
   SLQ_KC_MATVEC = 0,       /* block matvec                              */
   SLQ_KC_DOTS = 1,         /* Gram-Schmidt coefficients Q^T w            */
@@ -258,6 +262,7 @@ typedef enum slq_kernel_class {
   SLQ_KC_COUNT = 7
 } slq_kernel_class;
 int slq_profile_enable(slq_context* ctx, int on);
+int slq_profile_select(slq_context* ctx, int mask);
 int slq_profile_read(slq_context* ctx, int kernel_class, double* total_ms, int64_t* launches);
 
 #ifdef __cplusplus

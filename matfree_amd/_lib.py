@@ -109,6 +109,7 @@ SIGNATURES = {
         c_int, [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_void_p, c_void_p],
     ),
     "slq_profile_enable": (c_int, [c_void_p, c_int]),
+    "slq_profile_select": (c_int, [c_void_p, c_int]),
     "slq_profile_read": (c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(c_i64)]),
 }
@@ -178,8 +179,15 @@ class Context:
         rc = getattr(load(), name)(self.ptr, *args)
         raise_for(rc, self.ptr)
 
-    def profile(self, on):
-        raise_for(load().slq_profile_enable(self.ptr, 1 if on else 0), self.ptr)
+    def profile(self, on, classes=None):
+        """Event hooks on (all kernel classes, or only the named ones) or off."""
+        if on and classes is not None:
+            mask = 0
+            for kc in classes:
+                mask |= 1 << KERNEL_CLASSES[kc]
+            raise_for(load().slq_profile_select(self.ptr, mask), self.ptr)
+        else:
+            raise_for(load().slq_profile_enable(self.ptr, 1 if on else 0), self.ptr)
 
     def profile_read(self, kernel_class):
         """(total device ms, launches) of one kernel class since the last read."""

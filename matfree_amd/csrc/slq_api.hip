@@ -36,7 +36,7 @@ struct slq_context {
   // pinned host word for the CG loop's count of active columns
   int* host_count = nullptr;
   // measurement hooks: event pairs per kernel class, reused across reads
-  bool profiling = false;
+  unsigned profile_mask = 0;  // bit kc: kernel class kc is bracketed by events
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events[SLQ_KC_COUNT];
   size_t used[SLQ_KC_COUNT] = {};
 };
@@ -128,10 +128,10 @@ int check_operator(slq_context* ctx, const slq_operator* op) {
   return SLQ_OK;
 }
 
-// The next (start, end) event pair of kernel class kc, or nulls when
-// profiling is off (or an event cannot be created).
+// The next (start, end) event pair of kernel class kc, or nulls when the
+// class is not profiled (or an event cannot be created).
 std::pair<hipEvent_t, hipEvent_t> take_events(slq_context* ctx, int kc) {
-  if (!ctx->profiling) return {nullptr, nullptr};
+  if (!((ctx->profile_mask >> kc) & 1u)) return {nullptr, nullptr};
   auto& vec = ctx->events[kc];
   size_t& u = ctx->used[kc];
   if (u == vec.size()) {
@@ -981,7 +981,15 @@ extern "C" {
 
 int slq_profile_enable(slq_context* ctx, int on) {
   if (!ctx) return SLQ_EINVAL;
-  ctx->profiling = (on != 0);
+  ctx->profile_mask = on != 0 ? (1u << SLQ_KC_COUNT) - 1u : 0u;
+  return SLQ_OK;
+}
+
+int slq_profile_select(slq_context* ctx, int mask) {
+  if (!ctx) return SLQ_EINVAL;
+  if (mask < 0 || mask >= (1 << SLQ_KC_COUNT))
+    return fail(ctx, SLQ_EINVAL, "profile mask has bits beyond the kernel classes");
+  ctx->profile_mask = unsigned(mask);
   return SLQ_OK;
 }
 

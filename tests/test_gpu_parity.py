@@ -371,3 +371,30 @@ def test_deterministic_repeat(cuda):
     a, _ = funm.logdet_per_probe(operators.dense_matvec, Z, k, (A,))
     b, _ = funm.logdet_per_probe(operators.dense_matvec, Z, k, (A,))
     assert torch.equal(a, b)
+
+
+def test_profile_hooks_bracket_selected_classes(cuda):
+    # measurement hooks: with only the matvec class selected, its k launches of
+    # a depth-k solve are timed and no other class records events
+    from matfree_amd import _runtime
+
+    n, P, k = 300, 4, 6
+    A = torch.as_tensor(sym_from_eigs(np.linspace(1.0, 3.0, n), 5), device=cuda)
+    Z = oprng.rademacher(oprng.prng_key(2), (P, n), np.float32)
+    ctx = _runtime.context(cuda)
+    classes = ("matvec", "dots", "project_dots", "project", "normalize", "quadrature")
+    ctx.profile(True, ("matvec",))
+    for kc in classes:
+        ctx.profile_read(kc)
+    funm.logdet_per_probe(operators.dense_matvec, torch.as_tensor(Z, device=cuda), k, (A,))
+    ms, cnt = ctx.profile_read("matvec")
+    assert cnt == k and ms > 0.0
+    for kc in classes[1:]:
+        assert ctx.profile_read(kc) == (0.0, 0)
+    ctx.profile(True)
+    funm.logdet_per_probe(operators.dense_matvec, torch.as_tensor(Z, device=cuda), k, (A,))
+    assert ctx.profile_read("matvec")[1] == k
+    assert ctx.profile_read("quadrature")[1] == 1
+    ctx.profile(False)
+    funm.logdet_per_probe(operators.dense_matvec, torch.as_tensor(Z, device=cuda), k, (A,))
+    assert ctx.profile_read("matvec") == (0.0, 0)
