@@ -34,7 +34,8 @@ struct slq_context {
             &lr_z,     &cg_r,     &cg_z,    &cg_d,    &cg_q,     &cg_parts, &cg_scal};
   }
   // pinned host word for the CG loop's count of active columns
-  int* host_count = nullptr;
+  int* host_count = nullptr;              // pinned, 16 ints (CG's polled counts)
+  hipEvent_t cg_done[2] = {nullptr, nullptr};  // CG: count copied to host_count[i]
   // measurement hooks: event pairs per kernel class, reused across reads
   unsigned profile_mask = 0;  // bit kc: kernel class kc is bracketed by events
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events[SLQ_KC_COUNT];
@@ -610,6 +611,8 @@ int slq_context_destroy(slq_context* ctx) {
   for (auto* b : ctx->all_bufs())
     if (b->ptr) (void)hipFree(b->ptr);
   if (ctx->host_count) (void)hipHostFree(ctx->host_count);
+  for (auto& ev : ctx->cg_done)
+    if (ev) (void)hipEventDestroy(ev);
   delete ctx;
   return SLQ_OK;
 }
@@ -855,6 +858,12 @@ int slq_cg(slq_context* ctx, const slq_operator* op, const float* B, int64_t P, 
     }
     ctx->host_count = static_cast<int*>(h);
   }
+  for (auto& ev : ctx->cg_done)
+    if (!ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+      ev = nullptr;
+      (void)hipGetLastError();
+      return fail(ctx, SLQ_EHIP, "event creation failed");
+    }
   const int64_t nb = cg_chunks(n);
   float *r, *z = nullptr, *d, *q, *parts, *scal, *mvs;
   SLQ_TRY(ensure_t(ctx, ctx->cg_r, size_t(P * n), &r));
@@ -900,14 +909,17 @@ int slq_cg(slq_context* ctx, const slq_operator* op, const float* B, int64_t P, 
   SLQ_HIP(ctx, launch_cg_dot(s, B, B, P, n, bb));
   SLQ_HIP(ctx, launch_cg_start(s, rz, bb, precond ? rr : nullptr, P, n, float(tol), float(atol),
                                maxiter, sc, nactive));
-  // The host reads the count of active columns after batches of 1, 2, 4, 8,
-  // 8, ... iterations; an iteration with no active column changes nothing.
-  // Iteration t reads scalar generation t % 2 and writes the other.
+  // Iterations run in batches of 1, 2, 4, 8, 8, ...; after batch i the count
+  // of active columns is copied to pinned host memory (slot (i + 1) % 2).
+  // While the batches grow the host waits for each batch's own count; once
+  // they are 8 long it keeps one batch in flight: it enqueues batch i, then
+  // waits for the count of batch i - 1, so that the GPU does not idle while
+  // the host polls.  It stops at a zero count; iterations that find no
+  // active column (the batch still in flight) change nothing.  Iteration t
+  // reads scalar generation t % 2 and writes the other.
+  int* hc = ctx->host_count;
   int64_t t = 0;
-  for (int64_t batch = 1;; batch = batch < 8 ? 2 * batch : 8) {
-    SLQ_HIP(ctx, hipMemcpyAsync(ctx->host_count, nactive, sizeof(int), hipMemcpyDeviceToHost, s));
-    SLQ_HIP(ctx, hipStreamSynchronize(s));
-    if (*ctx->host_count == 0) break;
+  for (int64_t i = 0, batch = 1;; ++i, batch = batch < 8 ? 2 * batch : 8) {
     for (int64_t b = 0; b < batch; ++b, ++t) {
       MvSplitSum qsum;
       {
@@ -919,6 +931,12 @@ int slq_cg(slq_context* ctx, const slq_operator* op, const float* B, int64_t P, 
                                   int(t & 1), pq, rr, rz, precond ? &M : nullptr));
     }
     SLQ_HIP(ctx, launch_cg_count(s, sc, int(t & 1), P, nactive));
+    const int slot = int((i + 1) & 1);
+    SLQ_HIP(ctx, hipMemcpyAsync(hc + slot, nactive, sizeof(int), hipMemcpyDeviceToHost, s));
+    SLQ_HIP(ctx, hipEventRecord(ctx->cg_done[slot], s));
+    const int wait = batch < 8 ? slot : int(i & 1);
+    SLQ_HIP(ctx, hipEventSynchronize(ctx->cg_done[wait]));
+    if (hc[wait] == 0) break;
   }
   if (iterations)
     SLQ_HIP(ctx, hipMemcpyAsync(iterations, sc.iters, size_t(P) * sizeof(int64_t),
