@@ -55,6 +55,7 @@ DEFAULT_PRECISION = {"rbf": "bf16x2", "dense": "fp32", "rbf-grad": "fp32", "rbf-
 SERVE_N = (2048, 32768)
 SERVE_DENSE_MAX_N = 16384
 SERVE_PROBES = (16, 32, 64, 128)
+SERVE_SLOTS = 4  # requests in flight at once in the serving stream
 SERVE_DEPTH = (16, 64)
 
 
@@ -70,8 +71,9 @@ def parse():
     ap.add_argument("--no-reference", action="store_true", help="skip the FP64 Cholesky check")
     ap.add_argument("--serve-requests", type=int, default=48,
                     help="requests of the serving measurement (0 skips it)")
-    ap.add_argument("--serve-load", type=float, default=0.5,
-                    help="offered load of the open-loop (Poisson) serving replay")
+    ap.add_argument("--serve-load", type=float, default=1.0,
+                    help="offered load of the Poisson request stream, relative to the "
+                         "mean service rate of one request at a time")
     return ap.parse_args()
 
 
@@ -196,23 +198,28 @@ def measured_traffic(kernel):
 
 
 def serve(args, device):
-    """Per-request latency of a stream of independent SLQ log-determinant solves.
+    """Latency of a stream of independent SLQ log-determinant solves.
 
     Each request draws its operator (RBF on fresh points, or a stored SPD
-    matrix), dimension, probe count and depth from the serving mix; its inputs
-    are made resident in HBM first, then the request (probe generation, solve,
-    result ready on the device) is timed on its own.  Closed-loop percentiles
-    are the service times; the open-loop figures replay Poisson arrivals at
-    the given offered load through one FIFO server with those service times.
+    matrix), dimension, probe count and depth from the serving mix; all
+    operators are made resident in HBM first.  Closed loop: every request
+    (probe generation, solve, result ready on the device) is timed alone.
+    Then the same requests arrive as a Poisson stream at the given offered
+    load (relative to the mean service time alone) and run through the
+    continuous-batching loop of matfree_amd.serving (4 slots, each a stream
+    with its own library context, admitted in arrival order by HBM
+    footprint): latency = completion - arrival.  Each request's per-probe
+    results in the mix are compared with its results alone (bit for bit
+    expected), and the same arrivals are replayed through one FIFO server with
+    the closed-loop service times for comparison.
     """
     import numpy as np
     import torch
 
-    from matfree_amd import funm, operators, problems, prng, stochtrace
+    from matfree_amd import funm, operators, problems, prng, serving, stochtrace
 
     rng = np.random.default_rng(args.seed + 1)
-    lat = []
-    meta = []
+    reqs, meta = [], []
     for r in range(args.serve_requests):
         n = int(round(np.exp(rng.uniform(np.log(SERVE_N[0]), np.log(SERVE_N[1])))))
         P = int(rng.choice(SERVE_PROBES))
@@ -224,26 +231,39 @@ def serve(args, device):
         else:
             A = problems.dense_spd_device(n, seed=1000 + r, shift=0.1, device=device)
             op, params = operators.DenseMatvec(precision="fp32"), (A,)
-        sampler = stochtrace.sampler_signs(np.zeros(n, np.float32), num=P)
+        reqs.append(serving.Request(op, params, n, P, k, seed=r))
+        meta.append((kind, n, P, k))
+    torch.cuda.synchronize()
+    lat, alone = [], []
+    for q in reqs:
+        sampler = stochtrace.sampler_signs(np.zeros(q.n, np.float32), num=q.P)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        Z = sampler.flat(prng.PRNGKey(r), device)
-        _, mean = funm.logdet_per_probe(op, Z, k, params)
+        Z = sampler.flat(prng.PRNGKey(q.seed), device)
+        per, mean = funm.logdet_per_probe(q.matvec, Z, q.k, q.params)
         torch.cuda.synchronize()
         lat.append((time.perf_counter() - t0) * 1e3)
-        meta.append((kind, n, P, k))
-        del op, params, Z
+        alone.append(per.cpu().numpy())
     lat = np.array(lat)
-    # open loop: Poisson arrivals at load rho of the mean service rate, FIFO
+    # Poisson arrivals at offered load rho of the mean service rate alone
     rho = args.serve_load
-    gaps = rng.exponential(lat.mean() / rho, size=len(lat))
-    arrive = np.cumsum(gaps)
+    arrive = np.cumsum(rng.exponential(lat.mean() / rho, size=len(lat))) / 1e3  # seconds
+    mixed = [q._replace(arrival=float(a)) for q, a in zip(reqs, arrive)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = serving.Server(device, slots=SERVE_SLOTS).run(mixed)
+    span = time.perf_counter() - t0
+    resp = np.array([(o.finish - o.arrival) * 1e3 for o in out])
+    diff = max(float(np.max(np.abs(o.per_probe.cpu().numpy() - a) / np.abs(a))) for o, a in
+               zip(out, alone))
+    # the same arrivals through one FIFO server with the service times alone
     finish = np.empty_like(arrive)
     busy = 0.0
     for i in range(len(lat)):
-        start = max(arrive[i], busy)
+        start = max(arrive[i] * 1e3, busy)
         finish[i] = busy = start + lat[i]
-    resp = finish - arrive
+    fifo = finish - arrive * 1e3
+    del reqs, mixed, out
     torch.cuda.empty_cache()
     return {
         "requests": len(lat),
@@ -251,9 +271,17 @@ def serve(args, device):
         "p99_ms": float(np.percentile(lat, 99)),
         "mean_ms": float(lat.mean()),
         "max_ms": float(lat.max()),
-        "open_loop_load": rho,
-        "open_loop_p50_ms": float(np.percentile(resp, 50)),
-        "open_loop_p99_ms": float(np.percentile(resp, 99)),
+        "stream": {
+            "offered_load": rho,
+            "slots": SERVE_SLOTS,
+            "p50_ms": float(np.percentile(resp, 50)),
+            "p99_ms": float(np.percentile(resp, 99)),
+            "requests_per_s": len(lat) / span,
+            "probe_steps_per_s": sum(m[2] * m[3] for m in meta) / span,
+            "max_rel_diff_vs_alone": diff,
+            "fifo_one_at_a_time_p50_ms": float(np.percentile(fifo, 50)),
+            "fifo_one_at_a_time_p99_ms": float(np.percentile(fifo, 99)),
+        },
         "mix": (f"n log-uniform [{SERVE_N[0]}, {SERVE_N[1]}], probes {list(SERVE_PROBES)}, "
                 f"depth U[{SERVE_DEPTH[0]}, {SERVE_DEPTH[1]}], RBF bf16x2 or dense fp32 "
                 f"(dense n <= {SERVE_DENSE_MAX_N}); {sum(m[0] == 'rbf' for m in meta)} RBF"),
@@ -580,7 +608,9 @@ def main():
         del op, params, Z, host
         torch.cuda.empty_cache()
         out["serve"] = serve(args, device)
-        log(f"bench: serve p50 {out['serve']['p50_ms']:.2f} ms, p99 {out['serve']['p99_ms']:.2f} ms")
+        sv = out["serve"]
+        log(f"bench: serve alone p50 {sv['p50_ms']:.2f} ms, p99 {sv['p99_ms']:.2f} ms; stream "
+            f"p50 {sv['stream']['p50_ms']:.2f} ms, p99 {sv['stream']['p99_ms']:.2f} ms")
     print(json.dumps(out), flush=True)
 
 

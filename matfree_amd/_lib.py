@@ -217,12 +217,18 @@ _contexts = threading.local()
 
 
 def context(device, stream_handle):
-    """The calling thread's context for ``device``, bound to ``stream_handle``."""
+    """The calling thread's context for ``device`` and ``stream_handle``.
+
+    One context per (device, stream): a context's scratch is reused by every
+    call made through it, in its stream's order, so work in flight on two
+    streams at once (the serving loop's slots) needs a context each.
+    """
     table = getattr(_contexts, "table", None)
     if table is None:
         table = _contexts.table = {}
-    ctx = table.get(device)
+    key = (int(device), int(stream_handle or 0))
+    ctx = table.get(key)
     if ctx is None:
-        ctx = table[device] = Context(device)
-    ctx.set_stream(stream_handle)
+        ctx = table[key] = Context(device)
+        ctx.set_stream(stream_handle)
     return ctx

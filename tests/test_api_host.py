@@ -117,3 +117,17 @@ def test_cg_rejects_foreign_preconditioners():
 
     with pytest.raises(TypeError):
         linalg.cg(lambda v: v, np.ones(3))
+
+
+def test_serving_admission_rule():
+    # HBM-footprint admission of the serving loop (matfree_amd.serving): a
+    # request starts if it fits beside the running ones, or if nothing runs
+    from matfree_amd import serving
+
+    r = serving.Request(None, (), 1000, 16, 20, seed=0)
+    fp = serving.footprint(r)
+    assert fp == 4 * (20 + 24) * 16 * 1000
+    assert serving.admits(r, 0, None) and serving.admits(r, 10**12, None)
+    assert serving.admits(r, 0, fp // 2)  # too big, but alone
+    assert serving.admits(r, 1, fp + 1)
+    assert not serving.admits(r, 2, fp + 1)
