@@ -7,6 +7,7 @@ the arithmetic runs in the hand-written HIP kernels of ``libslq.so``
 (C ABI in ``include/slq.h``).
 """
 
-from . import decomp, funm, gp, linalg, low_rank, operators, prng, stochtrace
+from . import decomp, funm, gp, linalg, low_rank, operators, prng, serving, stochtrace
 
-__all__ = ["decomp", "funm", "gp", "linalg", "low_rank", "operators", "prng", "stochtrace"]
+__all__ = ["decomp", "funm", "gp", "linalg", "low_rank", "operators", "prng", "serving",
+           "stochtrace"]

@@ -21,6 +21,9 @@ import collections
 import time
 from typing import Any, NamedTuple, Optional
 
+import numpy as np
+import torch
+
 from . import funm, prng, stochtrace
 
 
@@ -64,8 +67,6 @@ class Server:
     """Runs requests with up to ``slots`` in flight under an HBM budget."""
 
     def __init__(self, device, slots: int = 4, hbm_budget: Optional[int] = None):
-        import torch
-
         if slots < 1:
             raise ValueError("slots must be at least 1")
         self.device = device
@@ -74,9 +75,6 @@ class Server:
         self.hbm_budget = hbm_budget
 
     def _launch(self, req: Request, stream):
-        import numpy as np
-        import torch
-
         stream.wait_stream(torch.cuda.current_stream(self.device))
         with torch.cuda.stream(stream):
             sampler = stochtrace.sampler_signs(np.zeros(req.n, np.float32), num=req.P)
