@@ -398,3 +398,45 @@ def test_profile_hooks_bracket_selected_classes(cuda):
     ctx.profile(False)
     funm.logdet_per_probe(operators.dense_matvec, torch.as_tensor(Z, device=cuda), k, (A,))
     assert ctx.profile_read("matvec") == (0.0, 0)
+
+
+# ---------------------------------------------------------------------------
+# full sizes: size-independent properties at the headline configurations
+# ---------------------------------------------------------------------------
+
+
+def test_full_size_rbf_logdet_within_1e3_of_fp64(cuda):
+    # n = 32768 RBF (l = 0.5, s2 = 1e-2), 128 probes, depth 64, bf16x2: the
+    # estimate is within 1e-3 of the FP64 dense-Cholesky log-determinant (the
+    # bar the BF16 form is held to)
+    from matfree_amd import problems as pr
+
+    n, P, k = 32768, 128, 64
+    X = torch.as_tensor(pr.rbf_points(n, seed=0), device=cuda)
+    Z = stochtrace.sampler_signs(np.zeros(n, np.float32), num=P).flat(prng.PRNGKey(0), cuda)
+    per, mean = funm.logdet_per_probe(operators.RBF(X, precision="bf16x2"), Z, k, (0.5, 1e-2))
+    K = pr.rbf_dense64_device(X.cpu().numpy(), 0.5, 1e-2, cuda)
+    exact = pr.logdet_cholesky_device(K)
+    del K
+    torch.cuda.empty_cache()
+    assert abs(float(mean) - exact) <= 1e-3 * abs(exact)
+    assert bool(torch.isfinite(per).all())
+
+
+def test_full_size_dense_decomposition_properties(cuda):
+    # n = 16384 stored dense A = B B^T / n + 0.1 I, FP32, depth 64 on 2 of the
+    # 256 probes: Q Q^T = I and A Q^T = Q^T T + r e_k^T (tests/test_decomp/
+    # test_tridiag_sym.py:62-65) to FP32 accuracy at the full dimension
+    from matfree_amd import problems as pr
+
+    n, k = 16384, 64
+    A = pr.dense_spd_device(n, seed=0, shift=0.1, device=cuda)
+    z = stochtrace.sampler_signs(np.zeros(n, np.float32), num=2).flat(prng.PRNGKey(0), cuda)[1]
+    Q, T, r, _ = decomp.tridiag_sym(k)(operators.dense_matvec, z, A)
+    Q64, T64, r64 = Q.double(), T.double(), r.double()
+    eye = torch.eye(k, dtype=torch.float64, device=cuda)
+    assert float(torch.linalg.matrix_norm(Q64 @ Q64.T - eye)) <= 1e-4
+    lhs = A.double() @ Q64.T
+    rhs = Q64.T @ T64
+    rhs[:, -1] += r64
+    assert float(torch.linalg.matrix_norm(lhs - rhs) / torch.linalg.matrix_norm(lhs)) <= 1e-4
