@@ -21,6 +21,8 @@
 //
 // Plain Lanczos (reortho="none", decomp.py:198-266) uses the single-column
 // dots, k_three_term and k_normalize: no basis is read.
+#include <initializer_list>
+
 #include "slq_internal.h"
 
 namespace slq {
@@ -89,6 +91,17 @@ struct VecIO<4> {
   }
   __device__ static __forceinline__ void store(float* p, const float (&r)[4]) {
     *reinterpret_cast<float4*>(p) = make_float4(r[0], r[1], r[2], r[3]);
+  }
+};
+
+template <>
+struct VecIO<2> {
+  __device__ static __forceinline__ void load(const float* p, float (&r)[2]) {
+    const float2 t = *reinterpret_cast<const float2*>(p);
+    r[0] = t.x; r[1] = t.y;
+  }
+  __device__ static __forceinline__ void store(float* p, const float (&r)[2]) {
+    *reinterpret_cast<float2*>(p) = make_float2(r[0], r[1]);
   }
 };
 
@@ -519,6 +532,18 @@ __global__ void __launch_bounds__(kStreamThreads)
 
 inline bool aligned16(const void* ptr) { return (reinterpret_cast<uintptr_t>(ptr) & 15) == 0; }
 
+// Vector width of the streaming kernels for rows of n floats at these
+// pointers: 4 (float4) when n % 4 == 0 and all are 16-byte aligned, 2 (float2)
+// when n is even and all are 8-byte aligned (odd-multiple-of-2 n: every row
+// then starts on an 8-byte boundary), else 1.
+inline int stream_width(int64_t n, int64_t ld, std::initializer_list<const void*> ptrs) {
+  uintptr_t bits = 0;
+  for (const void* q : ptrs) bits |= reinterpret_cast<uintptr_t>(q);
+  if (n % 4 == 0 && ld % 4 == 0 && (bits & 15) == 0) return 4;
+  if (n % 2 == 0 && ld % 2 == 0 && (bits & 7) == 0) return 2;
+  return 1;
+}
+
 template <int V>
 hipError_t launch_dots_v(hipStream_t s, const float* Q, int64_t ldc, const float* x, int64_t P,
                          int64_t n, int ncols, float* partial, int pstride) {
@@ -544,17 +569,20 @@ hipError_t launch_dots_v(hipStream_t s, const float* Q, int64_t ldc, const float
 
 hipError_t launch_dots(hipStream_t s, const float* Q, int64_t ldc, const float* x, int64_t P,
                        int64_t n, int ncols, float* partial, int pstride) {
-  const bool vec = (n % 4 == 0) && aligned16(x) && (Q == nullptr || (aligned16(Q) && ldc % 4 == 0));
+  const int V = Q ? stream_width(n, ldc, {x, Q}) : stream_width(n, n, {x});
   if (Q == nullptr) {
     const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
-    if (vec)
+    if (V == 4)
       hipLaunchKernelGGL(k_sumsq<4>, grid, dim3(kStreamThreads), 0, s, x, n, partial);
+    else if (V == 2)
+      hipLaunchKernelGGL(k_sumsq<2>, grid, dim3(kStreamThreads), 0, s, x, n, partial);
     else
       hipLaunchKernelGGL(k_sumsq<1>, grid, dim3(kStreamThreads), 0, s, x, n, partial);
     return hipGetLastError();
   }
-  return vec ? launch_dots_v<4>(s, Q, ldc, x, P, n, ncols, partial, pstride)
-             : launch_dots_v<1>(s, Q, ldc, x, P, n, ncols, partial, pstride);
+  if (V == 4) return launch_dots_v<4>(s, Q, ldc, x, P, n, ncols, partial, pstride);
+  if (V == 2) return launch_dots_v<2>(s, Q, ldc, x, P, n, ncols, partial, pstride);
+  return launch_dots_v<1>(s, Q, ldc, x, P, n, ncols, partial, pstride);
 }
 
 hipError_t launch_dots_sum(hipStream_t s, const float* Q, int64_t ldc, const MvSplitSum& sum,
@@ -580,9 +608,12 @@ hipError_t launch_project(hipStream_t s, const float* Q, int64_t ldc, const floa
                           int64_t hsub_stride) {
   const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
   const size_t shmem = size_t(ncols > 0 ? ncols : 1) * sizeof(float);
-  const bool vec = (n % 4 == 0) && aligned16(x) && aligned16(y) && aligned16(Q) && ldc % 4 == 0;
-  if (vec)
+  const int V = stream_width(n, ldc, {x, y, Q});
+  if (V == 4)
     hipLaunchKernelGGL(k_project<4>, grid, dim3(kStreamThreads), shmem, s, Q, ldc, x, y, n, ncols,
+                       hpart, pstride, norm2part, hout, hout_stride, hsub, hsub_stride);
+  else if (V == 2)
+    hipLaunchKernelGGL(k_project<2>, grid, dim3(kStreamThreads), shmem, s, Q, ldc, x, y, n, ncols,
                        hpart, pstride, norm2part, hout, hout_stride, hsub, hsub_stride);
   else
     hipLaunchKernelGGL(k_project<1>, grid, dim3(kStreamThreads), shmem, s, Q, ldc, x, y, n, ncols,
@@ -667,13 +698,15 @@ hipError_t launch_normalize(hipStream_t s, const float* x, float* out, int64_t P
                             int64_t hslot_stride, float* inv_out, float* hslot2,
                             const PlaneOut* planes) {
   const PlaneOut pl = planes ? *planes : PlaneOut{};
-  const bool vec = (n % 4 == 0) && aligned16(x) && aligned16(out);
-  const int V = vec ? 4 : 1;
+  const int V = stream_width(n, n, {x, out});
   const int64_t per_block = int64_t(kStreamThreads) * V * kNormVec;
   const dim3 grid(unsigned((n + per_block - 1) / per_block), unsigned(P));
   const int64_t nbn = num_chunks(n);
-  if (vec)
+  if (V == 4)
     hipLaunchKernelGGL(k_normalize<4>, grid, dim3(kStreamThreads), 0, s, x, out, n, nbn, norm2part,
+                       norm_out, hslot, hslot_stride, hslot2, inv_out, pl);
+  else if (V == 2)
+    hipLaunchKernelGGL(k_normalize<2>, grid, dim3(kStreamThreads), 0, s, x, out, n, nbn, norm2part,
                        norm_out, hslot, hslot_stride, hslot2, inv_out, pl);
   else
     hipLaunchKernelGGL(k_normalize<1>, grid, dim3(kStreamThreads), 0, s, x, out, n, nbn, norm2part,

@@ -213,8 +213,10 @@ def fp32_spread(A, V0, k, ref_Q, ref_H, perms=3):
     return worst_q, worst_h
 
 
-@pytest.mark.parametrize("n,P,k", [(1024, 64, 30), (1001, 7, 20), (200, 3, 70), (64, 5, 48)])
+@pytest.mark.parametrize("n,P,k", [(1024, 64, 30), (1001, 7, 20), (1002, 9, 24), (200, 3, 70),
+                                   (64, 5, 48)])
 def test_hessenberg_matches_oracle(cuda, n, P, k):
+    # n = 1024: float4 streaming kernels; 1002: float2; 1001: scalar
     A = problems.dense_spd(n, seed=0, shift=1.0)
     V = oprng.rademacher(oprng.prng_key(n), (P, n), np.float32)
     V0 = V / np.sqrt(n, dtype=np.float32)
@@ -440,3 +442,16 @@ def test_full_size_dense_decomposition_properties(cuda):
     rhs = Q64.T @ T64
     rhs[:, -1] += r64
     assert float(torch.linalg.matrix_norm(lhs - rhs) / torch.linalg.matrix_norm(lhs)) <= 1e-4
+
+
+@pytest.mark.parametrize("n", [2050, 2051, 2052])
+def test_logdet_same_across_stream_widths(cuda, n):
+    # the Gram-Schmidt kernels stream rows in float4, float2 or scalar groups
+    # depending on n; each width reproduces the oracle on the same problem
+    P, k = 6, 20
+    A = problems.dense_spd(n, seed=3, shift=0.5)
+    Z = oprng.rademacher(oprng.prng_key(11), (P, n), np.float32)
+    per, _ = funm.logdet_per_probe(operators.dense_matvec, torch.as_tensor(Z, device=cuda), k,
+                                   (torch.as_tensor(A, device=cuda),))
+    o_vals, _, _ = oslq.quadform_logdet(oslq.DenseOperator(A), k, Z, np.float32)
+    assert np.allclose(per.cpu().numpy(), o_vals, rtol=1e-5)
