@@ -210,6 +210,69 @@ __global__ void __launch_bounds__(kStreamThreads)
   }
 }
 
+// Rows whose length n is not a multiple of 4 start at every alignment (probe
+// rows are contiguous).  The *_shifted kernels read row p from its first
+// 16-byte boundary: element i of chunk b's float4 groups is sh_p + b kChunk +
+// (e kStreamThreads + t) 4 with sh_p = (-p n) mod 4; the sh_p head elements
+// (block 0, threads < sh_p) and a last group cut by n are read as single
+// floats.  The basis stride ldc is a multiple of 4, so every column of the
+// row shares its alignment.
+__device__ __forceinline__ int row_shift(int64_t p, int64_t n) { return int((4 - (p * n) % 4) % 4); }
+
+// Elements i .. i + 3 of a row of n (zeros past n): one float4 load for a
+// whole group, single loads for a cut one.
+__device__ __forceinline__ void load_group4(const float* row, int64_t i, int64_t n, float (&r)[4]) {
+  if (i + 4 <= n) {
+    VecIO<4>::load(row + i, r);
+  } else {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) r[v] = (i + v < n) ? row[i + v] : 0.f;
+  }
+}
+
+template <int NC>
+__global__ void __launch_bounds__(kStreamThreads)
+    k_dots_shifted(const float* __restrict__ Q, int64_t ldc, const float* __restrict__ x,
+                   int64_t n, int ncols, float* __restrict__ partial, int pstride) {
+  constexpr int NE = Layout<4>::NE;
+  __shared__ float red[kWaves * NC];
+  const int64_t p = blockIdx.y, b = blockIdx.x, nb = gridDim.x;
+  const int t = threadIdx.x;
+  const int sh = row_shift(p, n);
+  const int64_t base = sh + b * kChunk;
+  const bool head = b == 0 && t < sh && t < n;
+  const float* xr = x + p * n;
+  float xv[NE][4];
+#pragma unroll
+  for (int e = 0; e < NE; ++e) load_group4(xr, base + (int64_t(e) * kStreamThreads + t) * 4, n, xv[e]);
+  const float xh = head ? xr[t] : 0.f;
+
+  const float* qp = Q + p * n;
+  for (int c0 = 0; c0 < ncols; c0 += NC) {
+    float acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      acc[c] = 0.f;
+      if (c0 + c < ncols) {
+        const float* q = qp + int64_t(c0 + c) * ldc;
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+          const int64_t i = base + (int64_t(e) * kStreamThreads + t) * 4;
+          if (i < n) {
+            float qv[4];
+            load_group4(q, i, n, qv);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[c] = fmaf(qv[v], xv[e][v], acc[c]);
+          }
+        }
+        if (head) acc[c] = fmaf(q[t], xh, acc[c]);
+      }
+    }
+    const int live = ncols - c0 < NC ? ncols - c0 : NC;
+    block_column_sums<NC>(acc, live, red, partial + (p * nb + b) * pstride + c0);
+  }
+}
+
 // sum of squares: partial[p][b] = sum over chunk b of x[p][:]^2
 template <int V>
 __global__ void __launch_bounds__(kStreamThreads)
@@ -304,6 +367,81 @@ __global__ void __launch_bounds__(kStreamThreads)
         acc = fmaf(xv[v], xv[v], acc);
       }
       VecIO<V>::store(y + p * n + i, xv);
+    }
+  }
+  if (norm2part) {
+    const float tot = block_sum(acc, red);
+    if (t == 0) norm2part[p * nb + b] = tot;
+  }
+}
+
+// k_project for rows that start at every alignment (see k_dots_shifted).
+__global__ void __launch_bounds__(kStreamThreads)
+    k_project_shifted(const float* __restrict__ Q, int64_t ldc, const float* __restrict__ x,
+                      float* __restrict__ y, int64_t n, int ncols,
+                      const float* __restrict__ hpart, int pstride,
+                      float* __restrict__ norm2part, float* __restrict__ hout,
+                      int64_t hout_stride, const float* __restrict__ hsub, int64_t hsub_stride) {
+  constexpr int NE = Layout<4>::NE;
+  extern __shared__ float hs[];
+  __shared__ float red[kWaves];
+  const int64_t p = blockIdx.y, b = blockIdx.x, nb = gridDim.x;
+  const int t = threadIdx.x;
+  const int sh = row_shift(p, n);
+  const int64_t base = sh + b * kChunk;
+  const bool head = b == 0 && t < sh && t < n;
+  load_coeffs(hpart, p, nb, ncols, pstride, hs, hout, hout_stride, b == 0, hsub, hsub_stride);
+
+  float s[NE][4];
+#pragma unroll
+  for (int e = 0; e < NE; ++e)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) s[e][v] = 0.f;
+  float sh_acc = 0.f;
+
+  const float* qp = Q + p * n;
+  for (int c = 0; c < ncols; ++c) {
+    const float hc = hs[c];
+    const float* q = qp + int64_t(c) * ldc;
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      const int64_t i = base + (int64_t(e) * kStreamThreads + t) * 4;
+      if (i < n) {
+        float qv[4];
+        load_group4(q, i, n, qv);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) s[e][v] = fmaf(qv[v], hc, s[e][v]);
+      }
+    }
+    if (head) sh_acc = fmaf(q[t], hc, sh_acc);
+  }
+
+  float acc = 0.f;
+  const float* xr = x + p * n;
+  float* yr = y + p * n;
+  if (head) {
+    const float v = xr[t] - sh_acc;
+    acc = fmaf(v, v, acc);
+    yr[t] = v;
+  }
+#pragma unroll
+  for (int e = 0; e < NE; ++e) {
+    const int64_t i = base + (int64_t(e) * kStreamThreads + t) * 4;
+    if (i < n) {
+      float xv[4];
+      load_group4(xr, i, n, xv);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        xv[v] = xv[v] - s[e][v];
+        if (i + v < n) acc = fmaf(xv[v], xv[v], acc);
+      }
+      if (i + 4 <= n) {
+        VecIO<4>::store(yr + i, xv);
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (i + v < n) yr[i + v] = xv[v];
+      }
     }
   }
   if (norm2part) {
@@ -536,6 +674,14 @@ inline bool aligned16(const void* ptr) { return (reinterpret_cast<uintptr_t>(ptr
 // pointers: 4 (float4) when n % 4 == 0 and all are 16-byte aligned, 2 (float2)
 // when n is even and all are 8-byte aligned (odd-multiple-of-2 n: every row
 // then starts on an 8-byte boundary), else 1.
+// The shifted float4 kernels (rows of any n): a basis stride that is a
+// multiple of 4 and 16-byte aligned bases.
+inline bool shifted_ok(int64_t ld, std::initializer_list<const void*> ptrs) {
+  uintptr_t bits = 0;
+  for (const void* q : ptrs) bits |= reinterpret_cast<uintptr_t>(q);
+  return ld % 4 == 0 && (bits & 15) == 0;
+}
+
 inline int stream_width(int64_t n, int64_t ld, std::initializer_list<const void*> ptrs) {
   uintptr_t bits = 0;
   for (const void* q : ptrs) bits |= reinterpret_cast<uintptr_t>(q);
@@ -582,6 +728,23 @@ hipError_t launch_dots(hipStream_t s, const float* Q, int64_t ldc, const float* 
   }
   if (V == 4) return launch_dots_v<4>(s, Q, ldc, x, P, n, ncols, partial, pstride);
   if (V == 2) return launch_dots_v<2>(s, Q, ldc, x, P, n, ncols, partial, pstride);
+  if (shifted_ok(ldc, {x, Q})) {
+    const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
+    const dim3 block(kStreamThreads);
+    if (ncols <= 8)
+      hipLaunchKernelGGL(k_dots_shifted<8>, grid, block, 0, s, Q, ldc, x, n, ncols, partial,
+                         pstride);
+    else if (ncols <= 16)
+      hipLaunchKernelGGL(k_dots_shifted<16>, grid, block, 0, s, Q, ldc, x, n, ncols, partial,
+                         pstride);
+    else if (ncols <= 32)
+      hipLaunchKernelGGL(k_dots_shifted<32>, grid, block, 0, s, Q, ldc, x, n, ncols, partial,
+                         pstride);
+    else
+      hipLaunchKernelGGL(k_dots_shifted<64>, grid, block, 0, s, Q, ldc, x, n, ncols, partial,
+                         pstride);
+    return hipGetLastError();
+  }
   return launch_dots_v<1>(s, Q, ldc, x, P, n, ncols, partial, pstride);
 }
 
@@ -615,6 +778,9 @@ hipError_t launch_project(hipStream_t s, const float* Q, int64_t ldc, const floa
   else if (V == 2)
     hipLaunchKernelGGL(k_project<2>, grid, dim3(kStreamThreads), shmem, s, Q, ldc, x, y, n, ncols,
                        hpart, pstride, norm2part, hout, hout_stride, hsub, hsub_stride);
+  else if (shifted_ok(ldc, {x, y, Q}))
+    hipLaunchKernelGGL(k_project_shifted, grid, dim3(kStreamThreads), shmem, s, Q, ldc, x, y, n,
+                       ncols, hpart, pstride, norm2part, hout, hout_stride, hsub, hsub_stride);
   else
     hipLaunchKernelGGL(k_project<1>, grid, dim3(kStreamThreads), shmem, s, Q, ldc, x, y, n, ncols,
                        hpart, pstride, norm2part, hout, hout_stride, hsub, hsub_stride);
