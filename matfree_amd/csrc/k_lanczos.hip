@@ -727,8 +727,7 @@ hipError_t launch_dots(hipStream_t s, const float* Q, int64_t ldc, const float* 
     return hipGetLastError();
   }
   if (V == 4) return launch_dots_v<4>(s, Q, ldc, x, P, n, ncols, partial, pstride);
-  if (V == 2) return launch_dots_v<2>(s, Q, ldc, x, P, n, ncols, partial, pstride);
-  if (shifted_ok(ldc, {x, Q})) {
+  if (shifted_ok(ldc, {x, Q})) {  // measured faster than the float2 kernels for even n
     const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
     const dim3 block(kStreamThreads);
     if (ncols <= 8)
@@ -745,6 +744,7 @@ hipError_t launch_dots(hipStream_t s, const float* Q, int64_t ldc, const float* 
                          pstride);
     return hipGetLastError();
   }
+  if (V == 2) return launch_dots_v<2>(s, Q, ldc, x, P, n, ncols, partial, pstride);
   return launch_dots_v<1>(s, Q, ldc, x, P, n, ncols, partial, pstride);
 }
 
@@ -775,12 +775,12 @@ hipError_t launch_project(hipStream_t s, const float* Q, int64_t ldc, const floa
   if (V == 4)
     hipLaunchKernelGGL(k_project<4>, grid, dim3(kStreamThreads), shmem, s, Q, ldc, x, y, n, ncols,
                        hpart, pstride, norm2part, hout, hout_stride, hsub, hsub_stride);
+  else if (shifted_ok(ldc, {x, y, Q}))  // measured faster than the float2 kernel for even n
+    hipLaunchKernelGGL(k_project_shifted, grid, dim3(kStreamThreads), shmem, s, Q, ldc, x, y, n,
+                       ncols, hpart, pstride, norm2part, hout, hout_stride, hsub, hsub_stride);
   else if (V == 2)
     hipLaunchKernelGGL(k_project<2>, grid, dim3(kStreamThreads), shmem, s, Q, ldc, x, y, n, ncols,
                        hpart, pstride, norm2part, hout, hout_stride, hsub, hsub_stride);
-  else if (shifted_ok(ldc, {x, y, Q}))
-    hipLaunchKernelGGL(k_project_shifted, grid, dim3(kStreamThreads), shmem, s, Q, ldc, x, y, n,
-                       ncols, hpart, pstride, norm2part, hout, hout_stride, hsub, hsub_stride);
   else
     hipLaunchKernelGGL(k_project<1>, grid, dim3(kStreamThreads), shmem, s, Q, ldc, x, y, n, ncols,
                        hpart, pstride, norm2part, hout, hout_stride, hsub, hsub_stride);

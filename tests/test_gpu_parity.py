@@ -216,7 +216,8 @@ def fp32_spread(A, V0, k, ref_Q, ref_H, perms=3):
 @pytest.mark.parametrize("n,P,k", [(1024, 64, 30), (1001, 7, 20), (1002, 9, 24), (200, 3, 70),
                                    (64, 5, 48)])
 def test_hessenberg_matches_oracle(cuda, n, P, k):
-    # n = 1024: float4 streaming kernels; 1002: float2; 1001: scalar
+    # n = 1024: aligned float4 streaming; 1002 and 1001: float4 groups from each row's
+    # 16-byte boundary for the dots and projection, float2 / single floats elsewhere
     A = problems.dense_spd(n, seed=0, shift=1.0)
     V = oprng.rademacher(oprng.prng_key(n), (P, n), np.float32)
     V0 = V / np.sqrt(n, dtype=np.float32)
@@ -446,8 +447,9 @@ def test_full_size_dense_decomposition_properties(cuda):
 
 @pytest.mark.parametrize("n", [2050, 2051, 2052])
 def test_logdet_same_across_stream_widths(cuda, n):
-    # the Gram-Schmidt kernels stream rows in float4, float2 or scalar groups
-    # depending on n; each width reproduces the oracle on the same problem
+    # the Gram-Schmidt kernels stream rows in aligned float4 groups (n % 4 == 0),
+    # float4 groups shifted to each row's 16-byte boundary, float2 or single
+    # floats depending on n; each reproduces the oracle on the same problem
     P, k = 6, 20
     A = problems.dense_spd(n, seed=3, shift=0.5)
     Z = oprng.rademacher(oprng.prng_key(11), (P, n), np.float32)
