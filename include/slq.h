@@ -78,6 +78,11 @@ int slq_context_set_stream(slq_context* ctx, void* stream);
 const char* slq_last_error(const slq_context* ctx);
 /* Bytes of device scratch the context currently holds. */
 int64_t slq_context_scratch_bytes(const slq_context* ctx);
+/* Synchronise the context's stream and free all of its pooled scratch (the
+ * next call allocates again).  No counterpart in the reference (JAX owns its
+ * buffers): for a host that admits work by device-memory footprint and must
+ * give an idle context's memory back. */
+int slq_context_release(slq_context* ctx);
 
 /* Rademacher probes, bit-identical to jax.random.rademacher(key, (num, n),
  * float32) with the partitionable threefry2x32 layout.
@@ -135,6 +140,42 @@ int slq_logdet(slq_context* ctx, const slq_operator* op, const float* Z, int64_t
  * symmetric).  Fails with SLQ_EADJOINT_K0 for k = 0 (decomp.py:483-486). */
 int slq_logdet_vjp(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P, int64_t k,
                    double* per_probe, double* mean, double* dparams, float* dmatrix);
+
+/* Saved forward state: the residuals of the custom VJP (decomp.py:398-401:
+ * basis, Hessenberg or tridiagonal matrix, residual) kept on the device
+ * between the forward and the reverse pass, so that the reverse pass need
+ * not recompute the forward (jax.custom_vjp stores them the same way).
+ *
+ * slq_logdet_save runs slq_logdet (reortho_full != 0) or slq_logdet_lanczos
+ * (reortho_full = 0, its k vectors kept) and moves the buffers the reverse
+ * pass reads out of the context's pool into *out; per_probe and mean as in
+ * slq_logdet.  slq_logdet_vjp_saved then computes exactly what
+ * slq_logdet_vjp / slq_logdet_lanczos_vjp computes for the same operator
+ * and probes (without the forward pass), on the context's stream, any
+ * number of times; op must be the operator of the save (same n, kind and
+ * parameters).  slq_saved_destroy synchronises ctx's stream (the device
+ * when ctx is NULL) and frees the state.  Errors as the corresponding
+ * slq_logdet_vjp call. */
+typedef struct slq_saved slq_saved;
+int slq_logdet_save(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P, int64_t k,
+                    int reortho_full, double* per_probe, double* mean, slq_saved** out);
+int slq_logdet_vjp_saved(slq_context* ctx, const slq_operator* op, const slq_saved* saved,
+                         double* dparams, float* dmatrix);
+int64_t slq_saved_bytes(const slq_saved* saved);
+int slq_saved_destroy(slq_context* ctx, slq_saved* saved);
+
+/* Device scratch (bytes) that one call of an entry point needs for (op, P,
+ * k), the caller's inputs and outputs excluded: what a fresh context grows
+ * to (up to 256-byte alignment per buffer).  No counterpart in the
+ * reference; for admission by device-memory footprint. */
+typedef enum slq_entry {
+  SLQ_ENTRY_LOGDET = 0,             /* slq_logdet             */
+  SLQ_ENTRY_LOGDET_LANCZOS = 1,     /* slq_logdet_lanczos     */
+  SLQ_ENTRY_LOGDET_VJP = 2,         /* slq_logdet_vjp         */
+  SLQ_ENTRY_LOGDET_LANCZOS_VJP = 3  /* slq_logdet_lanczos_vjp */
+} slq_entry;
+int slq_workspace_bytes(slq_context* ctx, int entry, const slq_operator* op, int64_t P, int64_t k,
+                        int64_t* bytes);
 
 /* Plain Lanczos (tridiag_sym(k, reortho="none"), decomp.py:155-266) for P
  * start vectors: v0 = V0 / |V0|; per step a = q . A q, r = A q - a q - b q_prev,

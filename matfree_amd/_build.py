@@ -5,6 +5,7 @@ unit with ``hipcc`` into one shared library next to this file.  The library is
 loaded with ctypes by ``matfree_amd._lib``; nothing is JIT-compiled at import.
 """
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -49,21 +50,48 @@ def hipcc():
     return path
 
 
-def build(verbose=False):
-    """Compile libslq.so for gfx950 (always recompiles)."""
-    tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(HERE, "..", "include"), *sources(), "-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
+def _run(cmd):
     proc = subprocess.run(cmd, capture_output=True, text=True)
     # The host pass does not know the packed-fp32 feature and says so; that
     # note is expected and not shown.
     noise = "is not a recognized feature for this target"
     msgs = [ln for ln in (proc.stdout + proc.stderr).splitlines() if noise not in ln]
-    if proc.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + "\n".join(msgs))
-    if verbose and msgs:
-        print("\n".join(msgs), flush=True)
+    return proc.returncode, msgs
+
+
+def build(verbose=False):
+    """Compile libslq.so for gfx950 (always recompiles).
+
+    Every translation unit is compiled to an object in ``_build/`` (in
+    parallel, one hipcc per file), then the objects are linked into the
+    shared library next to this file.
+    """
+    objdir = os.path.join(HERE, "..", "_build")
+    os.makedirs(objdir, exist_ok=True)
+    inc = ["-I", os.path.join(HERE, "..", "include")]
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    jobs = []
+    for src in sources():
+        obj = os.path.join(objdir, os.path.basename(src)[:-4] + ".o")
+        jobs.append((src, obj, [hipcc(), *compile_flags, *inc, "-c", src, "-o", obj]))
+    workers = max(1, min(len(jobs), os.cpu_count() or 1, 8))
+    with concurrent.futures.ThreadPoolExecutor(workers) as pool:
+        results = list(pool.map(lambda job: (job, _run(job[2])), jobs))
+    for (src, _, cmd), (rc, msgs) in results:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        if rc != 0:
+            raise RuntimeError(f"hipcc failed on {os.path.basename(src)}:\n" + "\n".join(msgs))
+        if verbose and msgs:
+            print("\n".join(msgs), flush=True)
+    tmp = LIB_PATH + ".tmp"
+    link = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *[obj for _, obj, _ in jobs],
+            "-o", tmp]
+    if verbose:
+        print(" ".join(link), flush=True)
+    rc, msgs = _run(link)
+    if rc != 0:
+        raise RuntimeError("hipcc link failed:\n" + "\n".join(msgs))
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 

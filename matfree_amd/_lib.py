@@ -108,11 +108,29 @@ SIGNATURES = {
     "slq_quadform_vjp": (
         c_int, [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_void_p, c_void_p],
     ),
+    "slq_context_release": (c_int, [c_void_p]),
+    "slq_logdet_save": (
+        c_int,
+        [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_i64, c_int, c_void_p,
+         c_void_p, ctypes.POINTER(c_void_p)],
+    ),
+    "slq_logdet_vjp_saved": (
+        c_int, [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_void_p, c_void_p],
+    ),
+    "slq_saved_bytes": (c_i64, [c_void_p]),
+    "slq_saved_destroy": (c_int, [c_void_p, c_void_p]),
+    "slq_workspace_bytes": (
+        c_int, [c_void_p, c_int, ctypes.POINTER(OperatorStruct), c_i64, c_i64,
+                ctypes.POINTER(c_i64)],
+    ),
     "slq_profile_enable": (c_int, [c_void_p, c_int]),
     "slq_profile_select": (c_int, [c_void_p, c_int]),
     "slq_profile_read": (c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(c_i64)]),
 }
+
+# entry points sized by slq_workspace_bytes (slq_entry)
+ENTRIES = {"logdet": 0, "logdet_lanczos": 1, "logdet_vjp": 2, "logdet_lanczos_vjp": 3}
 
 # kernel classes of the measurement hooks (slq_kernel_class)
 KERNEL_CLASSES = {"matvec": 0, "dots": 1, "project_dots": 2, "project": 3, "normalize": 4,
@@ -200,6 +218,18 @@ class Context:
 
     def scratch_bytes(self):
         return int(load().slq_context_scratch_bytes(self.ptr))
+
+    def release(self):
+        """Free the context's pooled scratch (after its stream has drained)."""
+        raise_for(load().slq_context_release(self.ptr), self.ptr)
+
+    def workspace_bytes(self, entry, desc, P, k):
+        """Scratch bytes one call of ``entry`` (see ENTRIES) needs for (desc, P, k)."""
+        out = c_i64()
+        rc = load().slq_workspace_bytes(self.ptr, ENTRIES[entry], ctypes.byref(desc), int(P),
+                                        int(k), ctypes.byref(out))
+        raise_for(rc, self.ptr)
+        return int(out.value)
 
     def close(self):
         if self.ptr:

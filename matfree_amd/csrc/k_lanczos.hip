@@ -859,6 +859,23 @@ hipError_t launch_scale_rows(hipStream_t s, const float* x, const float* scale, 
   return hipGetLastError();
 }
 
+// out[0 .. count) = 0 (a kernel rather than hipMemsetAsync, so that the
+// clear is an ordinary kernel node when a caller captures the stream)
+__global__ void __launch_bounds__(kStreamThreads)
+    k_fill_zero(float* __restrict__ out, int64_t count) {
+  const int64_t stride = int64_t(gridDim.x) * kStreamThreads;
+  for (int64_t i = int64_t(blockIdx.x) * kStreamThreads + threadIdx.x; i < count; i += stride)
+    out[i] = 0.f;
+}
+
+hipError_t launch_fill_zero(hipStream_t s, float* out, int64_t count) {
+  if (count <= 0) return hipSuccess;
+  int64_t blocks = (count + kStreamThreads - 1) / kStreamThreads;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(k_fill_zero, dim3(unsigned(blocks)), dim3(kStreamThreads), 0, s, out, count);
+  return hipGetLastError();
+}
+
 hipError_t launch_normalize(hipStream_t s, const float* x, float* out, int64_t P, int64_t n,
                             const float* norm2part, float* norm_out, float* hslot,
                             int64_t hslot_stride, float* inv_out, float* hslot2,

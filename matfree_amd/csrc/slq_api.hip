@@ -36,10 +36,31 @@ struct slq_context {
   // pinned host word for the CG loop's count of active columns
   int* host_count = nullptr;              // pinned, 16 ints (CG's polled counts)
   hipEvent_t cg_done[2] = {nullptr, nullptr};  // CG: count copied to host_count[i]
+  // Arena mode: ensure() carves buffers out of [arena, arena + arena_size)
+  // instead of the pool (nothing is allocated).  With dry_run set the arena
+  // is only counted (its base is a placeholder) and nothing is launched:
+  // arena_off then says how many bytes the call needs.
+  bool arena_mode = false;
+  bool dry_run = false;
+  char* arena = nullptr;
+  size_t arena_size = 0, arena_off = 0;
   // measurement hooks: event pairs per kernel class, reused across reads
   unsigned profile_mask = 0;  // bit kc: kernel class kc is bracketed by events
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events[SLQ_KC_COUNT];
   size_t used[SLQ_KC_COUNT] = {};
+};
+
+// Device state of a forward solve kept for a later reverse pass (the
+// residuals of the custom VJP): buffers moved out of a context's pool.
+struct slq_saved {
+  int device = 0;
+  int full = 1;  // reortho="full" (Arnoldi) or "none" (plain Lanczos)
+  int32_t kind = 0;
+  int64_t n = 0, P = 0, k = 0, ldc = 0;
+  slq_context::Buf basis, hmat, resid, scal1, scal2, blast;
+  std::vector<slq_context::Buf*> all_bufs() {
+    return {&basis, &hmat, &resid, &scal1, &scal2, &blast};
+  }
 };
 
 namespace slq {
@@ -55,16 +76,33 @@ int hip_fail(slq_context* ctx, hipError_t err, const char* where) {
   return fail(ctx, SLQ_EHIP, msg);
 }
 
-#define SLQ_HIP(ctx, expr)                                     \
-  do {                                                         \
-    hipError_t _e = (expr);                                    \
-    if (_e != hipSuccess) return hip_fail((ctx), _e, #expr);   \
+// A HIP call or launch, skipped in a dry run (which only sizes scratch).
+#define SLQ_HIP(ctx, expr)                                       \
+  do {                                                           \
+    if (!(ctx)->dry_run) {                                       \
+      hipError_t _e = (expr);                                    \
+      if (_e != hipSuccess) return hip_fail((ctx), _e, #expr);   \
+    }                                                            \
   } while (0)
 
 // Grow a pooled buffer to at least `bytes`.  The stream is synchronised
 // before an old allocation is released so in-flight work never sees it freed.
 int ensure(slq_context* ctx, slq_context::Buf& b, size_t bytes, void** out) {
   if (bytes == 0) bytes = 16;
+  if (ctx->arena_mode) {
+    if (b.bytes < bytes) {
+      const size_t off = (ctx->arena_off + 255) / 256 * 256;
+      if (off + bytes > ctx->arena_size)
+        return fail(ctx, SLQ_ENOMEM,
+                    "the call needs more than the " + std::to_string(ctx->arena_size) +
+                        " bytes of scratch available to it");
+      b.ptr = ctx->arena + off;
+      b.bytes = bytes;
+      ctx->arena_off = off + bytes;
+    }
+    *out = b.ptr;
+    return SLQ_OK;
+  }
   if (b.bytes < bytes) {
     if (b.ptr) {
       SLQ_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -187,7 +225,7 @@ int hessenberg_core(slq_context* ctx, const slq_operator& op, const float* V0, i
   SLQ_TRY(ensure_t(ctx, ctx->work_w, size_t(P * n), &w));
   SLQ_TRY(ensure_t(ctx, ctx->work_w2, size_t(P * n), &w2));
 
-  if (k > 0) SLQ_HIP(ctx, hipMemsetAsync(H, 0, size_t(P * k * k) * sizeof(float), s));
+  if (k > 0) SLQ_HIP(ctx, launch_fill_zero(s, H, P * k * k));
 
   // initlength = sqrt(<v, v>) (decomp.py:435); c = 1 / initlength (:450)
   SLQ_HIP(ctx, launch_dots(s, nullptr, 0, V0, P, n, 1, norm2, 1));
@@ -263,7 +301,7 @@ int lanczos_core(slq_context* ctx, const slq_operator& op, const float* V0, int6
   SLQ_TRY(ensure_t(ctx, ctx->work_w, size_t(P * n), &w));
   SLQ_TRY(ensure_t(ctx, ctx->work_w2, size_t(P * n), &r));
   SLQ_TRY(ensure_t(ctx, ctx->ring, size_t(3 * P * n), &ring));
-  if (k > 0) SLQ_HIP(ctx, hipMemsetAsync(H, 0, size_t(P * k * k) * sizeof(float), s));
+  if (k > 0) SLQ_HIP(ctx, launch_fill_zero(s, H, P * k * k));
   // vector j: basis column j when kept (j < k), else ring slot j % 3
   auto vec = [&](int64_t j) -> float* {
     if (Q && j < k) return Q + j * ldc;
@@ -425,8 +463,8 @@ int logdet_reverse(slq_context* ctx, const slq_operator& op, int64_t P, int64_t 
   SLQ_TRY(ensure_t(ctx, ctx->adj_sum, size_t(2 * outer_contract_blocks(n)), &dsum));
 
   // dH from the eigh VJP of |z|^2 [log T]_00 / P (funm.py:239-241, 331-333)
-  SLQ_HIP(ctx, hipMemsetAsync(dH, 0, size_t(P * k * k) * sizeof(float), s));
-  SLQ_HIP(ctx, hipMemsetAsync(Gam, 0, size_t(P * k * k) * sizeof(float), s));
+  SLQ_HIP(ctx, launch_fill_zero(s, dH, P * k * k));
+  SLQ_HIP(ctx, launch_fill_zero(s, Gam, P * k * k));
   SLQ_HIP(ctx, launch_tridiag_vjp(s, st.H, st.ell, P, k, vwork, dH));
 
   // lambda = dr + Q gamma, gamma = dH e_K - Q^T dr with dr = 0 (decomp.py:498-499)
@@ -526,7 +564,7 @@ int logdet_lanczos_reverse(slq_context* ctx, const slq_operator& op, int64_t P, 
     SLQ_TRY(ensure_t(ctx, ctx->adj_sum, size_t(2 * outer_contract_blocks(n)), &dsum));
 
   // dT from the eigh VJP of |z|^2 [log T]_00 / P (funm.py:239-241, 331-333)
-  SLQ_HIP(ctx, hipMemsetAsync(dH, 0, size_t(P * kk) * sizeof(float), s));
+  SLQ_HIP(ctx, launch_fill_zero(s, dH, P * kk));
   SLQ_HIP(ctx, launch_tridiag_vjp(s, st.H, st.ell, P, k, vwork, dH));
 
   // reverse scan i = k-1 .. 0 (decomp.py:290-300) from xi = 0, lambda_plus = 0
@@ -895,7 +933,7 @@ int slq_cg(slq_context* ctx, const slq_operator* op, const float* B, int64_t P, 
     SLQ_TRY(ensure_t(ctx, ctx->lr_z, size_t(P * rank), &M.Z));
   }
   // x0 = 0; r0 = b - A x0 = b; p0 = z0 = M r0; gamma0 = r0 . z0
-  SLQ_HIP(ctx, hipMemsetAsync(X, 0, size_t(P * n) * sizeof(float), s));
+  SLQ_HIP(ctx, launch_fill_zero(s, X, P * n));
   SLQ_HIP(ctx, hipMemcpyAsync(r, B, size_t(P * n) * sizeof(float), hipMemcpyDeviceToDevice, s));
   if (precond) {
     SLQ_HIP(ctx, launch_woodbury_apply(s, L, n, int(rank), shift, M.C, r, P, M.Wp, M.Z, z));
@@ -991,6 +1029,150 @@ int slq_logdet_lanczos_vjp(slq_context* ctx, const slq_operator* op, const float
   SLQ_TRY(logdet_lanczos_forward(ctx, *op, Z, P, k, per_probe, mean, nullptr, nullptr, true, st));
   Timed tm(ctx, SLQ_KC_ADJOINT);
   return logdet_lanczos_reverse(ctx, *op, P, k, st, dparams, dmatrix);
+}
+
+int slq_context_release(slq_context* ctx) {
+  if (!ctx) return SLQ_EINVAL;
+  SLQ_TRY(set_device(ctx));
+  SLQ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (auto* b : ctx->all_bufs()) {
+    if (b->ptr) SLQ_HIP(ctx, hipFree(b->ptr));
+    b->ptr = nullptr;
+    b->bytes = 0;
+  }
+  return SLQ_OK;
+}
+
+int slq_workspace_bytes(slq_context* ctx, int entry, const slq_operator* op, int64_t P, int64_t k,
+                        int64_t* bytes) {
+  if (!ctx) return SLQ_EINVAL;
+  if (!bytes) return fail(ctx, SLQ_EINVAL, "output pointer is NULL");
+  *bytes = 0;
+  SLQ_TRY(check_operator(ctx, op));
+  const int64_t n = op->n;
+  if (k < 0 || k > n) return fail(ctx, SLQ_EINVAL_K, exceeds_msg(k, n));
+  if (k == 0) return fail(ctx, SLQ_EINVAL, "the log-determinant integrand needs num_matvecs >= 1");
+  if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of probes P must be positive");
+  // the call itself in a dry run: every buffer it would take is counted and
+  // nothing is launched, so the placeholders below are never dereferenced
+  slq_context v;
+  v.device = ctx->device;
+  v.stream = ctx->stream;
+  v.arena_mode = true;
+  v.dry_run = true;
+  v.arena = reinterpret_cast<char*>(uintptr_t(4096));  // placeholder base (never dereferenced)
+  v.arena_size = SIZE_MAX / 2;
+  const float* Z = reinterpret_cast<const float*>(op);
+  double dd = 0.0;
+  float df = 0.f;
+  ForwardState st;
+  int rc;
+  switch (entry) {
+    case SLQ_ENTRY_LOGDET:
+      rc = logdet_forward(&v, *op, Z, P, k, &dd, &dd, nullptr, nullptr, st);
+      break;
+    case SLQ_ENTRY_LOGDET_LANCZOS:
+      rc = logdet_lanczos_forward(&v, *op, Z, P, k, &dd, &dd, nullptr, nullptr, false, st);
+      break;
+    case SLQ_ENTRY_LOGDET_VJP:
+      rc = logdet_forward(&v, *op, Z, P, k, &dd, &dd, nullptr, nullptr, st);
+      if (rc == SLQ_OK) rc = logdet_reverse(&v, *op, P, k, st, &dd, &df);
+      break;
+    case SLQ_ENTRY_LOGDET_LANCZOS_VJP:
+      rc = logdet_lanczos_forward(&v, *op, Z, P, k, &dd, &dd, nullptr, nullptr, true, st);
+      if (rc == SLQ_OK) rc = logdet_lanczos_reverse(&v, *op, P, k, st, &dd, &df);
+      break;
+    default:
+      return fail(ctx, SLQ_EINVAL, "unknown entry " + std::to_string(entry));
+  }
+  if (rc != SLQ_OK) return fail(ctx, rc, v.error);
+  *bytes = int64_t(v.arena_off);
+  return SLQ_OK;
+}
+
+int slq_logdet_save(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P, int64_t k,
+                    int reortho_full, double* per_probe, double* mean, slq_saved** out) {
+  if (!ctx) return SLQ_EINVAL;
+  if (!out) return fail(ctx, SLQ_EINVAL, "output pointer is NULL");
+  *out = nullptr;
+  SLQ_TRY(check_operator(ctx, op));
+  const int64_t n = op->n;
+  if (k == 0 && reortho_full)
+    return fail(ctx, SLQ_EADJOINT_K0,
+                "Custom Hessenberg-adjoints are not implemented for num_matvecs = 0.");
+  if (k < 0 || k > n) return fail(ctx, SLQ_EINVAL_K, exceeds_msg(k, n));
+  if (k == 0) return fail(ctx, SLQ_EINVAL, "the log-determinant integrand needs num_matvecs >= 1");
+  if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of probes P must be positive");
+  if (!Z) return fail(ctx, SLQ_EINVAL, "probe pointer is NULL");
+  SLQ_TRY(set_device(ctx));
+  ForwardState st;
+  if (reortho_full)
+    SLQ_TRY(logdet_forward(ctx, *op, Z, P, k, per_probe, mean, nullptr, nullptr, st));
+  else
+    SLQ_TRY(logdet_lanczos_forward(ctx, *op, Z, P, k, per_probe, mean, nullptr, nullptr, true, st));
+  // the forward's buffers leave the pool: later calls through ctx take new ones
+  slq_saved* sv = new slq_saved();
+  sv->device = ctx->device;
+  sv->full = reortho_full ? 1 : 0;
+  sv->kind = op->kind;
+  sv->n = n;
+  sv->P = P;
+  sv->k = k;
+  sv->ldc = st.ldc;
+  slq_context::Buf* from[] = {&ctx->basis, &ctx->hmat, &ctx->resid, &ctx->scal1, &ctx->scal2,
+                              &ctx->blast};
+  auto to = sv->all_bufs();
+  for (size_t i = 0; i < to.size(); ++i) {
+    *to[i] = *from[i];
+    *from[i] = slq_context::Buf{};
+  }
+  *out = sv;
+  return SLQ_OK;
+}
+
+int slq_logdet_vjp_saved(slq_context* ctx, const slq_operator* op, const slq_saved* saved,
+                         double* dparams, float* dmatrix) {
+  if (!ctx) return SLQ_EINVAL;
+  if (!saved) return fail(ctx, SLQ_EINVAL, "saved state is NULL");
+  SLQ_TRY(check_operator(ctx, op));
+  if (op->n != saved->n || op->kind != saved->kind || ctx->device != saved->device)
+    return fail(ctx, SLQ_EINVAL, "the operator or device differs from the saved forward pass");
+  if (op->kind == SLQ_OP_RBF && !dparams)
+    return fail(ctx, SLQ_EINVAL, "dparams (2 doubles) is NULL");
+  if (op->kind == SLQ_OP_DENSE && !dmatrix)
+    return fail(ctx, SLQ_EINVAL, "dmatrix (n x n floats) is NULL");
+  SLQ_TRY(set_device(ctx));
+  ForwardState st;
+  st.Q = static_cast<float*>(saved->basis.ptr);
+  st.H = static_cast<float*>(saved->hmat.ptr);
+  st.R = static_cast<float*>(saved->resid.ptr);
+  st.c = static_cast<float*>(saved->scal1.ptr);
+  st.ell = static_cast<float*>(saved->scal2.ptr);
+  st.blast = static_cast<float*>(saved->blast.ptr);
+  st.ldc = saved->ldc;
+  Timed tm(ctx, SLQ_KC_ADJOINT);
+  if (saved->full) return logdet_reverse(ctx, *op, saved->P, saved->k, st, dparams, dmatrix);
+  return logdet_lanczos_reverse(ctx, *op, saved->P, saved->k, st, dparams, dmatrix);
+}
+
+int64_t slq_saved_bytes(const slq_saved* saved) {
+  if (!saved) return 0;
+  int64_t total = 0;
+  for (auto* b : const_cast<slq_saved*>(saved)->all_bufs()) total += int64_t(b->bytes);
+  return total;
+}
+
+int slq_saved_destroy(slq_context* ctx, slq_saved* saved) {
+  if (!saved) return SLQ_OK;
+  (void)hipSetDevice(saved->device);
+  if (ctx)
+    (void)hipStreamSynchronize(ctx->stream);
+  else
+    (void)hipDeviceSynchronize();
+  for (auto* b : saved->all_bufs())
+    if (b->ptr) (void)hipFree(b->ptr);
+  delete saved;
+  return SLQ_OK;
 }
 
 }  // extern "C"

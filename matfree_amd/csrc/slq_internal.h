@@ -80,6 +80,8 @@ hipError_t launch_normalize(hipStream_t s, const float* x, float* out, int64_t P
                             const float* norm2part, float* norm_out, float* hslot,
                             int64_t hslot_stride, float* inv_out, float* hslot2 = nullptr,
                             const PlaneOut* planes = nullptr);
+// out[0 .. count) = 0 by a kernel (no memset node in a captured stream).
+hipError_t launch_fill_zero(hipStream_t s, float* out, int64_t count);
 // Plain Lanczos step: r = w - a q - b_prev qm with a = sum_b apart[p][b]
 // (single-column dots of q and w), b_prev = bprev[p * bstride] (qm may be
 // null), norm^2 partials of r, a written to aout[p * astride].

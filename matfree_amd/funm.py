@@ -152,29 +152,145 @@ def _needs_grad(params):
     )
 
 
+# Keep-vs-recompute for the reverse pass: "keep" retains the forward's basis
+# on the device until backward runs, "recompute" runs the forward again in
+# backward (holding nothing in between), "auto" keeps the basis when it fits
+# in KEEP_FRACTION of the device's free memory.
+REVERSE_POLICY = "auto"
+KEEP_FRACTION = 0.25
+
+
+class _SavedForward:
+    """Device residuals of one forward pass (``slq_logdet_save``), freed with
+    the object."""
+
+    def __init__(self, ctx, ptr):
+        self.ctx = ctx
+        self.ptr = ptr
+
+    def nbytes(self):
+        return int(_lib.load().slq_saved_bytes(self.ptr)) if self.ptr else 0
+
+    def close(self):
+        if self.ptr:
+            _lib.load().slq_saved_destroy(self.ctx.ptr if self.ctx.ptr else None, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _grad_buffers(op, n, device):
+    if isinstance(op, operators.RBF):
+        return torch.empty((2,), dtype=torch.float64, device=device), None
+    return None, torch.empty((n, n), dtype=torch.float32, device=device)
+
+
+def _grads_of(op, dparams, dmatrix):
+    return (dparams[0], dparams[1]) if dparams is not None else (dmatrix,)
+
+
+def logdet_save(matvec, Z, num_matvecs, params, reortho="full"):
+    """Forward pass that keeps its residuals for :func:`logdet_vjp_saved`:
+    ``(per_probe (P,), mean (), saved)`` via ``slq_logdet_save``."""
+    op = operators.resolve(matvec)
+    shape = tuple(Z.shape)
+    P, n = (1, shape[0]) if len(shape) == 1 else shape
+    k = int(num_matvecs)
+    device = _runtime.target_device(Z, *params)
+    Z = _runtime.as_f32(Z, device).reshape(P, n)
+    desc, keep = op.descriptor(n, params, device)
+    per_probe = torch.empty((P,), dtype=torch.float64, device=device)
+    mean = torch.empty((), dtype=torch.float64, device=device)
+    ctx = _runtime.context(device)
+    handle = ctypes.c_void_p()
+    ctx.call("slq_logdet_save", ctypes.byref(desc), _runtime.ptr(Z), P, k,
+             1 if reortho == "full" else 0, _runtime.ptr(per_probe), _runtime.ptr(mean),
+             ctypes.byref(handle))
+    del keep
+    return per_probe, mean, _SavedForward(ctx, handle)
+
+
+def logdet_vjp_saved(matvec, saved, n, params):
+    """Cotangents of the probe mean from a :func:`logdet_save` state (the same
+    values :func:`logdet_value_and_grad` returns)."""
+    op = operators.resolve(matvec)
+    device = _runtime.target_device(*params)
+    desc, keep = op.descriptor(n, params, device)
+    dparams, dmatrix = _grad_buffers(op, n, device)
+    ctx = _runtime.context(device)
+    ctx.call("slq_logdet_vjp_saved", ctypes.byref(desc), saved.ptr, _runtime.ptr(dparams),
+             _runtime.ptr(dmatrix))
+    del keep
+    return _grads_of(op, dparams, dmatrix)
+
+
+def _keep_basis(matvec, Z, num_matvecs, params, reortho):
+    if REVERSE_POLICY in ("keep", "recompute"):
+        return REVERSE_POLICY == "keep"
+    op = operators.resolve(matvec)
+    P, n = (1, Z.shape[0]) if Z.ndim == 1 else tuple(Z.shape)
+    device = _runtime.target_device(Z, *params)
+    desc, keep = op.descriptor(n, params, device)
+    # the basis dominates both the forward's scratch and the kept state
+    need = _runtime.context(device).workspace_bytes("logdet", desc, P, num_matvecs)
+    del keep
+    free, _ = torch.cuda.mem_get_info(device)
+    return need <= KEEP_FRACTION * free
+
+
 class _LogdetMean(torch.autograd.Function):
     """The probe mean as a differentiable function of the operator parameters.
 
-    The forward pass runs ``slq_logdet_vjp`` (value and cotangents in one
-    call, as a custom VJP stores its residuals); the backward pass scales the
-    stored cotangents by the incoming gradient.
+    The forward pass computes the value only; the reverse pass (the GPU
+    adjoint of the decomposition) runs in backward, from the forward's basis
+    when it was kept (``slq_logdet_save``) or by recomputing the forward
+    (``slq_logdet_vjp``), as ``REVERSE_POLICY`` decides.  Parameters that are
+    not tensors requiring grad get no gradient.
     """
 
     @staticmethod
     def forward(ctx, matvec, samples, num_matvecs, reortho, *params):
-        _, mean, grads = logdet_value_and_grad(matvec, samples, num_matvecs, params, reortho)
-        shaped = []
-        for p, g in zip(params, grads):
-            if isinstance(p, torch.Tensor):
-                g = g.to(dtype=p.dtype, device=p.device).reshape(p.shape)
-            shaped.append(g)
-        ctx.save_for_backward(*shaped)
+        if _keep_basis(matvec, samples, num_matvecs, params, reortho):
+            _, mean, saved = logdet_save(matvec, samples, num_matvecs, params, reortho)
+        else:
+            _, mean = logdet_per_probe(matvec, samples, num_matvecs, params, reortho=reortho)
+            saved = None
+        ctx.matvec, ctx.num_matvecs, ctx.reortho, ctx.saved = matvec, num_matvecs, reortho, saved
+        is_tensor = [isinstance(p, torch.Tensor) for p in params]
+        ctx.is_tensor = is_tensor
+        ctx.plain = [None if t else p for p, t in zip(params, is_tensor)]
+        ctx.save_for_backward(samples if isinstance(samples, torch.Tensor) else None,
+                              *[p for p, t in zip(params, is_tensor) if t])
+        ctx.samples_host = None if isinstance(samples, torch.Tensor) else samples
         return mean
 
     @staticmethod
     def backward(ctx, gmean):
-        grads = ctx.saved_tensors
-        return (None, None, None, None, *[(gmean.to(g.dtype) * g) for g in grads])
+        saved_tensors = list(ctx.saved_tensors)
+        samples = saved_tensors.pop(0)
+        if samples is None:
+            samples = ctx.samples_host
+        it = iter(saved_tensors)
+        params = tuple(next(it) if t else p for p, t in zip(ctx.plain, ctx.is_tensor))
+        if ctx.saved is not None:
+            n = int(samples.shape[-1])
+            grads = logdet_vjp_saved(ctx.matvec, ctx.saved, n, params)
+        else:
+            _, _, grads = logdet_value_and_grad(ctx.matvec, samples, ctx.num_matvecs, params,
+                                                ctx.reortho)
+        out = []
+        for p, t, g in zip(params, ctx.is_tensor, grads):
+            if not t or not ctx.needs_input_grad[4 + len(out)]:
+                out.append(None)
+                continue
+            g = (gmean.to(torch.float64) * g.to(torch.float64)).to(dtype=p.dtype,
+                                                                   device=p.device)
+            out.append(g.reshape(p.shape))
+        return (None, None, None, None, *out)
 
 
 class _QuadFormSym:

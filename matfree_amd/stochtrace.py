@@ -76,7 +76,15 @@ def estimator_monte_carlo_mean_and_sem(integrand, /, sampler):
             samples = sampler.flat(key, _runtime.target_device(*parameters))
         else:
             samples = sampler(key)
-        values = integrand.batched(matvecs, samples, *parameters).contiguous()
+        values = integrand.batched(matvecs, samples, *parameters)
+        if (not isinstance(values, torch.Tensor) or values.ndim != 1
+                or values.dtype != torch.float64 or not values.is_cuda):
+            raise TypeError(
+                "matfree_amd's mean-and-SEM estimator reduces scalar integrands (one float64 "
+                "value per probe on the GPU, e.g. monte_carlo_funm_sym_logdet); "
+                f"{type(integrand).__name__} returns "
+                f"{tuple(values.shape) if isinstance(values, torch.Tensor) else type(values)}")
+        values = values.contiguous()
         mean = torch.empty((), dtype=torch.float64, device=values.device)
         sem = torch.empty((), dtype=torch.float64, device=values.device)
         ctx = _runtime.context(values.device)

@@ -121,13 +121,30 @@ def test_cg_rejects_foreign_preconditioners():
 
 def test_serving_admission_rule():
     # HBM-footprint admission of the serving loop (matfree_amd.serving): a
-    # request starts if it fits beside the running ones, or if nothing runs
+    # request starts if it fits beside the running requests and what the idle
+    # slots hold, after releasing idle slots (largest first) if needed; it
+    # waits if it still does not fit while anything runs, and runs alone
+    # otherwise
     from matfree_amd import serving
 
-    r = serving.Request(None, (), 1000, 16, 20, seed=0)
-    fp = serving.footprint(r)
-    assert fp == 4 * (20 + 24) * 16 * 1000
-    assert serving.admits(r, 0, None) and serving.admits(r, 10**12, None)
-    assert serving.admits(r, 0, fp // 2)  # too big, but alone
-    assert serving.admits(r, 1, fp + 1)
-    assert not serving.admits(r, 2, fp + 1)
+    adm = serving.admission
+    held = {0: 0, 1: 0, 2: 0}
+    assert adm(100, 0, held, [0, 1, 2], 0, False, None) == (True, [])
+    assert adm(100, 0, held, [0, 1, 2], 0, False, 150) == (True, [])
+    assert adm(100, 0, held, [0, 1], 60, True, 150) == (False, [])  # 160 > 150: wait
+    assert adm(100, 0, held, [0, 1], 50, True, 150) == (True, [])  # fits exactly
+    assert adm(500, 0, held, [0, 1, 2], 0, False, 150) == (True, [])  # too big: alone
+    # a slot that already holds more than the request needs is charged its
+    # holding; here that does not fit, so the slot's own scratch is released
+    assert adm(10, 0, {0: 120, 1: 0}, [0, 1], 50, True, 150) == (True, [0])
+    assert adm(10, 0, {0: 90, 1: 0}, [0, 1], 50, True, 150) == (True, [])
+    # idle slots give their memory back, largest first, only as far as needed
+    held = {0: 0, 1: 80, 2: 40, 3: 10}
+    assert adm(50, 0, held, [0, 1, 2, 3], 20, True, 150) == (True, [1])
+    assert adm(100, 0, held, [0, 1, 2, 3], 20, True, 150) == (True, [1, 2])
+    # when releasing everything is still not enough, nothing is released: wait
+    assert adm(200, 0, held, [0, 1, 2, 3], 20, True, 150) == (False, [])
+    # the chosen slot's own holding is released only when the request needs
+    # less than it holds
+    assert adm(100, 1, held, [0, 1, 2, 3], 0, False, 120) == (True, [2])
+    assert adm(30, 1, held, [0, 1, 2, 3], 0, False, 90) == (True, [1])

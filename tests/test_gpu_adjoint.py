@@ -172,3 +172,98 @@ def test_dense_cotangent_autograd_direction(cuda):
           - float(estimate(operators.dense_matvec, prng.PRNGKey(4), A - h * D))) / (2 * h)
     got = float(torch.sum(Ap.grad * D))
     assert abs(got - fd) <= 1e-2 * abs(fd), (got, fd)
+
+
+def test_autograd_with_some_parameters_plain(cuda):
+    # a requires-grad lengthscale beside a plain float noise: the noise gets
+    # no gradient and the lengthscale the same as the direct call
+    n, P, k = 400, 6, 14
+    X = problems.rbf_points(n, seed=8)
+    op = operators.RBF(X)
+    integrand = funm.monte_carlo_funm_sym_logdet(decomp.tridiag_sym(k))
+    sampler = stochtrace.sampler_signs(np.zeros(n, np.float32), num=P)
+    estimate = stochtrace.estimator_monte_carlo(integrand, sampler)
+    ls = torch.tensor(0.45, dtype=torch.float32, device=cuda, requires_grad=True)
+    estimate(op, prng.PRNGKey(1), ls, 3e-2).backward()
+    Z = sampler.flat(prng.PRNGKey(1), cuda)
+    _, _, grads = funm.logdet_value_and_grad(op, Z, k, (0.45, 3e-2))
+    assert ls.grad.dtype == torch.float32
+    assert float(ls.grad) == pytest.approx(float(grads[0]), rel=1e-6)
+    # and the other way round: a plain lengthscale, a requires-grad noise
+    s2 = torch.tensor(3e-2, dtype=torch.float64, device=cuda, requires_grad=True)
+    estimate(op, prng.PRNGKey(1), 0.45, s2).backward()
+    assert float(s2.grad) == float(grads[1])
+
+
+@pytest.mark.parametrize("reortho", ["full", "none"])
+def test_kept_and_recomputed_reverse_pass_agree(cuda, reortho):
+    # the reverse pass from the kept forward state and the one that recomputes
+    # the forward give the same bits; with the basis kept, the forward holds
+    # the state until backward
+    n, P, k = 700, 8, 18
+    X = problems.rbf_points(n, seed=9)
+    op = operators.RBF(X)
+    integrand = funm.monte_carlo_funm_sym_logdet(decomp.tridiag_sym(k, reortho=reortho))
+    sampler = stochtrace.sampler_signs(np.zeros(n, np.float32), num=P)
+    estimate = stochtrace.estimator_monte_carlo(integrand, sampler)
+    out = {}
+    old = funm.REVERSE_POLICY
+    try:
+        for policy in ("keep", "recompute"):
+            funm.REVERSE_POLICY = policy
+            ls = torch.tensor(0.5, dtype=torch.float64, device=cuda, requires_grad=True)
+            s2 = torch.tensor(0.1, dtype=torch.float64, device=cuda, requires_grad=True)
+            val = estimate(op, prng.PRNGKey(2), ls, s2)
+            saved = val.grad_fn.saved
+            assert (saved is not None) == (policy == "keep")
+            if saved is not None:
+                assert saved.nbytes() >= 4 * k * P * n
+            val.backward()
+            out[policy] = (float(val.detach()), float(ls.grad), float(s2.grad))
+    finally:
+        funm.REVERSE_POLICY = old
+    assert out["keep"] == out["recompute"]
+    Z = sampler.flat(prng.PRNGKey(2), cuda)
+    _, mean, grads = funm.logdet_value_and_grad(op, Z, k, (0.5, 0.1), reortho)
+    assert out["keep"] == (float(mean), float(grads[0]), float(grads[1]))
+
+
+def test_saved_forward_dense_matches_direct(cuda):
+    n, P, k = 300, 4, 16
+    A = torch.as_tensor(problems.dense_spd(n, seed=3, shift=1.0), device=cuda)
+    Z = torch.as_tensor(oprng.rademacher(oprng.prng_key(6), (P, n), np.float32), device=cuda)
+    per, mean, saved = funm.logdet_save(operators.dense_matvec, Z, k, (A,))
+    (G1,) = funm.logdet_vjp_saved(operators.dense_matvec, saved, n, (A,))
+    (G2,) = funm.logdet_vjp_saved(operators.dense_matvec, saved, n, (A,))  # reusable
+    saved.close()
+    per2, mean2, (G,) = funm.logdet_value_and_grad(operators.dense_matvec, Z, k, (A,))
+    assert torch.equal(per, per2) and float(mean) == float(mean2)
+    assert torch.equal(G1, G) and torch.equal(G2, G)
+
+
+def test_workspace_query_matches_pool_growth(cuda):
+    # the scratch slq_workspace_bytes reports is what a fresh context grows to
+    # (within the 256-byte alignment of each buffer)
+    from matfree_amd import _lib
+
+    n, P, k = 1500, 16, 20
+    X = torch.as_tensor(problems.rbf_points(n, seed=2), device=cuda)
+    for entry, reortho, grad in [("logdet", "full", False), ("logdet_lanczos", "none", False),
+                                 ("logdet_vjp", "full", True),
+                                 ("logdet_lanczos_vjp", "none", True)]:
+        op = operators.RBF(X, precision="bf16x2")
+        desc, keep = op.descriptor(n, (0.5, 0.1), cuda)
+        stream = torch.cuda.Stream(cuda)
+        with torch.cuda.stream(stream):
+            ctx = _lib.context(cuda.index, stream.cuda_stream)
+            need = ctx.workspace_bytes(entry, desc, P, k)
+            Z = torch.ones((P, n), device=cuda)
+            if grad:
+                funm.logdet_value_and_grad(op, Z, k, (0.5, 0.1), reortho)
+            else:
+                funm.logdet_per_probe(op, Z, k, (0.5, 0.1), reortho=reortho)
+            held = ctx.scratch_bytes()
+            stream.synchronize()
+            ctx.release()
+            assert ctx.scratch_bytes() == 0
+        assert held <= need <= held + 256 * 64, (entry, held, need)

@@ -1,0 +1,88 @@
+"""Depths above 64 (run on an MI355X).
+
+Above 64 Arnoldi steps the Gram-Schmidt passes run unfused (the fused pass
+keeps at most 64 basis columns in registers) and the FP64 tridiagonal kernels
+keep their workspace in global memory instead of LDS
+(k_tridiag_quadrature_global, k_tridiag_vjp_global, k_tridiag_funm_global).
+Odd n also runs the projection pass that starts each row's float4 groups at
+its 16-byte boundary, with the coefficient output of the unfused step.
+
+Tolerances: the FP32 GPU path against the FP32 restatement -- per-probe
+log-determinant relative 1e-5, alpha / beta within 1e-5 of max|alpha|; the
+reverse pass (FP32) against the FP64 restatement normwise 2e-4; f(A) v at
+least as close to FP64 as the FP32 restatement (up to 2x).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from matfree_amd import decomp, funm, operators, problems
+from oracle import prng as oprng
+from oracle import slq as oslq
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("n,P,k", [(301, 5, 65), (997, 6, 96), (1000, 3, 96)])
+def test_logdet_beyond_64_matches_oracle(cuda, n, P, k):
+    A = problems.dense_spd(n, seed=n, shift=1.0)
+    Z = oprng.rademacher(oprng.prng_key(n), (P, n), np.float32)
+    per, mean, alpha, beta = funm.logdet_per_probe(
+        operators.dense_matvec, torch.as_tensor(Z, device=cuda), k,
+        (torch.as_tensor(A, device=cuda),), want_coeffs=True)
+    o_vals, o_alpha, o_beta = oslq.quadform_logdet(oslq.DenseOperator(A), k, Z, np.float32)
+    scale = float(np.abs(o_alpha).max())
+    assert np.abs(alpha.cpu().numpy() - o_alpha).max() <= 1e-5 * scale
+    assert np.abs(beta.cpu().numpy() - o_beta).max() <= 1e-5 * scale
+    np.testing.assert_allclose(per.cpu().numpy(), o_vals, rtol=1e-5)
+    # the FP64 quadrature of the GPU's own tridiagonal (global workspace)
+    q = oslq.tridiag_eig_quadrature(alpha.cpu().numpy(), beta.cpu().numpy(),
+                                    np.sqrt(np.full(P, n, np.float64)))
+    np.testing.assert_allclose(per.cpu().numpy(), q, rtol=1e-10)
+
+
+@pytest.mark.parametrize("n,P,k", [(301, 4, 65), (511, 3, 96)])
+def test_rbf_cotangents_beyond_64_match_oracle(cuda, n, P, k):
+    X = problems.rbf_points(n, seed=11)
+    theta = (0.3, 0.2)
+    Z = oprng.rademacher(oprng.prng_key(k), (P, n), np.float32)
+    _, mean, grads = funm.logdet_value_and_grad(operators.RBF(X), torch.as_tensor(Z, device=cuda),
+                                                k, theta)
+    est64, g64 = oslq.slq_logdet_vjp(oslq.RBFOperator(X, *theta, np.float64), k, Z)
+    g = np.array([float(grads[0]), float(grads[1])])
+    assert rel(g, g64) < 2e-4, (g, g64)
+    assert abs(float(mean) - est64) <= 2e-5 * abs(est64)
+
+
+def test_dense_cotangent_beyond_64_matches_oracle(cuda):
+    n, P, k = 257, 3, 80
+    A = problems.dense_spd(n, seed=9, shift=1.0)
+    Z = oprng.rademacher(oprng.prng_key(3), (P, n), np.float32)
+    _, mean, (G,) = funm.logdet_value_and_grad(
+        operators.dense_matvec, torch.as_tensor(Z, device=cuda), k,
+        (torch.as_tensor(A, device=cuda),))
+    est64, G64 = oslq.slq_logdet_vjp(oslq.DenseOperator(A, np.float64), k, Z)
+    assert rel(G.cpu().numpy(), G64) < 2e-4
+    assert abs(float(mean) - est64) <= 2e-5 * abs(est64)
+
+
+@pytest.mark.parametrize("fun", [np.log, np.exp, np.reciprocal])
+@pytest.mark.parametrize("reortho", ["full", "none"])
+def test_funm_vectors_beyond_64_match_oracle(cuda, fun, reortho):
+    n, P, k = 401, 4, 72
+    A = problems.dense_spd(n, seed=4, shift=1.0)
+    V = np.random.default_rng(5).standard_normal((P, n)).astype(np.float32)
+    f = funm.funm_lanczos_sym(funm.dense_funm_sym_eigh(fun), decomp.tridiag_sym(k, reortho=reortho))
+    got = f.batched(operators.dense_matvec, torch.as_tensor(V, device=cuda),
+                    torch.as_tensor(A, device=cuda)).cpu().numpy()
+    o32 = oslq.funm_lanczos_sym(oslq.DenseOperator(A), k, V, fun, np.float32, reortho=reortho)
+    o64 = oslq.funm_lanczos_sym(oslq.DenseOperator(A, np.float64), k, V, fun, np.float64,
+                                reortho=reortho)
+    assert rel(got, o64) <= 2 * rel(o32, o64) + 1e-6, (rel(got, o64), rel(o32, o64))

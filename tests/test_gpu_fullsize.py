@@ -1,0 +1,86 @@
+"""Parity at the headline sizes (run on an MI355X).
+
+The GPU runs the whole launch the benchmark runs (every probe of the
+configuration, so the same kernel instances), and a subset of its probes is
+compared with the oracle's FP32 restatement of the reference's formulas on
+the same probes (probes are independent: each probe's coefficients depend on
+its own start vector and the operator only).
+
+Tolerances (FP32 GPU path vs FP32 restatement; the two differ in summation
+order only):
+  * alpha / beta: |delta| <= 1e-5 * max|alpha|;
+  * per-probe log-determinant: relative 1e-5.
+BF16 forms are compared with the GPU's own FP32 form per probe (tolerance in
+the test).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from matfree_amd import funm, operators, problems, prng, stochtrace
+from oracle import slq as oslq
+
+pytestmark = pytest.mark.gpu
+
+# probes compared with the oracle: first, second, middle and last
+SUBSET = lambda P: [0, 1, P // 2, P - 1]  # noqa: E731
+
+
+def compare(per, alpha, beta, sub, o_vals, o_alpha, o_beta, label):
+    g_per = per.cpu().numpy()[sub]
+    g_alpha = alpha.cpu().numpy()[sub]
+    g_beta = beta.cpu().numpy()[sub]
+    scale = float(np.abs(o_alpha).max())
+    da = float(np.abs(g_alpha - o_alpha).max())
+    db = float(np.abs(g_beta - o_beta).max())
+    dv = float(np.max(np.abs(g_per - o_vals) / np.abs(o_vals)))
+    print(f"{label}: max|dalpha| {da:.3e} max|dbeta| {db:.3e} (1e-5 max|alpha| = "
+          f"{1e-5 * scale:.3e}); per-probe rel {dv:.3e}", flush=True)
+    assert da <= 1e-5 * scale, (label, da, scale)
+    assert db <= 1e-5 * scale, (label, db, scale)
+    assert dv <= 1e-5, (label, dv)
+
+
+def test_c2_dense_fp32_matches_oracle(cuda):
+    # stored dense A = B B^T / n + 0.1 I, n = 16384, 256 probes, depth 64,
+    # FP32: the k_matvec_f32<4, false, 1, true> instance of the benchmark
+    n, P, k = 16384, 256, 64
+    A = problems.dense_spd_device(n, seed=0, shift=0.1, device=cuda)
+    Z = stochtrace.sampler_signs(np.zeros(n, np.float32), num=P).flat(prng.PRNGKey(0), cuda)
+    per, mean, alpha, beta = funm.logdet_per_probe(operators.dense_matvec, Z, k, (A,),
+                                                   want_coeffs=True)
+    assert bool(torch.isfinite(per).all())
+    sub = SUBSET(P)
+    Ah = A.cpu().numpy()
+    Zs = Z[sub].cpu().numpy()
+    o_vals, o_alpha, o_beta = oslq.quadform_logdet(oslq.DenseOperator(Ah), k, Zs, np.float32)
+    compare(per, alpha, beta, sub, o_vals, o_alpha, o_beta, "C2 fp32")
+
+
+@pytest.mark.parametrize("precision,tol", [("bf16x2", 5e-4)])
+def test_c3_rbf_fp32_matches_oracle_and_bf16_tracks_fp32(cuda, precision, tol):
+    # RBF n = 32768 points in [0, 1]^3, l = 0.5, s2 = 1e-2, 128 probes, depth
+    # 64: the plain FP32 form against the oracle on four probes, then the
+    # BF16 form of the headline against the FP32 form on every probe
+    n, P, k = 32768, 128, 64
+    X = torch.as_tensor(problems.rbf_points(n, seed=0), device=cuda)
+    Z = stochtrace.sampler_signs(np.zeros(n, np.float32), num=P).flat(prng.PRNGKey(0), cuda)
+    theta = (0.5, 1e-2)
+    per, mean, alpha, beta = funm.logdet_per_probe(operators.RBF(X, precision="fp32"), Z, k,
+                                                   theta, want_coeffs=True)
+    assert bool(torch.isfinite(per).all())
+    bper, bmean = funm.logdet_per_probe(operators.RBF(X, precision=precision), Z, k, theta)
+    d = (bper - per).abs() / per.abs()
+    print(f"C3 {precision} vs fp32: per-probe rel max {float(d.max()):.3e}, mean rel "
+          f"{abs(float(bmean) - float(mean)) / abs(float(mean)):.3e}", flush=True)
+    assert float(d.max()) <= tol
+    # the FP32 operator matrix for the oracle: FP64 entries rounded once
+    K = problems.rbf_dense64_device(X.cpu().numpy(), *theta, device=cuda)
+    Kh = K.to(torch.float32).cpu().numpy()
+    del K
+    torch.cuda.empty_cache()
+    sub = SUBSET(P)
+    Zs = Z[sub].cpu().numpy()
+    o_vals, o_alpha, o_beta = oslq.quadform_logdet(oslq.DenseOperator(Kh), k, Zs, np.float32)
+    compare(per, alpha, beta, sub, o_vals, o_alpha, o_beta, "C3 fp32")

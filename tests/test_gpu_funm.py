@@ -100,3 +100,14 @@ def test_mean_and_sem(cuda):
     assert abs(float(mean) - o_mean) <= 1e-12 * abs(o_mean)
     assert abs(float(sem) - o_sem) <= 1e-10 * abs(o_sem)
     assert float(mean) == pytest.approx(float(m2), rel=1e-14)
+
+
+def test_mean_and_sem_rejects_vector_integrands(cuda):
+    # the device reduction takes one float64 value per probe
+    n = 64
+    A = torch.as_tensor(problems.dense_spd(n, seed=1, shift=1.0), device=cuda)
+    f = funm.funm_lanczos_sym(funm.dense_funm_sym_eigh(np.exp), decomp.tridiag_sym(5))
+    sampler = stochtrace.sampler_signs(np.zeros(n, np.float32), num=3)
+    with pytest.raises(TypeError, match="scalar integrands"):
+        stochtrace.estimator_monte_carlo_mean_and_sem(f, sampler)(operators.dense_matvec,
+                                                                  prng.PRNGKey(0), A)

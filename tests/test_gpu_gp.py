@@ -250,3 +250,22 @@ def test_quadform_gradient_is_exact(cuda):
     dK = np.exp(-d2 / (2 * 0.3**2)) * d2 / 0.3**3
     assert np.isclose(float(ls.grad), -alpha @ dK @ alpha, rtol=1e-3)
     assert np.isclose(float(s2.grad), -alpha @ alpha, rtol=1e-3)
+
+
+def test_gaussian_logpdf_with_only_the_lengthscale_learned(cuda):
+    # a requires-grad lengthscale and a plain float noise: the gradient of the
+    # log-density in the lengthscale alone, against the dense closed form
+    n = 40
+    xs = np.linspace(0.0, 1.0, n).astype(np.float32)
+    op = operators.RBF(xs[:, None])
+    logpdf = gp.create_logpdf(op, np.zeros(n, np.float32))
+    y = np.random.default_rng(3).standard_normal(n).astype(np.float32)
+    ell = torch.tensor(0.8, dtype=torch.float64, device=cuda, requires_grad=True)
+    noise = 0.1
+    logpdf(torch.as_tensor(y, device=cuda), prng.PRNGKey(4), ell, noise).backward()
+    A = rbf_dense(xs[:, None], 0.8, noise)
+    d2 = (xs[:, None].astype(np.float64) - xs[None].astype(np.float64)) ** 2
+    dA = np.exp(-d2 / (2 * 0.8**2)) * d2 / 0.8**3
+    alpha = np.linalg.solve(A, y.astype(np.float64))
+    exact = 0.5 * (alpha @ dA @ alpha - np.trace(np.linalg.solve(A, dA)))
+    assert abs(float(ell.grad) - exact) <= 0.1 * abs(exact) + 0.05, (float(ell.grad), exact)
