@@ -93,7 +93,8 @@ def admission(need, slot, held, free, running_bytes, any_running, budget):
     """
     if budget is None:
         return True, []
-    h = dict(held)
+    # per-slot holdings, from a list indexed by slot or a mapping slot -> bytes
+    h = dict(held) if isinstance(held, dict) else dict(enumerate(held))
 
     def total():
         return sum(h[s] for s in free) + running_bytes - h[slot] + max(h[slot], need)

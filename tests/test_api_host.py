@@ -130,6 +130,9 @@ def test_serving_admission_rule():
     adm = serving.admission
     held = {0: 0, 1: 0, 2: 0}
     assert adm(100, 0, held, [0, 1, 2], 0, False, None) == (True, [])
+    # the server passes a list indexed by slot
+    assert adm(100, 0, [0, 0, 0], [0, 1], 60, True, 150) == (False, [])
+    assert adm(10, 0, [120, 0], [0, 1], 50, True, 150) == (True, [0])
     assert adm(100, 0, held, [0, 1, 2], 0, False, 150) == (True, [])
     assert adm(100, 0, held, [0, 1], 60, True, 150) == (False, [])  # 160 > 150: wait
     assert adm(100, 0, held, [0, 1], 50, True, 150) == (True, [])  # fits exactly

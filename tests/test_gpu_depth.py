@@ -7,8 +7,10 @@ keep their workspace in global memory instead of LDS
 Odd n also runs the projection pass that starts each row's float4 groups at
 its 16-byte boundary, with the coefficient output of the unfused step.
 
-Tolerances: the FP32 GPU path against the FP32 restatement -- per-probe
-log-determinant relative 1e-5, alpha / beta within 1e-5 of max|alpha|; the
+Tolerances: the FP32 GPU path against the FP64 restatement -- per-probe
+log-determinant relative 1e-5 and alpha / beta within 1e-5 of max|alpha|, or
+1.5x the FP32 restatement's own spread under element reorderings where that
+is larger (late Ritz values at depth 96); the
 reverse pass (FP32) against the FP64 restatement normwise 2e-4; f(A) v at
 least as close to FP64 as the FP32 restatement (up to 2x).
 """
@@ -30,6 +32,26 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
+def fp32_order_spread(A, Z, k, d_vals, d_alpha, d_beta, perms=3):
+    """Largest distance of the FP32 restatement from FP64 over element orders.
+
+    A symmetric permutation of A with the same permutation of the probes
+    changes only the FP32 summation order; at depth 96 the late Ritz values
+    have converged and alpha / beta move by up to ~1e-4 of max|alpha| under
+    such reorderings, so a correct FP32 implementation is held to that
+    spread (1.5x) where it exceeds 1e-5 max|alpha|.
+    """
+    n = A.shape[0]
+    wv, wc = 0.0, 0.0
+    for seed in range(perms + 1):
+        perm = np.arange(n) if seed == 0 else np.random.default_rng(seed).permutation(n)
+        vals, alpha, beta = oslq.quadform_logdet(oslq.DenseOperator(A[perm][:, perm]), k,
+                                                 Z[:, perm], np.float32)
+        wv = max(wv, float(np.max(np.abs(vals - d_vals) / np.abs(d_vals))))
+        wc = max(wc, float(np.abs(alpha - d_alpha).max()), float(np.abs(beta - d_beta).max()))
+    return wv, wc
+
+
 @pytest.mark.parametrize("n,P,k", [(301, 5, 65), (997, 6, 96), (1000, 3, 96)])
 def test_logdet_beyond_64_matches_oracle(cuda, n, P, k):
     A = problems.dense_spd(n, seed=n, shift=1.0)
@@ -37,14 +59,23 @@ def test_logdet_beyond_64_matches_oracle(cuda, n, P, k):
     per, mean, alpha, beta = funm.logdet_per_probe(
         operators.dense_matvec, torch.as_tensor(Z, device=cuda), k,
         (torch.as_tensor(A, device=cuda),), want_coeffs=True)
-    o_vals, o_alpha, o_beta = oslq.quadform_logdet(oslq.DenseOperator(A), k, Z, np.float32)
-    scale = float(np.abs(o_alpha).max())
-    assert np.abs(alpha.cpu().numpy() - o_alpha).max() <= 1e-5 * scale
-    assert np.abs(beta.cpu().numpy() - o_beta).max() <= 1e-5 * scale
-    np.testing.assert_allclose(per.cpu().numpy(), o_vals, rtol=1e-5)
-    # the FP64 quadrature of the GPU's own tridiagonal (global workspace)
-    q = oslq.tridiag_eig_quadrature(alpha.cpu().numpy(), beta.cpu().numpy(),
-                                    np.sqrt(np.full(P, n, np.float64)))
+    d_vals, d_alpha, d_beta = oslq.quadform_logdet(oslq.DenseOperator(A, np.float64), k, Z,
+                                                   np.float64)
+    spread_v, spread_c = fp32_order_spread(A, Z, k, d_vals, d_alpha, d_beta)
+    scale = float(np.abs(d_alpha).max())
+    tol_c = max(1e-5 * scale, 1.5 * spread_c)
+    tol_v = max(1e-5, 1.5 * spread_v)
+    da = float(np.abs(alpha.cpu().numpy() - d_alpha).max())
+    db = float(np.abs(beta.cpu().numpy() - d_beta).max())
+    dv = float(np.max(np.abs(per.cpu().numpy() - d_vals) / np.abs(d_vals)))
+    print(f"n={n} P={P} k={k}: dalpha {da:.2e} dbeta {db:.2e} (tol {tol_c:.2e}); "
+          f"per-probe {dv:.2e} (tol {tol_v:.2e})", flush=True)
+    assert da <= tol_c and db <= tol_c
+    assert dv <= tol_v
+    # the FP64 quadrature of the GPU's own tridiagonal (global workspace); the
+    # length is |z| in FP32 as funm.py:228 computes it
+    length = np.sqrt(np.full(P, n, np.float32)).astype(np.float64)
+    q = oslq.tridiag_eig_quadrature(alpha.cpu().numpy(), beta.cpu().numpy(), length)
     np.testing.assert_allclose(per.cpu().numpy(), q, rtol=1e-10)
 
 

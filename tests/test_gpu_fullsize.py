@@ -58,8 +58,11 @@ def test_c2_dense_fp32_matches_oracle(cuda):
     compare(per, alpha, beta, sub, o_vals, o_alpha, o_beta, "C2 fp32")
 
 
-@pytest.mark.parametrize("precision,tol", [("bf16x2", 5e-4)])
-def test_c3_rbf_fp32_matches_oracle_and_bf16_tracks_fp32(cuda, precision, tol):
+# bf16x2 rounds every Krylov vector to BF16 before the matvec: single probes'
+# quadratures move by up to 1.4e-3 (measured max over the 128 probes), the
+# probe mean by 3.6e-5 (per-probe errors of either sign average out)
+@pytest.mark.parametrize("precision,tol,tol_mean", [("bf16x2", 3e-3, 2e-4)])
+def test_c3_rbf_fp32_matches_oracle_and_bf16_tracks_fp32(cuda, precision, tol, tol_mean):
     # RBF n = 32768 points in [0, 1]^3, l = 0.5, s2 = 1e-2, 128 probes, depth
     # 64: the plain FP32 form against the oracle on four probes, then the
     # BF16 form of the headline against the FP32 form on every probe
@@ -72,9 +75,11 @@ def test_c3_rbf_fp32_matches_oracle_and_bf16_tracks_fp32(cuda, precision, tol):
     assert bool(torch.isfinite(per).all())
     bper, bmean = funm.logdet_per_probe(operators.RBF(X, precision=precision), Z, k, theta)
     d = (bper - per).abs() / per.abs()
+    dm = abs(float(bmean) - float(mean)) / abs(float(mean))
     print(f"C3 {precision} vs fp32: per-probe rel max {float(d.max()):.3e}, mean rel "
-          f"{abs(float(bmean) - float(mean)) / abs(float(mean)):.3e}", flush=True)
+          f"{dm:.3e}", flush=True)
     assert float(d.max()) <= tol
+    assert dm <= tol_mean
     # the FP32 operator matrix for the oracle: FP64 entries rounded once
     K = problems.rbf_dense64_device(X.cpu().numpy(), *theta, device=cuda)
     Kh = K.to(torch.float32).cpu().numpy()
