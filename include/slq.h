@@ -123,6 +123,22 @@ int slq_hessenberg(slq_context* ctx, const slq_operator* op, const float* V0, in
 int slq_logdet(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P, int64_t k,
                double* per_probe, double* mean, float* alpha, float* beta);
 
+/* slq_logdet without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, hosts that forbid it).  Same arguments,
+ * checks and results as slq_logdet (the same kernels in the same order), but
+ * every scratch buffer is carved from a static device array of
+ * SLQ_NOALLOC_SCRATCH_BYTES bytes per GPU (held by each GPU the library runs
+ * on until the process ends) instead of the context's pool; the call makes no
+ * hipMalloc/hipFree.  A call needing more scratch than that (roughly
+ * 4 (k + 6) P n bytes) fails with SLQ_EINVAL before anything is queued.  The
+ * array is located when the first context on a device is created.  All calls
+ * on one GPU share its array: two calls whose work overlaps on the device
+ * (different streams, or different contexts) race on it; serialise them. */
+#define SLQ_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int slq_logdet_noalloc(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P,
+                       int64_t k, double* per_probe, double* mean, float* alpha, float* beta);
+
 /* Value and parameter cotangent of the SLQ log-determinant estimate
  *   mean = (1/P) sum_p |z_p|^2 [log T_p]_00
  * for P probes (reverse mode through the estimator): the custom VJP of

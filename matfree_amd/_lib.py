@@ -19,6 +19,8 @@ SLQ_ENOMEM = -3
 SLQ_EHIP = -4
 SLQ_EUNSUPPORTED = -5
 SLQ_EADJOINT_K0 = -6
+# bytes of the static device scratch of slq_logdet_noalloc (SLQ_NOALLOC_SCRATCH_BYTES)
+NOALLOC_SCRATCH_BYTES = 256 << 20
 
 # operator kinds / precisions
 SLQ_OP_DENSE = 0
@@ -64,6 +66,11 @@ SIGNATURES = {
          c_void_p, c_void_p],
     ),
     "slq_logdet": (
+        c_int,
+        [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_i64, c_void_p, c_void_p,
+         c_void_p, c_void_p],
+    ),
+    "slq_logdet_noalloc": (
         c_int,
         [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_i64, c_void_p, c_void_p,
          c_void_p, c_void_p],

@@ -2,6 +2,7 @@
 // of the SLQ log-determinant path.  Kernels live in k_*.hip.
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -62,6 +63,43 @@ struct slq_saved {
     return {&basis, &hmat, &resid, &scal1, &scal2, &blast};
   }
 };
+
+// Static scratch of slq_logdet_noalloc: a file-scope device array, one copy
+// per GPU the library runs on, held until the process ends.  On the host the
+// name `scratch` only stands for the symbol (it is not a device address and is
+// never passed to a kernel or a copy): the device address of each GPU's copy
+// is looked up once per device with hipGetSymbolAddress, with that device
+// current, when the first context on the device is created, and that address
+// is what the entry point hands to its kernels.
+static __device__ __align__(256) char scratch[SLQ_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+struct StaticScratch {
+  bool looked_up = false;
+  hipError_t status = hipSuccess;
+  void* ptr = nullptr;  // device address of this GPU's copy of `scratch`
+};
+StaticScratch g_scratch[kMaxDevices];
+
+// Called with `device` current: the lookup happens once per device.
+void setup_static_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  StaticScratch& st = g_scratch[device];
+  if (st.looked_up) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    p = nullptr;
+  }
+  st.looked_up = true;
+  st.status = e;
+  st.ptr = p;
+}
+}  // namespace
 
 namespace slq {
 namespace {
@@ -630,6 +668,7 @@ int slq_context_create(int device, void* stream, slq_context** out) {
     return SLQ_EINVAL;
   }
   if (hipSetDevice(device) != hipSuccess) return SLQ_EHIP;
+  setup_static_scratch(device);  // a failed lookup only disables slq_logdet_noalloc
   slq_context* ctx = new slq_context();
   ctx->device = device;
   ctx->stream = static_cast<hipStream_t>(stream);
@@ -718,6 +757,60 @@ int slq_logdet(slq_context* ctx, const slq_operator* op, const float* Z, int64_t
   SLQ_TRY(set_device(ctx));
   ForwardState st;
   return logdet_forward(ctx, *op, Z, P, k, per_probe, mean, alpha, beta, st);
+}
+
+int slq_logdet_noalloc(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P,
+                       int64_t k, double* per_probe, double* mean, float* alpha, float* beta) {
+  if (!ctx) return SLQ_EINVAL;
+  SLQ_TRY(check_operator(ctx, op));
+  const int64_t n = op->n;
+  if (k < 0 || k > n) return fail(ctx, SLQ_EINVAL_K, exceeds_msg(k, n));
+  if (k == 0)
+    return fail(ctx, SLQ_EINVAL, "the log-determinant integrand needs num_matvecs >= 1");
+  if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of probes P must be positive");
+  if (!Z) return fail(ctx, SLQ_EINVAL, "probe pointer is NULL");
+  char* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (ctx->device < 0 || ctx->device >= kMaxDevices || !g_scratch[ctx->device].looked_up)
+      return fail(ctx, SLQ_EINVAL, "slq_logdet_noalloc: no static scratch on device " +
+                                       std::to_string(ctx->device));
+    const StaticScratch& st = g_scratch[ctx->device];
+    if (st.status != hipSuccess || !st.ptr)
+      return fail(ctx, SLQ_EHIP,
+                  std::string("slq_logdet_noalloc: the static scratch could not be located: ") +
+                      hipGetErrorString(st.status));
+    base = static_cast<char*>(st.ptr);
+  }
+  // The call's buffers are carved from the static scratch in the order the
+  // call takes them; a dry run (nothing launched) sizes them first, so a call
+  // that does not fit is rejected before any work is queued.
+  slq_context sizing;
+  sizing.device = ctx->device;
+  sizing.stream = ctx->stream;
+  sizing.arena_mode = true;
+  sizing.dry_run = true;
+  sizing.arena = base;
+  sizing.arena_size = SIZE_MAX / 2;
+  ForwardState st;
+  int rc = logdet_forward(&sizing, *op, Z, P, k, per_probe, mean, alpha, beta, st);
+  if (rc != SLQ_OK) return fail(ctx, rc, sizing.error);
+  if (sizing.arena_off > size_t(SLQ_NOALLOC_SCRATCH_BYTES))
+    return fail(ctx, SLQ_EINVAL,
+                "slq_logdet_noalloc: the call needs " + std::to_string(sizing.arena_off) +
+                    " bytes of scratch, more than the " +
+                    std::to_string(size_t(SLQ_NOALLOC_SCRATCH_BYTES)) +
+                    " bytes of the static scratch (use slq_logdet)");
+  slq_context run;
+  run.device = ctx->device;
+  run.stream = ctx->stream;
+  run.arena_mode = true;
+  run.arena = base;
+  run.arena_size = size_t(SLQ_NOALLOC_SCRATCH_BYTES);
+  rc = set_device(&run);
+  if (rc == SLQ_OK) rc = logdet_forward(&run, *op, Z, P, k, per_probe, mean, alpha, beta, st);
+  if (rc != SLQ_OK) return fail(ctx, rc, run.error);
+  return SLQ_OK;
 }
 
 int slq_lanczos(slq_context* ctx, const slq_operator* op, const float* V0, int64_t P, int64_t k,
