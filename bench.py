@@ -57,6 +57,7 @@ SERVE_DENSE_MAX_N = 16384
 SERVE_PROBES = (16, 32, 64, 128)
 SERVE_SLOTS = 4  # requests in flight at once in the serving stream
 SERVE_DEPTH = (16, 64)
+SERVE_GRAD_EVERY = 3  # every third request asks for the gradient too
 
 
 def parse():
@@ -71,9 +72,10 @@ def parse():
     ap.add_argument("--no-reference", action="store_true", help="skip the FP64 Cholesky check")
     ap.add_argument("--serve-requests", type=int, default=48,
                     help="requests of the serving measurement (0 skips it)")
-    ap.add_argument("--serve-load", type=float, default=1.0,
-                    help="offered load of the Poisson request stream, relative to the "
-                         "mean service rate of one request at a time")
+    ap.add_argument("--serve-rate", type=float, default=40.0,
+                    help="arrival rate of the Poisson request stream (requests/s)")
+    ap.add_argument("--serve-budget-gib", type=float, default=2.0,
+                    help="HBM budget of the serving loop for the solves' scratch (GiB)")
     return ap.parse_args()
 
 
@@ -198,18 +200,19 @@ def measured_traffic(kernel):
 
 
 def serve(args, device):
-    """Latency of a stream of independent SLQ log-determinant solves.
+    """Latency of a stream of independent SLQ log-determinant (+ gradient) solves.
 
     Each request draws its operator (RBF on fresh points, or a stored SPD
-    matrix), dimension, probe count and depth from the serving mix; all
-    operators are made resident in HBM first.  Closed loop: every request
-    (probe generation, solve, result ready on the device) is timed alone.
-    Then the same requests arrive as a Poisson stream at the given offered
-    load (relative to the mean service time alone) and run through the
-    continuous-batching loop of matfree_amd.serving (4 slots, each a stream
-    with its own library context, admitted in arrival order by HBM
-    footprint): latency = completion - arrival.  Each request's per-probe
-    results in the mix are compared with its results alone (bit for bit
+    matrix), dimension, probe count and depth from the serving mix, and every
+    third request asks for the gradient too (value and d/d parameters through
+    the reverse pass); all operators are made resident in HBM first.  Closed
+    loop: every request (probe generation, solve, result ready on the device)
+    is timed alone.  Then the same requests arrive as a Poisson stream at a
+    fixed rate (requests/s) and run through the continuous-batching loop of
+    matfree_amd.serving (4 slots, each a stream with its own library context,
+    admitted in arrival order within an HBM budget for the solves' scratch):
+    latency = completion - arrival.  Each request's per-probe results (and
+    gradients) in the mix are compared with its results alone (bit for bit
     expected), and the same arrivals are replayed through one FIFO server with
     the closed-loop service times for comparison.
     """
@@ -231,8 +234,9 @@ def serve(args, device):
         else:
             A = problems.dense_spd_device(n, seed=1000 + r, shift=0.1, device=device)
             op, params = operators.DenseMatvec(precision="fp32"), (A,)
-        reqs.append(serving.Request(op, params, n, P, k, seed=r))
-        meta.append((kind, n, P, k))
+        grad = r % SERVE_GRAD_EVERY == SERVE_GRAD_EVERY - 1
+        reqs.append(serving.Request(op, params, n, P, k, seed=r, grad=grad))
+        meta.append((kind, n, P, k, grad))
     torch.cuda.synchronize()
     lat, alone = [], []
     for q in reqs:
@@ -240,22 +244,41 @@ def serve(args, device):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         Z = sampler.flat(prng.PRNGKey(q.seed), device)
-        per, mean = funm.logdet_per_probe(q.matvec, Z, q.k, q.params)
+        grads = None
+        if q.grad:
+            per, mean, grads = funm.logdet_value_and_grad(q.matvec, Z, q.k, q.params)
+        else:
+            per, mean = funm.logdet_per_probe(q.matvec, Z, q.k, q.params)
         torch.cuda.synchronize()
         lat.append((time.perf_counter() - t0) * 1e3)
-        alone.append(per.cpu().numpy())
+        alone.append((per.cpu().numpy(), None if grads is None else
+                      [g.cpu().numpy() for g in grads]))
+        del Z, per, mean, grads
     lat = np.array(lat)
-    # Poisson arrivals at offered load rho of the mean service rate alone
-    rho = args.serve_load
-    arrive = np.cumsum(rng.exponential(lat.mean() / rho, size=len(lat))) / 1e3  # seconds
+    # Poisson arrivals at a fixed rate
+    rate = args.serve_rate
+    arrive = np.cumsum(rng.exponential(1.0 / rate, size=len(lat)))  # seconds
     mixed = [q._replace(arrival=float(a)) for q, a in zip(reqs, arrive)]
+    budget = int(args.serve_budget_gib * 2**30)
     torch.cuda.synchronize()
+    server = serving.Server(device, slots=SERVE_SLOTS, hbm_budget=budget)
     t0 = time.perf_counter()
-    out = serving.Server(device, slots=SERVE_SLOTS).run(mixed)
+    out = server.run(mixed)
     span = time.perf_counter() - t0
+    errors = [o.error for o in out if o.error is not None]
     resp = np.array([(o.finish - o.arrival) * 1e3 for o in out])
-    diff = max(float(np.max(np.abs(o.per_probe.cpu().numpy() - a) / np.abs(a))) for o, a in
-               zip(out, alone))
+
+    def rdiff(a, b):
+        a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+        return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+    diff = 0.0
+    for o, (per, grads) in zip(out, alone):
+        if o.error is None:
+            diff = max(diff, rdiff(o.per_probe.cpu().numpy(), per))
+            if grads is not None:
+                for g, gr in zip(o.grads, grads):
+                    diff = max(diff, rdiff(g.cpu().numpy(), gr))
     # the same arrivals through one FIFO server with the service times alone
     finish = np.empty_like(arrive)
     busy = 0.0
@@ -263,28 +286,36 @@ def serve(args, device):
         start = max(arrive[i] * 1e3, busy)
         finish[i] = busy = start + lat[i]
     fifo = finish - arrive * 1e3
-    del reqs, mixed, out
+    st = server.stats
+    del reqs, mixed, out, server
     torch.cuda.empty_cache()
     return {
         "requests": len(lat),
+        "gradient_requests": int(sum(m[4] for m in meta)),
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
         "mean_ms": float(lat.mean()),
         "max_ms": float(lat.max()),
         "stream": {
-            "offered_load": rho,
+            "arrival_rate_per_s": rate,
             "slots": SERVE_SLOTS,
+            "hbm_budget_bytes": budget,
             "p50_ms": float(np.percentile(resp, 50)),
             "p99_ms": float(np.percentile(resp, 99)),
             "requests_per_s": len(lat) / span,
             "probe_steps_per_s": sum(m[2] * m[3] for m in meta) / span,
             "max_rel_diff_vs_alone": diff,
+            "errors": len(errors),
+            "admissions_waited_for_memory": st.get("waited_for_memory"),
+            "scratch_releases": st.get("releases"),
+            "peak_resident_bytes": st.get("peak_resident"),
             "fifo_one_at_a_time_p50_ms": float(np.percentile(fifo, 50)),
             "fifo_one_at_a_time_p99_ms": float(np.percentile(fifo, 99)),
         },
         "mix": (f"n log-uniform [{SERVE_N[0]}, {SERVE_N[1]}], probes {list(SERVE_PROBES)}, "
                 f"depth U[{SERVE_DEPTH[0]}, {SERVE_DEPTH[1]}], RBF bf16x2 or dense fp32 "
-                f"(dense n <= {SERVE_DENSE_MAX_N}); {sum(m[0] == 'rbf' for m in meta)} RBF"),
+                f"(dense n <= {SERVE_DENSE_MAX_N}); {sum(m[0] == 'rbf' for m in meta)} RBF; "
+                f"every {SERVE_GRAD_EVERY}rd request value + gradient"),
     }
 
 
