@@ -7,9 +7,11 @@ the same probes (probes are independent: each probe's coefficients depend on
 its own start vector and the operator only).
 
 Tolerances (FP32 GPU path vs FP32 restatement; the two differ in summation
-order only):
+order and, for the RBF operator, in the last bit of the kernel entries):
   * alpha / beta: |delta| <= 1e-5 * max|alpha|;
-  * per-probe log-determinant: relative 1e-5.
+  * per-probe log-determinant: relative 1e-5 for the stored matrix (C2); for
+    the RBF operator (C3) 3x the restatement's own spread under a symmetric
+    permutation of the problem (summation order only), when that is larger.
 BF16 forms are compared with the GPU's own FP32 form per probe (tolerance in
 the test).
 """
@@ -27,7 +29,7 @@ pytestmark = pytest.mark.gpu
 SUBSET = lambda P: [0, 1, P // 2, P - 1]  # noqa: E731
 
 
-def compare(per, alpha, beta, sub, o_vals, o_alpha, o_beta, label):
+def compare(per, alpha, beta, sub, o_vals, o_alpha, o_beta, label, tol_vals=1e-5):
     g_per = per.cpu().numpy()[sub]
     g_alpha = alpha.cpu().numpy()[sub]
     g_beta = beta.cpu().numpy()[sub]
@@ -36,10 +38,10 @@ def compare(per, alpha, beta, sub, o_vals, o_alpha, o_beta, label):
     db = float(np.abs(g_beta - o_beta).max())
     dv = float(np.max(np.abs(g_per - o_vals) / np.abs(o_vals)))
     print(f"{label}: max|dalpha| {da:.3e} max|dbeta| {db:.3e} (1e-5 max|alpha| = "
-          f"{1e-5 * scale:.3e}); per-probe rel {dv:.3e}", flush=True)
+          f"{1e-5 * scale:.3e}); per-probe rel {dv:.3e} (tol {tol_vals:.3e})", flush=True)
     assert da <= 1e-5 * scale, (label, da, scale)
     assert db <= 1e-5 * scale, (label, db, scale)
-    assert dv <= 1e-5, (label, dv)
+    assert dv <= tol_vals, (label, dv)
 
 
 def test_c2_dense_fp32_matches_oracle(cuda):
@@ -88,4 +90,13 @@ def test_c3_rbf_fp32_matches_oracle_and_bf16_tracks_fp32(cuda, precision, tol, t
     sub = SUBSET(P)
     Zs = Z[sub].cpu().numpy()
     o_vals, o_alpha, o_beta = oslq.quadform_logdet(oslq.DenseOperator(Kh), k, Zs, np.float32)
-    compare(per, alpha, beta, sub, o_vals, o_alpha, o_beta, "C3 fp32")
+    # At this conditioning (eigenvalues from 1e-2 to ~1e4) a single probe's
+    # quadrature moves by ~1e-4 under any FP32-level change of the process:
+    # the oracle itself, run on a symmetric permutation of K and the probes
+    # (summation order only), is the yardstick for the per-probe values
+    perm = np.random.default_rng(1).permutation(n)
+    p_vals, _, _ = oslq.quadform_logdet(oslq.DenseOperator(Kh[perm][:, perm]), k, Zs[:, perm],
+                                        np.float32)
+    spread = float(np.max(np.abs(p_vals - o_vals) / np.abs(o_vals)))
+    compare(per, alpha, beta, sub, o_vals, o_alpha, o_beta, "C3 fp32",
+            tol_vals=max(1e-5, 3.0 * spread))
