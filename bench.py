@@ -44,11 +44,15 @@ WORKLOADS = {
     "dense": dict(n=16384, probes=256, depth=64, shift=0.1),
     "rbf-grad": dict(n=8192, probes=32, depth=32, lengthscale=0.5, noise=1e-2),
     "rbf-none": dict(n=32768, probes=128, depth=64, lengthscale=0.5, noise=1e-2),
+    "rbf-stored": dict(n=32768, probes=128, depth=64, lengthscale=0.5, noise=1e-2),
     "rbf-funm": dict(n=32768, probes=128, depth=64, lengthscale=0.5, noise=1e-2),
     "gp-logpdf": dict(n=8192, probes=32, depth=32, lengthscale=0.5, noise=1e-2, rank=128),
 }
 DEFAULT_PRECISION = {"rbf": "bf16x2", "dense": "fp32", "rbf-grad": "fp32", "rbf-none": "bf16x2",
-                     "rbf-funm": "bf16x2", "gp-logpdf": "fp32"}
+                     "rbf-funm": "bf16x2", "gp-logpdf": "fp32", "rbf-stored": "fp32"}
+# packed storage of the stored matrix (dense, rbf-stored): fp32 = the plain
+# FP32 matrix; bf16x2 / bf16 / fp8x2 / fp8 = operators.pack forms
+DEFAULT_STORAGE = {"dense": "fp32", "rbf-stored": "fp8x2"}
 # Serving mix: n log-uniform in [2048, 32768], probes in {16, 32, 64, 128},
 # depth uniform in [16, 64], RBF (bf16x2) or stored dense (fp32, n capped at
 # 16384 to bound the per-request matrix at 1 GB).
@@ -66,7 +70,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="rbf")
-    ap.add_argument("--precision", default=None, help="fp32 | bf16 | bf16x3")
+    ap.add_argument("--precision", default=None, help="fp32 | bf16 | bf16x2 | bf16x3")
+    ap.add_argument("--storage", default=None,
+                    help="stored matrix (dense, rbf-stored): fp32 | bf16x2 | bf16 | fp8x2 | fp8")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-reference", action="store_true", help="skip the FP64 Cholesky check")
@@ -92,16 +98,35 @@ def setup(args, device):
     w = WORKLOADS[args.workload]
     n, P, k = w["n"], w["probes"], w["depth"]
     prec = args.precision or DEFAULT_PRECISION[args.workload]
+    storage = args.storage or DEFAULT_STORAGE.get(args.workload, "fp32")
     if args.workload in ("rbf", "rbf-none", "rbf-funm"):
         X = problems.rbf_points(n, seed=args.seed)
         op = operators.RBF(X, precision=prec)
         params = (w["lengthscale"], w["noise"])
         host = {"points": X}
     else:
-        A = problems.dense_spd_device(n, seed=args.seed, shift=w["shift"], device=device)
-        op = operators.DenseMatvec(precision=prec)
-        params = (A,)
-        host = {"matrix": A}
+        if args.workload == "rbf-stored":
+            # the RBF kernel matrix of the points, stored (FP64 entries rounded once)
+            X = problems.rbf_points(n, seed=args.seed)
+            K = problems.rbf_dense64_device(X, w["lengthscale"], w["noise"], device)
+            A = K.to(torch.float32)
+            del K
+            torch.cuda.empty_cache()
+            host = {"points": X}
+        else:
+            A = problems.dense_spd_device(n, seed=args.seed, shift=w["shift"], device=device)
+            host = {"matrix": A}
+        if storage == "fp32":
+            op = operators.DenseMatvec(precision=prec)
+            params = (A,)
+            host["matrix"] = A
+        else:
+            op = operators.packed_matvec
+            params = (operators.pack(A, storage),)
+            prec = "stored-" + storage
+            if args.workload == "rbf-stored":
+                del A
+                torch.cuda.empty_cache()
     Z = stochtrace.sampler_signs(np.zeros(n, np.float32), num=P).flat(prng.PRNGKey(args.seed),
                                                                        device)
     torch.cuda.synchronize()
@@ -114,7 +139,7 @@ def fp64_reference(args, host, device):
     from matfree_amd import problems
 
     w = WORKLOADS[args.workload]
-    if args.workload in ("rbf", "rbf-none"):
+    if args.workload in ("rbf", "rbf-none", "rbf-stored"):
         K = problems.rbf_dense64_device(host["points"], w["lengthscale"], w["noise"], device)
     else:
         K = host["matrix"].to(torch.float64)
@@ -169,6 +194,12 @@ def roofline(kernels, n, P, k, prec, workload):
         entry = {"kernel": "block_matvec_" + workload + "_" + prec, "bound": "mfma",
                  "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                  "algorithmic_per_launch": flops, "avg_launch_ms": avg_s * 1e3}
+        if prec.startswith("stored-"):
+            # the stored planes streamed once per probe block
+            per_el = {"bf16x2": 4, "bf16": 2, "fp8x2": 2, "fp8": 1}[prec[len("stored-"):]]
+            blocks = -(-P // 128)
+            entry["matrix_bytes_per_launch"] = float(per_el) * n * n * blocks
+            entry["matrix_GBs"] = entry["matrix_bytes_per_launch"] / avg_s / 1e9
     else:
         # Streaming Gram-Schmidt kernels: bytes per launch averaged over the
         # k steps (column j reads j + 1 basis vectors of n floats per probe).
@@ -542,7 +573,7 @@ def cpu_baseline(args, host, device, n, k):
 
     w = WORKLOADS[args.workload]
     probes = 96  # about 10-20 s of host work at the restatement's rate
-    if args.workload in ("rbf", "rbf-none"):
+    if args.workload in ("rbf", "rbf-none", "rbf-stored"):
         from matfree_amd import problems
 
         K = problems.rbf_dense64_device(host["points"], w["lengthscale"], w["noise"], device)

@@ -54,6 +54,36 @@ typedef enum slq_precision {
   SLQ_PREC_BF16X2 = 3  /* matrix split hi+lo, vectors BF16: hi*hi + lo*hi           */
 } slq_precision;
 
+/* Storage of a DENSE operator's matrix.  SLQ_STORE_FP32 is the plain n x n
+ * FP32 matrix (`matrix`, products in the slq_precision `precision`).  The
+ * other forms are packed once by slq_pack_matrix into `packed` (+
+ * `tile_scales`); their block matvec runs BF16 MFMAs on the stored planes
+ * with the vectors rounded to BF16 and FP32 accumulation (`precision` is
+ * ignored), reading 4 (BF16X2), 2 (BF16, FP8X2) or 1 (FP8) bytes per matrix
+ * element from HBM instead of 4 with an FP32 matrix.
+ *   BF16X2: A = hi + lo, hi = bf16(A), lo = bf16(A - hi) (~16 significant bits);
+ *   BF16:   hi only (8 bits);
+ *   FP8X2:  A = s_hi q_hi + s_lo q_lo, q = OCP FP8 E4M3 (round to nearest
+ *           even), one power-of-two scale per 32 x 32 tile and plane
+ *           (max |A_tile| / s <= 448), q_lo from the exact residual A - s_hi q_hi;
+ *   FP8:    s q only (4 significant bits).
+ * Packed layout: the matrix zero-padded to npad = 32 ceil(n / 32) in both
+ * dimensions, cut into 32 x 32 tiles, tile (rb, ct) (rows 32 rb.., columns
+ * 32 ct..) at index rb * (npad / 32) + ct; planes one after the other (hi,
+ * then lo), each npad^2 elements; within a tile the elements are in the
+ * order the MFMA fragments read them (lane l = r + 32 h, r < 32, h < 2, holds
+ * row 32 rb + r, columns 32 ct + {8 h .. 8 h + 7} and 32 ct + {16 + 8 h ..
+ * 23 + 8 h}): BF16 planes as [half (first / second column group)][lane][8],
+ * FP8 planes as [lane][16] (the first column group in bytes 0-7).
+ * tile_scales: per FP8 plane (npad / 32)^2 floats, tile index as above. */
+typedef enum slq_storage {
+  SLQ_STORE_FP32 = 0,
+  SLQ_STORE_BF16X2 = 1,
+  SLQ_STORE_BF16 = 2,
+  SLQ_STORE_FP8X2 = 3,
+  SLQ_STORE_FP8 = 4
+} slq_storage;
+
 /* Operator descriptor: the reference's matvec(v, *params) plugin
  * (decomp.py:460, user code such as tests/test_decomp/test_tridiag_sym.py:16-18
  * and tutorials/8_gaussian_logpdf.py:21-31), restricted to the two operator
@@ -62,13 +92,20 @@ typedef struct slq_operator {
   int32_t kind;        /* slq_operator_kind                                   */
   int32_t precision;   /* slq_precision of the block matvec                    */
   int64_t n;           /* dimension                                            */
-  const float* matrix; /* DENSE: n x n symmetric, row-major                    */
+  const float* matrix; /* DENSE, SLQ_STORE_FP32: n x n symmetric, row-major    */
   const float* points; /* RBF: n x dim points                                  */
   int32_t dim;         /* RBF: point dimension, 1..4                           */
-  int32_t reserved;
+  int32_t storage;     /* DENSE: slq_storage (0 for RBF)                       */
   double lengthscale;  /* RBF: l > 0                                          */
   double noise;        /* RBF: s2 (added on the diagonal in FP32)             */
+  const void* packed;  /* DENSE, packed storage: the planes (see slq_storage)  */
+  const float* tile_scales; /* DENSE, FP8 storage: the per-tile scales        */
 } slq_operator;
+
+/* Sizes of the packed form of an n x n matrix in `storage` (not FP32):
+ * *packed_bytes for `packed`, *scale_count floats for `tile_scales` (0 for
+ * the BF16 forms).  Host only. */
+int slq_packed_size(int64_t n, int storage, int64_t* packed_bytes, int64_t* scale_count);
 
 /* Context (device, stream, pooled scratch). stream is a hipStream_t or NULL. */
 typedef struct slq_context slq_context;
@@ -83,6 +120,14 @@ int64_t slq_context_scratch_bytes(const slq_context* ctx);
  * buffers): for a host that admits work by device-memory footprint and must
  * give an idle context's memory back. */
 int slq_context_release(slq_context* ctx);
+
+/* Pack the n x n FP32 matrix A (device, row-major) into `storage` (see
+ * slq_storage; buffers of the sizes slq_packed_size gives, device).  No
+ * counterpart in the reference (its operator is a user function): the
+ * low-precision stored-matrix variant of the operator plugin of
+ * decomp.py:460. */
+int slq_pack_matrix(slq_context* ctx, const float* A, int64_t n, int storage, void* packed,
+                    float* tile_scales);
 
 /* Rademacher probes, bit-identical to jax.random.rademacher(key, (num, n),
  * float32) with the partitionable threefry2x32 layout.

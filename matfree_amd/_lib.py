@@ -26,6 +26,8 @@ NOALLOC_SCRATCH_BYTES = 256 << 20
 SLQ_OP_DENSE = 0
 SLQ_OP_RBF = 1
 PRECISIONS = {"fp32": 0, "bf16": 1, "bf16x3": 2, "bf16x2": 3}
+# packed storage of a stored matrix (slq_storage)
+STORAGES = {"fp32": 0, "bf16x2": 1, "bf16": 2, "fp8x2": 3, "fp8": 4}
 # matrix functions of slq_funm_vectors (slq_matfun)
 MATFUNS = {"log": 0, "exp": 1, "sin": 2, "cos": 3, "sqrt": 4, "reciprocal": 5}
 
@@ -45,9 +47,11 @@ class OperatorStruct(ctypes.Structure):
         ("matrix", c_void_p),
         ("points", c_void_p),
         ("dim", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("storage", ctypes.c_int32),
         ("lengthscale", ctypes.c_double),
         ("noise", ctypes.c_double),
+        ("packed", c_void_p),
+        ("tile_scales", c_void_p),
     ]
 
 
@@ -58,6 +62,8 @@ SIGNATURES = {
     "slq_context_set_stream": (c_int, [c_void_p, c_void_p]),
     "slq_last_error": (ctypes.c_char_p, [c_void_p]),
     "slq_context_scratch_bytes": (c_i64, [c_void_p]),
+    "slq_packed_size": (c_int, [c_i64, c_int, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)]),
+    "slq_pack_matrix": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_void_p, c_void_p]),
     "slq_sampler_signs": (c_int, [c_void_p, c_u32, c_u32, c_i64, c_i64, c_void_p]),
     "slq_matvec": (c_int, [c_void_p, ctypes.POINTER(OperatorStruct), c_void_p, c_i64, c_void_p]),
     "slq_hessenberg": (

@@ -660,6 +660,273 @@ __global__ void __launch_bounds__(kThreads)
   store_out<NACC, VEC, false>(Wout, V, P, n, i0w, p0, acc, 0.f);
 }
 
+// ---------------------------------------------------------------------------
+// Packed storage of a stored matrix (slq_storage in slq.h): 32 x 32 tiles of
+// the zero-padded matrix in MFMA fragment order, so that the 64 lanes of a
+// wave read a tile plane as contiguous 1 KB blocks (lane l = r + 32 h holds
+// row r and the 16 contraction slots frag_k(s, h) of that row).
+//   BF16 planes: 2 KB per tile, [half kb][lane][8 bf16] (slots 8 kb .. 8 kb + 7);
+//   FP8 planes:  1 KB per tile, [lane][16 bytes] (byte s = slot s), OCP FP8
+//                E4M3 with one power-of-two scale per tile and plane.
+// ---------------------------------------------------------------------------
+constexpr int64_t kTileBf16Bytes = 2048;
+constexpr int64_t kTileFp8Bytes = 1024;
+
+constexpr __host__ __device__ int storage_planes(int storage) {
+  return (storage == SLQ_STORE_BF16X2 || storage == SLQ_STORE_FP8X2) ? 2 : 1;
+}
+constexpr __host__ __device__ bool storage_fp8(int storage) {
+  return storage == SLQ_STORE_FP8X2 || storage == SLQ_STORE_FP8;
+}
+
+// Exponent k of the power-of-two tile scale: the smallest k with
+// m 2^-k <= 448 (the largest finite E4M3 value).  m = f 2^e, f in [0.5, 1):
+// m 2^-(e - 9) = f 2^9 <= 448 iff f <= 0.875.
+__device__ __forceinline__ int fp8_scale_exp(float m) {
+  if (!(m > 0.f)) return 0;
+  int e;
+  const float f = frexpf(m, &e);
+  const int k = f <= 0.875f ? e - 9 : e - 8;
+  return k < -120 ? -120 : k;
+}
+
+__device__ __forceinline__ float wave_max_f32(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// 16 values -> 16 E4M3 bytes of v 2^-k (round to nearest even), byte s =
+// value s; and back to floats (times 2^k).
+__device__ __forceinline__ u32x4 fp8_quantize16(const float (&v)[16], int k) {
+  u32x4 q;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    int word = __builtin_amdgcn_cvt_pk_fp8_f32(ldexpf(v[4 * w + 0], -k), ldexpf(v[4 * w + 1], -k),
+                                               0, false);
+    word = __builtin_amdgcn_cvt_pk_fp8_f32(ldexpf(v[4 * w + 2], -k), ldexpf(v[4 * w + 3], -k),
+                                           word, true);
+    q[w] = uint32_t(word);
+  }
+  return q;
+}
+
+__device__ __forceinline__ void fp8_dequantize16(const u32x4& q, float scale, float (&v)[16]) {
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const f32x2 lo = __builtin_amdgcn_cvt_pk_f32_fp8(int(q[w]), false);
+    const f32x2 hi = __builtin_amdgcn_cvt_pk_f32_fp8(int(q[w]), true);
+    v[4 * w + 0] = lo[0] * scale;
+    v[4 * w + 1] = lo[1] * scale;
+    v[4 * w + 2] = hi[0] * scale;
+    v[4 * w + 3] = hi[1] * scale;
+  }
+}
+
+// One wave per tile: reads the tile's 1024 FP32 entries (zero past n) and
+// writes its planes (and scales).
+template <int STORAGE>
+__global__ void __launch_bounds__(256)
+    k_pack_matrix(const float* __restrict__ A, int64_t n, int64_t nt, void* __restrict__ packed,
+                  float* __restrict__ scales) {
+  const int lane = threadIdx.x & 63;
+  const int64_t tile = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (tile >= nt * nt) return;  // the whole wave
+  const int64_t rb = tile / nt, ct = tile - rb * nt;
+  const int r = lane & 31, h = lane >> 5;
+  const int64_t row = rb * BK + r;
+  float a[16];
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int64_t col = ct * BK + frag_k(s, h);
+    a[s] = (row < n && col < n) ? A[row * n + col] : 0.f;
+  }
+  const int64_t plane_tiles = nt * nt;
+  if constexpr (!storage_fp8(STORAGE)) {
+    __bf16* hi = static_cast<__bf16*>(packed);
+    __bf16* lo = hi + plane_tiles * (kTileBf16Bytes / 2);
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+      u32x4 hv, lv;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        uint32_t hw, lw;
+        split_pair(a[8 * kb + 2 * e], a[8 * kb + 2 * e + 1], hw, lw);
+        hv[e] = hw;
+        lv[e] = lw;
+      }
+      const int64_t off = tile * (kTileBf16Bytes / 2) + kb * 512 + lane * 8;
+      *reinterpret_cast<u32x4*>(hi + off) = hv;
+      if (STORAGE == SLQ_STORE_BF16X2) *reinterpret_cast<u32x4*>(lo + off) = lv;
+    }
+  } else {
+    uint8_t* q = static_cast<uint8_t*>(packed);
+    float m = 0.f;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) m = fmaxf(m, fabsf(a[s]));
+    const int k = fp8_scale_exp(wave_max_f32(m));
+    const u32x4 qh = fp8_quantize16(a, k);
+    *reinterpret_cast<u32x4*>(q + tile * kTileFp8Bytes + lane * 16) = qh;
+    if (lane == 0) scales[tile] = ldexpf(1.f, k);
+    if (STORAGE == SLQ_STORE_FP8X2) {
+      // the residual a - s q is exact (both within a factor 2 of each other,
+      // or q = 0)
+      float d[16];
+      fp8_dequantize16(qh, ldexpf(1.f, k), d);
+      float res[16], ml = 0.f;
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        res[s] = a[s] - d[s];
+        ml = fmaxf(ml, fabsf(res[s]));
+      }
+      const int kl = fp8_scale_exp(wave_max_f32(ml));
+      *reinterpret_cast<u32x4*>(q + plane_tiles * kTileFp8Bytes + tile * kTileFp8Bytes +
+                                lane * 16) = fp8_quantize16(res, kl);
+      if (lane == 0) scales[plane_tiles + tile] = ldexpf(1.f, kl);
+    }
+  }
+}
+
+// The lane's BF16 fragment (two halves) of one tile plane, from its raw
+// packed form: BF16 words as stored, or E4M3 bytes times the tile scale
+// (exact in BF16: 4 significant bits).
+template <bool FP8>
+struct PackedRaw;
+template <>
+struct PackedRaw<false> {
+  u32x4 w[2];
+  __device__ __forceinline__ void load(const uint8_t* tile_base, int lane) {
+    const __bf16* b = reinterpret_cast<const __bf16*>(tile_base);
+    w[0] = *reinterpret_cast<const u32x4*>(b + lane * 8);
+    w[1] = *reinterpret_cast<const u32x4*>(b + 512 + lane * 8);
+  }
+  __device__ __forceinline__ void zero() { w[0] = w[1] = u32x4{0u, 0u, 0u, 0u}; }
+  __device__ __forceinline__ void to_frag(float, u32x4 (&f)[2]) const {
+    f[0] = w[0];
+    f[1] = w[1];
+  }
+};
+template <>
+struct PackedRaw<true> {
+  u32x4 w;
+  __device__ __forceinline__ void load(const uint8_t* tile_base, int lane) {
+    w = *reinterpret_cast<const u32x4*>(tile_base + lane * 16);
+  }
+  __device__ __forceinline__ void zero() { w = u32x4{0u, 0u, 0u, 0u}; }
+  __device__ __forceinline__ void to_frag(float scale, u32x4 (&f)[2]) const {
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int src = int(w[2 * kb + (e >> 1)]);
+        f32x2 x = (e & 1) ? __builtin_amdgcn_cvt_pk_f32_fp8(src, true)
+                          : __builtin_amdgcn_cvt_pk_f32_fp8(src, false);
+        x[0] *= scale;
+        x[1] *= scale;
+        f[kb][e] = __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2));
+      }
+  }
+};
+
+// Stored-matrix matvec on packed storage: each wave streams its row block's
+// tile planes (DEPTH tiles in flight, in registers), forms the BF16
+// fragments (FP8: converted with the tile scale) and runs the BF16 MFMAs of
+// tile_mfmas (NPL = 2: lo*hi + hi*hi as bf16x2) against the BF16 plane of the
+// probes, staged through LDS as in k_matvec_bf16_dense.
+template <int NACC, bool VEC, bool FP8, int NPL, int DEPTH>
+__global__ void __launch_bounds__(kThreads)
+    k_matvec_packed(const uint8_t* __restrict__ packed, const float* __restrict__ scales,
+                    int64_t nt, const __bf16* __restrict__ Vhi_g, int64_t Ppad,
+                    const float* __restrict__ V, int64_t P, int64_t n, float* __restrict__ W) {
+  constexpr int BPW = 32 * NACC;
+  constexpr int PT = PlaneTile<NACC>::kPerThread;
+  constexpr int64_t TB = FP8 ? kTileFp8Bytes : kTileBf16Bytes;
+  __shared__ __attribute__((aligned(16))) __bf16 Vhi[2][BPW * LDB];
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6, r = lane & 31, h = lane >> 5;
+  const int64_t i0w = int64_t(blockIdx.x) * BI + 32 * wave;
+  const int64_t rb = i0w / BK;
+  const bool rows_live = rb < nt;  // a wave past the last row block reads nothing
+  const int64_t p0 = int64_t(blockIdx.y) * BPW;
+  const int64_t kt_begin = nt * blockIdx.z / gridDim.z;
+  const int64_t kt_end = nt * (blockIdx.z + 1) / gridDim.z;
+  const int64_t ntiles = kt_end - kt_begin;
+  const bool split = gridDim.z > 1;
+  float* Wout = split ? W + int64_t(blockIdx.z) * P * n : W;
+  const int64_t plane_bytes = nt * nt * TB;
+  const uint8_t* base = packed + ((rows_live ? rb : 0) * nt + kt_begin) * TB;
+  const float* sbase = scales ? scales + (rows_live ? rb : 0) * nt + kt_begin : nullptr;
+
+  floatx16 acc[NACC];
+#pragma unroll
+  for (int c = 0; c < NACC; ++c)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[c][e] = 0.f;
+
+  PackedRaw<FP8> raw[DEPTH][NPL];
+  float rsc[DEPTH][NPL];
+  auto load_tile = [&](int64_t kt, PackedRaw<FP8> (&rw)[NPL], float (&sc)[NPL]) {
+#pragma unroll
+    for (int pl = 0; pl < NPL; ++pl) {
+      if (rows_live && kt < ntiles) {
+        rw[pl].load(base + pl * plane_bytes + kt * TB, lane);
+        sc[pl] = FP8 ? sbase[pl * nt * nt + kt] : 1.f;
+      } else {
+        rw[pl].zero();
+        sc[pl] = 0.f;
+      }
+    }
+  };
+#pragma unroll
+  for (int d = 0; d < DEPTH; ++d) load_tile(d, raw[d], rsc[d]);
+
+  u32x4 hA[PT], hB[PT];
+  load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin, hA);
+  load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin + (ntiles > 1 ? 1 : 0), hB);
+  store_plane_tile<NACC>(Vhi[0], hA);
+  __syncthreads();
+
+  // step kt uses ring slot d = kt % DEPTH and the probe tile in LDS buffer
+  // kt & 1; (ih, sh): plane registers loaded for kt + 2 / staged for kt + 1
+  auto step = [&](int64_t kt, PackedRaw<FP8> (&rw)[NPL], float (&sc)[NPL], u32x4 (&ih)[PT],
+                  const u32x4 (&sh)[PT]) {
+    const int buf = int(kt & 1);
+    u32x4 ahi[2], alo[2];
+    rw[0].to_frag(sc[0], ahi);
+    if constexpr (NPL == 2) rw[1].to_frag(sc[1], alo);
+    load_tile(kt + DEPTH, rw, sc);
+    const int64_t kt2 = (kt + 2 < ntiles) ? kt + 2 : ntiles - 1;
+    load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin + kt2, ih);
+    tile_mfmas<NACC, NPL == 2 ? 2 : 1>(Vhi[buf] + r * LDB + 8 * h, Vhi[buf] + r * LDB + 8 * h,
+                                        ahi, alo, acc);
+    store_plane_tile<NACC>(Vhi[buf ^ 1], sh);
+    __syncthreads();
+  };
+
+  // the ring slot and the plane register set of a step are fixed by kt mod
+  // (DEPTH * 2 / gcd) -- unroll by 2 DEPTH so both stay static
+  int64_t kt = 0;
+  for (; kt + 2 * DEPTH <= ntiles; kt += 2 * DEPTH) {
+#pragma unroll
+    for (int u = 0; u < 2 * DEPTH; ++u) {
+      if (u % 2 == 0)
+        step(kt + u, raw[u % DEPTH], rsc[u % DEPTH], hA, hB);
+      else
+        step(kt + u, raw[u % DEPTH], rsc[u % DEPTH], hB, hA);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 2 * DEPTH; ++u) {
+    if (kt + u < ntiles) {
+      if (u % 2 == 0)
+        step(kt + u, raw[u % DEPTH], rsc[u % DEPTH], hA, hB);
+      else
+        step(kt + u, raw[u % DEPTH], rsc[u % DEPTH], hB, hA);
+    }
+  }
+  store_out<NACC, VEC, false>(Wout, V, P, n, i0w, p0, acc, 0.f);
+}
+
 // RBF BF16 matvec.  Per tile and wave: two BF16 MFMAs give the exponents of
 // the NEXT tile, the 2 NACC NPROD BF16 MFMAs of this tile follow, and the
 // exp2 / split of the next tile's fragment is interleaved between them.  The
@@ -1073,6 +1340,53 @@ hipError_t launch_f32(hipStream_t s, const float* A, const RBFParams& prm, const
 
 int nacc_for(int64_t P) { return P <= 32 ? 1 : (P <= 64 ? 2 : 4); }
 
+// Ring depth of the packed matvec: 8 KB of tile planes in flight per wave
+// (BF16 planes 2 KB per tile, FP8 planes 1 KB).
+template <int NACC, bool FP8, int NPL>
+hipError_t launch_packed_n(hipStream_t s, const uint8_t* packed, const float* scales, int64_t nt,
+                           const __bf16* Vhi, int64_t Ppad, const float* V, int64_t P, int64_t n,
+                           float* W, bool vec, int S) {
+  constexpr int DEPTH = FP8 ? (NPL == 2 ? 4 : 8) : (NPL == 2 ? 2 : 4);
+  const dim3 grid(unsigned((n + BI - 1) / BI), unsigned((P + 32 * NACC - 1) / (32 * NACC)),
+                  unsigned(S));
+  if (vec)
+    hipLaunchKernelGGL((k_matvec_packed<NACC, true, FP8, NPL, DEPTH>), grid, dim3(kThreads), 0, s,
+                       packed, scales, nt, Vhi, Ppad, V, P, n, W);
+  else
+    hipLaunchKernelGGL((k_matvec_packed<NACC, false, FP8, NPL, DEPTH>), grid, dim3(kThreads), 0,
+                       s, packed, scales, nt, Vhi, Ppad, V, P, n, W);
+  return hipGetLastError();
+}
+
+template <int NACC>
+hipError_t launch_packed_acc(hipStream_t s, int storage, const uint8_t* packed,
+                             const float* scales, int64_t nt, const __bf16* Vhi, int64_t Ppad,
+                             const float* V, int64_t P, int64_t n, float* W, bool vec, int S) {
+  switch (storage) {
+    case SLQ_STORE_BF16X2:
+      return launch_packed_n<NACC, false, 2>(s, packed, scales, nt, Vhi, Ppad, V, P, n, W, vec, S);
+    case SLQ_STORE_BF16:
+      return launch_packed_n<NACC, false, 1>(s, packed, scales, nt, Vhi, Ppad, V, P, n, W, vec, S);
+    case SLQ_STORE_FP8X2:
+      return launch_packed_n<NACC, true, 2>(s, packed, scales, nt, Vhi, Ppad, V, P, n, W, vec, S);
+    default:
+      return launch_packed_n<NACC, true, 1>(s, packed, scales, nt, Vhi, Ppad, V, P, n, W, vec, S);
+  }
+}
+
+hipError_t launch_packed(hipStream_t s, int storage, const uint8_t* packed, const float* scales,
+                         int64_t nt, const __bf16* Vhi, int64_t Ppad, const float* V, int64_t P,
+                         int64_t n, float* W, bool vec, int S) {
+  switch (nacc_for(P)) {
+    case 1:
+      return launch_packed_acc<1>(s, storage, packed, scales, nt, Vhi, Ppad, V, P, n, W, vec, S);
+    case 2:
+      return launch_packed_acc<2>(s, storage, packed, scales, nt, Vhi, Ppad, V, P, n, W, vec, S);
+    default:
+      return launch_packed_acc<4>(s, storage, packed, scales, nt, Vhi, Ppad, V, P, n, W, vec, S);
+  }
+}
+
 template <bool RBF>
 hipError_t launch_bf16(hipStream_t s, const float* A, const u32x4* rec, const RBFParams& prm,
                        const __bf16* Vhi, const __bf16* Vlo, int64_t Ppad, const float* V,
@@ -1102,6 +1416,43 @@ int choose_splitk(int64_t n, int64_t P, int64_t avail_floats) {
 
 }  // namespace
 
+bool packed_size(int64_t n, int storage, int64_t* bytes, int64_t* scale_count) {
+  if (n <= 0 || storage <= SLQ_STORE_FP32 || storage > SLQ_STORE_FP8) return false;
+  const int64_t nt = (n + BK - 1) / BK;
+  const int planes = storage_planes(storage);
+  const bool fp8 = storage_fp8(storage);
+  *bytes = planes * nt * nt * (fp8 ? kTileFp8Bytes : kTileBf16Bytes);
+  *scale_count = fp8 ? planes * nt * nt : 0;
+  return true;
+}
+
+hipError_t launch_pack_matrix(hipStream_t s, const float* A, int64_t n, int storage, void* packed,
+                              float* scales) {
+  const int64_t nt = (n + BK - 1) / BK;
+  const dim3 grid(unsigned((nt * nt + 3) / 4));
+  switch (storage) {
+    case SLQ_STORE_BF16X2:
+      hipLaunchKernelGGL(k_pack_matrix<SLQ_STORE_BF16X2>, grid, dim3(256), 0, s, A, n, nt, packed,
+                         scales);
+      break;
+    case SLQ_STORE_BF16:
+      hipLaunchKernelGGL(k_pack_matrix<SLQ_STORE_BF16>, grid, dim3(256), 0, s, A, n, nt, packed,
+                         scales);
+      break;
+    case SLQ_STORE_FP8X2:
+      hipLaunchKernelGGL(k_pack_matrix<SLQ_STORE_FP8X2>, grid, dim3(256), 0, s, A, n, nt, packed,
+                         scales);
+      break;
+    case SLQ_STORE_FP8:
+      hipLaunchKernelGGL(k_pack_matrix<SLQ_STORE_FP8>, grid, dim3(256), 0, s, A, n, nt, packed,
+                         scales);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 int64_t matvec_scratch_floats(int64_t n, int64_t P) {
   const int64_t bpw = 32 * nacc_for(P);
   const int64_t Ppad = (P + bpw - 1) / bpw * bpw;
@@ -1113,12 +1464,13 @@ int64_t matvec_scratch_floats(int64_t n, int64_t P) {
 PlaneOut matvec_planes(const slq_operator& op, int64_t P, float* scratch) {
   PlaneOut out;
   const bool rows = op.kind == SLQ_OP_RBF && P <= kRowsMaxP && op.dim >= 1 && op.dim <= 4;
-  if (rows || op.precision == SLQ_PREC_FP32 || !scratch) return out;
+  const bool packed = op.kind == SLQ_OP_DENSE && op.storage != SLQ_STORE_FP32;
+  if (rows || (op.precision == SLQ_PREC_FP32 && !packed) || !scratch) return out;
   const int64_t npad = (op.n + BK - 1) / BK * BK;
   const int64_t bpw = 32 * nacc_for(P);
   out.Ppad = (P + bpw - 1) / bpw * bpw;
   out.hi = reinterpret_cast<__bf16*>(scratch);
-  out.lo = op.precision == SLQ_PREC_BF16X3 ? out.hi + out.Ppad * npad : nullptr;
+  out.lo = (!packed && op.precision == SLQ_PREC_BF16X3) ? out.hi + out.Ppad * npad : nullptr;
   out.perm = op.kind == SLQ_OP_RBF ? 1 : 0;
   return out;
 }
@@ -1151,8 +1503,12 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
     return hipGetLastError();
   }
   const bool vec = (n % 4 == 0) && aligned16(V) && aligned16(W);
-  const bool fp32 = op.precision == SLQ_PREC_FP32;
-  const int nprod = op.precision == SLQ_PREC_BF16X3 ? 3 : (op.precision == SLQ_PREC_BF16X2 ? 2 : 1);
+  const bool packed = op.kind == SLQ_OP_DENSE && op.storage != SLQ_STORE_FP32;
+  const bool fp32 = op.precision == SLQ_PREC_FP32 && !packed;
+  const int nprod = packed ? 1
+                           : (op.precision == SLQ_PREC_BF16X3
+                                  ? 3
+                                  : (op.precision == SLQ_PREC_BF16X2 ? 2 : 1));
   // scratch layout: [BF16 planes, RBF point records (bf16 path only)]
   // [split-K partial sums]
   int64_t planes_floats = 0;
@@ -1196,7 +1552,10 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
     if (err != hipSuccess) return err;
   }
   if (ev0) (void)hipEventRecord(ev0, s);
-  if (op.kind == SLQ_OP_DENSE) {
+  if (packed) {
+    err = launch_packed(s, op.storage, static_cast<const uint8_t*>(op.packed), op.tile_scales,
+                        npad / BK, Vhi, Ppad, V, P, n, out, vec, S);
+  } else if (op.kind == SLQ_OP_DENSE) {
     const bool v2 = vec && aligned16(op.matrix);
     err = fp32 ? launch_f32<false, 1>(s, op.matrix, prm, V, P, n, out, v2, S)
                : launch_bf16<false>(s, op.matrix, nullptr, prm, Vhi, Vlo, Ppad, V, P, n, out, v2,

@@ -191,7 +191,15 @@ int check_operator(slq_context* ctx, const slq_operator* op) {
     return fail(ctx, SLQ_EUNSUPPORTED,
                 "precision " + std::to_string(op->precision) + " is not available in this build");
   if (op->kind == SLQ_OP_DENSE) {
-    if (!op->matrix) return fail(ctx, SLQ_EINVAL, "dense operator without a matrix");
+    if (op->storage == SLQ_STORE_FP32) {
+      if (!op->matrix) return fail(ctx, SLQ_EINVAL, "dense operator without a matrix");
+    } else if (op->storage >= SLQ_STORE_BF16X2 && op->storage <= SLQ_STORE_FP8) {
+      if (!op->packed) return fail(ctx, SLQ_EINVAL, "packed dense operator without its planes");
+      if ((op->storage == SLQ_STORE_FP8 || op->storage == SLQ_STORE_FP8X2) && !op->tile_scales)
+        return fail(ctx, SLQ_EINVAL, "FP8 dense operator without its tile scales");
+    } else {
+      return fail(ctx, SLQ_EINVAL, "unknown matrix storage " + std::to_string(op->storage));
+    }
   } else if (op->kind == SLQ_OP_RBF) {
     if (!op->points) return fail(ctx, SLQ_EINVAL, "RBF operator without points");
     if (op->dim < 1 || op->dim > 4)
@@ -709,6 +717,27 @@ int64_t slq_context_scratch_bytes(const slq_context* ctx) {
   return total;
 }
 
+int slq_packed_size(int64_t n, int storage, int64_t* packed_bytes, int64_t* scale_count) {
+  if (!packed_bytes || !scale_count) return SLQ_EINVAL;
+  *packed_bytes = 0;
+  *scale_count = 0;
+  return packed_size(n, storage, packed_bytes, scale_count) ? SLQ_OK : SLQ_EINVAL;
+}
+
+int slq_pack_matrix(slq_context* ctx, const float* A, int64_t n, int storage, void* packed,
+                    float* tile_scales) {
+  if (!ctx) return SLQ_EINVAL;
+  int64_t bytes = 0, scales = 0;
+  if (!packed_size(n, storage, &bytes, &scales))
+    return fail(ctx, SLQ_EINVAL, "slq_pack_matrix: n must be positive and storage one of the "
+                                 "packed forms (not SLQ_STORE_FP32)");
+  if (!A || !packed || (scales > 0 && !tile_scales))
+    return fail(ctx, SLQ_EINVAL, "input or output pointer is NULL");
+  SLQ_TRY(set_device(ctx));
+  SLQ_HIP(ctx, launch_pack_matrix(ctx->stream, A, n, storage, packed, tile_scales));
+  return SLQ_OK;
+}
+
 int slq_sampler_signs(slq_context* ctx, uint32_t key0, uint32_t key1, int64_t num, int64_t n,
                       float* out) {
   if (!ctx) return SLQ_EINVAL;
@@ -929,6 +958,9 @@ int slq_cholesky_partial(slq_context* ctx, const slq_operator* op, int64_t rank,
                          float* L, int64_t* perm, int* success) {
   if (!ctx) return SLQ_EINVAL;
   SLQ_TRY(check_operator(ctx, op));
+  if (op->kind == SLQ_OP_DENSE && op->storage != SLQ_STORE_FP32)
+    return fail(ctx, SLQ_EUNSUPPORTED,
+                "the partial Cholesky factorisation reads matrix entries: it needs FP32 storage");
   const int64_t n = op->n;
   SLQ_TRY(check_rank(ctx, rank, n));
   if (!L || !success) return fail(ctx, SLQ_EINVAL, "output pointer is NULL");

@@ -144,6 +144,11 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
                          MvSplitSum* defer = nullptr);
 // The planes launch_matvec reads for (op, P) in scratch.
 PlaneOut matvec_planes(const slq_operator& op, int64_t P, float* scratch);
+// Packed storage of a stored matrix (slq_storage): sizes (false for FP32 or
+// an unknown storage) and the packing kernel.
+bool packed_size(int64_t n, int storage, int64_t* bytes, int64_t* scale_count);
+hipError_t launch_pack_matrix(hipStream_t s, const float* A, int64_t n, int storage, void* packed,
+                              float* scales);
 
 // ---- tridiagonal eigen + quadrature (k_tridiag.hip) ------------------------
 // For each probe p: T = (H + H^T)/2 restricted to its tridiagonal; returns

@@ -11,7 +11,12 @@ callable is rejected with ``TypeError`` (there is no second code path).
   with ``K_ij = exp(-|x_i - x_j|^2 / (2 l^2))``, the covariance of
   tutorials/8_gaussian_logpdf.py:21-31, generated tile by tile on the fly.
 
-Both are callables: ``op(v, *params)`` applies the block matvec to a vector
+* :data:`packed_matvec` -- ``matvec(v, M) = M @ v`` for a stored matrix
+  packed once by :func:`pack` into a low-precision form (BF16 hi/lo planes,
+  BF16, FP8 E4M3 with per-tile scales, or two FP8 planes): the stored-matrix
+  variant whose block matvec reads 1-4 bytes per entry instead of 4.
+
+All are callables: ``op(v, *params)`` applies the block matvec to a vector
 ``(n,)`` or a batch ``(P, n)`` and returns a float32 tensor on the GPU.
 """
 
@@ -119,11 +124,89 @@ class RBF(Operator):
         return d, X
 
 
+class PackedMatrix:
+    """A stored symmetric ``n x n`` matrix in a packed form of ``slq_storage``.
+
+    ``storage``: ``"bf16x2"`` (A = hi + lo, two BF16 planes, ~16 significant
+    bits), ``"bf16"`` (hi only), ``"fp8x2"`` (two OCP FP8 E4M3 planes with one
+    power-of-two scale per 32 x 32 tile and plane, ~8 bits) or ``"fp8"`` (one
+    plane, 4 bits).  ``packed`` (uint8) and ``scales`` (float32, FP8 forms)
+    are device tensors in the layout documented in include/slq.h."""
+
+    def __init__(self, A, storage="bf16x2"):
+        if storage not in _lib.STORAGES or storage == "fp32":
+            raise ValueError(f"storage must be one of "
+                             f"{sorted(k for k in _lib.STORAGES if k != 'fp32')}, got {storage!r}")
+        device = _runtime.target_device(A)
+        A32 = _runtime.as_f32(A, device)
+        if A32.ndim != 2 or A32.shape[0] != A32.shape[1]:
+            raise ValueError(f"expected a square matrix, got shape {tuple(A32.shape)}")
+        n = int(A32.shape[0])
+        code = _lib.STORAGES[storage]
+        nbytes, nscales = ctypes.c_int64(), ctypes.c_int64()
+        rc = _lib.load().slq_packed_size(n, code, ctypes.byref(nbytes), ctypes.byref(nscales))
+        if rc != _lib.SLQ_OK:
+            raise ValueError(f"cannot pack an {n} x {n} matrix as {storage}")
+        self.n = n
+        self.storage = storage
+        self.device = device
+        self.packed = torch.empty((nbytes.value,), dtype=torch.uint8, device=device)
+        self.scales = (torch.empty((nscales.value,), dtype=torch.float32, device=device)
+                       if nscales.value else None)
+        ctx = _runtime.context(device)
+        ctx.call("slq_pack_matrix", _runtime.ptr(A32), n, code, _runtime.ptr(self.packed),
+                 _runtime.ptr(self.scales))
+
+    def nbytes(self):
+        return self.packed.numel() + (4 * self.scales.numel() if self.scales is not None else 0)
+
+
+def pack(A, storage="bf16x2"):
+    """Pack a stored symmetric matrix for :data:`packed_matvec`."""
+    return PackedMatrix(A, storage)
+
+
+class PackedDense(Operator):
+    """``matvec(v, M) = M @ v`` for a :class:`PackedMatrix` ``M`` (the
+    parameter): BF16 MFMAs on the stored planes, vectors rounded to BF16,
+    FP32 accumulation."""
+
+    kind = _lib.SLQ_OP_DENSE
+    num_params = 1
+
+    def descriptor(self, n, params, device):
+        if len(params) != 1:
+            raise TypeError(f"packed_matvec expects one parameter (the packed matrix), "
+                            f"got {len(params)}")
+        M = params[0]
+        if not isinstance(M, PackedMatrix):
+            raise TypeError(f"packed_matvec expects a PackedMatrix (operators.pack), "
+                            f"got {type(M).__name__}")
+        if M.n != n:
+            raise ValueError(f"matrix of shape ({M.n}, {M.n}) does not act on vectors of size {n}")
+        if M.device != device:
+            raise ValueError(f"packed matrix on {M.device}, vectors on {device}")
+        d = _lib.OperatorStruct()
+        d.kind = self.kind
+        d.precision = _lib.PRECISIONS["fp32"]
+        d.n = n
+        d.matrix = None
+        d.points = None
+        d.dim = 0
+        d.storage = _lib.STORAGES[M.storage]
+        d.packed = M.packed.data_ptr()
+        d.tile_scales = M.scales.data_ptr() if M.scales is not None else None
+        return d, M
+
+
+packed_matvec = PackedDense()
+
+
 def resolve(matvec):
     """Check that ``matvec`` is a GPU operator of this package."""
     if not isinstance(matvec, Operator):
         raise TypeError(
             "matfree_amd runs the matvec inside its HIP block matvec; pass one of its "
-            f"operators (dense_matvec, RBF(points)), not {type(matvec).__name__}"
+            f"operators (dense_matvec, packed_matvec, RBF(points)), not {type(matvec).__name__}"
         )
     return matvec

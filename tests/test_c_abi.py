@@ -50,12 +50,13 @@ def test_operator_struct_layout_matches_c():
 #include <stdio.h>
 #include "slq.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(slq_operator),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(slq_operator),
          offsetof(slq_operator, kind), offsetof(slq_operator, precision),
          offsetof(slq_operator, n), offsetof(slq_operator, matrix),
          offsetof(slq_operator, points), offsetof(slq_operator, dim),
-         offsetof(slq_operator, reserved), offsetof(slq_operator, lengthscale),
-         offsetof(slq_operator, noise));
+         offsetof(slq_operator, storage), offsetof(slq_operator, lengthscale),
+         offsetof(slq_operator, noise), offsetof(slq_operator, packed),
+         offsetof(slq_operator, tile_scales));
   return 0;
 }
 """
@@ -67,8 +68,8 @@ int main(void) {
         got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True).stdout.split()]
     S = _lib.OperatorStruct
     want = [ctypes.sizeof(S)] + [getattr(S, f).offset for f in
-                                 ["kind", "precision", "n", "matrix", "points", "dim", "reserved",
-                                  "lengthscale", "noise"]]
+                                 ["kind", "precision", "n", "matrix", "points", "dim", "storage",
+                                  "lengthscale", "noise", "packed", "tile_scales"]]
     assert got == want
 
 
