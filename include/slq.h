@@ -154,7 +154,41 @@ int slq_matvec(slq_context* ctx, const slq_operator* op, const float* V, int64_t
 int slq_hessenberg(slq_context* ctx, const slq_operator* op, const float* V0, int64_t P, int64_t k,
                    float* Q, float* H, float* R, float* c);
 
-/* SLQ log-determinant integrand and estimator for P probes z_p:
+/* slq_hessenberg with the forward's reorthogonalisation selectable:
+ * reortho = 0 runs one classical Gram-Schmidt pass per step (the forward of
+ * decomp.hessenberg(..., reortho_vjp="none"), decomp.py:394-396, 466-468),
+ * reortho != 0 two (as slq_hessenberg). */
+int slq_arnoldi(slq_context* ctx, const slq_operator* op, const float* V0, int64_t P, int64_t k,
+                int reortho, float* Q, float* H, float* R, float* c);
+
+/* Reverse mode of the Arnoldi decomposition for general output cotangents:
+ * the custom VJP of decomp.hessenberg (decomp.py:398-422) and
+ * _hessenberg_adjoint (decomp.py:480-600).  The forward pass on V0 (as
+ * slq_arnoldi with `reortho`) is recomputed, then the adjoint runs with its
+ * reorthogonalisation on (reortho_adjoint != 0, the reference's
+ * reortho="full") or off.  Cotangents (device, any may be NULL = zero):
+ *   dQ: k x P x n (the layout of slq_hessenberg's Q), dH: P x k x k
+ *   (column-major like H), dr: P x n, dc: P.
+ * Outputs: dv (P x n, cotangent of V0 = c lambda, decomp.py:602; may be
+ * NULL), and the parameter cotangents as in slq_logdet_vjp (dparams for
+ * RBF, dmatrix for stored matrices).  Fails with SLQ_EADJOINT_K0 for k = 0. */
+int slq_arnoldi_vjp(slq_context* ctx, const slq_operator* op, const float* V0, int64_t P,
+                    int64_t k, int reortho, int reortho_adjoint, const float* dQ, const float* dH,
+                    const float* dr, const float* dc, float* dv, double* dparams, float* dmatrix);
+
+/* Reverse mode of plain Lanczos (tridiag_sym(k, reortho="none")) for general
+ * cotangents: the custom VJP of decomp.py:181-197 and _tridiag_adjoint
+ * (decomp.py:268-348).  The forward (slq_lanczos) is recomputed.
+ * Cotangents (device, any may be NULL = zero): dQ (k x P x n, the vectors
+ * x_0 .. x_{k-1}), dalpha (P x k), dbeta (P x k: the k - 1 off-diagonals,
+ * then the remainder's norm b_k), dxk (P x n, the remainder's direction x_k,
+ * residual = b_k x_k).  Outputs: dv (P x n, cotangent of V0, decomp.py:305;
+ * may be NULL) and the parameter cotangents as in slq_logdet_vjp. */
+int slq_lanczos_vjp(slq_context* ctx, const slq_operator* op, const float* V0, int64_t P,
+                    int64_t k, const float* dQ, const float* dalpha, const float* dbeta,
+                    const float* dxk, float* dv, double* dparams, float* dmatrix);
+
+/* SLQ log-determinant integrand and estimator for P probes z_p:/* SLQ log-determinant integrand and estimator for P probes z_p:
  *   per_probe[p] = |z_p|^2 [log T_p]_00 with T_p = (H_p + H_p^T)/2 from
  *   slq_hessenberg on z_p / |z_p|, evaluated in FP64 (eigenvalues and first
  *   eigenvector components of the tridiagonal T_p);

@@ -45,7 +45,8 @@ def ravel(tree, device):
     for leaf in leaves:
         t = leaf if isinstance(leaf, torch.Tensor) else torch.as_tensor(np.asarray(leaf))
         shapes.append(tuple(t.shape))
-        tensors.append(t.detach().to(device=device, dtype=torch.float32).reshape(-1))
+        # not detached: autograd reaches the leaves through the flattening
+        tensors.append(t.to(device=device, dtype=torch.float32).reshape(-1))
     flat = torch.cat(tensors) if len(tensors) > 1 else tensors[0].contiguous()
     sizes = [int(np.prod(s, dtype=np.int64)) for s in shapes]
 

@@ -32,10 +32,16 @@ namespace {
 // Output per probe (stride cs = 2 k + 4): gq[0..k), gl[k..2k), and
 // (cr, cl, cv, scale) for r, lambda, A^T lambda and the final division.
 // ---------------------------------------------------------------------------
+//
+// General cotangents (decomp.py:497-506): Pg (P x k x k, column-major) holds
+// Pi_gamma = -dc c e_1 e_1^T + H dH^T - dQ^T Q and gam (P x k) the vector
+// gamma = dH e_K - Q^T dr; when they are null, dQ = dr = dc = 0 and dH is
+// tridiagonal (the SLQ composition) and both are formed here.
 __global__ void __launch_bounds__(256)
     k_adjoint_coef(const float* __restrict__ H, const float* __restrict__ dH,
                    float* __restrict__ Gam, const float* __restrict__ part, int64_t nb,
-                   int pstride, int64_t k, int64_t idx, float* __restrict__ coef) {
+                   int pstride, int64_t k, int64_t idx, float* __restrict__ coef,
+                   const float* __restrict__ Pg, const float* __restrict__ gam) {
   extern __shared__ float grow[];  // Gamma[idx, 0..idx]
   const int64_t p = blockIdx.x;
   const float* Hp = H + p * k * k;    // column-major: H[row + col * k]
@@ -45,9 +51,14 @@ __global__ void __launch_bounds__(256)
   for (int64_t j = threadIdx.x; j <= idx; j += blockDim.x) {
     float s = 0.f;
     for (int64_t b = 0; b < nb; ++b) s += part[(p * nb + b) * pstride + j];
-    float pg = Hp[idx + j * k] * dHp[j + j * k];
-    if (j > 0) pg += Hp[idx + (j - 1) * k] * dHp[j + (j - 1) * k];
-    if (j + 1 < k) pg += Hp[idx + (j + 1) * k] * dHp[j + (j + 1) * k];
+    float pg;
+    if (Pg) {
+      pg = Pg[p * k * k + idx + j * k];
+    } else {
+      pg = Hp[idx + j * k] * dHp[j + j * k];
+      if (j > 0) pg += Hp[idx + (j - 1) * k] * dHp[j + (j - 1) * k];
+      if (j + 1 < k) pg += Hp[idx + (j + 1) * k] * dHp[j + (j + 1) * k];
+    }
     const float w = (j < idx) ? 1.f : 0.5f;
     const float g = w * (pg - s);
     grow[j] = g;
@@ -66,22 +77,23 @@ __global__ void __launch_bounds__(256)
     cp[k + j] = (j > idx) ? -Hp[idx + j * k] : 0.f;
   }
   if (threadIdx.x == 0) {
-    cp[2 * k + 0] = dHp[idx + (k - 1) * k];  // gamma_idx
+    cp[2 * k + 0] = gam ? gam[p * k + idx] : dHp[idx + (k - 1) * k];  // gamma_idx
     cp[2 * k + 1] = -Hp[idx + idx * k];      // -alpha_idx
     cp[2 * k + 2] = 1.f;                     // + A^T lambda
     cp[2 * k + 3] = 1.f / (idx == 0 ? 1.f : Hp[idx + (idx - 1) * k]);
   }
 }
 
-// y = scale (cr r + cl x + cv v + sum_{j < ncq} gq_j Q_j + sum_{jl0 <= j < k} gl_j L_j)
-// per probe, coefficients from coef (stride 2 k + 4); r, x, v may be null
+// y = scale (cr r + cl x + cv v + d + sum_{j < ncq} gq_j Q_j + sum_{jl0 <= j < k} gl_j L_j)
+// per probe, coefficients from coef (stride 2 k + 4); r, x, v, d may be null
 // (their terms are skipped).  With save (non-null) x is also copied there.
 template <int V>
 __global__ void __launch_bounds__(kStreamThreads)
     k_adjoint_combine(const float* __restrict__ Q, const float* __restrict__ Lam, int64_t ldc,
                       const float* __restrict__ r, const float* __restrict__ x,
                       const float* __restrict__ v, float* __restrict__ y, float* __restrict__ save,
-                      int64_t n, int64_t k, int ncq, int jl0, const float* __restrict__ coef) {
+                      int64_t n, int64_t k, int ncq, int jl0, const float* __restrict__ coef,
+                      const float* __restrict__ d) {
   extern __shared__ float cs[];
   const int64_t p = blockIdx.y;
   const int t = threadIdx.x;
@@ -112,6 +124,7 @@ __global__ void __launch_bounds__(kStreamThreads)
   if (r) axpy(cr, r + off);
   if (x) axpy(cl, x + off);
   if (v) axpy(cv, v + off);
+  if (d) axpy(1.f, d + off);
   float out[V];
 #pragma unroll
   for (int u = 0; u < V; ++u) out[u] = sc * acc[u];
@@ -153,6 +166,58 @@ __global__ void k_adjoint_init_coef(const float* __restrict__ dH, int64_t k,
   for (int64_t j = threadIdx.x; j < 2 * k + 4; j += blockDim.x) {
     float v = 0.f;
     if (j < k) v = dHp[j + (k - 1) * k];
+    if (j == 2 * k + 3) v = 1.f;
+    cp[j] = v;
+  }
+}
+
+// Row i of Pi_gamma = -dc c e_1 e_1^T + H dH^T - dQ^T Q (decomp.py:506) per
+// probe: (H dH^T)[i, j] = sum_m H[i, m] dH[j, m] (both column-major), and
+// (dQ^T Q)[i, j] = dQ_i . q_j from the chunk partials of launch_dots with
+// x = dQ_i (null: dQ = 0); dc / c per probe (dc null: 0).  Pg column-major.
+__global__ void __launch_bounds__(256)
+    k_adjoint_pigamma_row(const float* __restrict__ H, const float* __restrict__ dH,
+                          const float* __restrict__ part, int64_t nb, int pstride,
+                          const float* __restrict__ c, const float* __restrict__ dc, int64_t k,
+                          int64_t i, float* __restrict__ Pg) {
+  const int64_t p = blockIdx.x;
+  const float* Hp = H + p * k * k;
+  const float* dHp = dH + p * k * k;
+  for (int64_t j = threadIdx.x; j < k; j += blockDim.x) {
+    float v = 0.f;
+    for (int64_t m = 0; m < k; ++m) v = fmaf(Hp[i + m * k], dHp[j + m * k], v);
+    if (part) {
+      float g = 0.f;
+      for (int64_t b = 0; b < nb; ++b) g += part[(p * nb + b) * pstride + j];
+      v -= g;
+    }
+    if (dc && i == 0 && j == 0) v -= dc[p] * c[p];
+    Pg[p * k * k + i + j * k] = v;
+  }
+}
+
+// gamma = dH e_K - Q^T dr (decomp.py:498; part: chunk partials of Q^T dr,
+// null for dr = 0) into gam (P x k), and the coefficients that make
+// k_adjoint_combine compute lambda = dr + Q gamma (decomp.py:499; x = dr).
+__global__ void __launch_bounds__(256)
+    k_adjoint_gamma_init(const float* __restrict__ dH, const float* __restrict__ part, int64_t nb,
+                         int pstride, int64_t k, float* __restrict__ gam,
+                         float* __restrict__ coef) {
+  const int64_t p = blockIdx.x;
+  float* cp = coef + p * (2 * k + 4);
+  const float* dHp = dH + p * k * k;
+  for (int64_t j = threadIdx.x; j < 2 * k + 4; j += blockDim.x) {
+    float v = 0.f;
+    if (j < k) {
+      v = dHp[j + (k - 1) * k];
+      if (part) {
+        float g = 0.f;
+        for (int64_t b = 0; b < nb; ++b) g += part[(p * nb + b) * pstride + j];
+        v -= g;
+      }
+      gam[p * k + j] = v;
+    }
+    if (j == 2 * k + 1) v = 1.f;  // x = dr
     if (j == 2 * k + 3) v = 1.f;
     cp[j] = v;
   }
@@ -338,30 +403,47 @@ hipError_t launch_outer_rbf_contract(hipStream_t s, const float* L, const float*
 
 hipError_t launch_adjoint_coef(hipStream_t s, const float* H, const float* dH, float* Gam,
                                const float* part, int pstride, int64_t P, int64_t n, int64_t k,
-                               int64_t idx, float* coef) {
+                               int64_t idx, float* coef, const float* Pg, const float* gam) {
   const size_t shmem = size_t(idx + 1) * sizeof(float);
   hipLaunchKernelGGL(k_adjoint_coef, dim3(unsigned(P)), dim3(256), shmem, s, H, dH, Gam, part,
-                     num_chunks(n), pstride, k, idx, coef);
+                     num_chunks(n), pstride, k, idx, coef, Pg, gam);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint_pigamma_row(hipStream_t s, const float* H, const float* dH,
+                                      const float* part, int pstride, const float* c,
+                                      const float* dc, int64_t P, int64_t n, int64_t k, int64_t i,
+                                      float* Pg) {
+  hipLaunchKernelGGL(k_adjoint_pigamma_row, dim3(unsigned(P)), dim3(256), 0, s, H, dH, part,
+                     num_chunks(n), pstride, c, dc, k, i, Pg);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint_gamma_init(hipStream_t s, const float* dH, const float* part, int pstride,
+                                     int64_t P, int64_t n, int64_t k, float* gam, float* coef) {
+  hipLaunchKernelGGL(k_adjoint_gamma_init, dim3(unsigned(P)), dim3(256), 0, s, dH, part,
+                     num_chunks(n), pstride, k, gam, coef);
   return hipGetLastError();
 }
 
 hipError_t launch_adjoint_combine(hipStream_t s, const float* Q, const float* Lam, int64_t ldc,
                                   const float* r, const float* x, const float* v, float* y,
                                   float* save, int64_t P, int64_t n, int64_t k, int ncq, int jl0,
-                                  const float* coef) {
+                                  const float* coef, const float* d) {
   const auto al = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
   const bool vec = n % 4 == 0 && ldc % 4 == 0 && al(Q) && al(y) && (!Lam || al(Lam)) &&
-                   (!r || al(r)) && (!x || al(x)) && (!v || al(v)) && (!save || al(save));
+                   (!r || al(r)) && (!x || al(x)) && (!v || al(v)) && (!save || al(save)) &&
+                   (!d || al(d));
   const int V = vec ? 4 : 1;
   const int64_t per_block = int64_t(kStreamThreads) * V;
   const dim3 grid(unsigned((n + per_block - 1) / per_block), unsigned(P));
   const size_t shmem = size_t(2 * k + 4) * sizeof(float);
   if (vec)
     hipLaunchKernelGGL(k_adjoint_combine<4>, grid, dim3(kStreamThreads), shmem, s, Q, Lam, ldc, r,
-                       x, v, y, save, n, k, ncq, jl0, coef);
+                       x, v, y, save, n, k, ncq, jl0, coef, d);
   else
     hipLaunchKernelGGL(k_adjoint_combine<1>, grid, dim3(kStreamThreads), shmem, s, Q, Lam, ldc, r,
-                       x, v, y, save, n, k, ncq, jl0, coef);
+                       x, v, y, save, n, k, ncq, jl0, coef, d);
   return hipGetLastError();
 }
 

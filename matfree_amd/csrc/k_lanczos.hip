@@ -921,9 +921,11 @@ __device__ __forceinline__ void ladj_coeffs(const LanczosAdjStep& st, int64_t p,
       d2 += pp[3 * bb + 2];
     }
   }
-  const float db = st.db1 ? st.db1[p * st.sstride] + st.db2[p * st.sstride] : 0.f;
+  const int64_t dbs = st.dbstride ? st.dbstride : st.sstride;
+  const float db =
+      st.db1 ? st.db1[p * dbs] + (st.db2 ? st.db2[p * dbs] : 0.f) : 0.f;
   mu = db - d0 + d1 / b;
-  nu = st.da[p * st.sstride] + d2 / b;
+  nu = st.da[p * (st.dastride ? st.dastride : st.sstride)] + d2 / b;
 }
 
 template <int V>
@@ -937,6 +939,7 @@ __global__ void __launch_bounds__(kStreamThreads)
   const int64_t base = int64_t(blockIdx.x) * kChunk;
   float b, mu, nu;
   ladj_coeffs(st, p, nb, b, mu, nu);
+  const float cmu = st.xplus_res ? mu / b : mu;
 #pragma unroll
   for (int e = 0; e < NE; ++e) {
     const int64_t i = base + (int64_t(e) * kStreamThreads + t) * V;
@@ -955,7 +958,7 @@ __global__ void __launch_bounds__(kStreamThreads)
         float yv[V];
         VecIO<V>::load(xplus + p * n + i, yv);
 #pragma unroll
-        for (int u = 0; u < V; ++u) out[u] = fmaf(mu, yv[u], out[u]);
+        for (int u = 0; u < V; ++u) out[u] = fmaf(cmu, yv[u], out[u]);
       }
       VecIO<V>::store(lam + p * n + i, out);
     }
@@ -997,6 +1000,12 @@ __global__ void __launch_bounds__(kStreamThreads)
       }
 #pragma unroll
       for (int u = 0; u < V; ++u) out[u] = fmaf(-cx, yv[u], out[u]);
+      if (st.dx) {
+        float dv[V];
+        VecIO<V>::load(st.dx + p * n + i, dv);
+#pragma unroll
+        for (int u = 0; u < V; ++u) out[u] -= dv[u];
+      }
       VecIO<V>::store(xi_out + p * n + i, out);
       if (part_out) {
         float xv[V], pv[V];
@@ -1014,7 +1023,75 @@ __global__ void __launch_bounds__(kStreamThreads)
   if (part_out) block_column_sums<4>(acc, 3, red, part_out + (p * nb + blockIdx.x) * 3);
 }
 
+// xi = -dxk and the partial dots [0, (R / blast) . xi, xkm1 . xi] (scalar).
+__global__ void __launch_bounds__(kStreamThreads)
+    k_ladj_init(const float* __restrict__ dxk, const float* __restrict__ R,
+                const float* __restrict__ blast, const float* __restrict__ xkm1, int64_t n,
+                float* __restrict__ xi, float* __restrict__ part_out) {
+  __shared__ float red[kWaves * 4];
+  const int64_t p = blockIdx.y, nb = gridDim.x;
+  const int t = threadIdx.x;
+  const int64_t base = int64_t(blockIdx.x) * kChunk;
+  const float inv_b = 1.f / blast[p];
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int64_t i = base + t; i < base + kChunk && i < n; i += kStreamThreads) {
+    const int64_t off = p * n + i;
+    const float z = dxk ? -dxk[off] : 0.f;
+    xi[off] = z;
+    acc[1] = fmaf(R[off] * inv_b, z, acc[1]);
+    acc[2] = fmaf(xkm1[off], z, acc[2]);
+  }
+  block_column_sums<4>(acc, 3, red, part_out + (p * nb + blockIdx.x) * 3);
+}
+
+// partial dots of xi . x0 (one column), then dv = c ((xi . x0) x0 - xi)
+__global__ void __launch_bounds__(kStreamThreads)
+    k_ladj_initvec_dot(const float* __restrict__ xi, const float* __restrict__ x0, int64_t n,
+                       float* __restrict__ part) {
+  __shared__ float red[kWaves];
+  const int64_t p = blockIdx.y, nb = gridDim.x;
+  const int t = threadIdx.x;
+  const int64_t base = int64_t(blockIdx.x) * kChunk;
+  float acc = 0.f;
+  for (int64_t i = base + t; i < base + kChunk && i < n; i += kStreamThreads)
+    acc = fmaf(xi[p * n + i], x0[p * n + i], acc);
+  const float s = block_sum(acc, red);
+  if (t == 0) part[p * nb + blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(kStreamThreads)
+    k_ladj_initvec(const float* __restrict__ xi, const float* __restrict__ x0,
+                   const float* __restrict__ c, const float* __restrict__ part, int64_t nb,
+                   int64_t n, float* __restrict__ dv) {
+  const int64_t p = blockIdx.y;
+  float d = 0.f;
+  for (int64_t b = 0; b < nb; ++b) d += part[p * nb + b];
+  const float cp = c[p];
+  const int64_t i = int64_t(blockIdx.x) * kStreamThreads + threadIdx.x;
+  if (i < n) dv[p * n + i] = cp * fmaf(d, x0[p * n + i], -xi[p * n + i]);
+}
+
 }  // namespace
+
+hipError_t launch_lanczos_adj_init(hipStream_t s, const float* dxk, const float* R,
+                                   const float* blast, const float* xkm1, int64_t P, int64_t n,
+                                   float* xi, float* part_out) {
+  const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
+  hipLaunchKernelGGL(k_ladj_init, grid, dim3(kStreamThreads), 0, s, dxk, R, blast, xkm1, n, xi,
+                     part_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_lanczos_adj_initvec(hipStream_t s, const float* xi, const float* x0,
+                                      const float* c, int64_t P, int64_t n, float* part,
+                                      float* dv) {
+  const int64_t nb = num_chunks(n);
+  hipLaunchKernelGGL(k_ladj_initvec_dot, dim3(unsigned(nb), unsigned(P)), dim3(kStreamThreads), 0,
+                     s, xi, x0, n, part);
+  const dim3 grid(unsigned((n + kStreamThreads - 1) / kStreamThreads), unsigned(P));
+  hipLaunchKernelGGL(k_ladj_initvec, grid, dim3(kStreamThreads), 0, s, xi, x0, c, part, nb, n, dv);
+  return hipGetLastError();
+}
 
 hipError_t launch_lanczos_adj_lambda(hipStream_t s, const float* xi, const float* x,
                                      const float* xplus, int64_t P, int64_t n,
@@ -1038,7 +1115,8 @@ hipError_t launch_lanczos_adj_xi(hipStream_t s, const float* W, const float* lam
   const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
   const bool vec = (n % 4 == 0) && aligned16(W) && aligned16(lam) && aligned16(xplus) &&
                    aligned16(xi) && (!lam_plus || aligned16(lam_plus)) &&
-                   (!part_out || (aligned16(x) && aligned16(x_prev)));
+                   (!part_out || (aligned16(x) && aligned16(x_prev))) &&
+                   (!st.dx || aligned16(st.dx));
   const int res = xplus_is_residual ? 1 : 0;
   if (vec)
     hipLaunchKernelGGL(k_ladj_xi<4>, grid, dim3(kStreamThreads), 0, s, W, lam, lam_plus, xplus,

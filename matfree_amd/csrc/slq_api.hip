@@ -23,6 +23,8 @@ struct slq_context {
   Buf adj_dh, adj_gam, adj_coef, adj_lam, adj_v1, adj_v2, adj_v3, adj_work, adj_sum;
   // plain Lanczos: three rotating vectors and the last off-diagonal
   Buf ring, blast;
+  // general-cotangent reverse passes: Pi_gamma, gamma, the internal dH
+  Buf adj_pg, adj_gvec;
   // partial Cholesky, Woodbury solve and CG
   Buf lr_lc, lr_perm, lr_bval, lr_bidx, lr_rowi, lr_small, lr_gp, lr_c, lr_wp, lr_z, cg_r, cg_z,
       cg_d, cg_q, cg_parts, cg_scal;
@@ -30,7 +32,8 @@ struct slq_context {
     return {&partial1, &partial2, &norm2,   &work_w,  &work_w2,  &basis,   &hmat,
             &resid,    &vec0,     &scal1,   &scal2,   &eig,      &per_probe, &mean,
             &splitk,   &adj_dh,   &adj_gam, &adj_coef, &adj_lam, &adj_v1,  &adj_v2,
-            &adj_v3,   &adj_work, &adj_sum, &ring,    &blast,    &lr_lc,   &lr_perm,
+            &adj_v3,   &adj_work, &adj_sum, &ring,    &blast,    &adj_pg,  &adj_gvec,
+            &lr_lc,    &lr_perm,
             &lr_bval,  &lr_bidx,  &lr_rowi, &lr_small, &lr_gp,   &lr_c,    &lr_wp,
             &lr_z,     &cg_r,     &cg_z,    &cg_d,    &cg_q,     &cg_parts, &cg_scal};
   }
@@ -256,8 +259,11 @@ inline int64_t basis_stride(int64_t P, int64_t n) {
 }
 
 // Forward Arnoldi (decomp.py:426-477) for P start vectors; see slq.h.
+// reortho = false: one classical Gram-Schmidt pass per step (the forward of
+// hessenberg(..., reortho_vjp="none"), decomp.py:466-468).
 int hessenberg_core(slq_context* ctx, const slq_operator& op, const float* V0, int64_t P,
-                    int64_t k, float* Q, int64_t ldc, float* H, float* R, float* c) {
+                    int64_t k, float* Q, int64_t ldc, float* H, float* R, float* c,
+                    bool reortho = true) {
   const int64_t n = op.n;
   hipStream_t s = ctx->stream;
   const int64_t nb = num_chunks(n);
@@ -301,6 +307,22 @@ int hessenberg_core(slq_context* ctx, const slq_operator& op, const float* V0, i
       Timed tm(ctx, SLQ_KC_DOTS);
       SLQ_HIP(ctx, launch_dots(s, Q, ldc, w, P, n, ncols, part1, pstride));
     }
+    const bool last = (j + 1 == k);
+    float* dst = last ? R : w;
+    if (!reortho) {
+      // w = w - Q h and <w, w> (decomp.py:464, 471), h into H[:, j]
+      {
+        Timed tm(ctx, SLQ_KC_PROJECT);
+        SLQ_HIP(ctx, launch_project(s, Q, ldc, w, dst, P, n, ncols, part1, pstride, norm2, hcol,
+                                    k * k));
+      }
+      if (!last) {
+        Timed tm(ctx, SLQ_KC_NORMALIZE);
+        SLQ_HIP(ctx, launch_normalize(s, w, Q + (j + 1) * ldc, P, n, norm2, nullptr, hcol + j + 1,
+                                      k * k, nullptr, nullptr, planes.hi ? &planes : nullptr));
+      }
+      continue;
+    }
     // w = w - Q h (decomp.py:464), then the second pass's Q^T w (decomp.py:468)
     if (ncols <= kFusedMaxCols) {
       Timed tm(ctx, SLQ_KC_PROJECT_DOTS);
@@ -311,8 +333,6 @@ int hessenberg_core(slq_context* ctx, const slq_operator& op, const float* V0, i
       SLQ_HIP(ctx, launch_dots(s, Q, ldc, w2, P, n, ncols, part2, pstride));
     }
     // w = w - Q (Q^T w) and <w, w> (decomp.py:468, 471)
-    const bool last = (j + 1 == k);
-    float* dst = last ? R : w;
     {
       Timed tm(ctx, SLQ_KC_PROJECT);
       SLQ_HIP(ctx, launch_project(s, Q, ldc, w2, dst, P, n, ncols, part2, pstride, norm2, nullptr, 0));
@@ -552,7 +572,177 @@ int logdet_reverse(slq_context* ctx, const slq_operator& op, int64_t P, int64_t 
   return SLQ_OK;
 }
 
-// Largest rank of the partial Cholesky / Woodbury kernels (LDS staging).
+// Parameter cotangents of a reverse pass: S = sum over steps and probes of
+// L_c R_c^T (the matvec VJPs), contracted for the RBF parameters or stored
+// as the matrix cotangent.
+int param_cotangents(slq_context* ctx, const slq_operator& op, const float* L, const float* R,
+                     int64_t ldc, int64_t P, int64_t k, double* dparams, float* dmatrix) {
+  const int64_t n = op.n;
+  hipStream_t s = ctx->stream;
+  if (op.kind == SLQ_OP_RBF) {
+    double* dsum;
+    SLQ_TRY(ensure_t(ctx, ctx->adj_sum, size_t(2 * outer_contract_blocks(n)), &dsum));
+    SLQ_HIP(ctx, launch_outer_rbf_contract(s, L, R, ldc, P, n, k * P, op.points, op.dim,
+                                           op.lengthscale, dsum));
+    const double ls = op.lengthscale;
+    SLQ_HIP(ctx, launch_sum_fixed(s, dsum, outer_contract_blocks(n), 2, 1.0 / (ls * ls * ls), 1.0,
+                                  dparams));
+  } else {
+    SLQ_HIP(ctx, launch_outer_sum(s, L, R, ldc, P, n, k * P, dmatrix));
+  }
+  return SLQ_OK;
+}
+
+// The Arnoldi adjoint for general cotangents (decomp.py:480-600): dQ (k x P
+// x n, column j of probe p at dQ + j P n + p n), dH (P x k x k column-major),
+// dr (P x n), dc (P); any may be null (zero).  adj_reortho: the
+// reorthogonalisation of the adjoint (decomp.py:575-584, reortho="full").
+// Writes dv = c lambda (P x n, decomp.py:602) when dv is non-null and the
+// parameter cotangents.
+int arnoldi_reverse(slq_context* ctx, const slq_operator& op, int64_t P, int64_t k,
+                    const ForwardState& st, bool adj_reortho, const float* dQ, const float* dH_in,
+                    const float* dr, const float* dc, float* dv, double* dparams,
+                    float* dmatrix) {
+  const int64_t n = op.n, ldc = st.ldc, ldq = P * n;
+  hipStream_t s = ctx->stream;
+  const int pstride = int(k);
+  const int64_t nb = num_chunks(n);
+  float *dH, *Gam, *coef, *Lam, *lam, *lam2, *vm, *part1, *part2, *mvs, *Pg, *gam;
+  const int64_t mv_floats = matvec_scratch_floats(n, P);
+  SLQ_TRY(ensure_t(ctx, ctx->splitk, size_t(mv_floats), &mvs));
+  SLQ_TRY(ensure_t(ctx, ctx->partial1, size_t(P * nb * pstride), &part1));
+  SLQ_TRY(ensure_t(ctx, ctx->partial2, size_t(P * nb * pstride), &part2));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_dh, size_t(P * k * k), &dH));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_gam, size_t(P * k * k), &Gam));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_pg, size_t(P * k * k), &Pg));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_gvec, size_t(P * k), &gam));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_coef, size_t(P * (2 * k + 4)), &coef));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_lam, size_t(k * ldc), &Lam));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_v1, size_t(P * n), &lam));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_v2, size_t(P * n), &lam2));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_v3, size_t(P * n), &vm));
+
+  if (dH_in)
+    SLQ_HIP(ctx, hipMemcpyAsync(dH, dH_in, size_t(P * k * k) * sizeof(float),
+                                hipMemcpyDeviceToDevice, s));
+  else
+    SLQ_HIP(ctx, launch_fill_zero(s, dH, P * k * k));
+  SLQ_HIP(ctx, launch_fill_zero(s, Gam, P * k * k));
+  // Pi_gamma = -dc c e_1 e_1^T + H dH^T - dQ^T Q, row by row (decomp.py:506)
+  for (int64_t i = 0; i < k; ++i) {
+    if (dQ) SLQ_HIP(ctx, launch_dots(s, st.Q, ldc, dQ + i * ldq, P, n, int(k), part1, pstride));
+    SLQ_HIP(ctx, launch_adjoint_pigamma_row(s, st.H, dH, dQ ? part1 : nullptr, pstride, st.c, dc,
+                                            P, n, k, i, Pg));
+  }
+  // gamma = dH e_K - Q^T dr; lambda = dr + Q gamma (decomp.py:498-499)
+  if (dr) SLQ_HIP(ctx, launch_dots(s, st.Q, ldc, dr, P, n, int(k), part1, pstride));
+  SLQ_HIP(ctx, launch_adjoint_gamma_init(s, dH, dr ? part1 : nullptr, pstride, P, n, k, gam,
+                                         coef));
+  SLQ_HIP(ctx, launch_adjoint_combine(s, st.Q, nullptr, ldc, nullptr, dr, nullptr, lam, nullptr,
+                                      P, n, k, int(k), int(k), coef));
+  // reverse scan (decomp.py:538, 575-599)
+  for (int64_t idx = k - 1; idx >= 0; --idx) {
+    const float* lin = lam;
+    if (adj_reortho) {
+      // lambda -= Q_m (Q_m^T lambda - dH[:, idx]_m), columns j <= idx + 1
+      const int ncols = int(idx + 2 < k ? idx + 2 : k);
+      SLQ_HIP(ctx, launch_dots(s, st.Q, ldc, lam, P, n, ncols, part1, pstride));
+      SLQ_HIP(ctx, launch_project(s, st.Q, ldc, lam, lam2, P, n, ncols, part1, pstride, nullptr,
+                                  nullptr, 0, dH + idx * k, k * k));
+      lin = lam2;
+    }
+    // A^T lambda (decomp.py:587-588; A symmetric)
+    SLQ_HIP(ctx, launch_matvec(s, op, lin, P, vm, mvs, mv_floats));
+    // Q^T (A^T lambda) for Gamma[idx, :] (decomp.py:592-593)
+    SLQ_HIP(ctx, launch_dots(s, st.Q, ldc, vm, P, n, int(idx + 1), part2, pstride));
+    SLQ_HIP(ctx, launch_adjoint_coef(s, st.H, dH, Gam, part2, pstride, P, n, k, idx, coef, Pg,
+                                     gam));
+    // Lambda[:, idx] = lambda and the next lambda, with Pi_xi[idx] = dQ_idx +
+    // gamma_idx r (decomp.py:505, 596-599)
+    SLQ_HIP(ctx, launch_adjoint_combine(s, st.Q, Lam, ldc, st.R, lin, vm, adj_reortho ? lam : lam2,
+                                        Lam + idx * ldc, P, n, k, int(k), int(idx + 1), coef,
+                                        dQ ? dQ + idx * ldq : nullptr));
+    if (!adj_reortho) std::swap(lam, lam2);
+  }
+  // dv = lambda c (decomp.py:602)
+  if (dv) SLQ_HIP(ctx, launch_scale_rows(s, lam, st.c, 1, P, n, dv));
+  return param_cotangents(ctx, op, Lam, st.Q, ldc, P, k, dparams, dmatrix);
+}
+
+// The plain-Lanczos adjoint for general cotangents (decomp.py:181-197,
+// 268-348): dQ (k x P x n: the vectors x_0 .. x_{k-1}), dalpha (P x k),
+// dbeta (P x k: the k - 1 off-diagonals, then the remainder's norm), dxk (P x
+// n: the remainder's direction x_k); any may be null (zero).  Writes dv (P x
+// n, decomp.py:305) when non-null and the parameter cotangents.
+int lanczos_reverse(slq_context* ctx, const slq_operator& op, int64_t P, int64_t k,
+                    const ForwardState& st, const float* dQ, const float* dalpha,
+                    const float* dbeta, const float* dxk, float* dv, double* dparams,
+                    float* dmatrix) {
+  const int64_t n = op.n, ldc = st.ldc, kk = k * k, ldq = P * n;
+  hipStream_t s = ctx->stream;
+  const int64_t nb = num_chunks(n);
+  float *db, *Lam, *xi, *W, *part1, *part2, *mvs;
+  const int64_t mv_floats = matvec_scratch_floats(n, P);
+  SLQ_TRY(ensure_t(ctx, ctx->splitk, size_t(mv_floats), &mvs));
+  SLQ_TRY(ensure_t(ctx, ctx->partial1, size_t(P * nb * 3), &part1));
+  SLQ_TRY(ensure_t(ctx, ctx->partial2, size_t(P * nb * 3), &part2));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_gvec, size_t(2 * P * k), &db));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_lam, size_t(k * ldc), &Lam));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_v1, size_t(P * n), &xi));
+  SLQ_TRY(ensure_t(ctx, ctx->adj_v2, size_t(P * n), &W));
+  // da and db as (P x k) arrays (zero when not given)
+  float* da = db + P * k;
+  if (dalpha)
+    SLQ_HIP(ctx, hipMemcpyAsync(da, dalpha, size_t(P * k) * sizeof(float), hipMemcpyDeviceToDevice,
+                                s));
+  else
+    SLQ_HIP(ctx, launch_fill_zero(s, da, P * k));
+  if (dbeta)
+    SLQ_HIP(ctx, hipMemcpyAsync(db, dbeta, size_t(P * k) * sizeof(float), hipMemcpyDeviceToDevice,
+                                s));
+  else
+    SLQ_HIP(ctx, launch_fill_zero(s, db, P * k));
+
+  // xi = -dx_k, lambda_plus = 0 (decomp.py:290-297) and the dots of step k - 1
+  float* part[2] = {part1, part2};
+  SLQ_HIP(ctx, launch_lanczos_adj_init(s, dxk, st.R, st.blast, st.Q + (k - 1) * ldc, P, n, xi,
+                                       part[(k - 1) & 1]));
+  for (int64_t i = k - 1; i >= 0; --i) {
+    const bool last = (i + 1 == k);
+    LanczosAdjStep sc;
+    sc.a = st.H + i * k + i;
+    sc.sstride = kk;
+    sc.da = da + i;
+    sc.dastride = k;
+    sc.db1 = db + i;
+    sc.dbstride = k;
+    if (last) {
+      sc.b = st.blast;  // the remainder's norm
+      sc.bstride = 1;
+      sc.xplus_res = 1;
+    } else {
+      sc.b = st.H + i * k + i + 1;  // T_{i+1,i}
+      sc.bstride = kk;
+    }
+    sc.part = part[i & 1];
+    sc.dx = dQ ? dQ + i * ldq : nullptr;
+    const float* x = st.Q + i * ldc;
+    const float* xplus = last ? st.R : st.Q + (i + 1) * ldc;
+    float* lam = Lam + i * ldc;
+    SLQ_HIP(ctx, launch_lanczos_adj_lambda(s, xi, x, xplus, P, n, sc, lam));
+    // A lambda (decomp.py:337; the operator is symmetric)
+    SLQ_HIP(ctx, launch_matvec(s, op, lam, P, W, mvs, mv_floats));
+    // xi for step i - 1 (decomp.py:341) and that step's dots
+    SLQ_HIP(ctx, launch_lanczos_adj_xi(s, W, lam, last ? nullptr : Lam + (i + 1) * ldc, xplus,
+                                       last, i > 0 ? st.Q + (i - 1) * ldc : x, x, P, n, sc, xi,
+                                       i > 0 ? part[(i - 1) & 1] : nullptr));
+  }
+  // dv = ((xi . x_0) x_0 - xi) / |v| (decomp.py:305)
+  if (dv) SLQ_HIP(ctx, launch_lanczos_adj_initvec(s, xi, st.Q, st.c, P, n, part1, dv));
+  return param_cotangents(ctx, op, st.Q, Lam, ldc, P, k, dparams, dmatrix);
+}
+
+// Largest rank of the partial Cholesky / Woodbury kernels (LDS staging).// Largest rank of the partial Cholesky / Woodbury kernels (LDS staging).
 constexpr int64_t kMaxRank = 4096;
 
 int check_rank(slq_context* ctx, int64_t rank, int64_t n) {
@@ -771,6 +961,75 @@ int slq_hessenberg(slq_context* ctx, const slq_operator* op, const float* V0, in
     return fail(ctx, SLQ_EINVAL, "input or output pointer is NULL");
   SLQ_TRY(set_device(ctx));
   return hessenberg_core(ctx, *op, V0, P, k, Q, P * op->n, H, R, c);
+}
+
+int slq_arnoldi(slq_context* ctx, const slq_operator* op, const float* V0, int64_t P, int64_t k,
+                int reortho, float* Q, float* H, float* R, float* c) {
+  if (!ctx) return SLQ_EINVAL;
+  SLQ_TRY(check_operator(ctx, op));
+  if (k < 0 || k > op->n) return fail(ctx, SLQ_EINVAL_K, exceeds_msg(k, op->n));
+  if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of vectors P must be positive");
+  if (!V0 || !R || !c || (k > 0 && (!Q || !H)))
+    return fail(ctx, SLQ_EINVAL, "input or output pointer is NULL");
+  SLQ_TRY(set_device(ctx));
+  return hessenberg_core(ctx, *op, V0, P, k, Q, P * op->n, H, R, c, reortho != 0);
+}
+
+int slq_arnoldi_vjp(slq_context* ctx, const slq_operator* op, const float* V0, int64_t P,
+                    int64_t k, int reortho, int reortho_adjoint, const float* dQ, const float* dH,
+                    const float* dr, const float* dc, float* dv, double* dparams, float* dmatrix) {
+  if (!ctx) return SLQ_EINVAL;
+  SLQ_TRY(check_operator(ctx, op));
+  const int64_t n = op->n;
+  if (k < 0 || k > n) return fail(ctx, SLQ_EINVAL_K, exceeds_msg(k, n));
+  if (k == 0)
+    return fail(ctx, SLQ_EADJOINT_K0,
+                "Custom Hessenberg-adjoints are not implemented for num_matvecs = 0.");
+  if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of vectors P must be positive");
+  if (!V0) return fail(ctx, SLQ_EINVAL, "start-vector pointer is NULL");
+  if (op->kind == SLQ_OP_RBF && !dparams)
+    return fail(ctx, SLQ_EINVAL, "dparams (2 doubles) is NULL");
+  if (op->kind == SLQ_OP_DENSE && !dmatrix)
+    return fail(ctx, SLQ_EINVAL, "dmatrix (n x n floats) is NULL");
+  SLQ_TRY(set_device(ctx));
+  ForwardState st;
+  st.ldc = basis_stride(P, n);
+  SLQ_TRY(ensure_t(ctx, ctx->basis, size_t(k * st.ldc), &st.Q));
+  SLQ_TRY(ensure_t(ctx, ctx->hmat, size_t(P * k * k), &st.H));
+  SLQ_TRY(ensure_t(ctx, ctx->resid, size_t(P * n), &st.R));
+  SLQ_TRY(ensure_t(ctx, ctx->scal1, size_t(P), &st.c));
+  SLQ_TRY(hessenberg_core(ctx, *op, V0, P, k, st.Q, st.ldc, st.H, st.R, st.c, reortho != 0));
+  Timed tm(ctx, SLQ_KC_ADJOINT);
+  return arnoldi_reverse(ctx, *op, P, k, st, reortho_adjoint != 0, dQ, dH, dr, dc, dv, dparams,
+                         dmatrix);
+}
+
+int slq_lanczos_vjp(slq_context* ctx, const slq_operator* op, const float* V0, int64_t P,
+                    int64_t k, const float* dQ, const float* dalpha, const float* dbeta,
+                    const float* dxk, float* dv, double* dparams, float* dmatrix) {
+  if (!ctx) return SLQ_EINVAL;
+  SLQ_TRY(check_operator(ctx, op));
+  const int64_t n = op->n;
+  if (k < 0 || k > n) return fail(ctx, SLQ_EINVAL_K, exceeds_msg(k, n));
+  if (k == 0)
+    return fail(ctx, SLQ_EADJOINT_K0, "the Lanczos adjoint needs num_matvecs >= 1");
+  if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of vectors P must be positive");
+  if (!V0) return fail(ctx, SLQ_EINVAL, "start-vector pointer is NULL");
+  if (op->kind == SLQ_OP_RBF && !dparams)
+    return fail(ctx, SLQ_EINVAL, "dparams (2 doubles) is NULL");
+  if (op->kind == SLQ_OP_DENSE && !dmatrix)
+    return fail(ctx, SLQ_EINVAL, "dmatrix (n x n floats) is NULL");
+  SLQ_TRY(set_device(ctx));
+  ForwardState st;
+  st.ldc = basis_stride(P, n);
+  SLQ_TRY(ensure_t(ctx, ctx->basis, size_t(k * st.ldc), &st.Q));
+  SLQ_TRY(ensure_t(ctx, ctx->hmat, size_t(P * k * k), &st.H));
+  SLQ_TRY(ensure_t(ctx, ctx->resid, size_t(P * n), &st.R));
+  SLQ_TRY(ensure_t(ctx, ctx->scal1, size_t(P), &st.c));
+  SLQ_TRY(ensure_t(ctx, ctx->blast, size_t(P), &st.blast));
+  SLQ_TRY(lanczos_core(ctx, *op, V0, P, k, st.Q, st.ldc, st.H, st.blast, st.R, st.c));
+  Timed tm(ctx, SLQ_KC_ADJOINT);
+  return lanczos_reverse(ctx, *op, P, k, st, dQ, dalpha, dbeta, dxk, dv, dparams, dmatrix);
 }
 
 int slq_logdet(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P, int64_t k,

@@ -98,6 +98,10 @@ hipError_t launch_scale_rows(hipStream_t s, const float* x, const float* scale, 
 // db_i = db1[p * sstride] + db2[p * sstride] (db1 null: 0); part: the step's
 // partial dots (P x num_chunks(n) x 3: [lambda_plus . x_i, x_{i+1} . xi,
 // x_i . xi], null: all zero).
+// General cotangents add dx (the cotangent of x_i, subtracted in the xi
+// update), separate strides for db1 / db2 and da (dbstride / dastride, 0:
+// sstride; db2 may be null) and xplus_res: the xplus of the lambda kernel is the residual
+// b x_{i+1} (the last step), so its coefficient is mu / b.
 struct LanczosAdjStep {
   const float* a = nullptr;
   const float* b = nullptr;
@@ -107,7 +111,22 @@ struct LanczosAdjStep {
   const float* db2 = nullptr;
   int64_t sstride = 0;
   const float* part = nullptr;
+  const float* dx = nullptr;
+  int64_t dbstride = 0;
+  int64_t dastride = 0;
+  int xplus_res = 0;
 };
+// Start of the general Lanczos adjoint (decomp.py:290-297): xi = -dxk (dxk
+// null: 0) and the partial dots of step k - 1 [0, x_k . xi, x_{k-1} . xi]
+// with x_k = R / blast.
+hipError_t launch_lanczos_adj_init(hipStream_t s, const float* dxk, const float* R,
+                                   const float* blast, const float* xkm1, int64_t P, int64_t n,
+                                   float* xi, float* part_out);
+// Cotangent of the start vector (decomp.py:305): dv = c ((xi . x0) x0 - xi)
+// with c = 1 / |v| per probe and the final xi.
+hipError_t launch_lanczos_adj_initvec(hipStream_t s, const float* xi, const float* x0,
+                                      const float* c, int64_t P, int64_t n, float* part,
+                                      float* dv);
 // lambda = -xi / b + mu x_{i+1} + nu x_i (xi null: 0; xplus null: the mu term
 // is dropped, valid where mu = 0)
 hipError_t launch_lanczos_adj_lambda(hipStream_t s, const float* xi, const float* x,
@@ -181,18 +200,33 @@ hipError_t launch_mean_and_sem(hipStream_t s, const double* v, int64_t P, double
 // ---- reverse pass (k_adjoint.hip) ------------------------------------------
 // One backward step idx: Gamma row idx and the combination coefficients
 // (coef, per probe 2 k + 4 floats) from the chunk partials of Q^T (A^T lambda).
+// Pg / gam (optional): Pi_gamma and gamma of general cotangents (see
+// launch_adjoint_pigamma_row / launch_adjoint_gamma_init); null: the SLQ
+// composition's dQ = dr = dc = 0 with a tridiagonal dH.
 hipError_t launch_adjoint_coef(hipStream_t s, const float* H, const float* dH, float* Gam,
                                const float* part, int pstride, int64_t P, int64_t n, int64_t k,
-                               int64_t idx, float* coef);
+                               int64_t idx, float* coef, const float* Pg = nullptr,
+                               const float* gam = nullptr);
+// Row i of Pi_gamma = -dc c e_1 e_1^T + H dH^T - dQ^T Q per probe (P x k x k,
+// column-major) from the chunk partials of Q^T dQ_i (part, null: dQ = 0);
+// c, dc per probe (dc null: 0).
+hipError_t launch_adjoint_pigamma_row(hipStream_t s, const float* H, const float* dH,
+                                      const float* part, int pstride, const float* c,
+                                      const float* dc, int64_t P, int64_t n, int64_t k, int64_t i,
+                                      float* Pg);
+// gamma = dH e_K - Q^T dr (part: chunk partials of Q^T dr, null: dr = 0) into
+// gam (P x k) and the coefficients of lambda = dr + Q gamma (x = dr).
+hipError_t launch_adjoint_gamma_init(hipStream_t s, const float* dH, const float* part, int pstride,
+                                     int64_t P, int64_t n, int64_t k, float* gam, float* coef);
 // Coefficients for lambda = Q dH[:, k - 1] (the start of the recursion).
 hipError_t launch_adjoint_init_coef(hipStream_t s, const float* dH, int64_t P, int64_t k,
                                     float* coef);
-// y = scale (cr r + cl x + cv v + sum_{j < ncq} gq_j Q_j + sum_{jl0 <= j < k} gl_j Lam_j), per
-// probe (coef layout above); r, x, v may be null; save (if given) receives x.
+// y = scale (cr r + cl x + cv v + d + sum_{j < ncq} gq_j Q_j + sum_{jl0 <= j < k} gl_j Lam_j),
+// per probe (coef layout above); r, x, v, d may be null; save (if given) receives x.
 hipError_t launch_adjoint_combine(hipStream_t s, const float* Q, const float* Lam, int64_t ldc,
                                   const float* r, const float* x, const float* v, float* y,
                                   float* save, int64_t P, int64_t n, int64_t k, int ncq, int jl0,
-                                  const float* coef);
+                                  const float* coef, const float* d = nullptr);
 // out (n x n, row-major) = sum_{c < K} L_c R_c^T over columns c = (idx, p) at
 // base + idx * ldc + p * n (K = k P), FP32 MFMA.
 hipError_t launch_outer_sum(hipStream_t s, const float* L, const float* R, int64_t ldc, int64_t P,

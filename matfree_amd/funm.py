@@ -50,11 +50,29 @@ class _DenseFunmSymEigh:
     def is_log(self):
         return self.code == _lib.MATFUNS["log"]
 
+    def torch_fun(self):
+        """The matrix function on torch tensors (torch's spelling of a known
+        function, else ``matfun`` itself)."""
+        code = self.code
+        for name, fn in (("log", torch.log), ("exp", torch.exp), ("sin", torch.sin),
+                         ("cos", torch.cos), ("sqrt", torch.sqrt),
+                         ("reciprocal", torch.reciprocal)):
+            if code == _lib.MATFUNS[name]:
+                return fn
+        return self.matfun
+
     def __call__(self, dense_matrix):
-        raise NotImplementedError(
-            "matfree_amd fuses f(T) into its FP64 tridiagonal eigen-solve kernel; "
-            "use it through monte_carlo_funm_sym / monte_carlo_funm_sym_logdet"
-        )
+        """``U diag(f(w)) U^T`` with ``w, U = eigh(dense_matrix)`` (funm.py:330-333).
+
+        For a small dense symmetric matrix given directly (the integrands fuse
+        this step into libslq's FP64 tridiagonal kernels instead): torch's
+        ``eigh`` in the input's precision on its device, differentiable by
+        autograd."""
+        M = dense_matrix if isinstance(dense_matrix, torch.Tensor) else torch.as_tensor(
+            np.asarray(dense_matrix))
+        w, U = torch.linalg.eigh(M)
+        fw = self.torch_fun()(w)
+        return (U * fw[..., None, :]) @ U.transpose(-1, -2)
 
 
 def dense_funm_sym_eigh(matfun):
@@ -373,16 +391,38 @@ class _FunmLanczosSym:
             )
         self.reortho = _reortho_of(tridiag_sym)
         self.code = dense_funm.code
+        self.fun = dense_funm.torch_fun()
+        self.tridiag_sym = tridiag_sym
         self.num_matvecs = tridiag_sym.num_matvecs
 
     def batched(self, matvec, vecs, *parameters):
-        """``f(A) v`` for every row of ``vecs`` (P, n) in one call."""
+        """``f(A) v`` for every row of ``vecs`` (P, n) in one call.
+
+        When ``vecs`` or a tensor parameter requires grad, the product is
+        composed as the reference writes it (funm.py:138-147): the
+        differentiable decomposition (GPU adjoint), then ``f(T) e_1`` by
+        torch's FP64 ``eigh`` of the small k x k T and ``|v| Q^T (.)``, so
+        autograd reaches ``vecs`` and the parameters; otherwise one fused
+        libslq call."""
+        if decomp._differentiable(vecs, parameters):
+            return self._composed(matvec, vecs, parameters)
         return funm_vectors(matvec, vecs, self.num_matvecs, parameters, self.code, self.reortho)
+
+    def _composed(self, matvec, vecs, parameters):
+        V = vecs if isinstance(vecs, torch.Tensor) else torch.as_tensor(vecs)
+        device = _runtime.target_device(V, *parameters)
+        V = V.to(device=device, dtype=torch.float32)
+        length = torch.linalg.vector_norm(V, dim=1, keepdim=True)  # funm.py:139
+        dec = self.tridiag_sym.batched(matvec, V / length, *parameters)  # :144
+        T = dec.J_small if self.tridiag_sym.materialize else decomp._todense_tridiag_sym(
+            *dec.J_small)
+        w, U = torch.linalg.eigh(T.to(torch.float64))  # dense_funm (:145)
+        y = U @ (self.fun(w) * U[:, 0, :])[..., None]  # f(T) e_1 (:146-147)
+        return length * (dec.Q_tall.transpose(1, 2) @ y.to(torch.float32))[..., 0]
 
     def __call__(self, matvec, vec, *parameters):
         flat, unravel = _tree.ravel(vec, decomp._host_or_gpu(vec, *parameters))
-        out = funm_vectors(matvec, flat[None, :], self.num_matvecs, parameters, self.code,
-                           self.reortho)
+        out = self.batched(matvec, flat[None, :], *parameters)
         return unravel(out[0])
 
 

@@ -351,12 +351,13 @@ def _lower(m):
     return m_tril - 0.5 * np.diag(np.diag(m_tril))
 
 
-def hessenberg_adjoint(operator, Q, H, r, c, dQ, dH, dr, dc):
-    """``_hessenberg_adjoint`` with reortho="full" (decomp.py:480-600), one probe.
+def hessenberg_adjoint(operator, Q, H, r, c, dQ, dH, dr, dc, reortho="full"):
+    """``_hessenberg_adjoint`` (decomp.py:480-600), one probe.
 
     Q: (n, k) columns = Krylov vectors; H: (k, k); r: (n,); c: scalar.
-    Returns ``(dv, dp)`` where dp is the accumulated parameter cotangent of
-    ``operator.param_vjp``.
+    ``reortho="none"`` skips the reorthogonalisation of the adjoint state
+    (decomp.py:575).  Returns ``(dv, dp)`` where dp is the accumulated
+    parameter cotangent of ``operator.param_vjp``.
     """
     n, k = Q.shape
     if k == 0:  # :483-486
@@ -381,9 +382,10 @@ def hessenberg_adjoint(operator, Q, H, r, c, dQ, dH, dr, dc):
 
     for idx in reversed(range(k)):  # :538 reverse scan
         # :576-584 reorthogonalisation of the adjoint state
-        Q_masked = reortho_mask[idx][None, :] * Q
-        rhs_masked = reortho_mask[idx] * dH.T[idx]
-        lambda_k = lambda_k - Q_masked @ (Q_masked.T @ lambda_k) + Q_masked @ rhs_masked
+        if reortho == "full":
+            Q_masked = reortho_mask[idx][None, :] * Q
+            rhs_masked = reortho_mask[idx] * dH.T[idx]
+            lambda_k = lambda_k - Q_masked @ (Q_masked.T @ lambda_k) + Q_masked @ rhs_masked
         # :587-589 transposed matvec and parameter cotangent
         q = Q[:, idx]
         vecmat_lambda = operator.apply_transpose(lambda_k[None, :].astype(operator.dtype))[0]

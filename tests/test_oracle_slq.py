@@ -249,6 +249,52 @@ def test_adjoint_full_depth_matches_dense_closed_form():
         assert np.isclose(np.sum(np.outer(y, y) * L * M), gi, rtol=1e-7)
 
 
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_hessenberg_adjoint_random_cotangents_central_differences(seed):
+    # general cotangents on every output (Q, H, r, c), the check of
+    # tests/test_decomp/test_hessenberg_adjoint.py: the adjoint's (dv, dA)
+    # against central differences of <cotangents, outputs> of the FP64
+    # forward along a random start-vector direction and a symmetric matrix
+    # direction (the adjoint applies A^T as A)
+    rng = np.random.default_rng(seed)
+    n, k = 12, 6
+    A0 = sym_from_eigs(rng.uniform(1.0, 2.0, n), seed)
+    v0 = rng.standard_normal(n)
+    dQ, dH, dr, dc = (rng.standard_normal((n, k)), rng.standard_normal((k, k)),
+                      rng.standard_normal(n), rng.standard_normal())
+    Q, H, r, c = slq.hessenberg_forward(slq.DenseOperator(A0, np.float64), k, v0, np.float64)
+    dv, dA = slq.hessenberg_adjoint(slq.DenseOperator(A0, np.float64), Q[0], H[0], r[0],
+                                    float(c[0]), dQ, dH, dr, dc)
+
+    def f(v, A):
+        Q, H, r, c = slq.hessenberg_forward(slq.DenseOperator(A, np.float64), k, v, np.float64)
+        return np.sum(dQ * Q[0]) + np.sum(dH * H[0]) + dr @ r[0] + dc * c[0]
+
+    dv_dir = rng.standard_normal(n)
+    D = rng.standard_normal((n, n))
+    D = D + D.T
+    h = 1e-6
+    fd = (f(v0 + h * dv_dir, A0 + h * D) - f(v0 - h * dv_dir, A0 - h * D)) / (2 * h)
+    assert np.isclose(dv @ dv_dir + np.sum(dA * D), fd, rtol=1e-6)
+
+
+def test_hessenberg_adjoint_without_reorthogonalisation_agrees_in_fp64():
+    # decomp.py:575: the adjoint's reorthogonalisation only stabilises; with
+    # an orthonormal FP64 basis both forms give the same cotangents
+    rng = np.random.default_rng(4)
+    n, k = 20, 8
+    A0 = sym_from_eigs(rng.uniform(1.0, 2.0, n), 4)
+    v0 = rng.standard_normal(n)
+    cot = (rng.standard_normal((n, k)), rng.standard_normal((k, k)), rng.standard_normal(n),
+           rng.standard_normal())
+    op = slq.DenseOperator(A0, np.float64)
+    Q, H, r, c = slq.hessenberg_forward(op, k, v0, np.float64)
+    full = slq.hessenberg_adjoint(op, Q[0], H[0], r[0], float(c[0]), *cot, reortho="full")
+    none = slq.hessenberg_adjoint(op, Q[0], H[0], r[0], float(c[0]), *cot, reortho="none")
+    for a, b in zip(full, none):
+        assert np.allclose(a, b, rtol=1e-8, atol=1e-10)
+
+
 def test_adjoint_k0_raises():
     with pytest.raises(ValueError, match="= 0"):
         slq.hessenberg_adjoint(None, np.zeros((3, 0)), np.zeros((0, 0)), np.zeros(3), 1.0,
