@@ -188,7 +188,7 @@ int slq_lanczos_vjp(slq_context* ctx, const slq_operator* op, const float* V0, i
                     int64_t k, const float* dQ, const float* dalpha, const float* dbeta,
                     const float* dxk, float* dv, double* dparams, float* dmatrix);
 
-/* SLQ log-determinant integrand and estimator for P probes z_p:/* SLQ log-determinant integrand and estimator for P probes z_p:
+/* SLQ log-determinant integrand and estimator for P probes z_p:
  *   per_probe[p] = |z_p|^2 [log T_p]_00 with T_p = (H_p + H_p^T)/2 from
  *   slq_hessenberg on z_p / |z_p|, evaluated in FP64 (eigenvalues and first
  *   eigenvector components of the tridiagonal T_p);
