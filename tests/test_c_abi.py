@@ -64,7 +64,8 @@ int main(void) {
         c = os.path.join(d, "layout.c")
         exe = os.path.join(d, "layout")
         open(c, "w").write(src)
-        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        subprocess.run(["gcc", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), c, "-o", exe],
+                       check=True)
         got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True).stdout.split()]
     S = _lib.OperatorStruct
     want = [ctypes.sizeof(S)] + [getattr(S, f).offset for f in
