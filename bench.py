@@ -268,6 +268,18 @@ def serve(args, device):
         grad = r % SERVE_GRAD_EVERY == SERVE_GRAD_EVERY - 1
         reqs.append(serving.Request(op, params, n, P, k, seed=r, grad=grad))
         meta.append((kind, n, P, k, grad))
+    # start-up: one small untimed value and value+gradient solve per operator
+    # kind, as a server does before taking traffic (loads the library and its
+    # kernels, creates the context); every request below is then timed
+    wn, wP, wk = 1000, 16, 8
+    wX = torch.as_tensor(problems.rbf_points(wn, seed=999), device=device)
+    wA = problems.dense_spd_device(wn, seed=999, shift=0.1, device=device)
+    wZ = torch.ones((wP, wn), device=device)
+    for wop, wpar in ((operators.RBF(wX, precision="bf16x2"), (0.5, 1e-2)),
+                      (operators.DenseMatvec(precision="fp32"), (wA,))):
+        funm.logdet_per_probe(wop, wZ, wk, wpar)
+        funm.logdet_value_and_grad(wop, wZ, wk, wpar)
+    del wX, wA, wZ
     torch.cuda.synchronize()
     lat, alone = [], []
     for q in reqs:
@@ -318,6 +330,8 @@ def serve(args, device):
         finish[i] = busy = start + lat[i]
     fifo = finish - arrive * 1e3
     st = server.stats
+    val = np.array([t for t, m in zip(lat, meta) if not m[4]])
+    grd = np.array([t for t, m in zip(lat, meta) if m[4]])
     del reqs, mixed, out, server
     torch.cuda.empty_cache()
     return {
@@ -327,6 +341,10 @@ def serve(args, device):
         "p99_ms": float(np.percentile(lat, 99)),
         "mean_ms": float(lat.mean()),
         "max_ms": float(lat.max()),
+        "value_only": {"requests": int(len(val)), "p50_ms": float(np.percentile(val, 50)),
+                       "p99_ms": float(np.percentile(val, 99))},
+        "with_gradient": {"requests": int(len(grd)), "p50_ms": float(np.percentile(grd, 50)),
+                          "p99_ms": float(np.percentile(grd, 99))},
         "stream": {
             "arrival_rate_per_s": rate,
             "slots": SERVE_SLOTS,
@@ -671,6 +689,9 @@ def main():
         torch.cuda.empty_cache()
         out["serve"] = serve(args, device)
         sv = out["serve"]
+        log(f"bench: serve alone value-only p50 {sv['value_only']['p50_ms']:.2f} ms, p99 "
+            f"{sv['value_only']['p99_ms']:.2f} ms; with gradient p50 "
+            f"{sv['with_gradient']['p50_ms']:.2f} ms, p99 {sv['with_gradient']['p99_ms']:.2f} ms")
         log(f"bench: serve alone p50 {sv['p50_ms']:.2f} ms, p99 {sv['p99_ms']:.2f} ms; stream "
             f"p50 {sv['stream']['p50_ms']:.2f} ms, p99 {sv['stream']['p99_ms']:.2f} ms")
     print(json.dumps(out), flush=True)

@@ -12,6 +12,8 @@
 //  * k_outer_sum: the parameter cotangents from S = sum lambda q^T -- the
 //    stored matrix's dA = S, or the RBF contractions <S, dK/dl>, <S, I>;
 //    k_sum_fixed: their deterministic FP64 reduction.
+#include <type_traits>
+
 #include "slq_internal.h"
 
 namespace slq {
@@ -50,7 +52,7 @@ __global__ void __launch_bounds__(256)
   float* cp = coef + p * (2 * k + 4);
   for (int64_t j = threadIdx.x; j <= idx; j += blockDim.x) {
     float s = 0.f;
-    for (int64_t b = 0; b < nb; ++b) s += part[(p * nb + b) * pstride + j];
+    s = ordered_sum(part + p * nb * pstride + j, nb, pstride);
     float pg;
     if (Pg) {
       pg = Pg[p * k * k + idx + j * k];
@@ -87,6 +89,10 @@ __global__ void __launch_bounds__(256)
 // y = scale (cr r + cl x + cv v + d + sum_{j < ncq} gq_j Q_j + sum_{jl0 <= j < k} gl_j L_j)
 // per probe, coefficients from coef (stride 2 k + 4); r, x, v, d may be null
 // (their terms are skipped).  With save (non-null) x is also copied there.
+// V consecutive elements per thread (V = 4: float4 groups, rows of n % 4 == 0
+// with 16-byte aligned bases; V = 1 otherwise, and for small P n, where one
+// element per thread gives the launch four times the waves); the basis
+// columns eight at a time, their loads in flight together.
 template <int V>
 __global__ void __launch_bounds__(kStreamThreads)
     k_adjoint_combine(const float* __restrict__ Q, const float* __restrict__ Lam, int64_t ldc,
@@ -94,6 +100,7 @@ __global__ void __launch_bounds__(kStreamThreads)
                       const float* __restrict__ v, float* __restrict__ y, float* __restrict__ save,
                       int64_t n, int64_t k, int ncq, int jl0, const float* __restrict__ coef,
                       const float* __restrict__ d) {
+  using VT = typename std::conditional<V == 4, float4, float>::type;
   extern __shared__ float cs[];
   const int64_t p = blockIdx.y;
   const int t = threadIdx.x;
@@ -101,38 +108,45 @@ __global__ void __launch_bounds__(kStreamThreads)
   for (int64_t j = t; j < 2 * k + 4; j += blockDim.x) cs[j] = cp[j];
   __syncthreads();
   const float cr = cs[2 * k], cl = cs[2 * k + 1], cv = cs[2 * k + 2], sc = cs[2 * k + 3];
-  const int64_t per_block = int64_t(kStreamThreads) * V;
-  const int64_t i = int64_t(blockIdx.x) * per_block + int64_t(t) * V;
+  const int64_t i = (int64_t(blockIdx.x) * kStreamThreads + t) * V;
   if (i >= n) return;
   const int64_t off = p * n + i;
   float acc[V];
 #pragma unroll
   for (int u = 0; u < V; ++u) acc[u] = 0.f;
-  auto axpy = [&](float a, const float* src) {
+  auto fma_in = [&](float a, const VT& q) {
     if constexpr (V == 4) {
-      const float4 q = *reinterpret_cast<const float4*>(src);
       acc[0] = fmaf(a, q.x, acc[0]);
       acc[1] = fmaf(a, q.y, acc[1]);
       acc[2] = fmaf(a, q.z, acc[2]);
       acc[3] = fmaf(a, q.w, acc[3]);
     } else {
-      acc[0] = fmaf(a, src[0], acc[0]);
+      acc[0] = fmaf(a, q, acc[0]);
     }
   };
-  for (int j = 0; j < ncq; ++j) axpy(cs[j], Q + int64_t(j) * ldc + off);
-  for (int64_t j = jl0; j < k; ++j) axpy(cs[k + j], Lam + j * ldc + off);
-  if (r) axpy(cr, r + off);
-  if (x) axpy(cl, x + off);
-  if (v) axpy(cv, v + off);
-  if (d) axpy(1.f, d + off);
-  float out[V];
+  auto cols = [&](const float* base, int64_t j0, int64_t j1, const float* c) {
+    int64_t j = j0;
+    for (; j + 8 <= j1; j += 8) {
+      VT q[8];
 #pragma unroll
-  for (int u = 0; u < V; ++u) out[u] = sc * acc[u];
+      for (int u = 0; u < 8; ++u) q[u] = *reinterpret_cast<const VT*>(base + (j + u) * ldc + off);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) fma_in(c[j + u], q[u]);
+    }
+    for (; j < j1; ++j) fma_in(c[j], *reinterpret_cast<const VT*>(base + j * ldc + off));
+  };
+  cols(Q, 0, ncq, cs);
+  if (Lam) cols(Lam, jl0, k, cs + k);
+  if (r) fma_in(cr, *reinterpret_cast<const VT*>(r + off));
+  if (x) fma_in(cl, *reinterpret_cast<const VT*>(x + off));
+  if (v) fma_in(cv, *reinterpret_cast<const VT*>(v + off));
+  if (d) fma_in(1.f, *reinterpret_cast<const VT*>(d + off));
   if constexpr (V == 4) {
-    *reinterpret_cast<float4*>(y + off) = make_float4(out[0], out[1], out[2], out[3]);
+    *reinterpret_cast<float4*>(y + off) =
+        make_float4(sc * acc[0], sc * acc[1], sc * acc[2], sc * acc[3]);
     if (save) *reinterpret_cast<float4*>(save + off) = *reinterpret_cast<const float4*>(x + off);
   } else {
-    y[off] = out[0];
+    y[off] = sc * acc[0];
     if (save) save[off] = x[off];
   }
 }
@@ -188,7 +202,7 @@ __global__ void __launch_bounds__(256)
     for (int64_t m = 0; m < k; ++m) v = fmaf(Hp[i + m * k], dHp[j + m * k], v);
     if (part) {
       float g = 0.f;
-      for (int64_t b = 0; b < nb; ++b) g += part[(p * nb + b) * pstride + j];
+      g = ordered_sum(part + p * nb * pstride + j, nb, pstride);
       v -= g;
     }
     if (dc && i == 0 && j == 0) v -= dc[p] * c[p];
@@ -212,7 +226,7 @@ __global__ void __launch_bounds__(256)
       v = dHp[j + (k - 1) * k];
       if (part) {
         float g = 0.f;
-        for (int64_t b = 0; b < nb; ++b) g += part[(p * nb + b) * pstride + j];
+        g = ordered_sum(part + p * nb * pstride + j, nb, pstride);
         v -= g;
       }
       gam[p * k + j] = v;
@@ -238,6 +252,55 @@ constexpr int kOT = 128;         // output tile edge
 constexpr int kOK = 8;           // columns per stage
 constexpr int kOLd = kOT + 4;    // padded LDS row
 
+// Order of the output tiles over the workgroups.  Workgroups go to the 8
+// XCDs round-robin by their index, and each XCD caches in its own L2, so
+// workgroup b (XCD b % 8, per-XCD slot b / 8) takes tile (r / S, r % S) of
+// square q = (b / 8) / S^2 * 8 + b % 8 with r = (b / 8) % S^2: the S x S
+// (S = 8) workgroups an XCD runs together read S row panels of L and S of R
+// instead of S^2 of each.  Squares are laid over the T x T tiles row-major;
+// workgroups whose tile lies outside (partial squares, the last ragged group
+// of eight squares) write zeros.
+struct TileMap {
+  int64_t T, S, ST, blocks;
+};
+
+inline TileMap tile_map(int64_t n) {
+  TileMap m;
+  m.T = (n + kOT - 1) / kOT;
+  m.S = m.T < 8 ? m.T : 8;
+  m.ST = (m.T + m.S - 1) / m.S;
+  m.blocks = (m.ST * m.ST + 7) / 8 * 8 * m.S * m.S;
+  return m;
+}
+
+__device__ __forceinline__ bool tile_of_block(const TileMap& m, int64_t b, int64_t& ti,
+                                              int64_t& tj) {
+  const int64_t slot = b / 8, x = b % 8;
+  const int64_t q = slot / (m.S * m.S), r = slot % (m.S * m.S);
+  const int64_t g = q * 8 + x;
+  if (g >= m.ST * m.ST) return false;
+  ti = (g / m.ST) * m.S + r / m.S;
+  tj = (g % m.ST) * m.S + r % m.S;
+  return ti < m.T && tj < m.T;
+}
+
+// Column c = (idx, p) of a Krylov-layout basis (c = idx P + p) walked in
+// steps of a fixed stride without a division per step.
+struct ColWalk {
+  int64_t idx, p;
+  __device__ __forceinline__ ColWalk(int64_t c, int64_t P) : idx(c / P), p(c % P) {}
+  __device__ __forceinline__ void advance(int64_t step, int64_t P) {
+    p += step;
+    while (p >= P) {
+      p -= P;
+      ++idx;
+    }
+  }
+  __device__ __forceinline__ const float* ptr(const float* base, int64_t ldc, int64_t n) const {
+    return base + idx * ldc + p * n;
+  }
+};
+
 // RBF contraction mode: instead of storing S = sum_c L_c R_c^T, reduce
 //   sum_ij S_ij |x_i - x_j|^2 exp2(c |x_i - x_j|^2)   (l^3 times <S, dK/dl>)
 // and sum_i S_ii (<S, I>) per workgroup into red[2 * block + {0, 1}] (FP64).
@@ -248,33 +311,85 @@ struct RBFContract {
   double* red;
 };
 
+// The points of rows r0 .. r0 + kOT - 1 as float4 (zeros past n and past dim)
+// into LDS, for the contraction epilogues.
+__device__ __forceinline__ void stage_points(float4* xs, const RBFContract& rc, int64_t r0,
+                                             int64_t n) {
+  for (int t = threadIdx.x; t < kOT; t += blockDim.x) {
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int64_t i = r0 + t;
+    if (i < n) {
+      const float* x = rc.X + i * rc.dim;
+      v.x = x[0];
+      if (rc.dim > 1) v.y = x[1];
+      if (rc.dim > 2) v.z = x[2];
+      if (rc.dim > 3) v.w = x[3];
+    }
+    xs[t] = v;
+  }
+}
+
+// One 32 x 32 accumulator of S in the contraction: the lane holds column
+// j0 + jl at rows i0 + ib + 8 (s / 4) + (s % 4), s = 0..15.  Adds
+// sum_s S_ij d2 exp2(coef d2) to part and the diagonal entries to trace.
+__device__ __forceinline__ void contract_acc(const floatx16& acc, const float4* xi,
+                                             const float4* xj, int ib, int jl, int64_t i0,
+                                             int64_t j0, int64_t n, float coef, float& part,
+                                             double& trace) {
+  const int64_t j = j0 + jl;
+  if (j >= n) return;
+  const float4 pj = xj[jl];
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int il = ib + 8 * (s / 4) + (s % 4);
+    const int64_t i = i0 + il;
+    if (i >= n) continue;
+    const float4 pi = xi[il];
+    float d2 = 0.f, df;
+    df = pi.x - pj.x;
+    d2 = fmaf(df, df, d2);
+    df = pi.y - pj.y;
+    d2 = fmaf(df, df, d2);
+    df = pi.z - pj.z;
+    d2 = fmaf(df, df, d2);
+    df = pi.w - pj.w;
+    d2 = fmaf(df, df, d2);
+    part = fmaf(acc[s], d2 * __builtin_amdgcn_exp2f(coef * d2), part);
+    if (i == j) trace += double(acc[s]);
+  }
+}
+
 template <bool CONTRACT>
 __global__ void __launch_bounds__(256)
     k_outer_sum(const float* __restrict__ L, const float* __restrict__ R, int64_t ldc, int64_t P,
-                int64_t n, int64_t K, float* __restrict__ out, RBFContract rc) {
+                int64_t n, int64_t K, float* __restrict__ out, RBFContract rc, TileMap tmap) {
   __shared__ __attribute__((aligned(16))) float Ls[2][kOK * kOLd];
   __shared__ __attribute__((aligned(16))) float Rs[2][kOK * kOLd];
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6, r = lane & 31, h = lane >> 5;
-  const int64_t i0 = int64_t(blockIdx.y) * kOT, j0 = int64_t(blockIdx.x) * kOT;
+  int64_t ti, tj;
+  if (!tile_of_block(tmap, blockIdx.x, ti, tj)) {
+    if (CONTRACT && t == 0) {
+      rc.red[2 * int64_t(blockIdx.x)] = 0.0;
+      rc.red[2 * int64_t(blockIdx.x) + 1] = 0.0;
+    }
+    return;
+  }
+  const int64_t i0 = ti * kOT, j0 = tj * kOT;
   // staging: thread t moves float4 (t & 31) of stage column t >> 5
   const int sc = t >> 5, s4 = (t & 31) * 4;
-  auto col_ptr = [&](const float* base, int64_t c) {
-    const int64_t idx = c / P, p = c - idx * P;
-    return base + idx * ldc + p * n;
-  };
-  auto load = [&](const float* base, int64_t c0, int64_t e0) {
+  ColWalk lw(sc, P), rw(sc, P);
+  auto load = [&](const float* base, const ColWalk& cw, int64_t c0, int64_t e0) {
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     const int64_t c = c0 + sc, e = e0 + s4;
     if (c < K) {
-      const float* src = col_ptr(base, c) + e;
-      if (e + 3 < n && (n & 3) == 0) {
-        v = *reinterpret_cast<const float4*>(src);
+      const float* src = cw.ptr(base, ldc, n) + e;
+      if (e + 4 <= n) {
+        __builtin_memcpy(&v, src, sizeof(v));  // dword alignment suffices on gfx950
       } else {
         if (e + 0 < n) v.x = src[0];
         if (e + 1 < n) v.y = src[1];
         if (e + 2 < n) v.z = src[2];
-        if (e + 3 < n) v.w = src[3];
       }
     }
     return v;
@@ -286,7 +401,7 @@ __global__ void __launch_bounds__(256)
     for (int e = 0; e < 16; ++e) acc[c][e] = 0.f;
 
   const int64_t nst = (K + kOK - 1) / kOK;
-  float4 lv = load(L, 0, i0), rv = load(R, 0, j0);
+  float4 lv = load(L, lw, 0, i0), rv = load(R, rw, 0, j0);
   *reinterpret_cast<float4*>(&Ls[0][sc * kOLd + s4]) = lv;
   *reinterpret_cast<float4*>(&Rs[0][sc * kOLd + s4]) = rv;
   __syncthreads();
@@ -294,8 +409,10 @@ __global__ void __launch_bounds__(256)
     const int b = int(st & 1);
     const bool more = st + 1 < nst;
     if (more) {
-      lv = load(L, (st + 1) * kOK, i0);
-      rv = load(R, (st + 1) * kOK, j0);
+      lw.advance(kOK, P);
+      rw.advance(kOK, P);
+      lv = load(L, lw, (st + 1) * kOK, i0);
+      rv = load(R, rw, (st + 1) * kOK, j0);
     }
 #pragma unroll
     for (int q = 0; q < kOK / 2; ++q) {
@@ -326,35 +443,15 @@ __global__ void __launch_bounds__(256)
       }
     }
   } else {
+    __shared__ float4 xi[kOT], xj[kOT];
+    stage_points(xi, rc, i0, n);
+    stage_points(xj, rc, j0, n);
+    __syncthreads();
     double sd = 0.0, st2 = 0.0;
-    float xi[16][3];
-#pragma unroll
-    for (int s = 0; s < 16; ++s) {
-      const int64_t i = i0 + 32 * wave + 8 * (s / 4) + 4 * h + (s % 4);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) xi[s][d] = (i < n && d < rc.dim) ? rc.X[i * rc.dim + d] : 0.f;
-    }
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      const int64_t j = j0 + 32 * c + r;
-      if (j >= n) continue;
-      float xj[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) xj[d] = d < rc.dim ? rc.X[j * rc.dim + d] : 0.f;
       float part = 0.f;
-#pragma unroll
-      for (int s = 0; s < 16; ++s) {
-        const int64_t i = i0 + 32 * wave + 8 * (s / 4) + 4 * h + (s % 4);
-        if (i >= n) continue;
-        float d2 = 0.f;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const float df = xi[s][d] - xj[d];
-          d2 = fmaf(df, df, d2);
-        }
-        part = fmaf(acc[c][s], d2 * __builtin_amdgcn_exp2f(rc.coef * d2), part);
-        if (i == j) st2 += double(acc[c][s]);
-      }
+      contract_acc(acc[c], xi, xj, 32 * wave + 4 * h, 32 * c + r, i0, j0, n, rc.coef, part, st2);
       sd += double(part);
     }
     __shared__ double red[2][256];
@@ -369,10 +466,200 @@ __global__ void __launch_bounds__(256)
       __syncthreads();
     }
     if (t == 0) {
-      const int64_t b = int64_t(blockIdx.y) * gridDim.x + blockIdx.x;
+      const int64_t b = blockIdx.x;
       rc.red[2 * b] = red[0][0];
       rc.red[2 * b + 1] = red[1][0];
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The RBF contraction of k_outer_sum<true> on the BF16 matrix cores
+// (v_mfma_f32_32x32x16_bf16), for operators whose block matvec runs in a
+// BF16 precision: both operands split into hi = bf16(x) and
+// lo = bf16(x - hi) and S accumulated in FP32 as hi*hi + hi*lo + lo*hi (the
+// products are exact in FP32; the dropped lo*lo term is ~2^-16 relative).
+// A workgroup owns a 128 x 128 tile of S, each wave a 64 x 64 quarter
+// (2 x 2 accumulators).  The contraction runs over the K Krylov columns in
+// stages of 16.  Thread t stages element t & 127 of the tile's rows of L and
+// of R for the eight columns 8 (t >> 7) .. + 7 of the stage: eight coalesced
+// 4-byte loads per operand (the column walk is the same for the whole wave),
+// split, and one 16-byte LDS write per plane into [element][column] rows of
+// 24 BF16 (48 bytes: the 16-byte accesses of 8 consecutive lanes cover all
+// banks); lane (r, h) of an MFMA then reads columns 8 h .. 8 h + 7 of its
+// element.  Elements past n are read clamped to the last one (their entries
+// of S are never used); columns past K are zeroed in L.  Double-buffered,
+// one barrier per stage; the next stage's loads are in flight during the
+// MFMAs.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+typedef float f32x2v __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
+constexpr int kOBK = 16;   // Krylov columns per stage
+constexpr int kOBLd = 24;  // LDS row of BF16 (48 bytes)
+
+__device__ __forceinline__ void split_bf16_pair(float x0, float x1, uint32_t& hi, uint32_t& lo) {
+  const f32x2v x = {x0, x1};
+  hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2v));
+  const f32x2v h = {__builtin_bit_cast(float, hi << 16), __builtin_bit_cast(float, hi & 0xffff0000u)};
+  lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(x - h, bf16x2v));
+}
+
+__global__ void __launch_bounds__(256)
+    k_outer_rbf_bf16(const float* __restrict__ L, const float* __restrict__ R, int64_t ldc,
+                     int64_t P, int64_t n, int64_t K, RBFContract rc, TileMap tmap) {
+  // [buffer][plane hi/lo][element][column]
+  __shared__ __attribute__((aligned(16))) __bf16 Ls[2][2][kOT * kOBLd];
+  __shared__ __attribute__((aligned(16))) __bf16 Rs[2][2][kOT * kOBLd];
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6, r = lane & 31, h = lane >> 5;
+  const int wr = wave >> 1, wc = wave & 1;  // the wave's 64 x 64 quarter
+  int64_t ti, tj;
+  if (!tile_of_block(tmap, blockIdx.x, ti, tj)) {
+    if (t == 0) {
+      rc.red[2 * int64_t(blockIdx.x)] = 0.0;
+      rc.red[2 * int64_t(blockIdx.x) + 1] = 0.0;
+    }
+    return;
+  }
+  const int64_t i0 = ti * kOT, j0 = tj * kOT;
+  const int el = t & (kOT - 1);
+  const int half = __builtin_amdgcn_readfirstlane(t >> 7);  // uniform in the wave
+  // the element offsets of this lane in L's and R's columns (clamped into the row)
+  const int64_t li = i0 + el < n ? i0 + el : n - 1;
+  const int64_t rj = j0 + el < n ? j0 + el : n - 1;
+  // first column of the half in the current stage, as (idx, p)
+  int64_t cidx = (8 * half) / P, cp = (8 * half) % P;
+  const int64_t step_i = kOBK / P, step_p = kOBK % P;
+
+  const int64_t last = ((K - 1) / P) * ldc + ((K - 1) % P) * n;  // column K - 1
+  auto load = [&](int64_t c0, float (&lv)[8], float (&rv)[8]) {
+    // the eight column offsets first (the same for the whole wave), columns
+    // past K read column K - 1; then all sixteen loads
+    int64_t off[8];
+    int64_t idx = cidx, p = cp;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      off[u] = c0 + 8 * half + u < K ? idx * ldc + p * n : last;
+      if (++p == P) {
+        p = 0;
+        ++idx;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      lv[u] = L[off[u] + li];
+      rv[u] = R[off[u] + rj];
+    }
+  };
+  // split and store to LDS; c0 >= 0: the L operand of the stage at column c0,
+  // whose columns past K are zeroed here (after the MFMAs that overlap the
+  // loads, not before)
+  auto stage = [&](__bf16 (*dst)[kOT * kOBLd], const float (&v)[8], int64_t c0) {
+    float x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = (c0 < 0 || c0 + 8 * half + u < K) ? v[u] : 0.f;
+    u32x4v hw, lw;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      uint32_t a, b;
+      split_bf16_pair(x[2 * q], x[2 * q + 1], a, b);
+      hw[q] = a;
+      lw[q] = b;
+    }
+    const int off = el * kOBLd + 8 * half;
+    *reinterpret_cast<u32x4v*>(&dst[0][off]) = hw;
+    *reinterpret_cast<u32x4v*>(&dst[1][off]) = lw;
+  };
+  auto advance = [&]() {
+    cidx += step_i;
+    cp += step_p;
+    if (cp >= P) {
+      cp -= P;
+      ++cidx;
+    }
+  };
+
+  floatx16 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.f;
+
+  const int64_t nst = (K + kOBK - 1) / kOBK;
+  float lv[8], rv[8];
+  load(0, lv, rv);
+  stage(Ls[0], lv, 0);
+  stage(Rs[0], rv, -1);
+  __syncthreads();
+  for (int64_t st = 0; st < nst; ++st) {
+    const int buf = int(st & 1);
+    const bool more = st + 1 < nst;
+    if (more) {
+      advance();
+      load((st + 1) * kOBK, lv, rv);
+    }
+    bf16x8v ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const int off = (64 * wr + 32 * a + r) * kOBLd + 8 * h;
+      ah[a] = __builtin_bit_cast(bf16x8v, *reinterpret_cast<const u32x4v*>(&Ls[buf][0][off]));
+      al[a] = __builtin_bit_cast(bf16x8v, *reinterpret_cast<const u32x4v*>(&Ls[buf][1][off]));
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int off = (64 * wc + 32 * b + r) * kOBLd + 8 * h;
+      bh[b] = __builtin_bit_cast(bf16x8v, *reinterpret_cast<const u32x4v*>(&Rs[buf][0][off]));
+      bl[b] = __builtin_bit_cast(bf16x8v, *reinterpret_cast<const u32x4v*>(&Rs[buf][1][off]));
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+      }
+    if (more) {
+      stage(Ls[buf ^ 1], lv, (st + 1) * kOBK);
+      stage(Rs[buf ^ 1], rv, -1);
+    }
+    __syncthreads();
+  }
+  // lane l of accumulator (a, b) holds column j0 + 64 wc + 32 b + (l & 31)
+  // at rows i0 + 64 wr + 32 a + 8 (s / 4) + 4 h + (s % 4)
+  __shared__ float4 xi[kOT], xj[kOT];
+  stage_points(xi, rc, i0, n);
+  stage_points(xj, rc, j0, n);
+  __syncthreads();
+  double sd = 0.0, st2 = 0.0;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      float part = 0.f;
+      contract_acc(acc[a][b], xi, xj, 64 * wr + 32 * a + 4 * h, 64 * wc + 32 * b + r, i0, j0, n,
+                   rc.coef, part, st2);
+      sd += double(part);
+    }
+  __shared__ double red[2][256];
+  red[0][t] = sd;
+  red[1][t] = st2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      red[0][t] += red[0][t + w];
+      red[1][t] += red[1][t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    const int64_t b = blockIdx.x;
+    rc.red[2 * b] = red[0][0];
+    rc.red[2 * b + 1] = red[1][0];
   }
 }
 
@@ -380,24 +667,25 @@ __global__ void __launch_bounds__(256)
 
 hipError_t launch_outer_sum(hipStream_t s, const float* L, const float* R, int64_t ldc, int64_t P,
                             int64_t n, int64_t K, float* out) {
-  const unsigned tiles = unsigned((n + kOT - 1) / kOT);
-  hipLaunchKernelGGL(k_outer_sum<false>, dim3(tiles, tiles), dim3(256), 0, s, L, R, ldc, P, n, K,
-                     out, RBFContract{});
+  const TileMap tm = tile_map(n);
+  hipLaunchKernelGGL(k_outer_sum<false>, dim3(unsigned(tm.blocks)), dim3(256), 0, s, L, R, ldc, P,
+                     n, K, out, RBFContract{}, tm);
   return hipGetLastError();
 }
 
-int64_t outer_contract_blocks(int64_t n) {
-  const int64_t tiles = (n + kOT - 1) / kOT;
-  return tiles * tiles;
-}
+int64_t outer_contract_blocks(int64_t n) { return tile_map(n).blocks; }
 
 hipError_t launch_outer_rbf_contract(hipStream_t s, const float* L, const float* R, int64_t ldc,
                                      int64_t P, int64_t n, int64_t K, const float* X, int dim,
-                                     double lengthscale, double* red) {
-  const unsigned tiles = unsigned((n + kOT - 1) / kOT);
+                                     double lengthscale, double* red, bool bf16) {
+  const TileMap tm = tile_map(n);
   RBFContract rc{X, dim, float(-1.4426950408889634 / (2.0 * lengthscale * lengthscale)), red};
-  hipLaunchKernelGGL(k_outer_sum<true>, dim3(tiles, tiles), dim3(256), 0, s, L, R, ldc, P, n, K,
-                     nullptr, rc);
+  if (bf16)
+    hipLaunchKernelGGL(k_outer_rbf_bf16, dim3(unsigned(tm.blocks)), dim3(256), 0, s, L, R, ldc, P,
+                       n, K, rc, tm);
+  else
+    hipLaunchKernelGGL(k_outer_sum<true>, dim3(unsigned(tm.blocks)), dim3(256), 0, s, L, R, ldc, P,
+                       n, K, nullptr, rc, tm);
   return hipGetLastError();
 }
 
@@ -433,7 +721,7 @@ hipError_t launch_adjoint_combine(hipStream_t s, const float* Q, const float* La
   const auto al = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
   const bool vec = n % 4 == 0 && ldc % 4 == 0 && al(Q) && al(y) && (!Lam || al(Lam)) &&
                    (!r || al(r)) && (!x || al(x)) && (!v || al(v)) && (!save || al(save)) &&
-                   (!d || al(d));
+                   (!d || al(d)) && P * n >= (int64_t(1) << 20);
   const int V = vec ? 4 : 1;
   const int64_t per_block = int64_t(kStreamThreads) * V;
   const dim3 grid(unsigned((n + per_block - 1) / per_block), unsigned(P));

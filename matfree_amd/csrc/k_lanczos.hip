@@ -22,6 +22,7 @@
 // Plain Lanczos (reortho="none", decomp.py:198-266) uses the single-column
 // dots, k_three_term and k_normalize: no basis is read.
 #include <initializer_list>
+#include <type_traits>
 
 #include "slq_internal.h"
 
@@ -158,11 +159,20 @@ __device__ __forceinline__ void load_x(float* x, const MvSplitSum& sm, int64_t P
   VecIO<V>::store(x + p * n + i, xv);
 }
 
+// Basis columns whose loads are issued together (one batch in flight per
+// lane before the first product): a loop that loads, waits and multiplies one
+// column at a time leaves each wave one load in flight, and the kernels then
+// reach HBM bandwidth only at full occupancy.  Column indices past the last
+// live column are clamped to it (the loads stay in bounds, the products are
+// not used), and element groups past n read the row's last group with x = 0.
+constexpr int kColBatch = 8;
+
 template <int V, int NC, bool SUM>
 __global__ void __launch_bounds__(kStreamThreads)
     k_dots(const float* __restrict__ Q, int64_t ldc, const float* __restrict__ x, float* xw,
            int64_t n, int ncols, float* __restrict__ partial, int pstride, MvSplitSum sm) {
   constexpr int NE = Layout<V>::NE;
+  constexpr int CB = NC < kColBatch ? NC : kColBatch;
   __shared__ float red[kWaves * NC];
   const int64_t p = blockIdx.y;
   const int64_t b = blockIdx.x;
@@ -171,9 +181,11 @@ __global__ void __launch_bounds__(kStreamThreads)
   const int64_t base = b * kChunk;
 
   float xv[NE][V];
+  int64_t off[NE];
 #pragma unroll
   for (int e = 0; e < NE; ++e) {
     const int64_t i = base + (int64_t(e) * kStreamThreads + t) * V;
+    off[e] = i < n ? i : n - V;  // V divides n
     if (i < n) {
       if constexpr (SUM)
         load_x<V>(xw, sm, int64_t(gridDim.y), n, p, i, xv[e]);
@@ -189,20 +201,24 @@ __global__ void __launch_bounds__(kStreamThreads)
   for (int c0 = 0; c0 < ncols; c0 += NC) {
     float acc[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      acc[c] = 0.f;
-      if (c0 + c < ncols) {
-        const float* q = qp + int64_t(c0 + c) * ldc;
+    for (int c = 0; c < NC; ++c) acc[c] = 0.f;
 #pragma unroll
-        for (int e = 0; e < NE; ++e) {
-          const int64_t i = base + (int64_t(e) * kStreamThreads + t) * V;
-          if (i < n) {
-            float qv[V];
-            VecIO<V>::load(q + i, qv);
+    for (int g = 0; g < NC; g += CB) {
+      if (c0 + g < ncols) {
+        float qv[CB][NE][V];
 #pragma unroll
-            for (int v = 0; v < V; ++v) acc[c] = fmaf(qv[v], xv[e][v], acc[c]);
-          }
+        for (int u = 0; u < CB; ++u) {
+          const int c = c0 + g + u < ncols ? c0 + g + u : ncols - 1;
+          const float* q = qp + int64_t(c) * ldc;
+#pragma unroll
+          for (int e = 0; e < NE; ++e) VecIO<V>::load(q + off[e], qv[u][e]);
         }
+#pragma unroll
+        for (int u = 0; u < CB; ++u)
+#pragma unroll
+          for (int e = 0; e < NE; ++e)
+#pragma unroll
+            for (int v = 0; v < V; ++v) acc[g + u] = fmaf(qv[u][e][v], xv[e][v], acc[g + u]);
       }
     }
     const int live = ncols - c0 < NC ? ncols - c0 : NC;
@@ -211,61 +227,88 @@ __global__ void __launch_bounds__(kStreamThreads)
 }
 
 // Rows whose length n is not a multiple of 4 start at every alignment (probe
-// rows are contiguous).  The *_shifted kernels read row p from its first
-// 16-byte boundary: element i of chunk b's float4 groups is sh_p + b kChunk +
-// (e kStreamThreads + t) 4 with sh_p = (-p n) mod 4; the sh_p head elements
-// (block 0, threads < sh_p) and a last group cut by n are read as single
-// floats.  The basis stride ldc is a multiple of 4, so every column of the
-// row shares its alignment.
+// rows are contiguous).  The *_shifted kernels read row p as float4 groups
+// from its first 16-byte boundary: group e of thread t in chunk b starts at
+// sh_p + b kChunk + (e kStreamThreads + t) 4, sh_p = (-p n) mod 4, and the
+// groups that end by n are the full ones, [sh_p, gend_p).  The up to three
+// head elements [0, sh_p) and up to three tail elements [gend_p, n) are
+// single floats, one per thread of block 0 (threads 0-2 the head, threads
+// 4-6 the tail).  The basis stride ldc is a multiple of 4, so every column of
+// the row shares its alignment.  Launched for n >= 8 (at least one full
+// group).
 __device__ __forceinline__ int row_shift(int64_t p, int64_t n) { return int((4 - (p * n) % 4) % 4); }
 
-// Elements i .. i + 3 of a row of n (zeros past n): one float4 load for a
-// whole group, single loads for a cut one.
-__device__ __forceinline__ void load_group4(const float* row, int64_t i, int64_t n, float (&r)[4]) {
-  if (i + 4 <= n) {
-    VecIO<4>::load(row + i, r);
-  } else {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) r[v] = (i + v < n) ? row[i + v] : 0.f;
+struct ShiftedRow {
+  int64_t gend;  // end of the full groups
+  int64_t ix;    // the thread's single element, or -1
+  __device__ __forceinline__ ShiftedRow(int64_t p, int64_t n, int64_t b, int t) {
+    const int sh = row_shift(p, n);
+    gend = sh + (n - sh) / 4 * 4;
+    ix = -1;
+    if (b == 0) {
+      if (t < sh) ix = t;
+      else if (t >= 4 && t - 4 < n - gend) ix = gend + (t - 4);
+    }
   }
-}
+  // start of group e of thread t in chunk b, and its in-bounds load offset
+  __device__ __forceinline__ static int64_t group(int64_t p, int64_t n, int64_t b, int e, int t) {
+    return row_shift(p, n) + b * kChunk + (int64_t(e) * kStreamThreads + t) * 4;
+  }
+};
 
 template <int NC>
 __global__ void __launch_bounds__(kStreamThreads)
     k_dots_shifted(const float* __restrict__ Q, int64_t ldc, const float* __restrict__ x,
                    int64_t n, int ncols, float* __restrict__ partial, int pstride) {
   constexpr int NE = Layout<4>::NE;
+  constexpr int CB = NC < kColBatch ? NC : kColBatch;
   __shared__ float red[kWaves * NC];
   const int64_t p = blockIdx.y, b = blockIdx.x, nb = gridDim.x;
   const int t = threadIdx.x;
-  const int sh = row_shift(p, n);
-  const int64_t base = sh + b * kChunk;
-  const bool head = b == 0 && t < sh && t < n;
+  const ShiftedRow row(p, n, b, t);
   const float* xr = x + p * n;
   float xv[NE][4];
+  int64_t off[NE];
 #pragma unroll
-  for (int e = 0; e < NE; ++e) load_group4(xr, base + (int64_t(e) * kStreamThreads + t) * 4, n, xv[e]);
-  const float xh = head ? xr[t] : 0.f;
+  for (int e = 0; e < NE; ++e) {
+    const int64_t i = ShiftedRow::group(p, n, b, e, t);
+    const bool full = i < row.gend;
+    off[e] = full ? i : row.gend - 4;
+    if (full) {
+      VecIO<4>::load(xr + i, xv[e]);
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) xv[e][v] = 0.f;
+    }
+  }
+  const int64_t ixo = row.ix >= 0 ? row.ix : 0;
+  const float xh = row.ix >= 0 ? xr[ixo] : 0.f;
 
   const float* qp = Q + p * n;
   for (int c0 = 0; c0 < ncols; c0 += NC) {
     float acc[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      acc[c] = 0.f;
-      if (c0 + c < ncols) {
-        const float* q = qp + int64_t(c0 + c) * ldc;
+    for (int c = 0; c < NC; ++c) acc[c] = 0.f;
 #pragma unroll
-        for (int e = 0; e < NE; ++e) {
-          const int64_t i = base + (int64_t(e) * kStreamThreads + t) * 4;
-          if (i < n) {
-            float qv[4];
-            load_group4(q, i, n, qv);
+    for (int g = 0; g < NC; g += CB) {
+      if (c0 + g < ncols) {
+        float qv[CB][NE][4], qh[CB];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) acc[c] = fmaf(qv[v], xv[e][v], acc[c]);
-          }
+        for (int u = 0; u < CB; ++u) {
+          const int c = c0 + g + u < ncols ? c0 + g + u : ncols - 1;
+          const float* q = qp + int64_t(c) * ldc;
+#pragma unroll
+          for (int e = 0; e < NE; ++e) VecIO<4>::load(q + off[e], qv[u][e]);
+          qh[u] = q[ixo];
         }
-        if (head) acc[c] = fmaf(q[t], xh, acc[c]);
+#pragma unroll
+        for (int u = 0; u < CB; ++u) {
+#pragma unroll
+          for (int e = 0; e < NE; ++e)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[g + u] = fmaf(qv[u][e][v], xv[e][v], acc[g + u]);
+          acc[g + u] = fmaf(qh[u], xh, acc[g + u]);
+        }
       }
     }
     const int live = ncols - c0 < NC ? ncols - c0 : NC;
@@ -307,7 +350,7 @@ __device__ __forceinline__ void load_coeffs(const float* __restrict__ hpart, int
                                             int64_t hsub_stride = 0) {
   for (int c = threadIdx.x; c < ncols; c += blockDim.x) {
     float s = 0.f;
-    for (int64_t bb = 0; bb < nb; ++bb) s += hpart[(p * nb + bb) * pstride + c];
+    s = ordered_sum(hpart + p * nb * pstride + c, nb, pstride);
     if (write_out && hout) hout[p * hout_stride + c] = s;
     if (hsub) s -= hsub[p * hsub_stride + c];
     hs[c] = s;
@@ -316,7 +359,9 @@ __device__ __forceinline__ void load_coeffs(const float* __restrict__ hpart, int
 }
 
 // ---------------------------------------------------------------------------
-// project: y = x - Q h (h from partials); optional norm^2 partials of y
+// project: y = x - Q h (h from partials); optional norm^2 partials of y.
+// Columns in batches of kColBatch loaded together, the last ncols % kColBatch
+// in batches of 4, 2 and 1 (no column is read twice).
 // ---------------------------------------------------------------------------
 template <int V>
 __global__ void __launch_bounds__(kStreamThreads)
@@ -333,26 +378,40 @@ __global__ void __launch_bounds__(kStreamThreads)
   load_coeffs(hpart, p, nb, ncols, pstride, hs, hout, hout_stride, b == 0, hsub, hsub_stride);
 
   float s[NE][V];
+  int64_t off[NE];
 #pragma unroll
-  for (int e = 0; e < NE; ++e)
+  for (int e = 0; e < NE; ++e) {
+    const int64_t i = base + (int64_t(e) * kStreamThreads + t) * V;
+    off[e] = i < n ? i : n - V;  // V divides n
 #pragma unroll
     for (int v = 0; v < V; ++v) s[e][v] = 0.f;
+  }
 
   const float* qp = Q + p * n;
-  for (int c = 0; c < ncols; ++c) {
-    const float hc = hs[c];
-    const float* q = qp + int64_t(c) * ldc;
+  // columns [c0, c0 + B), loaded together, added in column order
+  auto batch = [&](auto bsz, int c0) {
+    constexpr int B = decltype(bsz)::value;
+    float qv[B][NE][V];
 #pragma unroll
-    for (int e = 0; e < NE; ++e) {
-      const int64_t i = base + (int64_t(e) * kStreamThreads + t) * V;
-      if (i < n) {
-        float qv[V];
-        VecIO<V>::load(q + i, qv);
+    for (int u = 0; u < B; ++u) {
+      const float* q = qp + int64_t(c0 + u) * ldc;
 #pragma unroll
-        for (int v = 0; v < V; ++v) s[e][v] = fmaf(qv[v], hc, s[e][v]);
-      }
+      for (int e = 0; e < NE; ++e) VecIO<V>::load(q + off[e], qv[u][e]);
     }
-  }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const float hc = hs[c0 + u];
+#pragma unroll
+      for (int e = 0; e < NE; ++e)
+#pragma unroll
+        for (int v = 0; v < V; ++v) s[e][v] = fmaf(qv[u][e][v], hc, s[e][v]);
+    }
+  };
+  int c0 = 0;
+  for (; c0 + kColBatch <= ncols; c0 += kColBatch) batch(std::integral_constant<int, kColBatch>{}, c0);
+  if (c0 + 4 <= ncols) { batch(std::integral_constant<int, 4>{}, c0); c0 += 4; }
+  if (c0 + 2 <= ncols) { batch(std::integral_constant<int, 2>{}, c0); c0 += 2; }
+  if (c0 < ncols) batch(std::integral_constant<int, 1>{}, c0);
 
   float acc = 0.f;
 #pragma unroll
@@ -387,61 +446,68 @@ __global__ void __launch_bounds__(kStreamThreads)
   __shared__ float red[kWaves];
   const int64_t p = blockIdx.y, b = blockIdx.x, nb = gridDim.x;
   const int t = threadIdx.x;
-  const int sh = row_shift(p, n);
-  const int64_t base = sh + b * kChunk;
-  const bool head = b == 0 && t < sh && t < n;
+  const ShiftedRow row(p, n, b, t);
   load_coeffs(hpart, p, nb, ncols, pstride, hs, hout, hout_stride, b == 0, hsub, hsub_stride);
 
   float s[NE][4];
+  int64_t off[NE];
 #pragma unroll
-  for (int e = 0; e < NE; ++e)
+  for (int e = 0; e < NE; ++e) {
+    const int64_t i = ShiftedRow::group(p, n, b, e, t);
+    off[e] = i < row.gend ? i : row.gend - 4;
 #pragma unroll
     for (int v = 0; v < 4; ++v) s[e][v] = 0.f;
-  float sh_acc = 0.f;
+  }
+  const int64_t ixo = row.ix >= 0 ? row.ix : 0;
+  float sx = 0.f;
 
   const float* qp = Q + p * n;
-  for (int c = 0; c < ncols; ++c) {
-    const float hc = hs[c];
-    const float* q = qp + int64_t(c) * ldc;
+  auto batch = [&](auto bsz, int c0) {
+    constexpr int B = decltype(bsz)::value;
+    float qv[B][NE][4], qh[B];
 #pragma unroll
-    for (int e = 0; e < NE; ++e) {
-      const int64_t i = base + (int64_t(e) * kStreamThreads + t) * 4;
-      if (i < n) {
-        float qv[4];
-        load_group4(q, i, n, qv);
+    for (int u = 0; u < B; ++u) {
+      const float* q = qp + int64_t(c0 + u) * ldc;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) s[e][v] = fmaf(qv[v], hc, s[e][v]);
-      }
+      for (int e = 0; e < NE; ++e) VecIO<4>::load(q + off[e], qv[u][e]);
+      qh[u] = q[ixo];
     }
-    if (head) sh_acc = fmaf(q[t], hc, sh_acc);
-  }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const float hc = hs[c0 + u];
+#pragma unroll
+      for (int e = 0; e < NE; ++e)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) s[e][v] = fmaf(qv[u][e][v], hc, s[e][v]);
+      sx = fmaf(qh[u], hc, sx);
+    }
+  };
+  int c0 = 0;
+  for (; c0 + kColBatch <= ncols; c0 += kColBatch) batch(std::integral_constant<int, kColBatch>{}, c0);
+  if (c0 + 4 <= ncols) { batch(std::integral_constant<int, 4>{}, c0); c0 += 4; }
+  if (c0 + 2 <= ncols) { batch(std::integral_constant<int, 2>{}, c0); c0 += 2; }
+  if (c0 < ncols) batch(std::integral_constant<int, 1>{}, c0);
 
   float acc = 0.f;
   const float* xr = x + p * n;
   float* yr = y + p * n;
-  if (head) {
-    const float v = xr[t] - sh_acc;
+  if (row.ix >= 0) {
+    const float v = xr[row.ix] - sx;
     acc = fmaf(v, v, acc);
-    yr[t] = v;
+    yr[row.ix] = v;
   }
 #pragma unroll
   for (int e = 0; e < NE; ++e) {
-    const int64_t i = base + (int64_t(e) * kStreamThreads + t) * 4;
-    if (i < n) {
+    const int64_t i = ShiftedRow::group(p, n, b, e, t);
+    if (i < row.gend) {
       float xv[4];
-      load_group4(xr, i, n, xv);
+      VecIO<4>::load(xr + i, xv);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         xv[v] = xv[v] - s[e][v];
-        if (i + v < n) acc = fmaf(xv[v], xv[v], acc);
+        acc = fmaf(xv[v], xv[v], acc);
       }
-      if (i + 4 <= n) {
-        VecIO<4>::store(yr + i, xv);
-      } else {
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-          if (i + v < n) yr[i + v] = xv[v];
-      }
+      VecIO<4>::store(yr + i, xv);
     }
   }
   if (norm2part) {
@@ -559,7 +625,7 @@ __global__ void __launch_bounds__(kStreamThreads)
   const int t = threadIdx.x;
   const int64_t base = b * kChunk;
   float a = 0.f;
-  for (int64_t bb = 0; bb < nb; ++bb) a += apart[p * nb + bb];
+  a = ordered_sum(apart + p * nb, nb, 1);
   const float bm = qm ? bprev[p * bstride] : 0.f;
   float acc = 0.f;
 #pragma unroll
@@ -640,7 +706,7 @@ __global__ void __launch_bounds__(kStreamThreads)
   const int t = threadIdx.x;
   if (t == 0) {
     float s = 0.f;
-    for (int64_t bb = 0; bb < nbn; ++bb) s += norm2part[p * nbn + bb];
+    s = ordered_sum(norm2part + p * nbn, nbn, 1);
     const float len = sqrtf(s);
     nrm = len;
     if (blockIdx.x == 0) {
@@ -674,12 +740,12 @@ inline bool aligned16(const void* ptr) { return (reinterpret_cast<uintptr_t>(ptr
 // pointers: 4 (float4) when n % 4 == 0 and all are 16-byte aligned, 2 (float2)
 // when n is even and all are 8-byte aligned (odd-multiple-of-2 n: every row
 // then starts on an 8-byte boundary), else 1.
-// The shifted float4 kernels (rows of any n): a basis stride that is a
+// The shifted float4 kernels (rows of any n >= 8): a basis stride that is a
 // multiple of 4 and 16-byte aligned bases.
-inline bool shifted_ok(int64_t ld, std::initializer_list<const void*> ptrs) {
+inline bool shifted_ok(int64_t n, int64_t ld, std::initializer_list<const void*> ptrs) {
   uintptr_t bits = 0;
   for (const void* q : ptrs) bits |= reinterpret_cast<uintptr_t>(q);
-  return ld % 4 == 0 && (bits & 15) == 0;
+  return n >= 8 && ld % 4 == 0 && (bits & 15) == 0;
 }
 
 inline int stream_width(int64_t n, int64_t ld, std::initializer_list<const void*> ptrs) {
@@ -727,7 +793,7 @@ hipError_t launch_dots(hipStream_t s, const float* Q, int64_t ldc, const float* 
     return hipGetLastError();
   }
   if (V == 4) return launch_dots_v<4>(s, Q, ldc, x, P, n, ncols, partial, pstride);
-  if (shifted_ok(ldc, {x, Q})) {  // measured faster than the float2 kernels for even n
+  if (shifted_ok(n, ldc, {x, Q})) {  // measured faster than the float2 kernels for even n
     const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
     const dim3 block(kStreamThreads);
     if (ncols <= 8)
@@ -775,7 +841,7 @@ hipError_t launch_project(hipStream_t s, const float* Q, int64_t ldc, const floa
   if (V == 4)
     hipLaunchKernelGGL(k_project<4>, grid, dim3(kStreamThreads), shmem, s, Q, ldc, x, y, n, ncols,
                        hpart, pstride, norm2part, hout, hout_stride, hsub, hsub_stride);
-  else if (shifted_ok(ldc, {x, y, Q}))  // measured faster than the float2 kernel for even n
+  else if (shifted_ok(n, ldc, {x, y, Q}))  // measured faster than the float2 kernel for even n
     hipLaunchKernelGGL(k_project_shifted, grid, dim3(kStreamThreads), shmem, s, Q, ldc, x, y, n,
                        ncols, hpart, pstride, norm2part, hout, hout_stride, hsub, hsub_stride);
   else if (V == 2)
@@ -915,11 +981,9 @@ __device__ __forceinline__ void ladj_coeffs(const LanczosAdjStep& st, int64_t p,
   float d0 = 0.f, d1 = 0.f, d2 = 0.f;
   if (st.part) {
     const float* pp = st.part + p * nb * 3;
-    for (int64_t bb = 0; bb < nb; ++bb) {
-      d0 += pp[3 * bb + 0];
-      d1 += pp[3 * bb + 1];
-      d2 += pp[3 * bb + 2];
-    }
+    d0 = ordered_sum(pp + 0, nb, 3);
+    d1 = ordered_sum(pp + 1, nb, 3);
+    d2 = ordered_sum(pp + 2, nb, 3);
   }
   const int64_t dbs = st.dbstride ? st.dbstride : st.sstride;
   const float db =
@@ -1065,7 +1129,7 @@ __global__ void __launch_bounds__(kStreamThreads)
                    int64_t n, float* __restrict__ dv) {
   const int64_t p = blockIdx.y;
   float d = 0.f;
-  for (int64_t b = 0; b < nb; ++b) d += part[p * nb + b];
+  d = ordered_sum(part + p * nb, nb, 1);
   const float cp = c[p];
   const int64_t i = int64_t(blockIdx.x) * kStreamThreads + threadIdx.x;
   if (i < n) dv[p * n + i] = cp * fmaf(d, x0[p * n + i], -xi[p * n + i]);

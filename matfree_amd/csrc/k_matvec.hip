@@ -44,6 +44,19 @@ struct RBFParams {
   float noise;
 };
 
+// 16 bytes at a 4-byte aligned address (rows of n % 4 != 0 floats start at
+// every dword alignment): a memcpy, which gfx950 code (unaligned global
+// access enabled by the HSA runtime) turns into one global_load_dwordx4 /
+// global_store_dwordx4 whatever the alignment.
+__device__ __forceinline__ float4 load4u(const float* p) {
+  float4 v;
+  __builtin_memcpy(&v, p, sizeof(v));
+  return v;
+}
+__device__ __forceinline__ void store4u(float* p, const float4& v) {
+  __builtin_memcpy(p, &v, sizeof(v));
+}
+
 // V tile: (32 NACC) probes x 32 m = 8 NACC float4, NACC per thread.
 template <int NACC, bool VEC>
 __device__ __forceinline__ void load_v_tile(const float* __restrict__ V, int64_t P, int64_t n,
@@ -59,6 +72,8 @@ __device__ __forceinline__ void load_v_tile(const float* __restrict__ V, int64_t
       const float* src = V + p * n + m;
       if (VEC) {
         if (m < n) v = *reinterpret_cast<const float4*>(src);
+      } else if (m + 4 <= n) {
+        v = load4u(src);
       } else {
         if (m + 0 < n) v.x = src[0];
         if (m + 1 < n) v.y = src[1];
@@ -88,10 +103,10 @@ __device__ __forceinline__ void load_a_frag(const float* __restrict__ A, int64_t
   const int64_t m = m0 + 16 * h;
   if (row < n) {
     const float* src = A + row * n + m;
-    if (VEC && m + 16 <= n) {
+    if (m + 16 <= n) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const float4 v = *reinterpret_cast<const float4*>(src + 4 * q);
+        const float4 v = VEC ? *reinterpret_cast<const float4*>(src + 4 * q) : load4u(src + 4 * q);
         a[4 * q + 0] = v.x;
         a[4 * q + 1] = v.y;
         a[4 * q + 2] = v.z;
@@ -150,6 +165,15 @@ __device__ __forceinline__ void store_out(float* __restrict__ W, const float* __
           }
           *reinterpret_cast<float4*>(W + p * n + i) = make_float4(o[0], o[1], o[2], o[3]);
         }
+      } else if (i + 4 <= n) {
+        if (NUGGET) {
+          const float4 v = load4u(V + p * n + i);
+          o[0] = fmaf(noise, v.x, o[0]);
+          o[1] = fmaf(noise, v.y, o[1]);
+          o[2] = fmaf(noise, v.z, o[2]);
+          o[3] = fmaf(noise, v.w, o[3]);
+        }
+        store4u(W + p * n + i, make_float4(o[0], o[1], o[2], o[3]));
       } else {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -405,12 +429,12 @@ __device__ __forceinline__ void load_a_frag16(const float* __restrict__ A, int64
                                               int64_t m0, int h, float (&a)[16]) {
   if (row < n) {
     const float* src = A + row * n + m0;
-    if (VEC && m0 + BK <= n) {
+    if (m0 + BK <= n) {
 #pragma unroll
       for (int half = 0; half < 2; ++half) {
         const float* q = src + 16 * half + 8 * h;
-        const float4 v0 = *reinterpret_cast<const float4*>(q);
-        const float4 v1 = *reinterpret_cast<const float4*>(q + 4);
+        const float4 v0 = VEC ? *reinterpret_cast<const float4*>(q) : load4u(q);
+        const float4 v1 = VEC ? *reinterpret_cast<const float4*>(q + 4) : load4u(q + 4);
         a[8 * half + 0] = v0.x; a[8 * half + 1] = v0.y; a[8 * half + 2] = v0.z;
         a[8 * half + 3] = v0.w; a[8 * half + 4] = v1.x; a[8 * half + 5] = v1.y;
         a[8 * half + 6] = v1.z; a[8 * half + 7] = v1.w;

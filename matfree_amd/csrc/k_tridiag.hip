@@ -8,9 +8,11 @@
 // stochtrace.py:50.  Only the first row of the eigenvector matrix enters
 // e1^T U log(L) U^T e1 = sum_j U_0j^2 log(theta_j) (Golub-Welsch).
 //
-// Algorithm (one wave per probe, one eigenvalue per lane):
+// Algorithm (per probe):
 //  * theta_j by bisection on the Sturm count of T - x I (the number of
-//    negative pivots of its LDL^T factorisation), to full FP64 resolution;
+//    negative pivots of its LDL^T factorisation), to full FP64 resolution --
+//    for depths up to 64 as a multisection with four lanes per eigenvalue
+//    (k_tridiag_quadrature_ms), beyond that one lane per eigenvalue;
 //  * the weight U_0j^2 from the twisted factorisation of T - theta_j I
 //    (Parlett and Dhillon, "Fernando's solution to Wilkinson's problem",
 //    1997): the stationary pivots D+_i (top-down LDL^T) and D-_i (bottom-up
@@ -354,15 +356,166 @@ __device__ __forceinline__ void quadrature_wave(const float* __restrict__ Hp, in
   if (lane == 0) *out = nonpositive ? __builtin_nan("") : len * len * acc;
 }
 
+// ---------------------------------------------------------------------------
+// Four-wave forms for depths up to 64: four lanes per eigenvalue.
+// ---------------------------------------------------------------------------
+// The tridiagonal of T = (H + H^T)/2 in FP32 (decomp.py:133-135; H
+// column-major) into LDS (a, |b|, b^2 and, if bs, the signed b), threads < k;
+// alpha / beta outputs if given.
+__device__ __forceinline__ void load_tridiag(const float* __restrict__ Hp, int k, double* a,
+                                             double* ab, double* b2, double* bs, float* alpha,
+                                             float* beta) {
+  const int t = threadIdx.x;
+  if (t < k) {
+    const float av = Hp[int64_t(t) * k + t];
+    a[t] = double(av);
+    if (alpha) alpha[t] = av;
+    if (t + 1 < k) {
+      const float b = 0.5f * (Hp[int64_t(t + 1) * k + t] + Hp[int64_t(t) * k + t + 1]);
+      ab[t] = fabs(double(b));
+      b2[t] = double(b) * double(b);
+      if (bs) bs[t] = double(b);
+      if (beta) beta[t] = b;
+    } else {
+      ab[t] = 0.0;
+      b2[t] = 0.0;
+      if (bs) bs[t] = 0.0;
+    }
+  }
+}
+
+// Leading unreduced block [0, m), Gershgorin bounds widened as LAPACK's
+// dstebz does, and the pivot threshold; every wave forms the same values.
+struct TriBounds {
+  int m;
+  double gl, gu, pivmin;
+};
+
+__device__ __forceinline__ TriBounds tri_bounds(const double* a, const double* ab,
+                                                const double* b2, int k) {
+  const int lane = threadIdx.x & 63;
+  int split = k;
+  double gl = DBL_MAX, gu = -DBL_MAX, bmax = 0.0;
+  if (lane < k) {
+    const int i = lane;
+    if (i + 1 < k && ab[i] <= DBL_EPSILON * (fabs(a[i]) + fabs(a[i + 1]))) split = i + 1;
+    const double rad = ab[i] + (i > 0 ? ab[i - 1] : 0.0);
+    gl = a[i] - rad;
+    gu = a[i] + rad;
+    bmax = b2[i];
+  }
+  split = wave_min_int(split);
+  gl = wave_min_f64(gl);
+  gu = wave_max_f64(gu);
+  bmax = wave_max_f64(bmax);
+  const double pivmin = DBL_MIN * fmax(1.0, bmax);
+  const double wide = fmax(fabs(gl), fabs(gu));
+  gl -= 2.0 * DBL_EPSILON * wide * k + 2.0 * pivmin;
+  gu += 2.0 * DBL_EPSILON * wide * k + 2.0 * pivmin;
+  return {split, gl, gu, pivmin};
+}
+
+// Eigenvalue e of the leading m x m block (ascending) by the four lanes of a
+// group (sub = lane & 3): each evaluates the Sturm count at one of the four
+// points that cut [lo, hi] into fifths, and the group keeps the fifth that
+// holds the eigenvalue (log2 5 bits per round, the four counts in parallel),
+// until the interval is as narrow as bisection leaves it or no interior
+// point is left.  Every lane of the wave takes part (e >= m: idle).
+__device__ __forceinline__ double multisect(const double* a, const double* b2, int m, int e,
+                                            int sub, const TriBounds& tb) {
+  const int lane = threadIdx.x & 63;
+  double lo = tb.gl, hi = tb.gu;
+  bool open = e < m;
+  for (int it = 0; it < kMaxBisect; ++it) {
+    const double w = (hi - lo) * 0.2;
+    double x[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) x[q] = fma(double(q + 1), w, lo);
+    if (open && (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) || !(lo < x[0]) ||
+                 !(x[0] < x[1]) || !(x[1] < x[2]) || !(x[2] < x[3]) || !(x[3] < hi)))
+      open = false;
+    if (__ballot(open) == 0) break;
+    const bool above = open && sturm_count(a, b2, m, x[sub], tb.pivmin) > e;
+    const unsigned g = unsigned(__ballot(above) >> (lane & ~3)) & 0xFu;
+    if (open) {
+      if (g & 1u) {
+        hi = x[0];
+      } else if (g & 2u) {
+        lo = x[0];
+        hi = x[1];
+      } else if (g & 4u) {
+        lo = x[1];
+        hi = x[2];
+      } else if (g & 8u) {
+        lo = x[2];
+        hi = x[3];
+      } else {
+        lo = x[3];
+      }
+    }
+  }
+  return 0.5 * (lo + hi);
+}
+
+// Quadrature: the group's first lane forms the twisted weight.  Probes
+// whose weights do not sum to one are flagged in retry[p] for
+// k_tridiag_quadrature_retry (the one-wave path with its QL fallback);
+// retry[p] is written for every probe.
+__global__ void __launch_bounds__(256)
+    k_tridiag_quadrature_ms(const float* __restrict__ H, const float* __restrict__ scale,
+                            int64_t k, double* __restrict__ per_probe,
+                            float* __restrict__ alpha, float* __restrict__ beta,
+                            int* __restrict__ retry) {
+  __shared__ double a[kQuadLdsDepth], ab[kQuadLdsDepth], b2[kQuadLdsDepth];
+  __shared__ double dm[64 * kQuadLdsDepth];
+  __shared__ double red[2][4];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int64_t p = blockIdx.x;
+  const int kk = int(k);
+  load_tridiag(H + p * k * k, kk, a, ab, b2, nullptr, alpha ? alpha + p * k : nullptr,
+               beta ? beta + p * (k - 1) : nullptr);
+  __syncthreads();
+  const TriBounds tb = tri_bounds(a, ab, b2, kk);
+  const int m = tb.m;
+  const int e = wave * 16 + (lane >> 2), sub = lane & 3;
+  const double theta = multisect(a, b2, m, e, sub, tb);
+  double acc = 0.0, wsum = 0.0;
+  if (sub == 0 && e < m) {
+    const double wt = twisted_weight(a, b2, m, theta, tb.pivmin, dm + e);
+    wsum = wt;
+    acc = wt * log(theta);
+  }
+  acc = wave_sum_f64(acc);
+  wsum = wave_sum_f64(wsum);
+  if (lane == 0) {
+    red[0][wave] = acc;
+    red[1][wave] = wsum;
+  }
+  __syncthreads();
+  if (t == 0) {
+    acc = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    wsum = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+    const bool ok = fabs(wsum - 1.0) <= kWeightSumTol;
+    retry[p] = ok ? 0 : 1;
+    if (ok) {
+      // any eigenvalue of T at or below zero makes the reference's log NaN
+      const bool nonpositive = sturm_count(a, b2, kk, 0.0, tb.pivmin) > 0;
+      const double len = double(scale[p]);
+      per_probe[p] = nonpositive ? __builtin_nan("") : len * len * acc;
+    }
+  }
+}
+
+// The one-wave path for the probes k_tridiag_quadrature_ms flagged.
 __global__ void __launch_bounds__(64)
-    k_tridiag_quadrature_lds(const float* __restrict__ H, const float* __restrict__ scale,
-                             int64_t k, double* __restrict__ per_probe,
-                             float* __restrict__ alpha, float* __restrict__ beta) {
+    k_tridiag_quadrature_retry(const float* __restrict__ H, const float* __restrict__ scale,
+                               int64_t k, const int* __restrict__ retry,
+                               double* __restrict__ per_probe) {
   __shared__ double tri[3 * kQuadLdsDepth];
   __shared__ double dm[64 * kQuadLdsDepth];
   const int64_t p = blockIdx.x;
-  quadrature_wave(H + p * k * k, int(k), tri, tri + k, tri + 2 * k, dm,
-                  alpha ? alpha + p * k : nullptr, beta ? beta + p * (k - 1) : nullptr,
+  if (!retry[p]) return;
+  quadrature_wave(H + p * k * k, int(k), tri, tri + k, tri + 2 * k, dm, nullptr, nullptr,
                   double(scale[p]), per_probe + p);
 }
 
@@ -435,13 +588,188 @@ __device__ __forceinline__ void tridiag_vjp_wave(const float* __restrict__ Hp, i
   }
 }
 
+// Unit eigenvector of the leading m x m block for the eigenvalue theta from
+// the twisted factorisation (see the file header): bottom-up pivots D-_i into
+// dm[i * 64], top-down pivots D+_i into z[i * ldz] (then overwritten), the
+// twist r = argmin |gamma_i|, and z_r = 1, z_i = -(b_i / D+_i) z_{i+1}
+// (i < r), z_{i+1} = -(b_i / D-_{i+1}) z_i (i >= r), normalised.  bs: the
+// signed off-diagonal.
+__device__ __forceinline__ void twisted_vector(const double* __restrict__ a,
+                                               const double* __restrict__ bs,
+                                               const double* __restrict__ b2, int m,
+                                               double theta, double pivmin, double* dm, double* z,
+                                               int ldz) {
+  double d = a[m - 1] - theta;
+  if (fabs(d) <= pivmin) d = -pivmin;
+  dm[(m - 1) * 64] = d;
+  for (int i = m - 2; i >= 0; --i) {
+    d = (a[i] - theta) - b2[i] / d;
+    if (fabs(d) <= pivmin) d = -pivmin;
+    dm[i * 64] = d;
+  }
+  double dp = a[0] - theta;
+  if (fabs(dp) <= pivmin) dp = -pivmin;
+  z[0] = dp;
+  double best = fabs(dp + dm[0] - (a[0] - theta));
+  int r = 0;
+  for (int i = 1; i < m; ++i) {
+    dp = (a[i] - theta) - b2[i - 1] / dp;
+    if (fabs(dp) <= pivmin) dp = -pivmin;
+    z[i * ldz] = dp;
+    const double g = fabs(dp + dm[i * 64] - (a[i] - theta));
+    if (g < best) {
+      best = g;
+      r = i;
+    }
+  }
+  double zi = 1.0, nrm = 1.0;
+  for (int i = r - 1; i >= 0; --i) {
+    zi = -(bs[i] / z[i * ldz]) * zi;
+    z[i * ldz] = zi;
+    nrm = fma(zi, zi, nrm);
+  }
+  z[r * ldz] = 1.0;
+  zi = 1.0;
+  for (int i = r; i + 1 < m; ++i) {
+    zi = -(bs[i] / dm[(i + 1) * 64]) * zi;
+    z[(i + 1) * ldz] = zi;
+    nrm = fma(zi, zi, nrm);
+  }
+  const double inv = 1.0 / sqrt(nrm);
+  for (int i = 0; i < m; ++i) z[i * ldz] *= inv;
+}
+
+// The eigen cotangent for depths up to 64, four waves per probe: the
+// eigenvalues of the leading unreduced block by multisection, their
+// eigenvectors by the twisted factorisation, accepted when the residuals
+// |T u - theta u| / |T| and the entries of U^T U - I are all below
+// kEigCheckTol (eigenvalues that coincide to working precision fail it);
+// otherwise retry[p] = 1 and k_tridiag_vjp_retry (implicit QL, one wave)
+// takes the probe.  Then Tbar = U (L o u u^T) U^T as in tridiag_vjp_wave; U
+// has no component outside the leading block (U_0j = 0 there), so neither
+// has Tbar.  retry[p] is written for every probe.
+constexpr double kEigCheckTol = 1e-10;
+
+// The eigenpairs of the leading m x m block into th[j], U[i * ldu + j]
+// (i, j < m) with the checks described above; every thread of the block
+// calls it and gets the same answer.  dm: 64 x 64 doubles of scratch.
+__device__ __forceinline__ bool eigpairs_ms(const double* a, const double* bs, const double* b2,
+                                            const TriBounds& tb, double* th, double* U, int ldu,
+                                            double* dm, double* worst_w) {
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int m = tb.m;
+  const int e = wave * 16 + (lane >> 2), sub = lane & 3;
+  const double theta = multisect(a, b2, m, e, sub, tb);
+  if (sub == 0 && e < m) {
+    th[e] = theta;
+    twisted_vector(a, bs, b2, m, theta, tb.pivmin, dm + e, U + e, ldu);
+  }
+  __syncthreads();
+  // residuals (thread t: vector t) and orthogonality (pairs j <= l)
+  double worst = 0.0;
+  if (t < m) {
+    const double tj = th[t];
+    const double tnorm = fmax(fmax(fabs(tb.gl), fabs(tb.gu)), DBL_MIN);
+    for (int i = 0; i < m; ++i) {
+      double r = (a[i] - tj) * U[i * ldu + t];
+      if (i > 0) r = fma(bs[i - 1], U[(i - 1) * ldu + t], r);
+      if (i + 1 < m) r = fma(bs[i], U[(i + 1) * ldu + t], r);
+      worst = fmax(worst, fabs(r) / tnorm);
+    }
+  }
+  for (int idx = t; idx < m * m; idx += blockDim.x) {
+    const int j = idx / m, l = idx - j * m;
+    if (l < j) continue;
+    double d = 0.0;
+    for (int i = 0; i < m; ++i) d = fma(U[i * ldu + j], U[i * ldu + l], d);
+    worst = fmax(worst, fabs(d - (j == l ? 1.0 : 0.0)));
+  }
+  // fmax drops a NaN operand: a NaN anywhere must fail the check
+  if (worst != worst) worst = DBL_MAX;
+  worst = wave_max_f64(worst);
+  if (lane == 0) worst_w[wave] = worst;
+  __syncthreads();
+  const double wmax = fmax(fmax(worst_w[0], worst_w[1]), fmax(worst_w[2], worst_w[3]));
+  return wmax <= kEigCheckTol;
+}
+
+__global__ void __launch_bounds__(256)
+    k_tridiag_vjp_ms(const float* __restrict__ H, const float* __restrict__ len, int64_t P,
+                     int64_t k, float* __restrict__ dH, int* __restrict__ retry) {
+  constexpr int ldu = eig_ld(kEigLdsDepth);
+  __shared__ double a[kEigLdsDepth], ab[kEigLdsDepth], b2[kEigLdsDepth], bs[kEigLdsDepth];
+  __shared__ double th[kEigLdsDepth], lg[kEigLdsDepth];
+  __shared__ double dm[64 * kEigLdsDepth];  // bottom-up pivots, then W
+  __shared__ double U[kEigLdsDepth * ldu];  // U[i * ldu + j]: component i of vector j
+  __shared__ double worst_w[4];
+  const int t = threadIdx.x;
+  const int64_t p = blockIdx.x;
+  const int kk = int(k);
+  load_tridiag(H + p * k * k, kk, a, ab, b2, bs, nullptr, nullptr);
+  __syncthreads();
+  const TriBounds tb = tri_bounds(a, ab, b2, kk);
+  const int m = tb.m;
+  const bool ok = eigpairs_ms(a, bs, b2, tb, th, U, ldu, dm, worst_w);
+  if (t == 0) retry[p] = ok ? 0 : 1;
+  if (!ok) return;  // the same decision in every thread of the block
+
+  // any eigenvalue of T at or below zero makes the reference's log, and so
+  // its cotangent, NaN
+  const bool nonpositive = sturm_count(a, b2, kk, 0.0, tb.pivmin) > 0;
+  if (t < m) lg[t] = log(th[t]);
+  __syncthreads();
+  // W_jl = u_j u_l L_jl (u = U[0, :], L the divided differences of log)
+  for (int idx = t; idx < m * m; idx += 256) {
+    const int j = idx / m, l = idx - j * m;
+    const double thj = th[j], thl = th[l], del = thl - thj;
+    double L;
+    if (j == l || del == 0.0) {
+      L = 1.0 / thj;
+    } else if (fabs(del) < 1e-3 * fabs(thj)) {
+      L = log1p(del / thj) / del;  // (log thl - log thj) / (thl - thj), no cancellation
+    } else {
+      L = (lg[l] - lg[j]) / del;
+    }
+    dm[idx] = U[j] * U[l] * L;
+  }
+  __syncthreads();
+  // Tbar_aa = sum_l U_al v_l, Tbar_{a,a+1} = sum_l U_{a+1,l} v_l with
+  // v_l = sum_j W_jl U_aj (zero outside the leading block)
+  const double l2 = double(len[p]) * double(len[p]);
+  const double scale = l2 / double(P);
+  float* dHp = dH + p * k * k;
+  if (t < kk) {
+    double tdd = 0.0, tdo = 0.0;
+    if (t < m) {
+      const double* ua = U + t * ldu;
+      const bool up = t + 1 < m;
+      const double* ub = U + (up ? t + 1 : t) * ldu;
+      for (int l = 0; l < m; ++l) {
+        double v = 0.0;
+        for (int j = 0; j < m; ++j) v = fma(dm[j * m + l], ua[j], v);
+        tdd = fma(ua[l], v, tdd);
+        if (up) tdo = fma(ub[l], v, tdo);
+      }
+    }
+    const float nan = __builtin_nanf("");
+    dHp[int64_t(t) * k + t] = nonpositive ? nan : float(scale * tdd);
+    if (t + 1 < kk) {
+      const float o = nonpositive ? nan : float(scale * tdo);
+      dHp[int64_t(t + 1) * k + t] = o;  // (t, t + 1)
+      dHp[int64_t(t) * k + t + 1] = o;  // (t + 1, t)
+    }
+  }
+}
+
+// The one-wave QL path for the probes k_tridiag_vjp_ms flagged.
 __global__ void __launch_bounds__(64)
-    k_tridiag_vjp_lds(const float* __restrict__ H, const float* __restrict__ len, int64_t P,
-                      int64_t k, float* __restrict__ dH) {
+    k_tridiag_vjp_retry(const float* __restrict__ H, const float* __restrict__ len, int64_t P,
+                        int64_t k, const int* __restrict__ retry, float* __restrict__ dH) {
   __shared__ double U[kEigLdsDepth * eig_ld(kEigLdsDepth)];
   __shared__ double Wm[kEigLdsDepth * kEigLdsDepth];
   __shared__ double d[kEigLdsDepth], e[kEigLdsDepth], lg[kEigLdsDepth];
   const int64_t p = blockIdx.x;
+  if (!retry[p]) return;
   const double l2 = double(len[p]) * double(len[p]);
   tridiag_vjp_wave(H + p * k * k, int(k), d, e, lg, U, Wm, l2 / double(P), dH + p * k * k);
 }
@@ -498,15 +826,6 @@ __device__ __forceinline__ void tridiag_funm_wave(const float* __restrict__ Hp, 
 }
 
 __global__ void __launch_bounds__(64)
-    k_tridiag_funm_lds(const float* __restrict__ H, const float* __restrict__ len, int64_t k, int f,
-                       float* __restrict__ coef) {
-  __shared__ double U[kEigLdsDepth * eig_ld(kEigLdsDepth)];
-  __shared__ double d[kEigLdsDepth], e[kEigLdsDepth], fd[kEigLdsDepth];
-  const int64_t p = blockIdx.x;
-  tridiag_funm_wave(H + p * k * k, int(k), d, e, fd, U, f, len[p], coef + p * (2 * k + 4));
-}
-
-__global__ void __launch_bounds__(64)
     k_tridiag_funm_global(const float* __restrict__ H, const float* __restrict__ len, int64_t k,
                           int f, double* __restrict__ work, float* __restrict__ coef) {
   const int64_t p = blockIdx.x;
@@ -514,6 +833,56 @@ __global__ void __launch_bounds__(64)
   double* w = work + p * (ku + 3 * k);
   tridiag_funm_wave(H + p * k * k, int(k), w + ku, w + ku + k, w + ku + 2 * k, w, f, len[p],
                     coef + p * (2 * k + 4));
+}
+
+// funm's small part for depths up to 64 (four waves per probe): the
+// eigenpairs of eigpairs_ms when T has no negligible off-diagonal (the whole
+// of T is the leading block) and they pass its checks; otherwise retry[p] = 1
+// and k_tridiag_funm_retry (implicit QL, one wave) takes the probe.
+__global__ void __launch_bounds__(256)
+    k_tridiag_funm_ms(const float* __restrict__ H, const float* __restrict__ len, int64_t k,
+                      int f, float* __restrict__ coef, int* __restrict__ retry) {
+  constexpr int ldu = eig_ld(kEigLdsDepth);
+  __shared__ double a[kEigLdsDepth], ab[kEigLdsDepth], b2[kEigLdsDepth], bs[kEigLdsDepth];
+  __shared__ double th[kEigLdsDepth], fd[kEigLdsDepth];
+  __shared__ double dm[64 * kEigLdsDepth];
+  __shared__ double U[kEigLdsDepth * ldu];
+  __shared__ double worst_w[4];
+  const int t = threadIdx.x;
+  const int64_t p = blockIdx.x;
+  const int kk = int(k);
+  load_tridiag(H + p * k * k, kk, a, ab, b2, bs, nullptr, nullptr);
+  __syncthreads();
+  const TriBounds tb = tri_bounds(a, ab, b2, kk);
+  const bool ok = tb.m == kk && eigpairs_ms(a, bs, b2, tb, th, U, ldu, dm, worst_w);
+  if (t == 0) retry[p] = ok ? 0 : 1;
+  if (!ok) return;  // the same decision in every thread of the block
+  if (t < kk) fd[t] = apply_matfun(f, th[t]);  // f(theta_j) once
+  __syncthreads();
+  float* cp = coef + p * (2 * k + 4);
+  if (t < kk) {
+    double y = 0.0;
+    const double* ua = U + t * ldu;
+    for (int j = 0; j < kk; ++j) y += ua[j] * fd[j] * U[j];
+    cp[t] = float(y);
+    cp[k + t] = 0.f;
+  }
+  if (t == 0) {
+    cp[2 * k + 0] = 0.f;
+    cp[2 * k + 1] = 0.f;
+    cp[2 * k + 2] = 0.f;
+    cp[2 * k + 3] = len[p];
+  }
+}
+
+__global__ void __launch_bounds__(64)
+    k_tridiag_funm_retry(const float* __restrict__ H, const float* __restrict__ len, int64_t k,
+                         int f, const int* __restrict__ retry, float* __restrict__ coef) {
+  __shared__ double U[kEigLdsDepth * eig_ld(kEigLdsDepth)];
+  __shared__ double d[kEigLdsDepth], e[kEigLdsDepth], fd[kEigLdsDepth];
+  const int64_t p = blockIdx.x;
+  if (!retry[p]) return;
+  tridiag_funm_wave(H + p * k * k, int(k), d, e, fd, U, f, len[p], coef + p * (2 * k + 4));
 }
 
 // mean and standard error of the mean (population std / sqrt(P)) in FP64,
@@ -579,12 +948,16 @@ hipError_t launch_tridiag_quadrature(hipStream_t s, const float* H, const float*
                                      float* alpha, float* beta) {
   if (P <= 0) return hipSuccess;
   if (k > 0) {
-    if (k <= kQuadLdsDepth)
-      hipLaunchKernelGGL(k_tridiag_quadrature_lds, dim3(unsigned(P)), dim3(64), 0, s, H, scale, k,
-                         per_probe, alpha, beta);
-    else
+    if (k <= kQuadLdsDepth) {
+      int* retry = reinterpret_cast<int*>(work);
+      hipLaunchKernelGGL(k_tridiag_quadrature_ms, dim3(unsigned(P)), dim3(256), 0, s, H, scale, k,
+                         per_probe, alpha, beta, retry);
+      hipLaunchKernelGGL(k_tridiag_quadrature_retry, dim3(unsigned(P)), dim3(64), 0, s, H, scale,
+                         k, retry, per_probe);
+    } else {
       hipLaunchKernelGGL(k_tridiag_quadrature_global, dim3(unsigned(P)), dim3(64), 0, s, H, scale,
                          k, work, per_probe, alpha, beta);
+    }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
   }
@@ -595,30 +968,42 @@ hipError_t launch_tridiag_quadrature(hipStream_t s, const float* H, const float*
 hipError_t launch_tridiag_vjp(hipStream_t s, const float* H, const float* len, int64_t P, int64_t k,
                               double* work, float* dH) {
   if (P <= 0 || k <= 0) return hipSuccess;
-  if (k <= kEigLdsDepth)
-    hipLaunchKernelGGL(k_tridiag_vjp_lds, dim3(unsigned(P)), dim3(64), 0, s, H, len, P, k, dH);
-  else
+  if (k <= kEigLdsDepth) {
+    int* retry = reinterpret_cast<int*>(work);
+    hipLaunchKernelGGL(k_tridiag_vjp_ms, dim3(unsigned(P)), dim3(256), 0, s, H, len, P, k, dH,
+                       retry);
+    hipLaunchKernelGGL(k_tridiag_vjp_retry, dim3(unsigned(P)), dim3(64), 0, s, H, len, P, k,
+                       retry, dH);
+  } else {
     hipLaunchKernelGGL(k_tridiag_vjp_global, dim3(unsigned(P)), dim3(64), 0, s, H, len, P, k, work,
                        dH);
+  }
   return hipGetLastError();
 }
 
 int64_t tridiag_quadrature_work_doubles(int64_t P, int64_t k) {
-  return k <= kQuadLdsDepth ? 0 : P * 67 * k;
+  // k <= kQuadLdsDepth: the retry flags (one int per probe, in doubles)
+  return k <= kQuadLdsDepth ? P : P * 67 * k;
 }
 
 int64_t tridiag_vjp_work_doubles(int64_t P, int64_t k) {
-  return k <= kEigLdsDepth ? 0 : P * (k * eig_ld(int(k)) + k * k + 3 * k);
+  // k <= kEigLdsDepth: the retry flags (one int per probe, in doubles)
+  return k <= kEigLdsDepth ? P : P * (k * eig_ld(int(k)) + k * k + 3 * k);
 }
 
 hipError_t launch_tridiag_funm(hipStream_t s, const float* H, const float* len, int64_t P,
                                int64_t k, int f, double* work, float* coef) {
   if (P <= 0 || k <= 0) return hipSuccess;
-  if (k <= kEigLdsDepth)
-    hipLaunchKernelGGL(k_tridiag_funm_lds, dim3(unsigned(P)), dim3(64), 0, s, H, len, k, f, coef);
-  else
+  if (k <= kEigLdsDepth) {
+    int* retry = reinterpret_cast<int*>(work);
+    hipLaunchKernelGGL(k_tridiag_funm_ms, dim3(unsigned(P)), dim3(256), 0, s, H, len, k, f, coef,
+                       retry);
+    hipLaunchKernelGGL(k_tridiag_funm_retry, dim3(unsigned(P)), dim3(64), 0, s, H, len, k, f,
+                       retry, coef);
+  } else {
     hipLaunchKernelGGL(k_tridiag_funm_global, dim3(unsigned(P)), dim3(64), 0, s, H, len, k, f,
                        work, coef);
+  }
   return hipGetLastError();
 }
 

@@ -561,7 +561,8 @@ int logdet_reverse(slq_context* ctx, const slq_operator& op, int64_t P, int64_t 
     // d/d lengthscale = <S, dK/dl> = <S, |x_i - x_j|^2 K> / l^3, in one pass
     // over the output tiles of S (never stored)
     SLQ_HIP(ctx, launch_outer_rbf_contract(s, Lam, st.Q, ldc, P, n, k * P, op.points, op.dim,
-                                           op.lengthscale, dsum));
+                                           op.lengthscale, dsum,
+                                           op.precision != SLQ_PREC_FP32));
     const double ls = op.lengthscale;
     SLQ_HIP(ctx, launch_sum_fixed(s, dsum, outer_contract_blocks(n), 2, 1.0 / (ls * ls * ls), 1.0,
                                   dparams));
@@ -583,7 +584,8 @@ int param_cotangents(slq_context* ctx, const slq_operator& op, const float* L, c
     double* dsum;
     SLQ_TRY(ensure_t(ctx, ctx->adj_sum, size_t(2 * outer_contract_blocks(n)), &dsum));
     SLQ_HIP(ctx, launch_outer_rbf_contract(s, L, R, ldc, P, n, k * P, op.points, op.dim,
-                                           op.lengthscale, dsum));
+                                           op.lengthscale, dsum,
+                                           op.precision != SLQ_PREC_FP32));
     const double ls = op.lengthscale;
     SLQ_HIP(ctx, launch_sum_fixed(s, dsum, outer_contract_blocks(n), 2, 1.0 / (ls * ls * ls), 1.0,
                                   dparams));
@@ -839,7 +841,8 @@ int logdet_lanczos_reverse(slq_context* ctx, const slq_operator& op, int64_t P, 
   // parameter cotangents: S = sum_{i, p} x_i lambda_i^T
   if (op.kind == SLQ_OP_RBF) {
     SLQ_HIP(ctx, launch_outer_rbf_contract(s, st.Q, Lam, ldc, P, n, k * P, op.points, op.dim,
-                                           op.lengthscale, dsum));
+                                           op.lengthscale, dsum,
+                                           op.precision != SLQ_PREC_FP32));
     const double ls = op.lengthscale;
     SLQ_HIP(ctx, launch_sum_fixed(s, dsum, outer_contract_blocks(n), 2, 1.0 / (ls * ls * ls), 1.0,
                                   dparams));
@@ -1383,7 +1386,8 @@ int slq_quadform_vjp(slq_context* ctx, const slq_operator* op, const float* X, i
     double* dsum;
     SLQ_TRY(ensure_t(ctx, ctx->adj_sum, size_t(2 * outer_contract_blocks(n)), &dsum));
     SLQ_HIP(ctx, launch_outer_rbf_contract(s, X, X, P * n, P, n, P, op->points, op->dim,
-                                           op->lengthscale, dsum));
+                                           op->lengthscale, dsum,
+                                           op->precision != SLQ_PREC_FP32));
     const double ls = op->lengthscale;
     SLQ_HIP(ctx, launch_sum_fixed(s, dsum, outer_contract_blocks(n), 2, 1.0 / (ls * ls * ls), 1.0,
                                   dparams));

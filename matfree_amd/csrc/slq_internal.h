@@ -26,6 +26,24 @@ constexpr int kFusedMaxCols = 64;
 
 inline int64_t num_chunks(int64_t n) { return (n + kChunk - 1) / kChunk; }
 
+// sum_{b < count} v[b * stride], added in the order b = 0, 1, ... like a
+// plain loop, but with the loads issued eight at a time: a serial loop costs
+// one memory latency per partial sum, which dominates the prologue of a
+// streaming kernel over many chunks.
+__device__ __forceinline__ float ordered_sum(const float* __restrict__ v, int64_t count,
+                                             int64_t stride) {
+  float s = 0.f;
+  for (int64_t b = 0; b < count; b += 8) {
+    float r[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) r[u] = v[(b + u < count ? b + u : count - 1) * stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (b + u < count) s += r[u];
+  }
+  return s;
+}
+
 // ---- probes (k_probes.hip) -------------------------------------------------
 hipError_t launch_rademacher(hipStream_t s, uint32_t k0, uint32_t k1, int64_t count, float* out);
 
@@ -235,9 +253,10 @@ hipError_t launch_outer_sum(hipStream_t s, const float* L, const float* R, int64
 // red[2 b] = sum_ij S_ij d2_ij exp(-d2_ij / (2 l^2)), d2_ij = |x_i - x_j|^2, and
 // red[2 b + 1] = sum_i S_ii, partial over the outer_contract_blocks(n) tiles b.
 int64_t outer_contract_blocks(int64_t n);
+// bf16: the split-BF16 matrix-core form (operators whose matvec runs in BF16).
 hipError_t launch_outer_rbf_contract(hipStream_t s, const float* L, const float* R, int64_t ldc,
                                      int64_t P, int64_t n, int64_t K, const float* X, int dim,
-                                     double lengthscale, double* red);
+                                     double lengthscale, double* red, bool bf16);
 // out[o] = f_o * sum_i partial[i * nout + o] (fixed order), nout <= 2.
 hipError_t launch_sum_fixed(hipStream_t s, const double* partial, int64_t count, int nout,
                             double f0, double f1, double* out);

@@ -24,9 +24,9 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-@pytest.mark.parametrize("n,P,k", [(256, 6, 12), (301, 5, 17)])
-def test_rbf_cotangents_match_oracle(cuda, n, P, k):
-    X = problems.rbf_points(n, seed=4)
+@pytest.mark.parametrize("n,P,k,dim", [(256, 6, 12, 3), (301, 5, 17, 3), (260, 4, 10, 4)])
+def test_rbf_cotangents_match_oracle(cuda, n, P, k, dim):
+    X = problems.rbf_points(n, seed=4, dim=dim)
     theta = (0.5, 0.1)
     Z = oprng.rademacher(oprng.prng_key(7), (P, n), np.float32)
     per, mean, grads = funm.logdet_value_and_grad(
@@ -55,6 +55,26 @@ def test_rbf_cotangents_match_oracle_at_depth_32(cuda):
     g = np.array([float(grads[0]), float(grads[1])])
     assert rel(g, g64) < 1e-3, (g, g64)
     assert abs(float(mean) - est64) <= 1e-4 * abs(est64)
+
+
+@pytest.mark.parametrize("precision,tol", [("bf16x3", 1e-3), ("bf16x2", 5e-3)])
+def test_rbf_cotangents_bf16_forms(cuda, precision, tol):
+    # the BF16 block matvec in the forward and reverse passes and the
+    # split-BF16 parameter contraction, at the configuration of the depth-32
+    # test, against the FP64 restatement (tolerance: the FP32 form's 1e-3, and
+    # 5e-3 where the probes are rounded to BF16 once per matvec)
+    n, P, k = 2048, 32, 32
+    X = problems.rbf_points(n, seed=0)
+    theta = (0.5, 1e-2)
+    Z = oprng.rademacher(oprng.prng_key(0), (P, n), np.float32)
+    _, mean, grads = funm.logdet_value_and_grad(
+        operators.RBF(X, precision=precision), torch.as_tensor(Z, device=cuda), k, theta
+    )
+    est64, g64 = oslq.slq_logdet_vjp(oslq.RBFOperator(X, *theta, np.float64), k, Z)
+    g = np.array([float(grads[0]), float(grads[1])])
+    print(f"{precision}: gradient rel {rel(g, g64):.2e}, value rel "
+          f"{abs(float(mean) - est64) / abs(est64):.2e}", flush=True)
+    assert rel(g, g64) < tol, (g, g64)
 
 
 def test_rbf_cotangents_match_central_differences(cuda):
@@ -108,6 +128,35 @@ def test_full_depth_matches_dense_closed_form(cuda):
     for dA, gi in zip([dA_dl, np.eye(n)], grads):
         expected = np.sum(np.outer(y, y) * L * (U.T @ dA @ U))
         assert np.isclose(float(gi), expected, rtol=2e-4), (float(gi), expected)
+
+
+def test_coincident_eigenvalues_cotangent_takes_the_ql_path(cuda):
+    # W21+ (shifted by 2) from e_1 at full depth: T = A exactly, whose largest
+    # eigenvalues pair up to ~1e-14; the eigen cotangent must come from the QL
+    # kernel (the twisted eigenvectors of a pair are not orthogonal).  The
+    # derivative of [log A]_00 along a symmetric direction D in closed form:
+    # sum_ij y_i y_j L_ij (U^T D U)_ij, y = U^T e_1, L the divided differences
+    # of log (1 / w_i where two eigenvalues coincide)
+    n = 21
+    a = np.abs(np.arange(n) - 10).astype(np.float64) + 2.0
+    A = np.diag(a) + np.diag(np.ones(n - 1), 1) + np.diag(np.ones(n - 1), -1)
+    z = np.zeros((1, n), np.float32)
+    z[0, 0] = 1.0
+    _, _, (G,) = funm.logdet_value_and_grad(
+        operators.dense_matvec, torch.as_tensor(z, device=cuda), n,
+        (torch.as_tensor(A.astype(np.float32), device=cuda),))
+    w, U = np.linalg.eigh(A)
+    L = np.empty((n, n))
+    for i in range(n):
+        for j in range(n):
+            close = abs(w[i] - w[j]) <= 1e-8 * w[i]
+            L[i, j] = 1 / w[i] if close else (np.log(w[i]) - np.log(w[j])) / (w[i] - w[j])
+    y = U.T @ z[0].astype(np.float64)
+    D = np.random.default_rng(3).standard_normal((n, n))
+    D = D + D.T
+    expected = np.sum(np.outer(y, y) * L * (U.T @ D @ U))
+    got = float(np.sum(G.cpu().numpy().astype(np.float64) * D))
+    assert np.isclose(got, expected, rtol=1e-4), (got, expected)
 
 
 def test_estimator_autograd(cuda):

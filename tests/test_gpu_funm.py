@@ -82,6 +82,33 @@ def test_log_quadform_matches_integrand(cuda):
     assert torch.allclose(quad, per, rtol=1e-5)
 
 
+def w21_plus():
+    # Wilkinson's W21+ shifted by 2: its largest eigenvalues come in pairs that
+    # agree to ~1e-14, below what bisection separates
+    n = 21
+    a = np.abs(np.arange(n) - 10).astype(np.float64) + 2.0
+    return np.diag(a) + np.diag(np.ones(n - 1), 1) + np.diag(np.ones(n - 1), -1)
+
+
+@pytest.mark.parametrize("fun", [np.log, np.exp])
+def test_coincident_eigenvalues_take_the_ql_path(cuda, fun):
+    # from e_1 and e_21 the Krylov tridiagonal is W21+ itself (exact in FP32);
+    # the twisted eigenvectors of its coincident pairs are not orthogonal, so
+    # the four-wave kernel must hand the probes to the QL kernel: f(A) e_1 and
+    # f(A) e_21 in closed form (FP32 output: 1e-6 relative)
+    A = w21_plus()
+    n = A.shape[0]
+    V = np.zeros((2, n), np.float32)
+    V[0, 0] = 1.0
+    V[1, -1] = 1.0
+    f = funm.funm_lanczos_sym(funm.dense_funm_sym_eigh(fun), decomp.tridiag_sym(n))
+    got = f.batched(operators.dense_matvec, torch.as_tensor(V, device=cuda),
+                    torch.as_tensor(A.astype(np.float32), device=cuda)).cpu().numpy()
+    w, U = np.linalg.eigh(A)
+    expected = V.astype(np.float64) @ ((U * fun(w)) @ U.T)
+    assert rel(got, expected) < 1e-6, rel(got, expected)
+
+
 def test_unsupported_matfun_raises(cuda):
     with pytest.raises(NotImplementedError):
         funm.funm_lanczos_sym(funm.dense_funm_sym_eigh(np.tanh), decomp.tridiag_sym(4))

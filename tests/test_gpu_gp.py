@@ -199,6 +199,32 @@ def test_quadform_vjp_matches_dense(cuda):
     assert np.allclose(D.cpu().numpy(), V64.T @ V64, rtol=1e-4, atol=1e-4)
 
 
+# The RBF parameter contraction <S, dK/dl>, <S, I> with S = sum_p x_p x_p^T,
+# against its FP64 closed form: the FP32 matrix-core form for FP32
+# operators, the split-BF16 form (hi*hi + hi*lo + lo*hi, ~2^-16 per product)
+# for BF16 operators; point dimensions 1-4 (4 in FP32 only), unaligned n,
+# several output tiles and contraction stages.
+@pytest.mark.parametrize("precision,dim,n,P", [
+    ("fp32", 1, 300, 3), ("fp32", 4, 301, 5), ("fp32", 4, 1000, 40),
+    ("bf16x2", 1, 300, 3), ("bf16x2", 2, 301, 5), ("bf16x2", 3, 1000, 40),
+    ("bf16", 3, 517, 17)])
+def test_quadform_vjp_contraction_forms(cuda, precision, dim, n, P):
+    X = problems.rbf_points(n, seed=6, dim=dim)
+    V = np.random.default_rng(dim).standard_normal((P, n)).astype(np.float32)
+    ls = 0.4
+    d = linalg.quadform_vjp(operators.RBF(X, precision=precision),
+                            torch.as_tensor(V, device=cuda), (ls, 0.1))
+    Xd = np.asarray(X, np.float64)
+    d2 = np.sum((Xd[:, None] - Xd[None]) ** 2, -1)
+    dK = np.exp(-d2 / (2 * ls**2)) * d2 / ls**3
+    V64 = V.astype(np.float64)
+    want = np.einsum("pi,ij,pj->", V64, dK, V64)
+    print(f"{precision} dim={dim} n={n} P={P}: rel {abs(float(d[0]) - want) / abs(want):.2e}",
+          flush=True)
+    assert np.isclose(float(d[0]), want, rtol=1e-4)
+    assert np.isclose(float(d[1]), np.sum(V64 * V64), rtol=1e-4)
+
+
 def test_gaussian_logpdf_tutorial(cuda):
     # tutorials/8_gaussian_logpdf.py: n = 30 points on [0, 1], log_ell = 0,
     # log_noise = -2, num_matvecs = 6, num_samples = 500; the value against
