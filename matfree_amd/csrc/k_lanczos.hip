@@ -139,20 +139,23 @@ __device__ __forceinline__ float block_sum(float v, float* red) {
 // x of probe p, elements i .. i + V - 1: loaded, or (sm.partial) formed
 // from a deferred split-K sum exactly as the matvec's reduction does and
 // stored to x.
+// (The partial planes z >= 1 start at z P n floats, which need not keep the
+// 16-byte alignment of plane 0 when P n % 4 != 0: they are read by memcpy,
+// one global_load_dwordx4 at any dword alignment on gfx950.)
 template <int V>
 __device__ __forceinline__ void load_x(float* x, const MvSplitSum& sm, int64_t P, int64_t n,
                                        int64_t p, int64_t i, float (&xv)[V]) {
   const int64_t count = P * n, off = p * n + i;
-  VecIO<V>::load(sm.partial + off, xv);
+  __builtin_memcpy(xv, sm.partial + off, sizeof(xv));
   for (int z = 1; z < sm.S; ++z) {
     float pz[V];
-    VecIO<V>::load(sm.partial + int64_t(z) * count + off, pz);
+    __builtin_memcpy(pz, sm.partial + int64_t(z) * count + off, sizeof(pz));
 #pragma unroll
     for (int v = 0; v < V; ++v) xv[v] += pz[v];
   }
   if (sm.nugget) {
     float vv[V];
-    VecIO<V>::load(sm.V + off, vv);
+    __builtin_memcpy(vv, sm.V + off, sizeof(vv));
 #pragma unroll
     for (int v = 0; v < V; ++v) xv[v] = fmaf(sm.noise, vv[v], xv[v]);
   }
@@ -256,10 +259,13 @@ struct ShiftedRow {
   }
 };
 
-template <int NC>
+// SUM: x is the deferred split-K sum `sm` (formed as the matvec's reduction
+// does it and written to xw), as in k_dots.
+template <int NC, bool SUM>
 __global__ void __launch_bounds__(kStreamThreads)
     k_dots_shifted(const float* __restrict__ Q, int64_t ldc, const float* __restrict__ x,
-                   int64_t n, int ncols, float* __restrict__ partial, int pstride) {
+                   float* xw, int64_t n, int ncols, float* __restrict__ partial, int pstride,
+                   MvSplitSum sm) {
   constexpr int NE = Layout<4>::NE;
   constexpr int CB = NC < kColBatch ? NC : kColBatch;
   __shared__ float red[kWaves * NC];
@@ -275,14 +281,26 @@ __global__ void __launch_bounds__(kStreamThreads)
     const bool full = i < row.gend;
     off[e] = full ? i : row.gend - 4;
     if (full) {
-      VecIO<4>::load(xr + i, xv[e]);
+      if constexpr (SUM)
+        load_x<4>(xw, sm, int64_t(gridDim.y), n, p, i, xv[e]);
+      else
+        VecIO<4>::load(xr + i, xv[e]);
     } else {
 #pragma unroll
       for (int v = 0; v < 4; ++v) xv[e][v] = 0.f;
     }
   }
   const int64_t ixo = row.ix >= 0 ? row.ix : 0;
-  const float xh = row.ix >= 0 ? xr[ixo] : 0.f;
+  float xh = 0.f;
+  if (row.ix >= 0) {
+    if constexpr (SUM) {
+      float one[1];
+      load_x<1>(xw, sm, int64_t(gridDim.y), n, p, ixo, one);
+      xh = one[0];
+    } else {
+      xh = xr[ixo];
+    }
+  }
 
   const float* qp = Q + p * n;
   for (int c0 = 0; c0 < ncols; c0 += NC) {
@@ -573,21 +591,24 @@ __global__ void __launch_bounds__(kStreamThreads)
   for (int64_t it = 0; it < kChunk / (kStreamThreads * E); ++it) {
     const int64_t i = base + (it * kStreamThreads + t) * E;
     if (i >= n) break;
-    // Column addresses are formed by a pointer chain that the compiler may
-    // not re-associate: otherwise it keeps NC loop-invariant 64-bit column
-    // offsets live across the loop (2 NC registers).
+    // Column offsets are formed by a chain that the compiler may not
+    // re-associate: otherwise it keeps NC loop-invariant 64-bit column
+    // offsets live across the loop (2 NC registers).  The chain runs on the
+    // integer offset, not on the pointer, so that the loads keep the global
+    // address space (a pointer through the asm becomes a generic pointer and
+    // its loads flat loads, which also wait on the LDS counter).
     VT q[NC];
-    const float* col = qp + i;
+    int64_t coff = i;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       if (c < ncols) {
-        q[c] = *reinterpret_cast<const VT*>(col);
+        q[c] = *reinterpret_cast<const VT*>(qp + coff);
       } else {
 #pragma unroll
         for (int e = 0; e < E; ++e) VecE<E>::set(q[c], e, 0.f);
       }
-      col += ldc;
-      asm volatile("" : "+v"(col));
+      coff += ldc;
+      asm volatile("" : "+v"(coff));
     }
     VT yv = *reinterpret_cast<const VT*>(x + p * n + i);
 #pragma unroll
@@ -797,17 +818,17 @@ hipError_t launch_dots(hipStream_t s, const float* Q, int64_t ldc, const float* 
     const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
     const dim3 block(kStreamThreads);
     if (ncols <= 8)
-      hipLaunchKernelGGL(k_dots_shifted<8>, grid, block, 0, s, Q, ldc, x, n, ncols, partial,
-                         pstride);
+      hipLaunchKernelGGL((k_dots_shifted<8, false>), grid, block, 0, s, Q, ldc, x, nullptr, n,
+                         ncols, partial, pstride, MvSplitSum{});
     else if (ncols <= 16)
-      hipLaunchKernelGGL(k_dots_shifted<16>, grid, block, 0, s, Q, ldc, x, n, ncols, partial,
-                         pstride);
+      hipLaunchKernelGGL((k_dots_shifted<16, false>), grid, block, 0, s, Q, ldc, x, nullptr, n,
+                         ncols, partial, pstride, MvSplitSum{});
     else if (ncols <= 32)
-      hipLaunchKernelGGL(k_dots_shifted<32>, grid, block, 0, s, Q, ldc, x, n, ncols, partial,
-                         pstride);
+      hipLaunchKernelGGL((k_dots_shifted<32, false>), grid, block, 0, s, Q, ldc, x, nullptr, n,
+                         ncols, partial, pstride, MvSplitSum{});
     else
-      hipLaunchKernelGGL(k_dots_shifted<64>, grid, block, 0, s, Q, ldc, x, n, ncols, partial,
-                         pstride);
+      hipLaunchKernelGGL((k_dots_shifted<64, false>), grid, block, 0, s, Q, ldc, x, nullptr, n,
+                         ncols, partial, pstride, MvSplitSum{});
     return hipGetLastError();
   }
   if (V == 2) return launch_dots_v<2>(s, Q, ldc, x, P, n, ncols, partial, pstride);
@@ -816,18 +837,39 @@ hipError_t launch_dots(hipStream_t s, const float* Q, int64_t ldc, const float* 
 
 hipError_t launch_dots_sum(hipStream_t s, const float* Q, int64_t ldc, const MvSplitSum& sum,
                            float* x, int64_t P, int64_t n, int ncols, float* partial, int pstride) {
-  // single column (the Lanczos coefficient q . A q); several columns would
-  // need the wider instantiations, which the Arnoldi pass does not use
-  if (ncols != 1) return hipErrorInvalidValue;
-  const bool vec = (n % 4 == 0) && aligned16(x) && aligned16(Q) && ldc % 4 == 0 &&
-                   aligned16(sum.partial) && (!sum.nugget || aligned16(sum.V));
   const dim3 grid(unsigned(num_chunks(n)), unsigned(P));
-  if (vec)
-    hipLaunchKernelGGL((k_dots<4, 8, true>), grid, dim3(kStreamThreads), 0, s, Q, ldc, nullptr, x,
-                       n, ncols, partial, pstride, sum);
-  else
-    hipLaunchKernelGGL((k_dots<1, 8, true>), grid, dim3(kStreamThreads), 0, s, Q, ldc, nullptr, x,
-                       n, ncols, partial, pstride, sum);
+  const dim3 block(kStreamThreads);
+  const std::initializer_list<const void*> ptrs = {x, Q, sum.partial, sum.nugget ? sum.V : x};
+  const int V = stream_width(n, ldc, ptrs);
+#define SLQ_DS(VV, NC)                                                                          \
+  hipLaunchKernelGGL((k_dots<VV, NC, true>), grid, block, 0, s, Q, ldc, nullptr, x, n, ncols, \
+                     partial, pstride, sum)
+#define SLQ_DSH(NC)                                                                          \
+  hipLaunchKernelGGL((k_dots_shifted<NC, true>), grid, block, 0, s, Q, ldc, nullptr, x, n, \
+                     ncols, partial, pstride, sum)
+#define SLQ_DS_NC(M)         \
+  if (ncols <= 8) M(8);      \
+  else if (ncols <= 16) M(16); \
+  else if (ncols <= 32) M(32); \
+  else M(64)
+  if (V == 4) {
+#define SLQ_DS4(NC) SLQ_DS(4, NC)
+    SLQ_DS_NC(SLQ_DS4);
+#undef SLQ_DS4
+  } else if (shifted_ok(n, ldc, ptrs)) {
+    SLQ_DS_NC(SLQ_DSH);
+  } else if (V == 2) {
+#define SLQ_DS2(NC) SLQ_DS(2, NC)
+    SLQ_DS_NC(SLQ_DS2);
+#undef SLQ_DS2
+  } else {
+#define SLQ_DS1(NC) SLQ_DS(1, NC)
+    SLQ_DS_NC(SLQ_DS1);
+#undef SLQ_DS1
+  }
+#undef SLQ_DS_NC
+#undef SLQ_DSH
+#undef SLQ_DS
   return hipGetLastError();
 }
 

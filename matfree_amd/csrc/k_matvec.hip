@@ -693,15 +693,7 @@ __global__ void __launch_bounds__(kThreads)
 //   FP8 planes:  1 KB per tile, [lane][16 bytes] (byte s = slot s), OCP FP8
 //                E4M3 with one power-of-two scale per tile and plane.
 // ---------------------------------------------------------------------------
-constexpr int64_t kTileBf16Bytes = 2048;
-constexpr int64_t kTileFp8Bytes = 1024;
-
-constexpr __host__ __device__ int storage_planes(int storage) {
-  return (storage == SLQ_STORE_BF16X2 || storage == SLQ_STORE_FP8X2) ? 2 : 1;
-}
-constexpr __host__ __device__ bool storage_fp8(int storage) {
-  return storage == SLQ_STORE_FP8X2 || storage == SLQ_STORE_FP8;
-}
+// (kTileBf16Bytes, kTileFp8Bytes, storage_planes and storage_fp8: slq_check.h)
 
 // Exponent k of the power-of-two tile scale: the smallest k with
 // m 2^-k <= 448 (the largest finite E4M3 value).  m = f 2^e, f in [0.5, 1):
@@ -1441,13 +1433,8 @@ int choose_splitk(int64_t n, int64_t P, int64_t avail_floats) {
 }  // namespace
 
 bool packed_size(int64_t n, int storage, int64_t* bytes, int64_t* scale_count) {
-  if (n <= 0 || storage <= SLQ_STORE_FP32 || storage > SLQ_STORE_FP8) return false;
-  const int64_t nt = (n + BK - 1) / BK;
-  const int planes = storage_planes(storage);
-  const bool fp8 = storage_fp8(storage);
-  *bytes = planes * nt * nt * (fp8 ? kTileFp8Bytes : kTileBf16Bytes);
-  *scale_count = fp8 ? planes * nt * nt : 0;
-  return true;
+  static_assert(kPackTile == BK, "packed tiles are the matvec's contraction tiles");
+  return packed_layout(n, storage, bytes, scale_count);
 }
 
 hipError_t launch_pack_matrix(hipStream_t s, const float* A, int64_t n, int storage, void* packed,

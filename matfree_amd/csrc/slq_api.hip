@@ -187,33 +187,16 @@ std::string exceeds_msg(int64_t k, int64_t n) {
 }
 
 int check_operator(slq_context* ctx, const slq_operator* op) {
-  if (!op) return fail(ctx, SLQ_EINVAL, "operator descriptor is NULL");
-  if (op->n <= 0) return fail(ctx, SLQ_EINVAL, "operator dimension n must be positive");
-  if (op->precision != SLQ_PREC_FP32 && op->precision != SLQ_PREC_BF16 &&
-      op->precision != SLQ_PREC_BF16X2 && op->precision != SLQ_PREC_BF16X3)
-    return fail(ctx, SLQ_EUNSUPPORTED,
-                "precision " + std::to_string(op->precision) + " is not available in this build");
-  if (op->kind == SLQ_OP_DENSE) {
-    if (op->storage == SLQ_STORE_FP32) {
-      if (!op->matrix) return fail(ctx, SLQ_EINVAL, "dense operator without a matrix");
-    } else if (op->storage >= SLQ_STORE_BF16X2 && op->storage <= SLQ_STORE_FP8) {
-      if (!op->packed) return fail(ctx, SLQ_EINVAL, "packed dense operator without its planes");
-      if ((op->storage == SLQ_STORE_FP8 || op->storage == SLQ_STORE_FP8X2) && !op->tile_scales)
-        return fail(ctx, SLQ_EINVAL, "FP8 dense operator without its tile scales");
-    } else {
-      return fail(ctx, SLQ_EINVAL, "unknown matrix storage " + std::to_string(op->storage));
-    }
-  } else if (op->kind == SLQ_OP_RBF) {
-    if (!op->points) return fail(ctx, SLQ_EINVAL, "RBF operator without points");
-    if (op->dim < 1 || op->dim > 4)
-      return fail(ctx, SLQ_EINVAL, "RBF point dimension must be in 1..4");
-    if (!(op->lengthscale > 0.0)) return fail(ctx, SLQ_EINVAL, "RBF lengthscale must be > 0");
-    if (op->precision != SLQ_PREC_FP32 && op->dim > 3)
-      return fail(ctx, SLQ_EUNSUPPORTED, "BF16 RBF kernels support point dimension <= 3");
-  } else {
-    return fail(ctx, SLQ_EINVAL, "unknown operator kind " + std::to_string(op->kind));
-  }
-  return SLQ_OK;
+  std::string msg;
+  const int rc = operator_status(op, &msg);
+  return rc == SLQ_OK ? SLQ_OK : fail(ctx, rc, msg);
+}
+
+// Sizes whose products the library forms (limits in slq_check.h).
+int check_footprint(slq_context* ctx, int64_t P, int64_t n, int64_t k) {
+  std::string msg;
+  const int rc = footprint_status(P, n, k, &msg);
+  return rc == SLQ_OK ? SLQ_OK : fail(ctx, rc, msg);
 }
 
 // The next (start, end) event pair of kernel class kc, or nulls when the
@@ -291,21 +274,27 @@ int hessenberg_core(slq_context* ctx, const slq_operator& op, const float* V0, i
   // after the first matvec (which splits its input and prepares the point
   // records) the normalisation writes the next input's BF16 planes itself
   const PlaneOut planes = matvec_planes(op, P, mvs);
+  MvSplitSum pend;
   for (int64_t j = 0; j < k; ++j) {
     const float* qj = Q + j * ldc;
     const int ncols = int(j + 1);
     float* hcol = H + j * k;  // column j of every probe, stride k * k
     // w = A v (decomp.py:460); the events bracket the block-matvec kernel
-    // itself (not the probe split or the split-K reduction around it)
+    // itself (not the probe split or the split-K reduction around it).  A
+    // split-K result is summed into w by the first dots pass (pend).
     {
       const auto ev = take_events(ctx, SLQ_KC_MATVEC);
       const int flags = j == 0 ? 0 : (kMvPointsReady | (planes.hi ? kMvPlanesReady : 0));
-      SLQ_HIP(ctx, launch_matvec(s, op, qj, P, w, mvs, mv_floats, ev.first, ev.second, flags));
+      SLQ_HIP(ctx, launch_matvec(s, op, qj, P, w, mvs, mv_floats, ev.first, ev.second, flags,
+                                 &pend));
     }
     // h = Q^T w (decomp.py:463)
     {
       Timed tm(ctx, SLQ_KC_DOTS);
-      SLQ_HIP(ctx, launch_dots(s, Q, ldc, w, P, n, ncols, part1, pstride));
+      if (pend.partial)
+        SLQ_HIP(ctx, launch_dots_sum(s, Q, ldc, pend, w, P, n, ncols, part1, pstride));
+      else
+        SLQ_HIP(ctx, launch_dots(s, Q, ldc, w, P, n, ncols, part1, pstride));
     }
     const bool last = (j + 1 == k);
     float* dst = last ? R : w;
@@ -748,6 +737,8 @@ int lanczos_reverse(slq_context* ctx, const slq_operator& op, int64_t P, int64_t
 constexpr int64_t kMaxRank = 4096;
 
 int check_rank(slq_context* ctx, int64_t rank, int64_t n) {
+  if (n <= 0 || n > kMaxDim)
+    return fail(ctx, SLQ_EINVAL, "dimension n must be in 1.." + std::to_string(kMaxDim));
   if (rank > n)
     return fail(ctx, SLQ_EINVAL,
                 "Rank exceeds nrows: " + std::to_string(rank) + " >= " + std::to_string(n) + ".");
@@ -935,6 +926,8 @@ int slq_sampler_signs(slq_context* ctx, uint32_t key0, uint32_t key1, int64_t nu
                       float* out) {
   if (!ctx) return SLQ_EINVAL;
   if (num < 0 || n < 0) return fail(ctx, SLQ_EINVAL, "num and n must be non-negative");
+  if (num > kMaxCount || n > kMaxDim)
+    return fail(ctx, SLQ_EINVAL, "num or n exceeds the library's size limits");
   if (num * n > 0 && !out) return fail(ctx, SLQ_EINVAL, "output pointer is NULL");
   SLQ_TRY(set_device(ctx));
   SLQ_HIP(ctx, launch_rademacher(ctx->stream, key0, key1, num * n, out));
@@ -945,6 +938,7 @@ int slq_matvec(slq_context* ctx, const slq_operator* op, const float* V, int64_t
   if (!ctx) return SLQ_EINVAL;
   SLQ_TRY(check_operator(ctx, op));
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of vectors P must be positive");
+  SLQ_TRY(check_footprint(ctx, P, op->n, 0));
   if (!V || !W) return fail(ctx, SLQ_EINVAL, "input or output pointer is NULL");
   SLQ_TRY(set_device(ctx));
   float* mvs;
@@ -960,6 +954,7 @@ int slq_hessenberg(slq_context* ctx, const slq_operator* op, const float* V0, in
   SLQ_TRY(check_operator(ctx, op));
   if (k < 0 || k > op->n) return fail(ctx, SLQ_EINVAL_K, exceeds_msg(k, op->n));
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of vectors P must be positive");
+  SLQ_TRY(check_footprint(ctx, P, op->n, k));
   if (!V0 || !R || !c || (k > 0 && (!Q || !H)))
     return fail(ctx, SLQ_EINVAL, "input or output pointer is NULL");
   SLQ_TRY(set_device(ctx));
@@ -972,6 +967,7 @@ int slq_arnoldi(slq_context* ctx, const slq_operator* op, const float* V0, int64
   SLQ_TRY(check_operator(ctx, op));
   if (k < 0 || k > op->n) return fail(ctx, SLQ_EINVAL_K, exceeds_msg(k, op->n));
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of vectors P must be positive");
+  SLQ_TRY(check_footprint(ctx, P, op->n, k));
   if (!V0 || !R || !c || (k > 0 && (!Q || !H)))
     return fail(ctx, SLQ_EINVAL, "input or output pointer is NULL");
   SLQ_TRY(set_device(ctx));
@@ -989,6 +985,7 @@ int slq_arnoldi_vjp(slq_context* ctx, const slq_operator* op, const float* V0, i
     return fail(ctx, SLQ_EADJOINT_K0,
                 "Custom Hessenberg-adjoints are not implemented for num_matvecs = 0.");
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of vectors P must be positive");
+  SLQ_TRY(check_footprint(ctx, P, op->n, k));
   if (!V0) return fail(ctx, SLQ_EINVAL, "start-vector pointer is NULL");
   if (op->kind == SLQ_OP_RBF && !dparams)
     return fail(ctx, SLQ_EINVAL, "dparams (2 doubles) is NULL");
@@ -1017,6 +1014,7 @@ int slq_lanczos_vjp(slq_context* ctx, const slq_operator* op, const float* V0, i
   if (k == 0)
     return fail(ctx, SLQ_EADJOINT_K0, "the Lanczos adjoint needs num_matvecs >= 1");
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of vectors P must be positive");
+  SLQ_TRY(check_footprint(ctx, P, op->n, k));
   if (!V0) return fail(ctx, SLQ_EINVAL, "start-vector pointer is NULL");
   if (op->kind == SLQ_OP_RBF && !dparams)
     return fail(ctx, SLQ_EINVAL, "dparams (2 doubles) is NULL");
@@ -1044,6 +1042,7 @@ int slq_logdet(slq_context* ctx, const slq_operator* op, const float* Z, int64_t
   if (k == 0)
     return fail(ctx, SLQ_EINVAL, "the log-determinant integrand needs num_matvecs >= 1");
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of probes P must be positive");
+  SLQ_TRY(check_footprint(ctx, P, op->n, k));
   if (!Z) return fail(ctx, SLQ_EINVAL, "probe pointer is NULL");
   SLQ_TRY(set_device(ctx));
   ForwardState st;
@@ -1059,6 +1058,7 @@ int slq_logdet_noalloc(slq_context* ctx, const slq_operator* op, const float* Z,
   if (k == 0)
     return fail(ctx, SLQ_EINVAL, "the log-determinant integrand needs num_matvecs >= 1");
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of probes P must be positive");
+  SLQ_TRY(check_footprint(ctx, P, op->n, k));
   if (!Z) return fail(ctx, SLQ_EINVAL, "probe pointer is NULL");
   char* base = nullptr;
   {
@@ -1111,6 +1111,7 @@ int slq_lanczos(slq_context* ctx, const slq_operator* op, const float* V0, int64
   const int64_t n = op->n;
   if (k < 0 || k > n) return fail(ctx, SLQ_EINVAL_K, exceeds_msg(k, n));
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of vectors P must be positive");
+  SLQ_TRY(check_footprint(ctx, P, op->n, k));
   if (!V0 || !R || !c || !beta || (k > 0 && !alpha))
     return fail(ctx, SLQ_EINVAL, "input or output pointer is NULL");
   SLQ_TRY(set_device(ctx));
@@ -1134,6 +1135,7 @@ int slq_logdet_lanczos(slq_context* ctx, const slq_operator* op, const float* Z,
   if (k == 0)
     return fail(ctx, SLQ_EINVAL, "the log-determinant integrand needs num_matvecs >= 1");
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of probes P must be positive");
+  SLQ_TRY(check_footprint(ctx, P, op->n, k));
   if (!Z) return fail(ctx, SLQ_EINVAL, "probe pointer is NULL");
   SLQ_TRY(set_device(ctx));
   ForwardState st;
@@ -1148,6 +1150,7 @@ int slq_funm_vectors(slq_context* ctx, const slq_operator* op, const float* V, i
   if (k < 0 || k > n) return fail(ctx, SLQ_EINVAL_K, exceeds_msg(k, n));
   if (k == 0) return fail(ctx, SLQ_EINVAL, "funm_lanczos_sym needs num_matvecs >= 1");
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of vectors P must be positive");
+  SLQ_TRY(check_footprint(ctx, P, op->n, k));
   if (!V || !out) return fail(ctx, SLQ_EINVAL, "input or output pointer is NULL");
   if (matfun < SLQ_FUNM_LOG || matfun > SLQ_FUNM_INV)
     return fail(ctx, SLQ_EUNSUPPORTED, "matrix function " + std::to_string(matfun) +
@@ -1188,6 +1191,7 @@ int slq_mean_and_sem(slq_context* ctx, const double* values, int64_t P, double* 
                      double* sem) {
   if (!ctx) return SLQ_EINVAL;
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of values P must be positive");
+  SLQ_TRY(check_footprint(ctx, P, 1, 0));
   if (!values) return fail(ctx, SLQ_EINVAL, "values pointer is NULL");
   SLQ_TRY(set_device(ctx));
   SLQ_HIP(ctx, launch_mean_and_sem(ctx->stream, values, P, mean, sem));
@@ -1204,6 +1208,7 @@ int slq_logdet_vjp(slq_context* ctx, const slq_operator* op, const float* Z, int
                 "Custom Hessenberg-adjoints are not implemented for num_matvecs = 0.");
   if (k < 0 || k > n) return fail(ctx, SLQ_EINVAL_K, exceeds_msg(k, n));
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of probes P must be positive");
+  SLQ_TRY(check_footprint(ctx, P, op->n, k));
   if (!Z) return fail(ctx, SLQ_EINVAL, "probe pointer is NULL");
   if (op->kind == SLQ_OP_RBF && !dparams)
     return fail(ctx, SLQ_EINVAL, "dparams (2 doubles) is NULL");
@@ -1247,6 +1252,7 @@ int slq_lowrank_solve(slq_context* ctx, const float* L, int64_t n, int64_t rank,
   SLQ_TRY(check_rank(ctx, rank, n));
   if (!(shift > 0.0)) return fail(ctx, SLQ_EINVAL, "the shift s must be positive");
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of vectors P must be positive");
+  SLQ_TRY(check_footprint(ctx, P, n, rank));
   if (!L || !V || !out) return fail(ctx, SLQ_EINVAL, "input or output pointer is NULL");
   SLQ_TRY(set_device(ctx));
   double *C, *z;
@@ -1265,6 +1271,7 @@ int slq_cg(slq_context* ctx, const slq_operator* op, const float* B, int64_t P, 
   SLQ_TRY(check_operator(ctx, op));
   const int64_t n = op->n;
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of right-hand sides P must be positive");
+  SLQ_TRY(check_footprint(ctx, P, op->n, 0));
   if (!B || !X) return fail(ctx, SLQ_EINVAL, "input or output pointer is NULL");
   if (!(tol >= 0.0) || !(atol >= 0.0) || maxiter < 0)
     return fail(ctx, SLQ_EINVAL, "tol, atol and maxiter must be non-negative");
@@ -1375,6 +1382,7 @@ int slq_quadform_vjp(slq_context* ctx, const slq_operator* op, const float* X, i
   SLQ_TRY(check_operator(ctx, op));
   const int64_t n = op->n;
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of vectors P must be positive");
+  SLQ_TRY(check_footprint(ctx, P, op->n, 0));
   if (!X) return fail(ctx, SLQ_EINVAL, "input pointer is NULL");
   if (op->kind == SLQ_OP_RBF && !dparams)
     return fail(ctx, SLQ_EINVAL, "dparams (2 doubles) is NULL");
@@ -1407,6 +1415,7 @@ int slq_logdet_lanczos_vjp(slq_context* ctx, const slq_operator* op, const float
   if (k == 0)
     return fail(ctx, SLQ_EINVAL, "the log-determinant integrand needs num_matvecs >= 1");
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of probes P must be positive");
+  SLQ_TRY(check_footprint(ctx, P, op->n, k));
   if (!Z) return fail(ctx, SLQ_EINVAL, "probe pointer is NULL");
   if (op->kind == SLQ_OP_RBF && !dparams)
     return fail(ctx, SLQ_EINVAL, "dparams (2 doubles) is NULL");
@@ -1441,6 +1450,7 @@ int slq_workspace_bytes(slq_context* ctx, int entry, const slq_operator* op, int
   if (k < 0 || k > n) return fail(ctx, SLQ_EINVAL_K, exceeds_msg(k, n));
   if (k == 0) return fail(ctx, SLQ_EINVAL, "the log-determinant integrand needs num_matvecs >= 1");
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of probes P must be positive");
+  SLQ_TRY(check_footprint(ctx, P, op->n, k));
   // the call itself in a dry run: every buffer it would take is counted and
   // nothing is launched, so the placeholders below are never dereferenced
   slq_context v;
@@ -1491,6 +1501,7 @@ int slq_logdet_save(slq_context* ctx, const slq_operator* op, const float* Z, in
   if (k < 0 || k > n) return fail(ctx, SLQ_EINVAL_K, exceeds_msg(k, n));
   if (k == 0) return fail(ctx, SLQ_EINVAL, "the log-determinant integrand needs num_matvecs >= 1");
   if (P <= 0) return fail(ctx, SLQ_EINVAL, "number of probes P must be positive");
+  SLQ_TRY(check_footprint(ctx, P, op->n, k));
   if (!Z) return fail(ctx, SLQ_EINVAL, "probe pointer is NULL");
   SLQ_TRY(set_device(ctx));
   ForwardState st;

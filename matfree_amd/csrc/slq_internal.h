@@ -8,6 +8,7 @@
 #include <string>
 
 #include "../../include/slq.h"
+#include "slq_check.h"
 
 namespace slq {
 
@@ -76,7 +77,7 @@ struct MvSplitSum {
 hipError_t launch_dots(hipStream_t s, const float* Q, int64_t ldc, const float* x, int64_t P,
                        int64_t n, int ncols, float* partial, int pstride);
 // launch_dots of x = the split-K sum `sum` (a deferred launch_matvec result),
-// which is also written to x (Q non-null).
+// which is also written to x (Q non-null), any ncols.
 hipError_t launch_dots_sum(hipStream_t s, const float* Q, int64_t ldc, const MvSplitSum& sum,
                            float* x, int64_t P, int64_t n, int ncols, float* partial, int pstride);
 // y = x - sum_c Q[c] h[c] with h[c] = sum_b hpart[p][b][c] - hsub[p][c];
