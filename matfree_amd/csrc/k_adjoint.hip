@@ -506,6 +506,9 @@ __device__ __forceinline__ void split_bf16_pair(float x0, float x1, uint32_t& hi
   lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(x - h, bf16x2v));
 }
 
+// P8: P >= 8, so that eight consecutive Krylov columns cross at most one
+// step boundary (the column offsets are then base + u n plus one gap).
+template <bool P8>
 __global__ void __launch_bounds__(256)
     k_outer_rbf_bf16(const float* __restrict__ L, const float* __restrict__ R, int64_t ldc,
                      int64_t P, int64_t n, int64_t K, RBFContract rc, TileMap tmap) {
@@ -527,30 +530,56 @@ __global__ void __launch_bounds__(256)
   const int el = t & (kOT - 1);
   const int half = __builtin_amdgcn_readfirstlane(t >> 7);  // uniform in the wave
   // the element offsets of this lane in L's and R's columns (clamped into the row)
-  const int64_t li = i0 + el < n ? i0 + el : n - 1;
-  const int64_t rj = j0 + el < n ? j0 + el : n - 1;
+  // (32-bit: n <= 2^30, so that the loads take a scalar column base and a
+  // 32-bit lane offset)
+  const uint32_t li = uint32_t(i0 + el < n ? i0 + el : n - 1);
+  const uint32_t rj = uint32_t(j0 + el < n ? j0 + el : n - 1);
   // first column of the half in the current stage, as (idx, p)
   int64_t cidx = (8 * half) / P, cp = (8 * half) % P;
   const int64_t step_i = kOBK / P, step_p = kOBK % P;
 
   const int64_t last = ((K - 1) / P) * ldc + ((K - 1) % P) * n;  // column K - 1
+  // a value the same in every lane, moved to scalar registers
+  auto uniform64 = [](int64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(uint64_t(v)));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(uint64_t(v) >> 32));
+    return int64_t((uint64_t(hi) << 32) | lo);
+  };
   auto load = [&](int64_t c0, float (&lv)[8], float (&rv)[8]) {
-    // the eight column offsets first (the same for the whole wave), columns
-    // past K read column K - 1; then all sixteen loads
+    // the eight column offsets first (the same for the whole wave, in scalar
+    // registers), columns past K read column K - 1; then all sixteen loads,
+    // each a scalar column base plus the lane's 32-bit element offset
     int64_t off[8];
-    int64_t idx = cidx, p = cp;
+    const int64_t p0 = uniform64(cp);
+    // columns u < valid exist (32-bit compares below: 64-bit ones run on the VALU)
+    const int64_t left = K - (c0 + 8 * half);
+    const int valid = left >= 8 ? 8 : (left <= 0 ? 0 : int(left));
+    if constexpr (P8) {
+      // at most one step to the next basis column within eight: the offsets
+      // are base + u n, plus the gap ldc - P n from u = P - p0 on
+      const int64_t base = uniform64(uniform64(cidx) * ldc + p0 * n);
+      const int64_t gap = ldc - P * n;
+      const int wrap = P - p0 >= 8 ? 8 : int(P - p0);
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      off[u] = c0 + 8 * half + u < K ? idx * ldc + p * n : last;
-      if (++p == P) {
-        p = 0;
-        ++idx;
+      for (int u = 0; u < 8; ++u) {
+        const int64_t o = base + u * n + (u >= wrap ? gap : 0);
+        off[u] = u < valid ? o : last;
+      }
+    } else {
+      int64_t idx = uniform64(cidx), p = p0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        off[u] = uniform64(c0 + 8 * half + u < K ? idx * ldc + p * n : last);
+        if (++p == P) {
+          p = 0;
+          ++idx;
+        }
       }
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      lv[u] = L[off[u] + li];
-      rv[u] = R[off[u] + rj];
+      lv[u] = (L + off[u])[li];
+      rv[u] = (R + off[u])[rj];
     }
   };
   // split and store to LDS; c0 >= 0: the L operand of the stage at column c0,
@@ -558,8 +587,10 @@ __global__ void __launch_bounds__(256)
   // loads, not before)
   auto stage = [&](__bf16 (*dst)[kOT * kOBLd], const float (&v)[8], int64_t c0) {
     float x[8];
+    const int64_t left = K - (c0 + 8 * half);
+    const int valid = c0 < 0 ? 8 : (left >= 8 ? 8 : (left <= 0 ? 0 : int(left)));
 #pragma unroll
-    for (int u = 0; u < 8; ++u) x[u] = (c0 < 0 || c0 + 8 * half + u < K) ? v[u] : 0.f;
+    for (int u = 0; u < 8; ++u) x[u] = u < valid ? v[u] : 0.f;
     u32x4v hw, lw;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -589,18 +620,30 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.f;
 
+  // The global loads run three stages ahead of the MFMAs: stage s's raw
+  // values sit in register slot s % 3 from iteration s - 3 until iteration
+  // s - 1 splits them into LDS buffer s % 2 (a load's latency is several
+  // times one stage's matrix-core work).
   const int64_t nst = (K + kOBK - 1) / kOBK;
-  float lv[8], rv[8];
-  load(0, lv, rv);
-  stage(Ls[0], lv, 0);
-  stage(Rs[0], rv, -1);
+  float lv[3][8], rv[3][8];
+  load(0, lv[0], rv[0]);
+  if (nst > 1) {
+    advance();
+    load(kOBK, lv[1], rv[1]);
+  }
+  if (nst > 2) {
+    advance();
+    load(2 * kOBK, lv[2], rv[2]);
+  }
+  stage(Ls[0], lv[0], 0);
+  stage(Rs[0], rv[0], -1);
   __syncthreads();
-  for (int64_t st = 0; st < nst; ++st) {
+  auto body = [&](auto U, int64_t st) {
+    constexpr int u = decltype(U)::value;  // st % 3
     const int buf = int(st & 1);
-    const bool more = st + 1 < nst;
-    if (more) {
+    if (st + 3 < nst) {  // slot u is free: stage st went to LDS one iteration ago
       advance();
-      load((st + 1) * kOBK, lv, rv);
+      load((st + 3) * kOBK, lv[u], rv[u]);
     }
     bf16x8v ah[2], al[2], bh[2], bl[2];
 #pragma unroll
@@ -623,11 +666,17 @@ __global__ void __launch_bounds__(256)
         acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
         acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
       }
-    if (more) {
-      stage(Ls[buf ^ 1], lv, (st + 1) * kOBK);
-      stage(Rs[buf ^ 1], rv, -1);
+    if (st + 1 < nst) {
+      constexpr int nx = (u + 1) % 3;
+      stage(Ls[buf ^ 1], lv[nx], (st + 1) * kOBK);
+      stage(Rs[buf ^ 1], rv[nx], -1);
     }
     __syncthreads();
+  };
+  for (int64_t st = 0; st < nst; st += 3) {
+    body(std::integral_constant<int, 0>{}, st);
+    if (st + 1 < nst) body(std::integral_constant<int, 1>{}, st + 1);
+    if (st + 2 < nst) body(std::integral_constant<int, 2>{}, st + 2);
   }
   // lane l of accumulator (a, b) holds column j0 + 64 wc + 32 b + (l & 31)
   // at rows i0 + 64 wr + 32 a + 8 (s / 4) + 4 h + (s % 4)
@@ -680,9 +729,12 @@ hipError_t launch_outer_rbf_contract(hipStream_t s, const float* L, const float*
                                      double lengthscale, double* red, bool bf16) {
   const TileMap tm = tile_map(n);
   RBFContract rc{X, dim, float(-1.4426950408889634 / (2.0 * lengthscale * lengthscale)), red};
-  if (bf16)
-    hipLaunchKernelGGL(k_outer_rbf_bf16, dim3(unsigned(tm.blocks)), dim3(256), 0, s, L, R, ldc, P,
-                       n, K, rc, tm);
+  if (bf16 && P >= 8)
+    hipLaunchKernelGGL(k_outer_rbf_bf16<true>, dim3(unsigned(tm.blocks)), dim3(256), 0, s, L, R,
+                       ldc, P, n, K, rc, tm);
+  else if (bf16)
+    hipLaunchKernelGGL(k_outer_rbf_bf16<false>, dim3(unsigned(tm.blocks)), dim3(256), 0, s, L, R,
+                       ldc, P, n, K, rc, tm);
   else
     hipLaunchKernelGGL(k_outer_sum<true>, dim3(unsigned(tm.blocks)), dim3(256), 0, s, L, R, ldc, P,
                        n, K, nullptr, rc, tm);
