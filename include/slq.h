@@ -271,6 +271,14 @@ typedef enum slq_entry {
 } slq_entry;
 int slq_workspace_bytes(slq_context* ctx, int entry, const slq_operator* op, int64_t P, int64_t k,
                         int64_t* bytes);
+/* The bytes by which ctx's pooled scratch grows in one call of entry for
+ * (op, P, k), given what the pool holds now (a pool keeps each buffer at the
+ * largest size any call needed): after the call slq_context_scratch_bytes(ctx)
+ * is at most its value before plus this (exactly, up to 256-byte alignment
+ * per grown buffer).  No counterpart in the reference; for admission by
+ * footprint when a request reuses a context that already holds scratch. */
+int slq_workspace_growth(slq_context* ctx, int entry, const slq_operator* op, int64_t P,
+                         int64_t k, int64_t* bytes);
 
 /* Plain Lanczos (tridiag_sym(k, reortho="none"), decomp.py:155-266) for P
  * start vectors: v0 = V0 / |V0|; per step a = q . A q, r = A q - a q - b q_prev,

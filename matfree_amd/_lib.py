@@ -151,6 +151,10 @@ SIGNATURES = {
         c_int, [c_void_p, c_int, ctypes.POINTER(OperatorStruct), c_i64, c_i64,
                 ctypes.POINTER(c_i64)],
     ),
+    "slq_workspace_growth": (
+        c_int, [c_void_p, c_int, ctypes.POINTER(OperatorStruct), c_i64, c_i64,
+                ctypes.POINTER(c_i64)],
+    ),
     "slq_profile_enable": (c_int, [c_void_p, c_int]),
     "slq_profile_select": (c_int, [c_void_p, c_int]),
     "slq_profile_read": (c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_double),
@@ -256,6 +260,15 @@ class Context:
         out = c_i64()
         rc = load().slq_workspace_bytes(self.ptr, ENTRIES[entry], ctypes.byref(desc), int(P),
                                         int(k), ctypes.byref(out))
+        raise_for(rc, self.ptr)
+        return int(out.value)
+
+    def workspace_growth(self, entry, desc, P, k):
+        """Bytes by which this context's pooled scratch grows in one call of
+        ``entry`` for (desc, P, k), given what it holds now."""
+        out = c_i64()
+        rc = load().slq_workspace_growth(self.ptr, ENTRIES[entry], ctypes.byref(desc), int(P),
+                                         int(k), ctypes.byref(out))
         raise_for(rc, self.ptr)
         return int(out.value)
 

@@ -10,8 +10,10 @@
 //    recursion (decomp.py:587-599) given the reductions of the streaming
 //    kernels (k_lanczos.hip).
 //  * k_outer_sum: the parameter cotangents from S = sum lambda q^T -- the
-//    stored matrix's dA = S, or the RBF contractions <S, dK/dl>, <S, I>;
-//    k_sum_fixed: their deterministic FP64 reduction.
+//    stored matrix's dA = S, or the RBF contractions <S, dK/dl>, <S, I> of
+//    FP32 operators (for BF16 operators the block matvec's derivative mode
+//    in k_matvec.hip computes them); k_sum_fixed: their deterministic FP64
+//    reduction.
 #include <type_traits>
 
 #include "slq_internal.h"
@@ -89,10 +91,12 @@ __global__ void __launch_bounds__(256)
 // y = scale (cr r + cl x + cv v + d + sum_{j < ncq} gq_j Q_j + sum_{jl0 <= j < k} gl_j L_j)
 // per probe, coefficients from coef (stride 2 k + 4); r, x, v, d may be null
 // (their terms are skipped).  With save (non-null) x is also copied there.
-// V consecutive elements per thread (V = 4: float4 groups, rows of n % 4 == 0
-// with 16-byte aligned bases; V = 1 otherwise, and for small P n, where one
-// element per thread gives the launch four times the waves); the basis
-// columns eight at a time, their loads in flight together.
+// V consecutive elements per thread (V = 4: 16-byte loads and stores at any
+// dword alignment, which gfx950 code issues as single dwordx4 accesses, so
+// rows of any length n stream as float4 groups, the last group of a row
+// element by element; V = 1 for small P n, where one element per thread
+// gives the launch four times the waves); the basis columns eight at a time,
+// their loads in flight together.
 template <int V>
 __global__ void __launch_bounds__(kStreamThreads)
     k_adjoint_combine(const float* __restrict__ Q, const float* __restrict__ Lam, int64_t ldc,
@@ -100,7 +104,6 @@ __global__ void __launch_bounds__(kStreamThreads)
                       const float* __restrict__ v, float* __restrict__ y, float* __restrict__ save,
                       int64_t n, int64_t k, int ncq, int jl0, const float* __restrict__ coef,
                       const float* __restrict__ d) {
-  using VT = typename std::conditional<V == 4, float4, float>::type;
   extern __shared__ float cs[];
   const int64_t p = blockIdx.y;
   const int t = threadIdx.x;
@@ -110,44 +113,60 @@ __global__ void __launch_bounds__(kStreamThreads)
   const float cr = cs[2 * k], cl = cs[2 * k + 1], cv = cs[2 * k + 2], sc = cs[2 * k + 3];
   const int64_t i = (int64_t(blockIdx.x) * kStreamThreads + t) * V;
   if (i >= n) return;
-  const int64_t off = p * n + i;
-  float acc[V];
+  // W elements from offset off (W = 4: one 16-byte access per operand)
+  auto combine = [&](auto width, int64_t off) {
+    constexpr int W = decltype(width)::value;
+    using VT = typename std::conditional<W == 4, float4, float>::type;
+    auto ld = [](const float* q) {
+      VT val;
+      __builtin_memcpy(&val, q, sizeof(VT));
+      return val;
+    };
+    float acc[W];
 #pragma unroll
-  for (int u = 0; u < V; ++u) acc[u] = 0.f;
-  auto fma_in = [&](float a, const VT& q) {
-    if constexpr (V == 4) {
-      acc[0] = fmaf(a, q.x, acc[0]);
-      acc[1] = fmaf(a, q.y, acc[1]);
-      acc[2] = fmaf(a, q.z, acc[2]);
-      acc[3] = fmaf(a, q.w, acc[3]);
-    } else {
-      acc[0] = fmaf(a, q, acc[0]);
+    for (int u = 0; u < W; ++u) acc[u] = 0.f;
+    auto fma_in = [&](float a, const VT& q) {
+      if constexpr (W == 4) {
+        acc[0] = fmaf(a, q.x, acc[0]);
+        acc[1] = fmaf(a, q.y, acc[1]);
+        acc[2] = fmaf(a, q.z, acc[2]);
+        acc[3] = fmaf(a, q.w, acc[3]);
+      } else {
+        acc[0] = fmaf(a, q, acc[0]);
+      }
+    };
+    auto cols = [&](const float* base, int64_t j0, int64_t j1, const float* c) {
+      int64_t j = j0;
+      for (; j + 8 <= j1; j += 8) {
+        VT q[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) q[u] = ld(base + (j + u) * ldc + off);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) fma_in(c[j + u], q[u]);
+      }
+      for (; j < j1; ++j) fma_in(c[j], ld(base + j * ldc + off));
+    };
+    cols(Q, 0, ncq, cs);
+    if (Lam) cols(Lam, jl0, k, cs + k);
+    if (r) fma_in(cr, ld(r + off));
+    if (x) fma_in(cl, ld(x + off));
+    if (v) fma_in(cv, ld(v + off));
+    if (d) fma_in(1.f, ld(d + off));
+    VT out;
+    if constexpr (W == 4)
+      out = make_float4(sc * acc[0], sc * acc[1], sc * acc[2], sc * acc[3]);
+    else
+      out = sc * acc[0];
+    __builtin_memcpy(y + off, &out, sizeof(VT));
+    if (save) {
+      const VT xs = ld(x + off);
+      __builtin_memcpy(save + off, &xs, sizeof(VT));
     }
   };
-  auto cols = [&](const float* base, int64_t j0, int64_t j1, const float* c) {
-    int64_t j = j0;
-    for (; j + 8 <= j1; j += 8) {
-      VT q[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) q[u] = *reinterpret_cast<const VT*>(base + (j + u) * ldc + off);
-#pragma unroll
-      for (int u = 0; u < 8; ++u) fma_in(c[j + u], q[u]);
-    }
-    for (; j < j1; ++j) fma_in(c[j], *reinterpret_cast<const VT*>(base + j * ldc + off));
-  };
-  cols(Q, 0, ncq, cs);
-  if (Lam) cols(Lam, jl0, k, cs + k);
-  if (r) fma_in(cr, *reinterpret_cast<const VT*>(r + off));
-  if (x) fma_in(cl, *reinterpret_cast<const VT*>(x + off));
-  if (v) fma_in(cv, *reinterpret_cast<const VT*>(v + off));
-  if (d) fma_in(1.f, *reinterpret_cast<const VT*>(d + off));
-  if constexpr (V == 4) {
-    *reinterpret_cast<float4*>(y + off) =
-        make_float4(sc * acc[0], sc * acc[1], sc * acc[2], sc * acc[3]);
-    if (save) *reinterpret_cast<float4*>(save + off) = *reinterpret_cast<const float4*>(x + off);
-  } else {
-    y[off] = sc * acc[0];
-    if (save) save[off] = x[off];
+  if (V == 1 || i + 4 <= n) {
+    combine(std::integral_constant<int, V>{}, p * n + i);
+  } else {  // the row's last, partial group
+    for (int64_t e = i; e < n; ++e) combine(std::integral_constant<int, 1>{}, p * n + e);
   }
 }
 
@@ -473,245 +492,6 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// ---------------------------------------------------------------------------
-// The RBF contraction of k_outer_sum<true> on the BF16 matrix cores
-// (v_mfma_f32_32x32x16_bf16), for operators whose block matvec runs in a
-// BF16 precision: both operands split into hi = bf16(x) and
-// lo = bf16(x - hi) and S accumulated in FP32 as hi*hi + hi*lo + lo*hi (the
-// products are exact in FP32; the dropped lo*lo term is ~2^-16 relative).
-// A workgroup owns a 128 x 128 tile of S, each wave a 64 x 64 quarter
-// (2 x 2 accumulators).  The contraction runs over the K Krylov columns in
-// stages of 16.  Thread t stages element t & 127 of the tile's rows of L and
-// of R for the eight columns 8 (t >> 7) .. + 7 of the stage: eight coalesced
-// 4-byte loads per operand (the column walk is the same for the whole wave),
-// split, and one 16-byte LDS write per plane into [element][column] rows of
-// 24 BF16 (48 bytes: the 16-byte accesses of 8 consecutive lanes cover all
-// banks); lane (r, h) of an MFMA then reads columns 8 h .. 8 h + 7 of its
-// element.  Elements past n are read clamped to the last one (their entries
-// of S are never used); columns past K are zeroed in L.  Double-buffered,
-// one barrier per stage; the next stage's loads are in flight during the
-// MFMAs.
-// ---------------------------------------------------------------------------
-typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
-typedef float f32x2v __attribute__((ext_vector_type(2)));
-typedef unsigned int u32x4v __attribute__((ext_vector_type(4)));
-typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
-constexpr int kOBK = 16;   // Krylov columns per stage
-constexpr int kOBLd = 24;  // LDS row of BF16 (48 bytes)
-
-__device__ __forceinline__ void split_bf16_pair(float x0, float x1, uint32_t& hi, uint32_t& lo) {
-  const f32x2v x = {x0, x1};
-  hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2v));
-  const f32x2v h = {__builtin_bit_cast(float, hi << 16), __builtin_bit_cast(float, hi & 0xffff0000u)};
-  lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(x - h, bf16x2v));
-}
-
-// P8: P >= 8, so that eight consecutive Krylov columns cross at most one
-// step boundary (the column offsets are then base + u n plus one gap).
-template <bool P8>
-__global__ void __launch_bounds__(256)
-    k_outer_rbf_bf16(const float* __restrict__ L, const float* __restrict__ R, int64_t ldc,
-                     int64_t P, int64_t n, int64_t K, RBFContract rc, TileMap tmap) {
-  // [buffer][plane hi/lo][element][column]
-  __shared__ __attribute__((aligned(16))) __bf16 Ls[2][2][kOT * kOBLd];
-  __shared__ __attribute__((aligned(16))) __bf16 Rs[2][2][kOT * kOBLd];
-  const int t = threadIdx.x;
-  const int lane = t & 63, wave = t >> 6, r = lane & 31, h = lane >> 5;
-  const int wr = wave >> 1, wc = wave & 1;  // the wave's 64 x 64 quarter
-  int64_t ti, tj;
-  if (!tile_of_block(tmap, blockIdx.x, ti, tj)) {
-    if (t == 0) {
-      rc.red[2 * int64_t(blockIdx.x)] = 0.0;
-      rc.red[2 * int64_t(blockIdx.x) + 1] = 0.0;
-    }
-    return;
-  }
-  const int64_t i0 = ti * kOT, j0 = tj * kOT;
-  const int el = t & (kOT - 1);
-  const int half = __builtin_amdgcn_readfirstlane(t >> 7);  // uniform in the wave
-  // the element offsets of this lane in L's and R's columns (clamped into the row)
-  // (32-bit: n <= 2^30, so that the loads take a scalar column base and a
-  // 32-bit lane offset)
-  const uint32_t li = uint32_t(i0 + el < n ? i0 + el : n - 1);
-  const uint32_t rj = uint32_t(j0 + el < n ? j0 + el : n - 1);
-  // first column of the half in the current stage, as (idx, p)
-  int64_t cidx = (8 * half) / P, cp = (8 * half) % P;
-  const int64_t step_i = kOBK / P, step_p = kOBK % P;
-
-  const int64_t last = ((K - 1) / P) * ldc + ((K - 1) % P) * n;  // column K - 1
-  // a value the same in every lane, moved to scalar registers
-  auto uniform64 = [](int64_t v) {
-    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(uint64_t(v)));
-    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(uint64_t(v) >> 32));
-    return int64_t((uint64_t(hi) << 32) | lo);
-  };
-  auto load = [&](int64_t c0, float (&lv)[8], float (&rv)[8]) {
-    // the eight column offsets first (the same for the whole wave, in scalar
-    // registers), columns past K read column K - 1; then all sixteen loads,
-    // each a scalar column base plus the lane's 32-bit element offset
-    int64_t off[8];
-    const int64_t p0 = uniform64(cp);
-    // columns u < valid exist (32-bit compares below: 64-bit ones run on the VALU)
-    const int64_t left = K - (c0 + 8 * half);
-    const int valid = left >= 8 ? 8 : (left <= 0 ? 0 : int(left));
-    if constexpr (P8) {
-      // at most one step to the next basis column within eight: the offsets
-      // are base + u n, plus the gap ldc - P n from u = P - p0 on
-      const int64_t base = uniform64(uniform64(cidx) * ldc + p0 * n);
-      const int64_t gap = ldc - P * n;
-      const int wrap = P - p0 >= 8 ? 8 : int(P - p0);
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int64_t o = base + u * n + (u >= wrap ? gap : 0);
-        off[u] = u < valid ? o : last;
-      }
-    } else {
-      int64_t idx = uniform64(cidx), p = p0;
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        off[u] = uniform64(c0 + 8 * half + u < K ? idx * ldc + p * n : last);
-        if (++p == P) {
-          p = 0;
-          ++idx;
-        }
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      lv[u] = (L + off[u])[li];
-      rv[u] = (R + off[u])[rj];
-    }
-  };
-  // split and store to LDS; c0 >= 0: the L operand of the stage at column c0,
-  // whose columns past K are zeroed here (after the MFMAs that overlap the
-  // loads, not before)
-  auto stage = [&](__bf16 (*dst)[kOT * kOBLd], const float (&v)[8], int64_t c0) {
-    float x[8];
-    const int64_t left = K - (c0 + 8 * half);
-    const int valid = c0 < 0 ? 8 : (left >= 8 ? 8 : (left <= 0 ? 0 : int(left)));
-#pragma unroll
-    for (int u = 0; u < 8; ++u) x[u] = u < valid ? v[u] : 0.f;
-    u32x4v hw, lw;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      uint32_t a, b;
-      split_bf16_pair(x[2 * q], x[2 * q + 1], a, b);
-      hw[q] = a;
-      lw[q] = b;
-    }
-    const int off = el * kOBLd + 8 * half;
-    *reinterpret_cast<u32x4v*>(&dst[0][off]) = hw;
-    *reinterpret_cast<u32x4v*>(&dst[1][off]) = lw;
-  };
-  auto advance = [&]() {
-    cidx += step_i;
-    cp += step_p;
-    if (cp >= P) {
-      cp -= P;
-      ++cidx;
-    }
-  };
-
-  floatx16 acc[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.f;
-
-  // The global loads run three stages ahead of the MFMAs: stage s's raw
-  // values sit in register slot s % 3 from iteration s - 3 until iteration
-  // s - 1 splits them into LDS buffer s % 2 (a load's latency is several
-  // times one stage's matrix-core work).
-  const int64_t nst = (K + kOBK - 1) / kOBK;
-  float lv[3][8], rv[3][8];
-  load(0, lv[0], rv[0]);
-  if (nst > 1) {
-    advance();
-    load(kOBK, lv[1], rv[1]);
-  }
-  if (nst > 2) {
-    advance();
-    load(2 * kOBK, lv[2], rv[2]);
-  }
-  stage(Ls[0], lv[0], 0);
-  stage(Rs[0], rv[0], -1);
-  __syncthreads();
-  auto body = [&](auto U, int64_t st) {
-    constexpr int u = decltype(U)::value;  // st % 3
-    const int buf = int(st & 1);
-    if (st + 3 < nst) {  // slot u is free: stage st went to LDS one iteration ago
-      advance();
-      load((st + 3) * kOBK, lv[u], rv[u]);
-    }
-    bf16x8v ah[2], al[2], bh[2], bl[2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      const int off = (64 * wr + 32 * a + r) * kOBLd + 8 * h;
-      ah[a] = __builtin_bit_cast(bf16x8v, *reinterpret_cast<const u32x4v*>(&Ls[buf][0][off]));
-      al[a] = __builtin_bit_cast(bf16x8v, *reinterpret_cast<const u32x4v*>(&Ls[buf][1][off]));
-    }
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int off = (64 * wc + 32 * b + r) * kOBLd + 8 * h;
-      bh[b] = __builtin_bit_cast(bf16x8v, *reinterpret_cast<const u32x4v*>(&Rs[buf][0][off]));
-      bl[b] = __builtin_bit_cast(bf16x8v, *reinterpret_cast<const u32x4v*>(&Rs[buf][1][off]));
-    }
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
-        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
-      }
-    if (st + 1 < nst) {
-      constexpr int nx = (u + 1) % 3;
-      stage(Ls[buf ^ 1], lv[nx], (st + 1) * kOBK);
-      stage(Rs[buf ^ 1], rv[nx], -1);
-    }
-    __syncthreads();
-  };
-  for (int64_t st = 0; st < nst; st += 3) {
-    body(std::integral_constant<int, 0>{}, st);
-    if (st + 1 < nst) body(std::integral_constant<int, 1>{}, st + 1);
-    if (st + 2 < nst) body(std::integral_constant<int, 2>{}, st + 2);
-  }
-  // lane l of accumulator (a, b) holds column j0 + 64 wc + 32 b + (l & 31)
-  // at rows i0 + 64 wr + 32 a + 8 (s / 4) + 4 h + (s % 4)
-  __shared__ float4 xi[kOT], xj[kOT];
-  stage_points(xi, rc, i0, n);
-  stage_points(xj, rc, j0, n);
-  __syncthreads();
-  double sd = 0.0, st2 = 0.0;
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      float part = 0.f;
-      contract_acc(acc[a][b], xi, xj, 64 * wr + 32 * a + 4 * h, 64 * wc + 32 * b + r, i0, j0, n,
-                   rc.coef, part, st2);
-      sd += double(part);
-    }
-  __shared__ double red[2][256];
-  red[0][t] = sd;
-  red[1][t] = st2;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (t < w) {
-      red[0][t] += red[0][t + w];
-      red[1][t] += red[1][t + w];
-    }
-    __syncthreads();
-  }
-  if (t == 0) {
-    const int64_t b = blockIdx.x;
-    rc.red[2 * b] = red[0][0];
-    rc.red[2 * b + 1] = red[1][0];
-  }
-}
-
 }  // namespace
 
 hipError_t launch_outer_sum(hipStream_t s, const float* L, const float* R, int64_t ldc, int64_t P,
@@ -726,18 +506,11 @@ int64_t outer_contract_blocks(int64_t n) { return tile_map(n).blocks; }
 
 hipError_t launch_outer_rbf_contract(hipStream_t s, const float* L, const float* R, int64_t ldc,
                                      int64_t P, int64_t n, int64_t K, const float* X, int dim,
-                                     double lengthscale, double* red, bool bf16) {
+                                     double lengthscale, double* red) {
   const TileMap tm = tile_map(n);
   RBFContract rc{X, dim, float(-1.4426950408889634 / (2.0 * lengthscale * lengthscale)), red};
-  if (bf16 && P >= 8)
-    hipLaunchKernelGGL(k_outer_rbf_bf16<true>, dim3(unsigned(tm.blocks)), dim3(256), 0, s, L, R,
-                       ldc, P, n, K, rc, tm);
-  else if (bf16)
-    hipLaunchKernelGGL(k_outer_rbf_bf16<false>, dim3(unsigned(tm.blocks)), dim3(256), 0, s, L, R,
-                       ldc, P, n, K, rc, tm);
-  else
-    hipLaunchKernelGGL(k_outer_sum<true>, dim3(unsigned(tm.blocks)), dim3(256), 0, s, L, R, ldc, P,
-                       n, K, nullptr, rc, tm);
+  hipLaunchKernelGGL(k_outer_sum<true>, dim3(unsigned(tm.blocks)), dim3(256), 0, s, L, R, ldc, P,
+                     n, K, nullptr, rc, tm);
   return hipGetLastError();
 }
 
@@ -770,10 +543,7 @@ hipError_t launch_adjoint_combine(hipStream_t s, const float* Q, const float* La
                                   const float* r, const float* x, const float* v, float* y,
                                   float* save, int64_t P, int64_t n, int64_t k, int ncq, int jl0,
                                   const float* coef, const float* d) {
-  const auto al = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
-  const bool vec = n % 4 == 0 && ldc % 4 == 0 && al(Q) && al(y) && (!Lam || al(Lam)) &&
-                   (!r || al(r)) && (!x || al(x)) && (!v || al(v)) && (!save || al(save)) &&
-                   (!d || al(d)) && P * n >= (int64_t(1) << 20);
+  const bool vec = P * n >= (int64_t(1) << 20);
   const int V = vec ? 4 : 1;
   const int64_t per_block = int64_t(kStreamThreads) * V;
   const dim3 grid(unsigned((n + per_block - 1) / per_block), unsigned(P));

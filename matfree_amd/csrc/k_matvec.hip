@@ -378,6 +378,43 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// The planes (layout and permuted order of k_split_probes<true>, Ppad
+// columns) of columns c0 .. c0 + Ppad of a Krylov-layout set: column c at
+// V + (c / Pin) ldc + (c % Pin) n, zero for c >= K.  One column per
+// blockIdx.y.
+__global__ void __launch_bounds__(256)
+    k_split_columns(const float* __restrict__ V, int64_t ldc, int64_t Pin, int64_t n, int64_t K,
+                    int64_t c0, int64_t Ppad, int64_t npad, __bf16* __restrict__ Vhi,
+                    __bf16* __restrict__ Vlo) {
+  const int64_t p = blockIdx.y, c = c0 + p;
+  const float* src = c < K ? V + (c / Pin) * ldc + (c % Pin) * n : nullptr;
+  const int64_t groups = npad / 4;
+  for (int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < groups;
+       g += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t m = 4 * g;
+    float x[4] = {0.f, 0.f, 0.f, 0.f};
+    if (src) {
+      if (m + 4 <= n) {
+        const float4 v = load4u(src + m);
+        x[0] = v.x;
+        x[1] = v.y;
+        x[2] = v.z;
+        x[3] = v.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (m + e < n) x[e] = src[m + e];
+      }
+    }
+    uint32_t h0, l0, h1, l1;
+    split_pair(x[0], x[1], h0, l0);
+    split_pair(x[2], x[3], h1, l1);
+    const int64_t dst = ((m / BK) * Ppad + p) * BK + 4 * perm_group(int((m % BK) / 4));
+    *reinterpret_cast<uint2*>(Vhi + dst) = make_uint2(h0, h1);
+    *reinterpret_cast<uint2*>(Vlo + dst) = make_uint2(l0, l1);
+  }
+}
+
 // One probe tile of one plane: BPW = 32 NACC rows x 32 bf16 = 4 BPW pieces of
 // 16 bytes, spread over the 256 threads (NACC / 2 pieces each, or one piece
 // for the first 128 threads when NACC = 1).
@@ -497,8 +534,9 @@ __device__ __forceinline__ floatx16 mfma_bf16(const u32x4& a, const u32x4& b, co
 // output gives lane (r, h) the exponents of its own row i = r at the 16
 // points m = 8 (s / 4) + 4 h + (s % 4), s = 0..15 -- exactly the BF16
 // A-fragment of that lane once the contraction index is ordered that way,
-// which the permuted probe planes provide.  Points past n carry -inf in the
-// slot of c |x_m|^2, so their entries are exactly 0.
+// which the permuted probe planes provide.  Points past n carry -2^100 in
+// the slot of c |x_m|^2 (finite, so that d exp2(d) of the derivative kernel
+// is 0 as well as exp2(d)), so their entries are exactly 0.
 __device__ __forceinline__ void split3(float x, __bf16 (&p)[3]) {
   p[0] = __bf16(x);
   const float r1 = x - float(p[0]);
@@ -530,10 +568,10 @@ __device__ __forceinline__ void exponent_slots(const float (&x)[3], float coef, 
     slot[18 + u] = side == 0 ? q[u] : one;
     slot[21 + u] = side == 0 ? one : q[u];
   }
-  if (pad) {  // a point past n: exponent -inf whatever the row
+  if (pad) {  // a point past n: exponent -2^100 whatever the row
 #pragma unroll
     for (int s = 0; s < 32; ++s) slot[s] = zero;
-    slot[18] = __bf16(-__builtin_inff());
+    slot[18] = __bf16(-0x1p100f);
     slot[21] = one;
     slot[22] = one;
     slot[23] = one;
@@ -943,19 +981,94 @@ __global__ void __launch_bounds__(kThreads)
   store_out<NACC, VEC, false>(Wout, V, P, n, i0w, p0, acc, 0.f);
 }
 
+// Derivative mode of the RBF BF16 matvec: the parameter contraction
+//   sum_c <L_c, D R_c>,  D_im = d_im exp2(d_im),  d_im = c |x_i - x_m|^2
+// (d_im exp2(d_im) = c l^3 dK_im/dl), together with the trace sum_c <L_c, R_c>,
+// over the columns c < K at base + (c / Pin) ldc + (c % Pin) n.  The planes of
+// R (columns c0 .. c0 + Ppad of this launch) are the staged operand; the
+// epilogue multiplies the product tile by L in FP32 instead of storing it.
+// Each workgroup writes its two partial sums (FP64) to red[2 wg + {0, 1}],
+// wg = (z * gridDim.y + y) * gridDim.x + x; the trace comes from the z = 0
+// slices only, so every (i, c) enters it once.
+struct DKArgs {
+  const float* L;
+  const float* R;
+  int64_t ldc, Pin, K, c0;
+  double* red;
+};
+
+template <int NACC>
+__device__ __forceinline__ void dk_epilogue(const floatx16 (&acc)[NACC], const DKArgs& dk,
+                                            int64_t i0w, int64_t p0, int64_t n, bool trace) {
+  const int t = threadIdx.x, lane = t & 63;
+  float part = 0.f, tr = 0.f;
+#pragma unroll
+  for (int c = 0; c < NACC; ++c) {
+    const int64_t cg = dk.c0 + p0 + 32 * c + (lane & 31);
+    if (cg >= dk.K) continue;
+    const int64_t off = (cg / dk.Pin) * dk.ldc + (cg % dk.Pin) * n;
+    const float* lc = dk.L + off;
+    const float* rc = dk.R + off;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int64_t i = i0w + 8 * g + 4 * (lane >> 5);
+      if (i + 4 <= n) {
+        const float4 lv = load4u(lc + i);
+        part = fmaf(acc[c][4 * g + 0], lv.x, part);
+        part = fmaf(acc[c][4 * g + 1], lv.y, part);
+        part = fmaf(acc[c][4 * g + 2], lv.z, part);
+        part = fmaf(acc[c][4 * g + 3], lv.w, part);
+        if (trace) {
+          const float4 rv = load4u(rc + i);
+          tr = fmaf(lv.x, rv.x, tr);
+          tr = fmaf(lv.y, rv.y, tr);
+          tr = fmaf(lv.z, rv.z, tr);
+          tr = fmaf(lv.w, rv.w, tr);
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (i + u < n) {
+            part = fmaf(acc[c][4 * g + u], lc[i + u], part);
+            if (trace) tr = fmaf(lc[i + u], rc[i + u], tr);
+          }
+        }
+      }
+    }
+  }
+  __shared__ double red[2][kThreads];
+  red[0][t] = double(part);
+  red[1][t] = double(tr);
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      red[0][t] += red[0][t + w];
+      red[1][t] += red[1][t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    const int64_t wg =
+        (int64_t(blockIdx.z) * gridDim.y + blockIdx.y) * int64_t(gridDim.x) + blockIdx.x;
+    dk.red[2 * wg] = red[0][0];
+    dk.red[2 * wg + 1] = red[1][0];
+  }
+}
+
 // RBF BF16 matvec.  Per tile and wave: two BF16 MFMAs give the exponents of
 // the NEXT tile, the 2 NACC NPROD BF16 MFMAs of this tile follow, and the
 // exp2 / split of the next tile's fragment is interleaved between them.  The
 // probe tiles and the point records are fetched two tiles ahead into
 // alternating register sets; the loop body is one basic block (clamped tile
 // indices, unconditional staging of the other LDS buffer), so that the
-// instruction order fixed below holds.
-template <int NACC, bool VEC, int NPROD>
+// instruction order fixed below holds.  DK: the derivative mode above (the
+// entries d exp2(d), the epilogue dk_epilogue).
+template <int NACC, bool VEC, int NPROD, bool DK = false>
 __global__ void __launch_bounds__(kThreads)
     k_matvec_rbf_bf16(const u32x4* __restrict__ rec, RBFParams prm,
                       const __bf16* __restrict__ Vhi_g, const __bf16* __restrict__ Vlo_g,
                       int64_t Ppad, const float* __restrict__ V, int64_t P, int64_t n,
-                      float* __restrict__ W) {
+                      float* __restrict__ W, DKArgs dk) {
   constexpr int BPW = 32 * NACC;
   constexpr bool LO = NPROD == 3;  // probes' low plane used
   constexpr int PT = PlaneTile<NACC>::kPerThread;
@@ -971,7 +1084,7 @@ __global__ void __launch_bounds__(kThreads)
   const int64_t kt_end = all_tiles * (blockIdx.z + 1) / gridDim.z;
   const int64_t ntiles = kt_end - kt_begin;
   const bool split = gridDim.z > 1;
-  float* Wout = split ? W + int64_t(blockIdx.z) * P * n : W;
+  float* Wout = (split && !DK) ? W + int64_t(blockIdx.z) * P * n : W;
   const int dim = prm.dim;
 
   floatx16 acc[NACC];
@@ -1023,7 +1136,8 @@ __global__ void __launch_bounds__(kThreads)
     const floatx16 d = rbf_exponents_mfma(x0, rr);
     float a[16];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) a[e] = __builtin_amdgcn_exp2f(d[e]);
+    for (int e = 0; e < 16; ++e)
+      a[e] = DK ? d[e] * __builtin_amdgcn_exp2f(d[e]) : __builtin_amdgcn_exp2f(d[e]);
     split_frag<NPROD>(a, fAh, fAl);
   }
   __syncthreads();
@@ -1091,8 +1205,12 @@ __global__ void __launch_bounds__(kThreads)
         const int j = i - kLead, slots = kM - kLead;
 #pragma unroll
         for (int e = j * 8 / slots; e < (j + 1) * 8 / slots; ++e) {
-          const float x0 = __builtin_amdgcn_exp2f(dnext[2 * e]);
-          const float x1 = __builtin_amdgcn_exp2f(dnext[2 * e + 1]);
+          float x0 = __builtin_amdgcn_exp2f(dnext[2 * e]);
+          float x1 = __builtin_amdgcn_exp2f(dnext[2 * e + 1]);
+          if (DK) {
+            x0 *= dnext[2 * e];
+            x1 *= dnext[2 * e + 1];
+          }
           uint32_t hw, lw;
           if (NPROD >= 2) {
             split_pair(x0, x1, hw, lw);
@@ -1117,7 +1235,9 @@ __global__ void __launch_bounds__(kThreads)
     step(kt + 1, hB, lB, hA, lA, xA, xB, fBh, fBl, fAh, fAl);
   }
   if (kt < ntiles) step(kt, hA, lA, hB, lB, xB, xA, fAh, fAl, fBh, fBl);
-  if (split)
+  if constexpr (DK)
+    dk_epilogue<NACC>(acc, dk, i0w, p0, n, blockIdx.z == 0);
+  else if (split)
     store_out<NACC, VEC, false>(Wout, V, P, n, i0w, p0, acc, 0.f);
   else
     store_out<NACC, VEC, true>(Wout, V, P, n, i0w, p0, acc, prm.noise);
@@ -1306,10 +1426,10 @@ hipError_t launch_tpl_bf16_n(hipStream_t s, const float* A, const u32x4* rec,
   if constexpr (RBF) {
     if (vec)
       hipLaunchKernelGGL((k_matvec_rbf_bf16<NACC, true, NPROD>), grid, block, 0, s, rec, prm, Vhi,
-                         Vlo, Ppad, V, P, n, W);
+                         Vlo, Ppad, V, P, n, W, DKArgs{});
     else
       hipLaunchKernelGGL((k_matvec_rbf_bf16<NACC, false, NPROD>), grid, block, 0, s, rec, prm,
-                         Vhi, Vlo, Ppad, V, P, n, W);
+                         Vhi, Vlo, Ppad, V, P, n, W, DKArgs{});
   } else {
     if (vec)
       hipLaunchKernelGGL((k_matvec_bf16_dense<NACC, true, NPROD>), grid, block, 0, s, A, Vhi, Vlo,
@@ -1428,6 +1548,27 @@ int choose_splitk(int64_t n, int64_t P, int64_t avail_floats) {
          int64_t(2 * S) * P * n <= avail_floats)
     S *= 2;
   return S;
+}
+
+// Derivative-kernel contraction: columns in launches of at most kDkCols
+// (planes of kDkCols x npad split-BF16 values), 128 per workgroup column
+// (NACC = 4); the split-K factor S fills the GPU (the slices add to the
+// partial sums, so they need no partial planes).
+constexpr int64_t kDkCols = 1024;
+struct DkPlan {
+  int64_t cols, groups, nbx;
+  int S;
+};
+DkPlan dk_plan(int64_t n, int64_t K) {
+  DkPlan p;
+  p.groups = (K + 127) / 128;
+  p.cols = p.groups * 128 < kDkCols ? p.groups * 128 : kDkCols;
+  p.nbx = (n + BI - 1) / BI;
+  const int64_t wgs = p.nbx * (p.cols / 128);
+  const int64_t tiles = (n + BK - 1) / BK;
+  p.S = 1;
+  while (p.S < kMaxSplitK && wgs * p.S < 1024 && tiles / (2 * p.S) >= 8) p.S *= 2;
+  return p;
 }
 
 }  // namespace
@@ -1604,6 +1745,53 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
     hipLaunchKernelGGL(k_splitk_reduce<false>, dim3(unsigned(blocks)), dim3(256), 0, s, partial,
                        S, count, V, 0.f, W);
   return hipGetLastError();
+}
+
+int64_t rbf_dk_scratch_floats(int64_t n, int64_t K) {
+  const int64_t npad = (n + BK - 1) / BK * BK;
+  return dk_plan(n, K).cols * npad + 16 * npad;
+}
+
+int64_t rbf_dk_partials(int64_t n, int64_t K) {
+  const DkPlan p = dk_plan(n, K);
+  return p.nbx * p.S * p.groups;
+}
+
+hipError_t launch_rbf_dk_contract(hipStream_t s, const float* X, int dim, double lengthscale,
+                                  const float* L, const float* R, int64_t ldc, int64_t Pin,
+                                  int64_t n, int64_t K, float* scratch, double* red) {
+  if (n <= 0 || K <= 0) return hipSuccess;
+  if (dim < 1 || dim > 3) return hipErrorInvalidValue;  // rejected by the API before launch
+  const DkPlan pl = dk_plan(n, K);
+  const int64_t npad = (n + BK - 1) / BK * BK;
+  __bf16* Vhi = reinterpret_cast<__bf16*>(scratch);
+  __bf16* Vlo = Vhi + pl.cols * npad;
+  u32x4* rec = reinterpret_cast<u32x4*>(scratch + pl.cols * npad);
+  RBFParams prm{};
+  prm.X = X;
+  prm.dim = dim;
+  prm.coef = float(-1.4426950408889634 / (2.0 * lengthscale * lengthscale));
+  hipLaunchKernelGGL(k_prep_points, dim3(unsigned((npad + 255) / 256)), dim3(256), 0, s, X, dim,
+                     prm.coef, n, npad, rec);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  const int64_t gx = (npad / 4 + 255) / 256;
+  int64_t g0 = 0;
+  for (int64_t c0 = 0; c0 < K; c0 += pl.cols) {
+    const int64_t left = (K - c0 + 127) / 128 * 128;
+    const int64_t cols = left < pl.cols ? left : pl.cols;
+    const int64_t ny = cols / 128;
+    hipLaunchKernelGGL(k_split_columns, dim3(unsigned(gx), unsigned(cols)), dim3(256), 0, s, R,
+                       ldc, Pin, n, K, c0, cols, npad, Vhi, Vlo);
+    DKArgs dk{L, R, ldc, Pin, K, c0, red + 2 * pl.nbx * pl.S * g0};
+    hipLaunchKernelGGL((k_matvec_rbf_bf16<4, false, 3, true>),
+                       dim3(unsigned(pl.nbx), unsigned(ny), unsigned(pl.S)), dim3(kThreads), 0, s,
+                       rec, prm, Vhi, Vlo, cols, nullptr, cols, n, nullptr, dk);
+    err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    g0 += ny;
+  }
+  return hipSuccess;
 }
 
 }  // namespace slq

@@ -20,7 +20,7 @@ struct slq_context {
   Buf partial1, partial2, norm2, work_w, work_w2, basis, hmat, resid, vec0, scal1, scal2, eig,
       per_probe, mean, splitk;
   // reverse pass
-  Buf adj_dh, adj_gam, adj_coef, adj_lam, adj_v1, adj_v2, adj_v3, adj_work, adj_sum;
+  Buf adj_dh, adj_gam, adj_coef, adj_lam, adj_v1, adj_v2, adj_v3, adj_work, adj_sum, adj_dk;
   // plain Lanczos: three rotating vectors and the last off-diagonal
   Buf ring, blast;
   // general-cotangent reverse passes: Pi_gamma, gamma, the internal dH
@@ -32,7 +32,7 @@ struct slq_context {
     return {&partial1, &partial2, &norm2,   &work_w,  &work_w2,  &basis,   &hmat,
             &resid,    &vec0,     &scal1,   &scal2,   &eig,      &per_probe, &mean,
             &splitk,   &adj_dh,   &adj_gam, &adj_coef, &adj_lam, &adj_v1,  &adj_v2,
-            &adj_v3,   &adj_work, &adj_sum, &ring,    &blast,    &adj_pg,  &adj_gvec,
+            &adj_v3,   &adj_work, &adj_sum, &adj_dk,  &ring,    &blast,    &adj_pg,  &adj_gvec,
             &lr_lc,    &lr_perm,
             &lr_bval,  &lr_bidx,  &lr_rowi, &lr_small, &lr_gp,   &lr_c,    &lr_wp,
             &lr_z,     &cg_r,     &cg_z,    &cg_d,    &cg_q,     &cg_parts, &cg_scal};
@@ -48,6 +48,9 @@ struct slq_context {
   bool dry_run = false;
   char* arena = nullptr;
   size_t arena_size = 0, arena_off = 0;
+  // arena mode: bytes of the buffers that were replaced by larger ones (a
+  // dry run seeded with a pool's sizes: the pool frees them when it grows)
+  size_t arena_replaced = 0;
   // measurement hooks: event pairs per kernel class, reused across reads
   unsigned profile_mask = 0;  // bit kc: kernel class kc is bracketed by events
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events[SLQ_KC_COUNT];
@@ -132,6 +135,7 @@ int ensure(slq_context* ctx, slq_context::Buf& b, size_t bytes, void** out) {
   if (bytes == 0) bytes = 16;
   if (ctx->arena_mode) {
     if (b.bytes < bytes) {
+      ctx->arena_replaced += b.bytes;
       const size_t off = (ctx->arena_off + 255) / 256 * 256;
       if (off + bytes > ctx->arena_size)
         return fail(ctx, SLQ_ENOMEM,
@@ -491,6 +495,35 @@ int logdet_lanczos_forward(slq_context* ctx, const slq_operator& op, const float
   return SLQ_OK;
 }
 
+// RBF parameter cotangents from S = sum_{c < K} L_c R_c^T (columns c at
+// base + (c / P) ldc + (c % P) n): dparams = (<S, dK/dl>, <S, I>), the VJPs of
+// the matvec in (lengthscale, noise).  FP32 operators: the FP32 matrix-core
+// contraction over the tiles of S; BF16 operators: the derivative kernel
+// d exp2(d) = c l^3 dK/dl (c = -log2(e) / (2 l^2)) applied to the columns of R
+// as a block matvec in split BF16 (hi*hi + hi*lo + lo*hi), dotted with L.
+int rbf_param_contract(slq_context* ctx, const slq_operator& op, const float* L, const float* R,
+                       int64_t ldc, int64_t P, int64_t K, double* dparams) {
+  const int64_t n = op.n;
+  hipStream_t s = ctx->stream;
+  const double ls = op.lengthscale;
+  double* dsum;
+  if (op.precision == SLQ_PREC_FP32) {
+    const int64_t cnt = outer_contract_blocks(n);
+    SLQ_TRY(ensure_t(ctx, ctx->adj_sum, size_t(2 * cnt), &dsum));
+    SLQ_HIP(ctx, launch_outer_rbf_contract(s, L, R, ldc, P, n, K, op.points, op.dim, ls, dsum));
+    SLQ_HIP(ctx, launch_sum_fixed(s, dsum, cnt, 2, 1.0 / (ls * ls * ls), 1.0, dparams));
+  } else {
+    const int64_t cnt = rbf_dk_partials(n, K);
+    float* dk;
+    SLQ_TRY(ensure_t(ctx, ctx->adj_sum, size_t(2 * cnt), &dsum));
+    SLQ_TRY(ensure_t(ctx, ctx->adj_dk, size_t(rbf_dk_scratch_floats(n, K)), &dk));
+    SLQ_HIP(ctx, launch_rbf_dk_contract(s, op.points, op.dim, ls, L, R, ldc, P, n, K, dk, dsum));
+    const double c = double(float(-1.4426950408889634 / (2.0 * ls * ls)));
+    SLQ_HIP(ctx, launch_sum_fixed(s, dsum, cnt, 2, 1.0 / (ls * ls * ls * c), 1.0, dparams));
+  }
+  return SLQ_OK;
+}
+
 // The reverse pass: cotangent of the probe mean w.r.t. H (eigh VJP), the
 // Arnoldi adjoint recursion (decomp.py:480-600, reortho="full"), and the
 // operator parameters' cotangents (the matvec VJPs of decomp.py:587-589).
@@ -501,7 +534,7 @@ int logdet_reverse(slq_context* ctx, const slq_operator& op, int64_t P, int64_t 
   const int pstride = int(k);
   const int64_t nb = num_chunks(n);
   float *dH, *Gam, *coef, *Lam, *lam, *lam2, *vm, *part1, *part2, *mvs;
-  double *vwork = nullptr, *dsum;
+  double* vwork = nullptr;
   const int64_t mv_floats = matvec_scratch_floats(n, P);
   SLQ_TRY(ensure_t(ctx, ctx->splitk, size_t(mv_floats), &mvs));
   SLQ_TRY(ensure_t(ctx, ctx->partial1, size_t(P * nb * pstride), &part1));
@@ -515,7 +548,6 @@ int logdet_reverse(slq_context* ctx, const slq_operator& op, int64_t P, int64_t 
   SLQ_TRY(ensure_t(ctx, ctx->adj_v3, size_t(P * n), &vm));
   const int64_t vw = tridiag_vjp_work_doubles(P, k);
   if (vw > 0) SLQ_TRY(ensure_t(ctx, ctx->adj_work, size_t(vw), &vwork));
-  SLQ_TRY(ensure_t(ctx, ctx->adj_sum, size_t(2 * outer_contract_blocks(n)), &dsum));
 
   // dH from the eigh VJP of |z|^2 [log T]_00 / P (funm.py:239-241, 331-333)
   SLQ_HIP(ctx, launch_fill_zero(s, dH, P * k * k));
@@ -549,12 +581,7 @@ int logdet_reverse(slq_context* ctx, const slq_operator& op, int64_t P, int64_t 
     // with S = sum_{idx, p} lambda q^T: d/d noise = <S, I> and
     // d/d lengthscale = <S, dK/dl> = <S, |x_i - x_j|^2 K> / l^3, in one pass
     // over the output tiles of S (never stored)
-    SLQ_HIP(ctx, launch_outer_rbf_contract(s, Lam, st.Q, ldc, P, n, k * P, op.points, op.dim,
-                                           op.lengthscale, dsum,
-                                           op.precision != SLQ_PREC_FP32));
-    const double ls = op.lengthscale;
-    SLQ_HIP(ctx, launch_sum_fixed(s, dsum, outer_contract_blocks(n), 2, 1.0 / (ls * ls * ls), 1.0,
-                                  dparams));
+    SLQ_TRY(rbf_param_contract(ctx, op, Lam, st.Q, ldc, P, k * P, dparams));
   } else {
     // d/dA = sum_idx sum_p lambda q^T (the VJP of A -> A q)
     SLQ_HIP(ctx, launch_outer_sum(s, Lam, st.Q, ldc, P, n, k * P, dmatrix));
@@ -570,14 +597,7 @@ int param_cotangents(slq_context* ctx, const slq_operator& op, const float* L, c
   const int64_t n = op.n;
   hipStream_t s = ctx->stream;
   if (op.kind == SLQ_OP_RBF) {
-    double* dsum;
-    SLQ_TRY(ensure_t(ctx, ctx->adj_sum, size_t(2 * outer_contract_blocks(n)), &dsum));
-    SLQ_HIP(ctx, launch_outer_rbf_contract(s, L, R, ldc, P, n, k * P, op.points, op.dim,
-                                           op.lengthscale, dsum,
-                                           op.precision != SLQ_PREC_FP32));
-    const double ls = op.lengthscale;
-    SLQ_HIP(ctx, launch_sum_fixed(s, dsum, outer_contract_blocks(n), 2, 1.0 / (ls * ls * ls), 1.0,
-                                  dparams));
+    SLQ_TRY(rbf_param_contract(ctx, op, L, R, ldc, P, k * P, dparams));
   } else {
     SLQ_HIP(ctx, launch_outer_sum(s, L, R, ldc, P, n, k * P, dmatrix));
   }
@@ -778,7 +798,7 @@ int logdet_lanczos_reverse(slq_context* ctx, const slq_operator& op, int64_t P, 
   hipStream_t s = ctx->stream;
   const int64_t nb = num_chunks(n);
   float *dH, *Lam, *xi, *W, *part1, *part2, *mvs;
-  double *vwork = nullptr, *dsum = nullptr;
+  double* vwork = nullptr;
   const int64_t mv_floats = matvec_scratch_floats(n, P);
   SLQ_TRY(ensure_t(ctx, ctx->splitk, size_t(mv_floats), &mvs));
   SLQ_TRY(ensure_t(ctx, ctx->partial1, size_t(P * nb * 3), &part1));
@@ -789,8 +809,6 @@ int logdet_lanczos_reverse(slq_context* ctx, const slq_operator& op, int64_t P, 
   SLQ_TRY(ensure_t(ctx, ctx->adj_v2, size_t(P * n), &W));
   const int64_t vw = tridiag_vjp_work_doubles(P, k);
   if (vw > 0) SLQ_TRY(ensure_t(ctx, ctx->adj_work, size_t(vw), &vwork));
-  if (op.kind == SLQ_OP_RBF)
-    SLQ_TRY(ensure_t(ctx, ctx->adj_sum, size_t(2 * outer_contract_blocks(n)), &dsum));
 
   // dT from the eigh VJP of |z|^2 [log T]_00 / P (funm.py:239-241, 331-333)
   SLQ_HIP(ctx, launch_fill_zero(s, dH, P * kk));
@@ -831,12 +849,7 @@ int logdet_lanczos_reverse(slq_context* ctx, const slq_operator& op, int64_t P, 
 
   // parameter cotangents: S = sum_{i, p} x_i lambda_i^T
   if (op.kind == SLQ_OP_RBF) {
-    SLQ_HIP(ctx, launch_outer_rbf_contract(s, st.Q, Lam, ldc, P, n, k * P, op.points, op.dim,
-                                           op.lengthscale, dsum,
-                                           op.precision != SLQ_PREC_FP32));
-    const double ls = op.lengthscale;
-    SLQ_HIP(ctx, launch_sum_fixed(s, dsum, outer_contract_blocks(n), 2, 1.0 / (ls * ls * ls), 1.0,
-                                  dparams));
+    SLQ_TRY(rbf_param_contract(ctx, op, st.Q, Lam, ldc, P, k * P, dparams));
   } else {
     // d/dA of <x, A lambda> is x lambda^T
     SLQ_HIP(ctx, launch_outer_sum(s, st.Q, Lam, ldc, P, n, k * P, dmatrix));
@@ -1391,14 +1404,7 @@ int slq_quadform_vjp(slq_context* ctx, const slq_operator* op, const float* X, i
   SLQ_TRY(set_device(ctx));
   hipStream_t s = ctx->stream;
   if (op->kind == SLQ_OP_RBF) {
-    double* dsum;
-    SLQ_TRY(ensure_t(ctx, ctx->adj_sum, size_t(2 * outer_contract_blocks(n)), &dsum));
-    SLQ_HIP(ctx, launch_outer_rbf_contract(s, X, X, P * n, P, n, P, op->points, op->dim,
-                                           op->lengthscale, dsum,
-                                           op->precision != SLQ_PREC_FP32));
-    const double ls = op->lengthscale;
-    SLQ_HIP(ctx, launch_sum_fixed(s, dsum, outer_contract_blocks(n), 2, 1.0 / (ls * ls * ls), 1.0,
-                                  dparams));
+    SLQ_TRY(rbf_param_contract(ctx, *op, X, X, P * n, P, P, dparams));
   } else {
     SLQ_HIP(ctx, launch_outer_sum(s, X, X, P * n, P, n, P, dmatrix));
   }
@@ -1440,8 +1446,11 @@ int slq_context_release(slq_context* ctx) {
   return SLQ_OK;
 }
 
-int slq_workspace_bytes(slq_context* ctx, int entry, const slq_operator* op, int64_t P, int64_t k,
-                        int64_t* bytes) {
+// The scratch of one call of entry, by running the call as a dry run: from an
+// empty pool (fresh), or from a pool with ctx's buffer sizes (grow: the bytes
+// by which ctx's pool grows).
+static int workspace_query(slq_context* ctx, int entry, const slq_operator* op, int64_t P,
+                           int64_t k, bool grow, int64_t* bytes) {
   if (!ctx) return SLQ_EINVAL;
   if (!bytes) return fail(ctx, SLQ_EINVAL, "output pointer is NULL");
   *bytes = 0;
@@ -1460,6 +1469,14 @@ int slq_workspace_bytes(slq_context* ctx, int entry, const slq_operator* op, int
   v.dry_run = true;
   v.arena = reinterpret_cast<char*>(uintptr_t(4096));  // placeholder base (never dereferenced)
   v.arena_size = SIZE_MAX / 2;
+  if (grow) {
+    const auto from = ctx->all_bufs();
+    const auto to = v.all_bufs();
+    for (size_t i = 0; i < from.size(); ++i) {
+      to[i]->bytes = from[i]->bytes;
+      to[i]->ptr = from[i]->bytes ? v.arena : nullptr;  // placeholder
+    }
+  }
   const float* Z = reinterpret_cast<const float*>(op);
   double dd = 0.0;
   float df = 0.f;
@@ -1484,8 +1501,18 @@ int slq_workspace_bytes(slq_context* ctx, int entry, const slq_operator* op, int
       return fail(ctx, SLQ_EINVAL, "unknown entry " + std::to_string(entry));
   }
   if (rc != SLQ_OK) return fail(ctx, rc, v.error);
-  *bytes = int64_t(v.arena_off);
+  *bytes = int64_t(v.arena_off - (grow ? v.arena_replaced : 0));
   return SLQ_OK;
+}
+
+int slq_workspace_bytes(slq_context* ctx, int entry, const slq_operator* op, int64_t P, int64_t k,
+                        int64_t* bytes) {
+  return workspace_query(ctx, entry, op, P, k, false, bytes);
+}
+
+int slq_workspace_growth(slq_context* ctx, int entry, const slq_operator* op, int64_t P,
+                         int64_t k, int64_t* bytes) {
+  return workspace_query(ctx, entry, op, P, k, true, bytes);
 }
 
 int slq_logdet_save(slq_context* ctx, const slq_operator* op, const float* Z, int64_t P, int64_t k,

@@ -254,10 +254,20 @@ hipError_t launch_outer_sum(hipStream_t s, const float* L, const float* R, int64
 // red[2 b] = sum_ij S_ij d2_ij exp(-d2_ij / (2 l^2)), d2_ij = |x_i - x_j|^2, and
 // red[2 b + 1] = sum_i S_ii, partial over the outer_contract_blocks(n) tiles b.
 int64_t outer_contract_blocks(int64_t n);
-// bf16: the split-BF16 matrix-core form (operators whose matvec runs in BF16).
+// FP32 matrix cores (operators whose matvec runs in FP32).
 hipError_t launch_outer_rbf_contract(hipStream_t s, const float* L, const float* R, int64_t ldc,
                                      int64_t P, int64_t n, int64_t K, const float* X, int dim,
-                                     double lengthscale, double* red, bool bf16);
+                                     double lengthscale, double* red);
+// The same two sums for an RBF operator with dim <= 3 in split BF16 from the
+// block matvec's derivative mode (k_matvec.hip): red receives
+// rbf_dk_partials(n, K) pairs (sum_ij S_ij d_ij exp2(d_ij), sum_i S_ii) with
+// d_ij = c d2_ij and c = float(-log2(e) / (2 l^2)); scratch holds
+// rbf_dk_scratch_floats(n, K) floats (16-byte aligned).
+int64_t rbf_dk_partials(int64_t n, int64_t K);
+int64_t rbf_dk_scratch_floats(int64_t n, int64_t K);
+hipError_t launch_rbf_dk_contract(hipStream_t s, const float* X, int dim, double lengthscale,
+                                  const float* L, const float* R, int64_t ldc, int64_t Pin,
+                                  int64_t n, int64_t K, float* scratch, double* red);
 // out[o] = f_o * sum_i partial[i * nout + o] (fixed order), nout <= 2.
 hipError_t launch_sum_fixed(hipStream_t s, const double* partial, int64_t count, int nout,
                             double f0, double f1, double* out);

@@ -11,11 +11,14 @@ fixed orders, and no scratch is shared).
 Admission by HBM footprint.  A request's footprint is the device scratch its
 library call needs (``slq_workspace_bytes``: the Krylov basis of k columns,
 the work vectors, split-K partials and probe planes; plus its probes and
-outputs), known exactly before it starts.  A slot's context keeps the scratch
-of the largest request it ran (the pool grows, it does not shrink), so the
-loop accounts for what every slot holds: resident = sum over slots of the
-scratch held (for a running request, the larger of what the slot held and
-what the request needs).  A request is admitted in arrival order once it has
+outputs), known exactly before it starts.  A slot's context keeps each
+scratch buffer at the largest size any request it ran needed (the pool grows,
+it does not shrink), so the loop accounts for what every slot holds:
+resident = sum over slots of the scratch held (for a running request, what
+the slot held plus the growth of its pool in this call, which the library
+also computes exactly before the call: ``slq_workspace_growth``; a pool can
+end above both its old holding and the request's fresh need when the two
+differ in which buffers are large).  A request is admitted in arrival order once it has
 arrived, a slot is free and the resident total with it fits the budget; if it
 does not fit, the idle slots' scratch is released first (largest first); if
 it still does not fit, the request waits for running ones to finish.  A
@@ -66,14 +69,20 @@ class Result(NamedTuple):
     waited_for_memory: bool = False
 
 
-def footprint(req: Request, device=None) -> int:
+def footprint(req: Request, device=None, context=None) -> int:
     """Device bytes a request needs beyond its operator: the library's scratch
-    for the call (exact) plus its probes and outputs."""
+    for the call (exact) plus its probes and outputs.  With ``context`` (a
+    library context): the growth of that context's pooled scratch in the call,
+    given what it holds, in place of a fresh context's scratch."""
     device = torch.device("cuda", torch.cuda.current_device()) if device is None else device
     op = operators.resolve(req.matvec)
     desc, keep = op.descriptor(req.n, req.params, device)
-    ctx = _lib.context(device.index, torch.cuda.current_stream(device).cuda_stream)
-    scratch = ctx.workspace_bytes("logdet_vjp" if req.grad else "logdet", desc, req.P, req.k)
+    entry = "logdet_vjp" if req.grad else "logdet"
+    if context is None:
+        ctx = _lib.context(device.index, torch.cuda.current_stream(device).cuda_stream)
+        scratch = ctx.workspace_bytes(entry, desc, req.P, req.k)
+    else:
+        scratch = context.workspace_growth(entry, desc, req.P, req.k)
     del keep
     extra = 4 * req.P * req.n + 8 * req.P + 64
     if req.grad and not isinstance(op, operators.RBF):
@@ -81,31 +90,36 @@ def footprint(req: Request, device=None) -> int:
     return int(scratch + extra)
 
 
-def admission(need, slot, held, free, running_bytes, any_running, budget):
+def admission(need, slot, held, free, running_bytes, any_running, budget, growth=None):
     """Whether a request needing ``need`` bytes may start in idle ``slot``.
 
     ``held[s]``: scratch bytes slot s's context holds; ``free``: the idle
-    slots; ``running_bytes``: bytes accounted to running requests.  Returns
-    ``(admit, release)``: admit or wait, and the idle slots whose scratch is
-    to be released first (largest first, only as many as needed; none when
-    the request waits).  A request that does not fit even with every idle
-    slot released waits while anything runs, and runs alone otherwise.
+    slots; ``running_bytes``: bytes accounted to running requests;
+    ``growth``: the bytes by which the slot's pool grows in the request (its
+    context's ``footprint``); None for the bound ``need`` (a pool never grows
+    by more than a fresh context needs).  The request is charged what the
+    slot holds plus the growth, or ``need`` once the slot holds nothing.
+    Returns ``(admit, release)``: admit or wait, and the idle slots whose
+    scratch is to be released first (largest first, only as many as needed;
+    none when the request waits).  A request that does not fit even with every
+    idle slot released waits while anything runs, and runs alone otherwise.
     """
     if budget is None:
         return True, []
     # per-slot holdings, from a list indexed by slot or a mapping slot -> bytes
     h = dict(held) if isinstance(held, dict) else dict(enumerate(held))
+    g = need if growth is None else growth
 
     def total():
-        return sum(h[s] for s in free) + running_bytes - h[slot] + max(h[slot], need)
+        return sum(h[s] for s in free) + running_bytes + (g if h[slot] > 0 else need)
 
     release = []
     with_it = total()
     for s in sorted(free, key=lambda s: -h[s]):
         if with_it <= budget or h[s] == 0:
             break
-        if s == slot and h[s] <= need:
-            continue  # the request takes this memory over: releasing it gains nothing
+        if s == slot and h[s] + g <= need:
+            continue  # the request reuses this memory: releasing it gains nothing
         release.append(s)
         h[s] = 0
         with_it = total()
@@ -207,8 +221,10 @@ class Server:
                     continue
                 slot = free[0]
                 running_bytes = sum(r[6] for r in running.values())
+                grow = (footprint(q, self.device, self.contexts[slot])
+                        if budget is not None and held[slot] > 0 else need[i])
                 admit, release = admission(need[i], slot, held, free, running_bytes,
-                                           bool(running), budget)
+                                           bool(running), budget, grow)
                 if not admit:
                     if i not in waited:
                         waited.add(i)
@@ -230,7 +246,8 @@ class Server:
                     free.insert(0, slot)
                     held[slot] = self._held(slot)
                     continue
-                running[slot] = (i, per, mean, grads, ev, now, max(held[slot], need[i]))
+                running[slot] = (i, per, mean, grads, ev, now,
+                                 held[slot] + grow if held[slot] > 0 else need[i])
                 stats["peak_resident"] = max(stats["peak_resident"], resident())
             if poll_sleep > 0.0:
                 time.sleep(poll_sleep)

@@ -137,17 +137,22 @@ def test_serving_admission_rule():
     assert adm(100, 0, held, [0, 1], 60, True, 150) == (False, [])  # 160 > 150: wait
     assert adm(100, 0, held, [0, 1], 50, True, 150) == (True, [])  # fits exactly
     assert adm(500, 0, held, [0, 1, 2], 0, False, 150) == (True, [])  # too big: alone
-    # a slot that already holds more than the request needs is charged its
-    # holding; here that does not fit, so the slot's own scratch is released
+    # a slot that holds scratch is charged its holding plus the growth of its
+    # pool (without the growth: the bound need); here that does not fit, so
+    # the slot's own scratch is released
     assert adm(10, 0, {0: 120, 1: 0}, [0, 1], 50, True, 150) == (True, [0])
     assert adm(10, 0, {0: 90, 1: 0}, [0, 1], 50, True, 150) == (True, [])
+    assert adm(10, 0, {0: 90, 1: 0}, [0, 1], 50, True, 140, 0) == (True, [])
+    assert adm(10, 0, {0: 90, 1: 0}, [0, 1], 50, True, 140) == (True, [0])
     # idle slots give their memory back, largest first, only as far as needed
     held = {0: 0, 1: 80, 2: 40, 3: 10}
     assert adm(50, 0, held, [0, 1, 2, 3], 20, True, 150) == (True, [1])
     assert adm(100, 0, held, [0, 1, 2, 3], 20, True, 150) == (True, [1, 2])
     # when releasing everything is still not enough, nothing is released: wait
     assert adm(200, 0, held, [0, 1, 2, 3], 20, True, 150) == (False, [])
-    # the chosen slot's own holding is released only when the request needs
-    # less than it holds
-    assert adm(100, 1, held, [0, 1, 2, 3], 0, False, 120) == (True, [2])
+    # the chosen slot's own holding is released only when its holding plus
+    # the growth exceeds what the request needs from an empty pool
+    assert adm(100, 1, held, [0, 1, 2, 3], 0, False, 120, 20) == (True, [2])
+    assert adm(100, 1, held, [0, 1, 2, 3], 0, False, 120) == (True, [1, 2])
     assert adm(30, 1, held, [0, 1, 2, 3], 0, False, 90) == (True, [1])
+    assert adm(30, 1, held, [0, 1, 2, 3], 0, False, 90, 0) == (True, [1])

@@ -316,3 +316,32 @@ def test_workspace_query_matches_pool_growth(cuda):
             ctx.release()
             assert ctx.scratch_bytes() == 0
         assert held <= need <= held + 256 * 64, (entry, held, need)
+
+
+def test_workspace_growth_matches_pool_growth(cuda):
+    # slq_workspace_growth: how far a context that already holds scratch grows
+    # in a call, given what it holds (the pool keeps each buffer at the
+    # largest size any call needed), over a sequence of forward and gradient
+    # calls of different shapes on one context
+    from matfree_amd import _lib
+
+    calls = [(1500, 16, 20, False), (1200, 48, 24, True), (1700, 8, 12, False),
+             (900, 64, 30, True), (1500, 16, 20, False)]
+    stream = torch.cuda.Stream(cuda)
+    with torch.cuda.stream(stream):
+        ctx = _lib.context(cuda.index, stream.cuda_stream)
+        for n, P, k, grad in calls:
+            X = torch.as_tensor(problems.rbf_points(n, seed=n), device=cuda)
+            op = operators.RBF(X, precision="bf16x2")
+            desc, keep = op.descriptor(n, (0.5, 0.1), cuda)
+            before = ctx.scratch_bytes()
+            grow = ctx.workspace_growth("logdet_vjp" if grad else "logdet", desc, P, k)
+            Z = torch.ones((P, n), device=cuda)
+            if grad:
+                funm.logdet_value_and_grad(op, Z, k, (0.5, 0.1))
+            else:
+                funm.logdet_per_probe(op, Z, k, (0.5, 0.1))
+            after = ctx.scratch_bytes()
+            assert after - before <= grow <= after - before + 256 * 64, (n, P, k, grad)
+        stream.synchronize()
+        ctx.release()
