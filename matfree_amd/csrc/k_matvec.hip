@@ -1062,13 +1062,13 @@ __device__ __forceinline__ void dk_epilogue(const floatx16 (&acc)[NACC], const D
 // alternating register sets; the loop body is one basic block (clamped tile
 // indices, unconditional staging of the other LDS buffer), so that the
 // instruction order fixed below holds.  DK: the derivative mode above (the
-// entries d exp2(d), the epilogue dk_epilogue).
-template <int NACC, bool VEC, int NPROD, bool DK = false>
-__global__ void __launch_bounds__(kThreads)
-    k_matvec_rbf_bf16(const u32x4* __restrict__ rec, RBFParams prm,
-                      const __bf16* __restrict__ Vhi_g, const __bf16* __restrict__ Vlo_g,
-                      int64_t Ppad, const float* __restrict__ V, int64_t P, int64_t n,
-                      float* __restrict__ W, DKArgs dk) {
+// entries d exp2(d), the epilogue dk_epilogue), run by k_rbf_dk_contract.
+template <int NACC, bool VEC, int NPROD, bool DK>
+__device__ __forceinline__ void rbf_bf16_body(const u32x4* __restrict__ rec, const RBFParams& prm,
+                                              const __bf16* __restrict__ Vhi_g,
+                                              const __bf16* __restrict__ Vlo_g, int64_t Ppad,
+                                              const float* __restrict__ V, int64_t P, int64_t n,
+                                              float* __restrict__ W, const DKArgs& dk) {
   constexpr int BPW = 32 * NACC;
   constexpr bool LO = NPROD == 3;  // probes' low plane used
   constexpr int PT = PlaneTile<NACC>::kPerThread;
@@ -1241,6 +1241,24 @@ __global__ void __launch_bounds__(kThreads)
     store_out<NACC, VEC, false>(Wout, V, P, n, i0w, p0, acc, 0.f);
   else
     store_out<NACC, VEC, true>(Wout, V, P, n, i0w, p0, acc, prm.noise);
+}
+
+template <int NACC, bool VEC, int NPROD>
+__global__ void __launch_bounds__(kThreads)
+    k_matvec_rbf_bf16(const u32x4* __restrict__ rec, RBFParams prm,
+                      const __bf16* __restrict__ Vhi_g, const __bf16* __restrict__ Vlo_g,
+                      int64_t Ppad, const float* __restrict__ V, int64_t P, int64_t n,
+                      float* __restrict__ W) {
+  rbf_bf16_body<NACC, VEC, NPROD, false>(rec, prm, Vhi_g, Vlo_g, Ppad, V, P, n, W, DKArgs{});
+}
+
+// The derivative-mode contraction over the Ppad (multiple of 128) columns of
+// the planes (hi and lo) of one launch.
+__global__ void __launch_bounds__(kThreads)
+    k_rbf_dk_contract(const u32x4* __restrict__ rec, RBFParams prm,
+                      const __bf16* __restrict__ Vhi_g, const __bf16* __restrict__ Vlo_g,
+                      int64_t Ppad, int64_t n, DKArgs dk) {
+  rbf_bf16_body<4, false, 3, true>(rec, prm, Vhi_g, Vlo_g, Ppad, nullptr, Ppad, n, nullptr, dk);
 }
 
 // RBF matvec for a few vectors (P <= kRowsMaxP), as CG's single right-hand
@@ -1426,10 +1444,10 @@ hipError_t launch_tpl_bf16_n(hipStream_t s, const float* A, const u32x4* rec,
   if constexpr (RBF) {
     if (vec)
       hipLaunchKernelGGL((k_matvec_rbf_bf16<NACC, true, NPROD>), grid, block, 0, s, rec, prm, Vhi,
-                         Vlo, Ppad, V, P, n, W, DKArgs{});
+                         Vlo, Ppad, V, P, n, W);
     else
       hipLaunchKernelGGL((k_matvec_rbf_bf16<NACC, false, NPROD>), grid, block, 0, s, rec, prm,
-                         Vhi, Vlo, Ppad, V, P, n, W, DKArgs{});
+                         Vhi, Vlo, Ppad, V, P, n, W);
   } else {
     if (vec)
       hipLaunchKernelGGL((k_matvec_bf16_dense<NACC, true, NPROD>), grid, block, 0, s, A, Vhi, Vlo,
@@ -1784,9 +1802,8 @@ hipError_t launch_rbf_dk_contract(hipStream_t s, const float* X, int dim, double
     hipLaunchKernelGGL(k_split_columns, dim3(unsigned(gx), unsigned(cols)), dim3(256), 0, s, R,
                        ldc, Pin, n, K, c0, cols, npad, Vhi, Vlo);
     DKArgs dk{L, R, ldc, Pin, K, c0, red + 2 * pl.nbx * pl.S * g0};
-    hipLaunchKernelGGL((k_matvec_rbf_bf16<4, false, 3, true>),
-                       dim3(unsigned(pl.nbx), unsigned(ny), unsigned(pl.S)), dim3(kThreads), 0, s,
-                       rec, prm, Vhi, Vlo, cols, nullptr, cols, n, nullptr, dk);
+    hipLaunchKernelGGL(k_rbf_dk_contract, dim3(unsigned(pl.nbx), unsigned(ny), unsigned(pl.S)),
+                       dim3(kThreads), 0, s, rec, prm, Vhi, Vlo, cols, n, dk);
     err = hipGetLastError();
     if (err != hipSuccess) return err;
     g0 += ny;
