@@ -1557,12 +1557,16 @@ hipError_t launch_bf16(hipStream_t s, const float* A, const u32x4* rec, const RB
 
 // Split-K factor: enough workgroups for two per CU (256 CUs), as long as
 // every slice keeps at least 8 tiles and the partial sums fit the scratch.
-int choose_splitk(int64_t n, int64_t P, int64_t avail_floats) {
+// The FP32 kernels for up to 32 probes use under 128 registers and hold four
+// waves per SIMD, so they get twice the workgroups (more waves to hide the
+// latency of their short per-tile chains).
+int choose_splitk(int64_t n, int64_t P, int64_t avail_floats, bool fp32) {
   const int64_t nacc = nacc_for(P);
   const int64_t wgs = ((n + BI - 1) / BI) * ((P + 32 * nacc - 1) / (32 * nacc));
   const int64_t tiles = (n + BK - 1) / BK;
+  const int64_t target = (fp32 && nacc == 1) ? 1024 : 512;
   int S = 1;
-  while (S < kMaxSplitK && wgs * S < 512 && tiles / (2 * S) >= 8 &&
+  while (S < kMaxSplitK && wgs * S < target && tiles / (2 * S) >= 8 &&
          int64_t(2 * S) * P * n <= avail_floats)
     S *= 2;
   return S;
@@ -1627,8 +1631,18 @@ int64_t matvec_scratch_floats(int64_t n, int64_t P) {
   const int64_t bpw = 32 * nacc_for(P);
   const int64_t Ppad = (P + bpw - 1) / bpw * bpw;
   const int64_t npad = (n + BK - 1) / BK * BK;
+  // the split-K slices a launch takes when scratch does not limit them (the
+  // larger of the FP32 and the BF16 choice, and the few-vector row kernel's)
+  const int64_t unlimited = int64_t(1) << 60;
+  int S = choose_splitk(n, P, unlimited, true);
+  const int S16 = choose_splitk(n, P, unlimited, false);
+  if (S16 > S) S = S16;
+  if (P <= kRowsMaxP) {
+    const int Sr = choose_rows_splitk(n, P, unlimited);
+    if (Sr > S) S = Sr;
+  }
   // partial sums + two BF16 planes + the RBF point records (16 floats each)
-  return kMaxSplitK * P * n + Ppad * npad + 16 * npad;
+  return int64_t(S) * P * n + Ppad * npad + 16 * npad;
 }
 
 PlaneOut matvec_planes(const slq_operator& op, int64_t P, float* scratch) {
@@ -1709,7 +1723,7 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
     }
   }
   float* partial = scratch ? scratch + planes_floats : nullptr;
-  const int S = partial ? choose_splitk(n, P, scratch_floats - planes_floats) : 1;
+  const int S = partial ? choose_splitk(n, P, scratch_floats - planes_floats, fp32) : 1;
   float* out = (S > 1) ? partial : W;
   RBFParams prm{};
   hipError_t err;

@@ -160,9 +160,10 @@ hipError_t launch_lanczos_adj_xi(hipStream_t s, const float* W, const float* lam
                                  const LanczosAdjStep& st, float* xi, float* part_out);
 
 // ---- block matvec (k_matvec.hip) -------------------------------------------
-// Largest split-K factor of the block matvec; its partial sums need
-// kMaxSplitK * P * n floats of scratch (split-K is off when scratch is null).
-constexpr int kMaxSplitK = 8;
+// Largest split-K factor of the block matvec; its partial sums need up to
+// kMaxSplitK * P * n floats of scratch (split-K is off when scratch is null;
+// matvec_scratch_floats sizes it for the factor the launch will take).
+constexpr int kMaxSplitK = 16;
 // Scratch (in floats) the block matvec can use for P vectors of length n:
 // the split-K partial sums and the BF16 hi/lo planes of the vectors.
 int64_t matvec_scratch_floats(int64_t n, int64_t P);
