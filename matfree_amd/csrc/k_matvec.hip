@@ -42,7 +42,14 @@ struct RBFParams {
   int dim;
   float coef;  // -log2(e) / (2 l^2)
   float noise;
+  float scale;  // sqrt(log2(e) / (2 l^2)) (FP32 kernels)
 };
+
+// Coordinate of the points past n in the FP32 kernels' staged tiles: far
+// enough that exp2(-|s x_i - s x_m|^2) is exactly 0 for every point in use
+// (the squared distance is at least ~1e36 times s^2, or +inf), so those
+// entries need no mask.
+constexpr float kFarPoint = 1e18f;
 
 // 16 bytes at a 4-byte aligned address (rows of n % 4 != 0 floats start at
 // every dword alignment): a memcpy, which gfx950 code (unaligned global
@@ -122,10 +129,12 @@ __device__ __forceinline__ void load_a_frag(const float* __restrict__ A, int64_t
   }
 }
 
-// The lane's 16 generated RBF values exp2(c |x_row - x_m|^2).
+// The lane's 16 generated RBF values exp2(-|x~_row - x~_m|^2) from scaled
+// coordinates x~ = s x, s = sqrt(log2(e) / (2 l^2)) (points past n staged at
+// kFarPoint, which gives exactly 0).
 template <int DIM>
-__device__ __forceinline__ void gen_rbf_frag(const float4* Xs, const float (&xr)[4], float coef,
-                                             int64_t n, int64_t m0, int h, float (&a)[16]) {
+__device__ __forceinline__ void gen_rbf_frag(const float4* Xs, const float (&xr)[4], int h,
+                                             float (&a)[16]) {
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
     const float4 xm = Xs[16 * h + s];
@@ -134,8 +143,7 @@ __device__ __forceinline__ void gen_rbf_frag(const float4* Xs, const float (&xr)
     if (DIM > 1) { d = xr[1] - xm.y; d2 = fmaf(d, d, d2); }
     if (DIM > 2) { d = xr[2] - xm.z; d2 = fmaf(d, d, d2); }
     if (DIM > 3) { d = xr[3] - xm.w; d2 = fmaf(d, d, d2); }
-    const float kv = __builtin_amdgcn_exp2f(coef * d2);
-    a[s] = (m0 + 16 * h + s < n) ? kv : 0.f;
+    a[s] = __builtin_amdgcn_exp2f(-d2);
   }
 }
 
@@ -218,17 +226,19 @@ __global__ void __launch_bounds__(kThreads)
   float xr[4] = {0.f, 0.f, 0.f, 0.f};
   if (RBF && row < n) {
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) xr[d] = prm.X[row * DIM + d];
+    for (int d = 0; d < DIM; ++d) xr[d] = prm.X[row * DIM + d] * prm.scale;
   }
   auto load_x = [&](int64_t m0) {
     float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
     const int64_t m = m0 + t;
     if (m < n) {
       const float* src = prm.X + m * DIM;
-      x.x = src[0];
-      if (DIM > 1) x.y = src[1];
-      if (DIM > 2) x.z = src[2];
-      if (DIM > 3) x.w = src[3];
+      x.x = src[0] * prm.scale;
+      if (DIM > 1) x.y = src[1] * prm.scale;
+      if (DIM > 2) x.z = src[2] * prm.scale;
+      if (DIM > 3) x.w = src[3] * prm.scale;
+    } else {
+      x = make_float4(kFarPoint, kFarPoint, kFarPoint, kFarPoint);
     }
     return x;
   };
@@ -255,7 +265,7 @@ __global__ void __launch_bounds__(kThreads)
     }
     float a[16];
     if (RBF) {
-      gen_rbf_frag<DIM>(Xs[buf], xr, prm.coef, n, m0, h, a);
+      gen_rbf_frag<DIM>(Xs[buf], xr, h, a);
     } else {
 #pragma unroll
       for (int s = 0; s < 16; ++s) a[s] = areg[s];
@@ -1748,6 +1758,7 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
     prm.X = op.points;
     prm.dim = op.dim;
     prm.coef = float(-1.4426950408889634 / (2.0 * op.lengthscale * op.lengthscale));
+    prm.scale = float(sqrt(1.4426950408889634 / (2.0 * op.lengthscale * op.lengthscale)));
     prm.noise = float(op.noise);
     if (!fp32) {
       if (op.dim > 3) return hipErrorInvalidValue;  // rejected by the API before launch
