@@ -219,6 +219,25 @@ def roofline(kernels, n, P, k, prec, workload):
     return entry
 
 
+def request_roofline_ms(kind, n, P, k, grad):
+    """Lower bound of one serving request's device time (ms) at the roofline:
+    its block-matvec work (2 n^2 P algorithmic flops per matvec, k matvecs;
+    a gradient request adds the reverse pass's k matvecs and the parameter
+    contraction, 2 n^2 P k flops) at the matrix-core peak of its precision
+    (BF16 for the RBF bf16x2 requests, FP32 for the stored FP32 ones), plus
+    the Gram-Schmidt streams at the HBM peak: forward step j reads the j + 1
+    basis columns three times and moves ~5 more P x n vectors; a reverse step
+    idx reads ~2 (idx + 2) + (idx + 1) + (2 k - idx - 1) columns and ~8 vectors
+    (dots, projection, the second dots, the adjoint combination)."""
+    peak = PEAK["bf16_mfma_TFs"] if kind == "rbf" else PEAK["fp32_mfma_TFs"]
+    vec = 4.0 * P * n
+    mv = 2.0 * n * n * P * k * (3 if grad else 1)
+    gs = vec * (3.0 * k * (k + 1) / 2.0 + 5.0 * k)
+    if grad:
+        gs += vec * sum(2 * (i + 2) + (i + 1) + (2 * k - i - 1) + 8 for i in range(k))
+    return (mv / (peak * 1e12) + gs / (PEAK["hbm_GBs"] * 1e9)) * 1e3
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
@@ -330,6 +349,7 @@ def serve(args, device):
         finish[i] = busy = start + lat[i]
     fifo = finish - arrive * 1e3
     st = server.stats
+    bound_ms = sum(request_roofline_ms(*m) for m in meta)
     val = np.array([t for t, m in zip(lat, meta) if not m[4]])
     grd = np.array([t for t, m in zip(lat, meta) if m[4]])
     del reqs, mixed, out, server
@@ -345,6 +365,10 @@ def serve(args, device):
                        "p99_ms": float(np.percentile(val, 99))},
         "with_gradient": {"requests": int(len(grd)), "p50_ms": float(np.percentile(grd, 50)),
                           "p99_ms": float(np.percentile(grd, 99))},
+        # the mix's work at the roofline (request_roofline_ms) against the
+        # requests' summed latencies alone and the stream's wall time
+        "roofline": {"lower_bound_ms": bound_ms, "frac_alone": bound_ms / float(lat.sum()),
+                     "frac_stream": bound_ms / (span * 1e3)},
         "stream": {
             "arrival_rate_per_s": rate,
             "slots": SERVE_SLOTS,
@@ -693,7 +717,9 @@ def main():
             f"{sv['value_only']['p99_ms']:.2f} ms; with gradient p50 "
             f"{sv['with_gradient']['p50_ms']:.2f} ms, p99 {sv['with_gradient']['p99_ms']:.2f} ms")
         log(f"bench: serve alone p50 {sv['p50_ms']:.2f} ms, p99 {sv['p99_ms']:.2f} ms; stream "
-            f"p50 {sv['stream']['p50_ms']:.2f} ms, p99 {sv['stream']['p99_ms']:.2f} ms")
+            f"p50 {sv['stream']['p50_ms']:.2f} ms, p99 {sv['stream']['p99_ms']:.2f} ms; "
+            f"roofline fraction alone {sv['roofline']['frac_alone']:.3f}, stream "
+            f"{sv['roofline']['frac_stream']:.3f}")
     print(json.dumps(out), flush=True)
 
 
