@@ -188,6 +188,25 @@ def test_lanczos_rbf_cotangents_match_oracle(cuda, n, P, k, theta):
     assert torch.equal(vals, per)
 
 
+@pytest.mark.parametrize("precision,tol", [("bf16x3", 1e-3), ("bf16x2", 5e-3)])
+def test_lanczos_rbf_cotangents_bf16_forms(cuda, precision, tol):
+    # the plain-Lanczos reverse pass with a BF16 operator: BF16 block matvecs
+    # forward and backward and the derivative-matvec parameter contraction
+    # (sum_i <x_i, D lambda_i>), against the FP64 restatement; tolerances as
+    # for the Arnoldi adjoint's BF16 forms (1e-3, and 5e-3 where the vectors
+    # are rounded to BF16 once per matvec)
+    n, P, k, theta = 512, 6, 10, (0.2, 0.5)
+    X = problems.rbf_points(n, seed=4)
+    Z = oprng.rademacher(oprng.prng_key(7), (P, n), np.float32)
+    _, mean, grads = funm.logdet_value_and_grad(
+        operators.RBF(X, precision=precision), torch.as_tensor(Z, device=cuda), k, theta,
+        reortho="none")
+    est64, g64 = oslq.slq_logdet_lanczos_vjp(oslq.RBFOperator(X, *theta, np.float64), k, Z)
+    g = np.array([float(grads[0]), float(grads[1])])
+    print(f"lanczos {precision}: gradient rel {rel(g, g64):.2e}", flush=True)
+    assert rel(g, g64) < tol, (g, g64)
+
+
 def test_lanczos_dense_cotangent_matches_oracle(cuda):
     n, P, k = 200, 4, 12
     A = problems.dense_spd(n, seed=5, shift=1.0)
