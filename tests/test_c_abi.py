@@ -80,4 +80,6 @@ def test_null_context_is_rejected_without_a_gpu():
     lib = _lib.load()
     assert lib.slq_hessenberg(None, None, None, 1, 1, None, None, None, None) == _lib.SLQ_EINVAL
     assert lib.slq_logdet(None, None, None, 1, 1, None, None, None, None) == _lib.SLQ_EINVAL
+    assert lib.slq_workspace_bytes(None, 0, None, 1, 1, None) == _lib.SLQ_EINVAL
+    assert lib.slq_workspace_growth(None, 0, None, 1, 1, None) == _lib.SLQ_EINVAL
     assert lib.slq_last_error(None) == b""
