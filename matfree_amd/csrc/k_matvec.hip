@@ -1379,6 +1379,116 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// The few-vector RBF matvec with the kernel's exponents from two BF16 MFMAs
+// per 32 x 32 tile (the split exponent slots of the block matvec: ~3e-6 at
+// most in the base-2 exponent, the size of an FP32 evaluation's rounding)
+// and everything else in FP32 on the vector units, for point dimensions <= 3
+// (the records' slots).  A workgroup owns 128 rows (32 per wave) and a column
+// slice, staged through LDS in chunks of kMxTiles tiles (the point records
+// and the vectors' entries, the next chunk's loads in flight while the
+// current one is used).  Lane (r, h) of a wave takes row i0w + r and, of each
+// tile, the 16 points m0 + 8 (s / 4) + 4 h + (s % 4) whose exponents the
+// MFMAs leave in it; it forms their exp2 and accumulates them times the
+// vectors' entries, and the two lane halves of a row are added at the end.
+// Per kernel entry: one exp2 and NP FMAs.  Split-K partial sums as in
+// k_matvec_rbf_rows.
+constexpr int kMxTiles = 8;
+template <int NP>
+__global__ void __launch_bounds__(kThreads)
+    k_matvec_rbf_rows_mx(const u32x4* __restrict__ rec, const float* __restrict__ X, int dim,
+                         float coef, float noise, const float* __restrict__ V, int64_t n,
+                         float* __restrict__ W) {
+  constexpr int CP = kMxTiles * BK;        // points per chunk
+  constexpr int RW = CP * 4 / kThreads;    // record words (u32x4) per thread and chunk
+  __shared__ u32x4 rs[CP * 4];
+  __shared__ __attribute__((aligned(16))) float vs[NP][CP];
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6, r = lane & 31, h = lane >> 5;
+  const int64_t row = int64_t(blockIdx.x) * BI + 32 * wave + r;
+  const int64_t all_tiles = (n + BK - 1) / BK;
+  const int64_t kt_begin = all_tiles * blockIdx.y / gridDim.y;
+  const int64_t kt_end = all_tiles * (blockIdx.y + 1) / gridDim.y;
+  const int64_t rec_words = all_tiles * BK * 4;
+  u32x4 rr[2];
+  {
+    float x[3] = {0.f, 0.f, 0.f};
+    if (row < n)
+      for (int d = 0; d < dim; ++d) x[d] = X[row * dim + d];
+    __bf16 slot[32];
+    exponent_slots(x, coef, 1, false, slot);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      bf16x8 v;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = slot[16 * q + 8 * h + j];
+      rr[q] = __builtin_bit_cast(u32x4, v);
+    }
+  }
+  // chunk starting at tile c0: record words (clamped to the array) and the
+  // vectors' entries (zero past n)
+  u32x4 wr[RW];
+  float wv[NP];
+  auto load_chunk = [&](int64_t c0) {
+#pragma unroll
+    for (int u = 0; u < RW; ++u) {
+      int64_t w = c0 * BK * 4 + int64_t(u) * kThreads + t;
+      w = w < rec_words ? w : rec_words - 1;
+      wr[u] = rec[w];
+    }
+    const int64_t m = c0 * BK + t;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) wv[q] = (t < CP && m < n) ? V[q * n + m] : 0.f;
+  };
+  float acc[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) acc[q] = 0.f;
+  if (kt_begin < kt_end) load_chunk(kt_begin);
+  for (int64_t c0 = kt_begin; c0 < kt_end; c0 += kMxTiles) {
+#pragma unroll
+    for (int u = 0; u < RW; ++u) rs[u * kThreads + t] = wr[u];
+    if (t < CP)
+#pragma unroll
+      for (int q = 0; q < NP; ++q) vs[q][t] = wv[q];
+    __syncthreads();
+    if (c0 + kMxTiles < kt_end) load_chunk(c0 + kMxTiles);
+    const int tiles = int(kt_end - c0 < kMxTiles ? kt_end - c0 : kMxTiles);
+    for (int j = 0; j < tiles; ++j) {
+      const u32x4 l[2] = {rs[(j * BK + r) * 4 + h], rs[(j * BK + r) * 4 + h + 2]};
+      float vq[NP][16];
+#pragma unroll
+      for (int q = 0; q < NP; ++q)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const float4 v4 = *reinterpret_cast<const float4*>(&vs[q][j * BK + 8 * g + 4 * h]);
+          vq[q][4 * g + 0] = v4.x;
+          vq[q][4 * g + 1] = v4.y;
+          vq[q][4 * g + 2] = v4.z;
+          vq[q][4 * g + 3] = v4.w;
+        }
+      const floatx16 d = rbf_exponents_mfma(l, rr);
+#pragma unroll
+      for (int sidx = 0; sidx < 16; ++sidx) {
+        const float k = __builtin_amdgcn_exp2f(d[sidx]);
+#pragma unroll
+        for (int q = 0; q < NP; ++q) acc[q] = fmaf(k, vq[q][sidx], acc[q]);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < NP; ++q) acc[q] += __shfl_xor(acc[q], 32);
+  if (h == 0 && row < n) {
+    const bool split = gridDim.y > 1;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+      if (split)
+        W[int64_t(blockIdx.y) * NP * n + q * n + row] = acc[q];
+      else
+        W[q * n + row] = fmaf(noise, V[q * n + row], acc[q]);
+    }
+  }
+}
+
 template <int DIM>
 void launch_rows_dim(hipStream_t s, const float* X, float scale, float noise, const float* V,
                      int64_t P, int64_t n, float* W, dim3 grid) {
@@ -1676,23 +1786,63 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
   const int64_t n = op.n;
   if (n <= 0 || P <= 0) return hipSuccess;
   if (op.kind == SLQ_OP_RBF && P <= kRowsMaxP && op.dim >= 1 && op.dim <= 4) {
-    // a few vectors: the row kernel in FP32 (every precision gets FP32 here)
-    const float scale = float(sqrt(1.4426950408889634 / (2.0 * op.lengthscale * op.lengthscale)));
+    // a few vectors: the row kernels in FP32 (every precision gets FP32
+    // here); for dimension <= 3 with the exponents from the matrix cores,
+    // the point records at the start of the scratch, the partial sums after
     const float noise = float(op.noise);
-    const int S = scratch ? choose_rows_splitk(n, P, scratch_floats) : 1;
-    float* out = S > 1 ? scratch : W;
-    if (ev0) (void)hipEventRecord(ev0, s);
-    hipError_t err = launch_rows(s, op.points, op.dim, scale, noise, V, P, n, out, S);
-    if (ev1) (void)hipEventRecord(ev1, s);
+    const int64_t npad = (n + BK - 1) / BK * BK;
+    const bool mx = op.dim <= 3 && scratch && scratch_floats >= 16 * npad;
+    float* partial = scratch ? (mx ? scratch + 16 * npad : scratch) : nullptr;
+    const int S = partial ? choose_rows_splitk(n, P, scratch_floats - (mx ? 16 * npad : 0)) : 1;
+    float* out = S > 1 ? partial : W;
+    hipError_t err;
+    if (mx) {
+      const float coef = float(-1.4426950408889634 / (2.0 * op.lengthscale * op.lengthscale));
+      u32x4* rec = reinterpret_cast<u32x4*>(scratch);
+      if (!(flags & kMvPointsReady)) {
+        hipLaunchKernelGGL(k_prep_points, dim3(unsigned((npad + 255) / 256)), dim3(256), 0, s,
+                           op.points, op.dim, coef, n, npad, rec);
+        err = hipGetLastError();
+        if (err != hipSuccess) return err;
+      }
+      const dim3 grid(unsigned((n + BI - 1) / BI), unsigned(S));
+      if (ev0) (void)hipEventRecord(ev0, s);
+      switch (P) {
+        case 1:
+          hipLaunchKernelGGL(k_matvec_rbf_rows_mx<1>, grid, dim3(kThreads), 0, s, rec, op.points,
+                             op.dim, coef, noise, V, n, out);
+          break;
+        case 2:
+          hipLaunchKernelGGL(k_matvec_rbf_rows_mx<2>, grid, dim3(kThreads), 0, s, rec, op.points,
+                             op.dim, coef, noise, V, n, out);
+          break;
+        case 3:
+          hipLaunchKernelGGL(k_matvec_rbf_rows_mx<3>, grid, dim3(kThreads), 0, s, rec, op.points,
+                             op.dim, coef, noise, V, n, out);
+          break;
+        default:
+          hipLaunchKernelGGL(k_matvec_rbf_rows_mx<4>, grid, dim3(kThreads), 0, s, rec, op.points,
+                             op.dim, coef, noise, V, n, out);
+          break;
+      }
+      err = hipGetLastError();
+      if (ev1) (void)hipEventRecord(ev1, s);
+    } else {
+      const float scale =
+          float(sqrt(1.4426950408889634 / (2.0 * op.lengthscale * op.lengthscale)));
+      if (ev0) (void)hipEventRecord(ev0, s);
+      err = launch_rows(s, op.points, op.dim, scale, noise, V, P, n, out, S);
+      if (ev1) (void)hipEventRecord(ev1, s);
+    }
     if (err != hipSuccess || S == 1) return err;
     if (defer) {
-      *defer = MvSplitSum{scratch, S, V, noise, 1};
+      *defer = MvSplitSum{partial, S, V, noise, 1};
       return hipSuccess;
     }
     const int64_t count = P * n;
     int64_t blocks = (count + 255) / 256;
     if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(k_splitk_reduce<true>, dim3(unsigned(blocks)), dim3(256), 0, s, scratch,
+    hipLaunchKernelGGL(k_splitk_reduce<true>, dim3(unsigned(blocks)), dim3(256), 0, s, partial,
                        S, count, V, noise, W);
     return hipGetLastError();
   }

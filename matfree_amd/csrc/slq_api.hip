@@ -1369,8 +1369,9 @@ int slq_cg(slq_context* ctx, const slq_operator* op, const float* B, int64_t P, 
       MvSplitSum qsum;
       {
         const auto ev = take_events(ctx, SLQ_KC_MATVEC);
-        SLQ_HIP(ctx, launch_matvec(s, *op, d, P, q, mvs, mv_floats, ev.first, ev.second, 0,
-                                   &qsum));
+        // the RBF point records of the first iteration's matvec are reused
+        SLQ_HIP(ctx, launch_matvec(s, *op, d, P, q, mvs, mv_floats, ev.first, ev.second,
+                                   t == 0 ? 0 : kMvPointsReady, &qsum));
       }
       SLQ_HIP(ctx, launch_cg_step(s, X, r, d, q, qsum, z, precond, P, n, maxiter, sc,
                                   int(t & 1), pq, rr, rz, precond ? &M : nullptr));
