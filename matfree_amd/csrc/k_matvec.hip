@@ -25,6 +25,8 @@
 //
 // RBF: K_im = exp2(c |x_i - x_m|^2), c = -log2(e) / (2 l^2); the nugget
 // s2 * v is added in FP32 in the epilogue (never folded into a tile).
+#include <cmath>
+
 #include "slq_internal.h"
 
 namespace slq {
@@ -1692,11 +1694,14 @@ int choose_splitk(int64_t n, int64_t P, int64_t avail_floats, bool fp32) {
   return S;
 }
 
-// Derivative-kernel contraction: columns in launches of at most kDkCols
-// (planes of kDkCols x npad split-BF16 values), 128 per workgroup column
-// (NACC = 4); the split-K factor S fills the GPU (the slices add to the
-// partial sums, so they need no partial planes).
-constexpr int64_t kDkCols = 1024;
+// Derivative-kernel contraction: columns in launches of G column blocks of
+// 128 (NACC = 4; planes of 128 G x npad split-BF16 values, G <= 8) and S
+// column slices (the slices add to the partial sums, so they need no partial
+// planes; any S, each slice keeping at least 8 tiles).  Two workgroups fit a
+// CU, so a launch runs in rounds of 512; (G, S) is chosen so that the
+// workgroup count nbx G S fills its last round best (nbx = n / 128 is rarely a
+// multiple of 256), among counts of at least one round and G S <= 32.
+constexpr int64_t kDkMaxGroups = 8;
 struct DkPlan {
   int64_t cols, groups, nbx;
   int S;
@@ -1704,12 +1709,27 @@ struct DkPlan {
 DkPlan dk_plan(int64_t n, int64_t K) {
   DkPlan p;
   p.groups = (K + 127) / 128;
-  p.cols = p.groups * 128 < kDkCols ? p.groups * 128 : kDkCols;
   p.nbx = (n + BI - 1) / BI;
-  const int64_t wgs = p.nbx * (p.cols / 128);
   const int64_t tiles = (n + BK - 1) / BK;
-  p.S = 1;
-  while (p.S < kMaxSplitK && wgs * p.S < 1024 && tiles / (2 * p.S) >= 8) p.S *= 2;
+  const int64_t gmax = p.groups < kDkMaxGroups ? p.groups : kDkMaxGroups;
+  int64_t bestG = gmax;
+  int bestS = 1;
+  double best = -1.0;
+  for (int64_t G = gmax; G >= 1; --G) {
+    for (int S = 1; S <= 16 && G * S <= 32; ++S) {
+      if (S > 1 && tiles / S < 8) break;
+      const double rounds = double(p.nbx * G * S) / 512.0;
+      if (rounds < 1.0 && !(G == gmax && S == 1)) continue;
+      const double fill = rounds / std::ceil(rounds);
+      if (fill > best + 1e-3) {
+        best = fill;
+        bestG = G;
+        bestS = S;
+      }
+    }
+  }
+  p.cols = bestG * 128;
+  p.S = bestS;
   return p;
 }
 
