@@ -542,7 +542,8 @@ __device__ __forceinline__ floatx16 mfma_bf16(const u32x4& a, const u32x4& b, co
 //   6 d + 0..5 (d < 3): (x_m,d ; -2c x_i,d) as hH, hM, mH, hL, lH, mM
 //   18..20: (c |x_m|^2 parts ; 1)     21..23: (1 ; c |x_i|^2 parts)
 // The point side ("L") of each point is precomputed once per matvec as 64
-// bytes; the row side ("R") is built per lane at kernel start.  The MFMA
+// bytes (32 in the compact form below); the row side ("R") is built per lane
+// at kernel start.  The MFMA
 // output gives lane (r, h) the exponents of its own row i = r at the 16
 // points m = 8 (s / 4) + 4 h + (s % 4), s = 0..15 -- exactly the BF16
 // A-fragment of that lane once the contraction index is ordered that way,
@@ -556,67 +557,130 @@ __device__ __forceinline__ void split3(float x, __bf16 (&p)[3]) {
   p[2] = __bf16(r1 - float(p[1]));
 }
 
-// The 32 slots of one side.  side 0: point side of x (the L factors), side 1:
-// row side (the R factors, from -2c x and c |x|^2).
-__device__ __forceinline__ void exponent_slots(const float (&x)[3], float coef, int side,
-                                               bool pad, __bf16 (&slot)[32]) {
+// The compact form (16 slots, one MFMA) for the split-BF16 matvec with the
+// probes in one plane, whose entries carry 2^-17 relative error anyway:
+// scaled coordinates y = s x (s = sqrt(-c)) are rounded to two BF16 parts,
+// y~ = y_h + y_l, and the exponent is that of the points y~,
+//   arg[m][i] = -|y~_m|^2 - |y~_i|^2 + sum_d y~_m,d (2 y~_i,d),
+// with the cross products exact (hH, hL, lH, lL: slots 4 d + 0..3) and each
+// squared norm (FP32, from y~) as two BF16 parts against an exact 1 (slots
+// 12, 13 for the point, 14, 15 for the row).  Both sides round the same
+// y~ for a point, so the tile is the RBF kernel of points moved by at most
+// 2^-18 of their scaled coordinates, times D_m D_i with D = exp2(delta),
+// |delta| <= 2^-18 |y~|^2, from the norms' rounding: a symmetric positive
+// definite matrix whose log-determinant differs from the exact one by ~1e-9
+// relative (measured in FP64 at n = 4096 and 8192, points in the unit cube,
+// l = 0.5, noise 1e-2).  Points past n carry -2^100 in slot 12.
+//
+// The slots of one side (NS = 32: the full form above, NS = 16: the
+// compact form).  side 0: point side of x (the L factors), side 1: row side
+// (the R factors).
+template <int NS>
+__device__ __forceinline__ void exponent_slots(const float (&x)[3], float coef, float scale,
+                                               int side, bool pad, __bf16 (&slot)[NS]) {
   const __bf16 one = __bf16(1.f), zero = __bf16(0.f);
+  if constexpr (NS == 16) {
+    float nrm = 0.f;
 #pragma unroll
-  for (int s = 0; s < 32; ++s) slot[s] = zero;
-  const float sq = fmaf(x[2] * x[2], coef, fmaf(x[1] * x[1], coef, x[0] * x[0] * coef));
-  __bf16 q[3];
+    for (int d = 0; d < 3; ++d) {
+      const float y = scale * x[d];
+      const __bf16 yh = __bf16(y);
+      const __bf16 yl = __bf16(y - float(yh));
+      const float yt = float(yh) + float(yl);
+      nrm = fmaf(yt, yt, nrm);
+      if (side == 0) {
+        slot[4 * d + 0] = yh;
+        slot[4 * d + 1] = yh;
+        slot[4 * d + 2] = yl;
+        slot[4 * d + 3] = yl;
+      } else {
+        const __bf16 h2 = __bf16(2.f * float(yh)), l2 = __bf16(2.f * float(yl));
+        slot[4 * d + 0] = h2;
+        slot[4 * d + 1] = l2;
+        slot[4 * d + 2] = h2;
+        slot[4 * d + 3] = l2;
+      }
+    }
+    const __bf16 nh = __bf16(-nrm);
+    const __bf16 nl = __bf16(-nrm - float(nh));
+    slot[12] = side == 0 ? nh : one;
+    slot[13] = side == 0 ? nl : one;
+    slot[14] = side == 0 ? one : nh;
+    slot[15] = side == 0 ? one : nl;
+    if (pad) {
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    split3(side == 0 ? x[d] : -2.f * coef * x[d], q);
-    // hH, hM, mH, hL, lH, mM: point side takes (h, h, m, h, l, m), row side
-    // (H, M, H, L, H, M)
-    const int lp[6] = {0, 0, 1, 0, 2, 1}, rp[6] = {0, 1, 0, 2, 0, 1};
-#pragma unroll
-    for (int u = 0; u < 6; ++u) slot[6 * d + u] = q[side == 0 ? lp[u] : rp[u]];
-  }
-  split3(sq, q);
-#pragma unroll
-  for (int u = 0; u < 3; ++u) {
-    slot[18 + u] = side == 0 ? q[u] : one;
-    slot[21 + u] = side == 0 ? one : q[u];
-  }
-  if (pad) {  // a point past n: exponent -2^100 whatever the row
+      for (int q = 0; q < 16; ++q) slot[q] = zero;
+      slot[12] = __bf16(-0x1p100f);
+      slot[14] = one;
+      slot[15] = one;
+    }
+  } else {
 #pragma unroll
     for (int s = 0; s < 32; ++s) slot[s] = zero;
-    slot[18] = __bf16(-0x1p100f);
-    slot[21] = one;
-    slot[22] = one;
-    slot[23] = one;
+    const float sq = fmaf(x[2] * x[2], coef, fmaf(x[1] * x[1], coef, x[0] * x[0] * coef));
+    __bf16 q[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      split3(side == 0 ? x[d] : -2.f * coef * x[d], q);
+      // hH, hM, mH, hL, lH, mM: point side takes (h, h, m, h, l, m), row side
+      // (H, M, H, L, H, M)
+      const int lp[6] = {0, 0, 1, 0, 2, 1}, rp[6] = {0, 1, 0, 2, 0, 1};
+#pragma unroll
+      for (int u = 0; u < 6; ++u) slot[6 * d + u] = q[side == 0 ? lp[u] : rp[u]];
+    }
+    split3(sq, q);
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      slot[18 + u] = side == 0 ? q[u] : one;
+      slot[21 + u] = side == 0 ? one : q[u];
+    }
+    if (pad) {  // a point past n: exponent -2^100 whatever the row
+#pragma unroll
+      for (int s = 0; s < 32; ++s) slot[s] = zero;
+      slot[18] = __bf16(-0x1p100f);
+      slot[21] = one;
+      slot[22] = one;
+      slot[23] = one;
+    }
   }
 }
 
-__device__ __forceinline__ floatx16 rbf_exponents_mfma(const u32x4 (&l)[2], const u32x4 (&r)[2]) {
+// Exponent slots of the split-BF16 RBF kernels: the compact form where the
+// entries are split into two BF16 parts and the probes kept in one (NPROD
+// <= 2), the full form for bf16x3 and the derivative kernel.
+constexpr int exponent_slot_count(int nprod, bool dk) { return (dk || nprod == 3) ? 32 : 16; }
+
+// The exponents of a tile from NK = NS / 16 BF16 MFMAs.
+template <int NK>
+__device__ __forceinline__ floatx16 rbf_exponents_mfma(const u32x4 (&l)[NK],
+                                                       const u32x4 (&r)[NK]) {
   floatx16 d;
 #pragma unroll
   for (int e = 0; e < 16; ++e) d[e] = 0.f;
-  d = mfma_bf16(l[0], r[0], d);
-  d = mfma_bf16(l[1], r[1], d);
+#pragma unroll
+  for (int k = 0; k < NK; ++k) d = mfma_bf16(l[k], r[k], d);
   return d;
 }
 
-// Point side of the exponent slots for every m < npad (64 bytes each, four
-// u32x4: slots 0-7, 8-15, 16-23, 24-31).
+// Point side of the exponent slots for every m < npad (NS / 8 u32x4 each:
+// slots 0-7, 8-15, ...).
+template <int NS>
 __global__ void __launch_bounds__(256)
-    k_prep_points(const float* __restrict__ X, int dim, float coef, int64_t n, int64_t npad,
-                  u32x4* __restrict__ rec) {
+    k_prep_points(const float* __restrict__ X, int dim, float coef, float scale, int64_t n,
+                  int64_t npad, u32x4* __restrict__ rec) {
   const int64_t m = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (m >= npad) return;
   float x[3] = {0.f, 0.f, 0.f};
   if (m < n)
     for (int d = 0; d < dim; ++d) x[d] = X[m * dim + d];
-  __bf16 slot[32];
-  exponent_slots(x, coef, 0, m >= n, slot);
+  __bf16 slot[NS];
+  exponent_slots<NS>(x, coef, scale, 0, m >= n, slot);
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < NS / 8; ++q) {
     bf16x8 v;
 #pragma unroll
     for (int j = 0; j < 8; ++j) v[j] = slot[8 * q + j];
-    rec[4 * m + q] = __builtin_bit_cast(u32x4, v);
+    rec[NS / 8 * m + q] = __builtin_bit_cast(u32x4, v);
   }
 }
 
@@ -1083,6 +1147,10 @@ __device__ __forceinline__ void rbf_bf16_body(const u32x4* __restrict__ rec, con
                                               float* __restrict__ W, const DKArgs& dk) {
   constexpr int BPW = 32 * NACC;
   constexpr bool LO = NPROD == 3;  // probes' low plane used
+  // exponent slots: the compact form for the forward matvec with the probes
+  // in one plane, the full form for bf16x3 and the derivative mode
+  constexpr int NS = exponent_slot_count(NPROD, DK);
+  constexpr int NK = NS / 16, RW = NS / 8;  // exponent MFMAs, record words
   constexpr int PT = PlaneTile<NACC>::kPerThread;
   __shared__ __attribute__((aligned(16))) __bf16 Vhi[2][BPW * LDB];
   __shared__ __attribute__((aligned(16))) __bf16 Vlo[LO ? 2 : 1][LO ? BPW * LDB : 8];
@@ -1106,28 +1174,33 @@ __device__ __forceinline__ void rbf_bf16_body(const u32x4* __restrict__ rec, con
     for (int e = 0; e < 16; ++e) acc[c][e] = 0.f;
 
   // lane constants: the row side of the exponent slots of this lane's row,
-  // as the B operands of the two exponent MFMAs (slots 16 t + 8 h + 0..7)
-  u32x4 rr[2];
+  // as the B operands of the NK exponent MFMAs (slots 16 k + 8 h + 0..7)
+  u32x4 rr[NK];
   {
     float x[3] = {0.f, 0.f, 0.f};
     if (row < n)
       for (int d = 0; d < dim; ++d) x[d] = prm.X[row * dim + d];
-    __bf16 slot[32];
-    exponent_slots(x, prm.coef, 1, false, slot);
+    __bf16 slot[NS];
+    exponent_slots<NS>(x, prm.coef, prm.scale, 1, false, slot);
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      bf16x8 v;
+    for (int q = 0; q < NK; ++q) {
+      bf16x8 v0, v1;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) v[j] = slot[16 * q + 8 * h + j];
-      rr[q] = __builtin_bit_cast(u32x4, v);
+      for (int j = 0; j < 8; ++j) {
+        v0[j] = slot[16 * q + j];
+        v1[j] = slot[16 * q + 8 + j];
+      }
+      const u32x4 w0 = __builtin_bit_cast(u32x4, v0), w1 = __builtin_bit_cast(u32x4, v1);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) rr[q][j] = h ? w1[j] : w0[j];
     }
   }
   // point side of this lane's tile point (m0 + r) for tile kt (relative to
-  // kt_begin): the A operands of the two exponent MFMAs
-  const u32x4* prow = rec + 4 * (kt_begin * BK + r) + h;
-  auto load_pts = [&](int64_t kt, u32x4 (&l)[2]) {
-    l[0] = prow[4 * BK * kt];
-    l[1] = prow[4 * BK * kt + 2];
+  // kt_begin): the A operands of the exponent MFMAs (records of RW words)
+  const u32x4* prow = rec + RW * (kt_begin * BK + r) + h;
+  auto load_pts = [&](int64_t kt, u32x4 (&l)[NK]) {
+#pragma unroll
+    for (int k = 0; k < NK; ++k) l[k] = prow[RW * BK * kt + 2 * k];
   };
 
   // prologue: probe tile 0 staged, tile 1 in flight, fragment of tile 0
@@ -1137,13 +1210,13 @@ __device__ __forceinline__ void rbf_bf16_body(const u32x4* __restrict__ rec, con
   if (LO) load_plane_tile<NACC>(Vlo_g, Ppad, p0, kt_begin, lA);
   load_plane_tile<NACC>(Vhi_g, Ppad, p0, kt_begin + clamp(1), hB);
   if (LO) load_plane_tile<NACC>(Vlo_g, Ppad, p0, kt_begin + clamp(1), lB);
-  u32x4 xA[2], xB[2];
+  u32x4 xA[NK], xB[NK];
   load_pts(clamp(1), xA);
   store_plane_tile<NACC>(Vhi[0], hA);
   if (LO) store_plane_tile<NACC>(&Vlo[0][0], lA);
   u32x4 fAh[2], fAl[2], fBh[2], fBl[2];  // fragments of the current / next tile
   {
-    u32x4 x0[2];
+    u32x4 x0[NK];
     load_pts(0, x0);
     const floatx16 d = rbf_exponents_mfma(x0, rr);
     float a[16];
@@ -1171,23 +1244,23 @@ __device__ __forceinline__ void rbf_bf16_body(const u32x4* __restrict__ rec, con
   // The loads of step kt read tile min(kt + 2, ntiles - 1): three pointers
   // advanced by one tile per step while a further tile exists (cheaper than
   // recomputing 64-bit tile offsets each step).
-  const u32x4* pnext = prow + 4 * BK * clamp(2);
+  const u32x4* pnext = prow + RW * BK * clamp(2);
   const int64_t pstep = Ppad * BK;
   const __bf16* hnext = Vhi_g + ((kt_begin + clamp(2)) * Ppad + p0) * BK;
   const __bf16* lnext = LO ? Vlo_g + ((kt_begin + clamp(2)) * Ppad + p0) * BK : nullptr;
   auto step = [&](int64_t kt, u32x4 (&ih)[PT], u32x4 (&il)[PT], const u32x4 (&sh)[PT],
-                  const u32x4 (&sl)[PT], u32x4 (&xi)[2], const u32x4 (&xs)[2],
+                  const u32x4 (&sl)[PT], u32x4 (&xi)[NK], const u32x4 (&xs)[NK],
                   const u32x4 (&ahi)[2], const u32x4 (&alo)[2], u32x4 (&nhi)[2],
                   u32x4 (&nlo)[2]) {
     const int buf = int(kt & 1);
     // points (needed one step later) ahead of the planes (needed at the end
     // of the next step)
-    xi[0] = pnext[0];
-    xi[1] = pnext[2];
+#pragma unroll
+    for (int k = 0; k < NK; ++k) xi[k] = pnext[2 * k];
     load_plane_tile_at<NACC>(hnext, ih);
     if (LO) load_plane_tile_at<NACC>(lnext, il);
     if (kt + 3 < ntiles) {
-      pnext += 4 * BK;
+      pnext += RW * BK;
       hnext += pstep;
       if (LO) lnext += pstep;
     }
@@ -1417,7 +1490,7 @@ __global__ void __launch_bounds__(kThreads)
     if (row < n)
       for (int d = 0; d < dim; ++d) x[d] = X[row * dim + d];
     __bf16 slot[32];
-    exponent_slots(x, coef, 1, false, slot);
+    exponent_slots<32>(x, coef, 0.f, 1, false, slot);
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       bf16x8 v;
@@ -1820,8 +1893,8 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
       const float coef = float(-1.4426950408889634 / (2.0 * op.lengthscale * op.lengthscale));
       u32x4* rec = reinterpret_cast<u32x4*>(scratch);
       if (!(flags & kMvPointsReady)) {
-        hipLaunchKernelGGL(k_prep_points, dim3(unsigned((npad + 255) / 256)), dim3(256), 0, s,
-                           op.points, op.dim, coef, n, npad, rec);
+        hipLaunchKernelGGL(k_prep_points<32>, dim3(unsigned((npad + 255) / 256)), dim3(256), 0,
+                           s, op.points, op.dim, coef, 0.f, n, npad, rec);
         err = hipGetLastError();
         if (err != hipSuccess) return err;
       }
@@ -1907,11 +1980,16 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
   float* out = (S > 1) ? partial : W;
   RBFParams prm{};
   hipError_t err;
+  const float rbf_coef = float(-1.4426950408889634 / (2.0 * op.lengthscale * op.lengthscale));
+  const float rbf_scale = float(sqrt(1.4426950408889634 / (2.0 * op.lengthscale * op.lengthscale)));
   if (op.kind == SLQ_OP_RBF && !fp32 && !(flags & kMvPointsReady)) {
-    hipLaunchKernelGGL(k_prep_points, dim3(unsigned((npad + 255) / 256)), dim3(256), 0, s,
-                       op.points, op.dim,
-                       float(-1.4426950408889634 / (2.0 * op.lengthscale * op.lengthscale)), n,
-                       npad, rec);
+    const dim3 pgrid(unsigned((npad + 255) / 256));
+    if (exponent_slot_count(nprod, false) == 16)
+      hipLaunchKernelGGL(k_prep_points<16>, pgrid, dim3(256), 0, s, op.points, op.dim, rbf_coef,
+                         rbf_scale, n, npad, rec);
+    else
+      hipLaunchKernelGGL(k_prep_points<32>, pgrid, dim3(256), 0, s, op.points, op.dim, rbf_coef,
+                         rbf_scale, n, npad, rec);
     err = hipGetLastError();
     if (err != hipSuccess) return err;
   }
@@ -1927,8 +2005,8 @@ hipError_t launch_matvec(hipStream_t s, const slq_operator& op, const float* V, 
   } else {
     prm.X = op.points;
     prm.dim = op.dim;
-    prm.coef = float(-1.4426950408889634 / (2.0 * op.lengthscale * op.lengthscale));
-    prm.scale = float(sqrt(1.4426950408889634 / (2.0 * op.lengthscale * op.lengthscale)));
+    prm.coef = rbf_coef;
+    prm.scale = rbf_scale;
     prm.noise = float(op.noise);
     if (!fp32) {
       if (op.dim > 3) return hipErrorInvalidValue;  // rejected by the API before launch
@@ -1984,8 +2062,9 @@ hipError_t launch_rbf_dk_contract(hipStream_t s, const float* X, int dim, double
   prm.X = X;
   prm.dim = dim;
   prm.coef = float(-1.4426950408889634 / (2.0 * lengthscale * lengthscale));
-  hipLaunchKernelGGL(k_prep_points, dim3(unsigned((npad + 255) / 256)), dim3(256), 0, s, X, dim,
-                     prm.coef, n, npad, rec);
+  static_assert(exponent_slot_count(3, true) == 32, "derivative mode: full exponent form");
+  hipLaunchKernelGGL(k_prep_points<32>, dim3(unsigned((npad + 255) / 256)), dim3(256), 0, s, X,
+                     dim, prm.coef, 0.f, n, npad, rec);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
   const int64_t gx = (npad / 4 + 255) / 256;
