@@ -543,7 +543,10 @@ __device__ __forceinline__ floatx16 mfma_bf16(const u32x4& a, const u32x4& b, co
 //   18..20: (c |x_m|^2 parts ; 1)     21..23: (1 ; c |x_i|^2 parts)
 // The point side ("L") of each point is precomputed once per matvec as 64
 // bytes (32 in the compact form below); the row side ("R") is built per lane
-// at kernel start.  The MFMA
+// at kernel start.  Both sides take the points relative to point 0 (the RBF
+// kernel is translation invariant; each side computes the same FP32
+// difference for a point), which keeps |x|^2, and with it the rounding of
+// the expanded exponent's sums, small for data far from the origin.  The MFMA
 // output gives lane (r, h) the exponents of its own row i = r at the 16
 // points m = 8 (s / 4) + 4 h + (s % 4), s = 0..15 -- exactly the BF16
 // A-fragment of that lane once the contraction index is ordered that way,
@@ -672,7 +675,7 @@ __global__ void __launch_bounds__(256)
   if (m >= npad) return;
   float x[3] = {0.f, 0.f, 0.f};
   if (m < n)
-    for (int d = 0; d < dim; ++d) x[d] = X[m * dim + d];
+    for (int d = 0; d < dim; ++d) x[d] = X[m * dim + d] - X[d];  // centred on point 0
   __bf16 slot[NS];
   exponent_slots<NS>(x, coef, scale, 0, m >= n, slot);
 #pragma unroll
@@ -1179,7 +1182,7 @@ __device__ __forceinline__ void rbf_bf16_body(const u32x4* __restrict__ rec, con
   {
     float x[3] = {0.f, 0.f, 0.f};
     if (row < n)
-      for (int d = 0; d < dim; ++d) x[d] = prm.X[row * dim + d];
+      for (int d = 0; d < dim; ++d) x[d] = prm.X[row * dim + d] - prm.X[d];
     __bf16 slot[NS];
     exponent_slots<NS>(x, prm.coef, prm.scale, 1, false, slot);
 #pragma unroll
@@ -1488,7 +1491,7 @@ __global__ void __launch_bounds__(kThreads)
   {
     float x[3] = {0.f, 0.f, 0.f};
     if (row < n)
-      for (int d = 0; d < dim; ++d) x[d] = X[row * dim + d];
+      for (int d = 0; d < dim; ++d) x[d] = X[row * dim + d] - X[d];
     __bf16 slot[32];
     exponent_slots<32>(x, coef, 0.f, 1, false, slot);
 #pragma unroll

@@ -14,7 +14,8 @@ positive definite matrix.  `compact_model` restates that arithmetic on the
 host in FP64.  The CPU test checks the claim that its log-determinant is the
 exact one to ~1e-9 relative; the GPU test checks the kernel against the model
 with probes of +-1 (exact in BF16), where only the hi + lo representation of
-the entries (2^-18 relative each) and FP32 accumulation remain.
+the entries (2^-18 relative each) and the exponents' sums remain.  Both sides
+of the kernel take the points relative to point 0, and so does the model.
 """
 
 import numpy as np
@@ -35,7 +36,7 @@ def _bf16(a):
 def compact_model(points, lengthscale, noise):
     """FP64 matrix of the compact exponent form (see the module docstring)."""
     X = np.zeros((len(points), 3), np.float32)
-    X[:, : points.shape[1]] = points
+    X[:, : points.shape[1]] = points - points[0]  # FP32, relative to point 0
     s = np.float32(np.sqrt(LOG2E / (2.0 * lengthscale * lengthscale)))
     y = (s * X).astype(np.float32)
     yh = _bf16(y)
@@ -79,17 +80,17 @@ def _split_entries(K):
 
 
 def test_compact_model_bound_discriminates():
-    # Points in [0, 4]^3 (scaled coordinates up to ~6.8): the GPU test's bound
-    # of 2^-14 (normwise, probes of +-1) sits above what the hi + lo entries
-    # leave (~3e-6) and what the matrix core's sums of the 16 slot products
-    # add at this range (3.5e-5 measured; summing them in sequence in FP32
-    # gives 5e-6 rounding to nearest, 1.6e-5 truncating) and below the
+    # Points in [0, 8]^3 (relative to point 0: squared scaled coordinates up
+    # to ~170): the GPU test's bound of 2^-13 (normwise, probes of +-1) sits
+    # above what the hi + lo entries leave (~3e-6) and what the matrix core's
+    # sums of the 16 slot products add at this range (up to 5.3e-5 measured;
+    # summing them in sequence in FP32 gives several times less) and below the
     # distance of the model to the exact kernel
     # and to the model with a planted mistake (a dropped y_l y_l product, or
     # one-part norms), so a kernel that met the bound against the wrong
     # matrix would fail it.
     n = 700
-    X = (problems.rbf_points(n, seed=6) * 4.0).astype(np.float32)
+    X = (problems.rbf_points(n, seed=6) * 8.0).astype(np.float32)
     K = compact_model(X, 0.5, 1e-2)
     V = np.random.default_rng(1).choice([-1.0, 1.0], size=(8, n))
 
@@ -97,7 +98,7 @@ def test_compact_model_bound_discriminates():
         return np.linalg.norm(V @ M - V @ K) / np.linalg.norm(V @ K)
 
     s = np.float32(np.sqrt(LOG2E / (2.0 * 0.25)))
-    y = (s * X).astype(np.float32)
+    y = (s * (X - X[0]).astype(np.float32)).astype(np.float32)
     yh = _bf16(y)
     yl = _bf16((y - yh).astype(np.float32)).astype(np.float64)
     eye = 1e-2 * np.eye(n)
@@ -106,9 +107,10 @@ def test_compact_model_bound_discriminates():
     N = np.float32(np.sum(yt * yt, axis=1)).astype(np.float64)
     N1 = -_bf16((-N).astype(np.float32)).astype(np.float64)  # one BF16 part
     one_part = np.exp2(2.0 * yt @ yt.T - N1[:, None] - N1[None, :]) + eye
-    assert dist(_split_entries(K)) < 2.0**-14 / 4
-    for wrong in (problems.rbf_dense64(X, 0.5, 1e-2), dropped_ll, one_part):
-        assert dist(wrong) > 2.0**-14 * 1.5
+    assert dist(_split_entries(K)) < 2.0**-13 / 8
+    assert dist(problems.rbf_dense64(X, 0.5, 1e-2)) > 2.0**-13
+    for wrong in (dropped_ll, one_part):
+        assert dist(wrong) > 2.0**-13 * 1.5
 
 
 @pytest.mark.gpu
@@ -116,8 +118,8 @@ def test_compact_model_bound_discriminates():
 @pytest.mark.parametrize("n,P,dim", [(1000, 16, 3), (2053, 128, 3), (777, 40, 2),
                                      (300, 70, 1), (4099, 33, 3)])
 def test_compact_exponent_matvec_matches_model(cuda, precision, n, P, dim):
-    # points in [0, 4]^dim, where the model stands apart from the exact kernel
-    X = (problems.rbf_points(n, seed=n + dim, dim=dim) * 4.0).astype(np.float32)
+    # points in [0, 8]^dim, where the model stands apart from the exact kernel
+    X = (problems.rbf_points(n, seed=n + dim, dim=dim) * 8.0).astype(np.float32)
     V = np.random.default_rng(P).choice([-1.0, 1.0], size=(P, n)).astype(np.float32)
     W = operators.RBF(X, precision=precision)(torch.as_tensor(V, device=cuda), 0.5, 1e-2)
     W = W.cpu().numpy().astype(np.float64)
@@ -130,7 +132,7 @@ def test_compact_exponent_matvec_matches_model(cuda, precision, n, P, dim):
     # bf16x2: hi + lo entries (2^-18 relative each) and the exponents' sums on
     # the matrix core; bf16: entries rounded once (2^-9 relative each)
     if precision == "bf16x2":
-        assert err <= 2.0**-14
-        assert err < 0.5 * err_exact  # the kernel's matrix is the model's
+        assert err <= 2.0**-13
+        assert err < 0.5 * err_exact  # the kernel's matrix is the model's (4-7x closer)
     else:
         assert err <= 2.0**-8
