@@ -225,20 +225,27 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[c][e] = 0.f;
 
+  // points relative to point 0 (translation invariance; both sides form the
+  // same FP32 difference), scaled
+  float x0[4] = {0.f, 0.f, 0.f, 0.f};
   float xr[4] = {0.f, 0.f, 0.f, 0.f};
+  if (RBF && n > 0) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) x0[d] = prm.X[d];
+  }
   if (RBF && row < n) {
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) xr[d] = prm.X[row * DIM + d] * prm.scale;
+    for (int d = 0; d < DIM; ++d) xr[d] = (prm.X[row * DIM + d] - x0[d]) * prm.scale;
   }
   auto load_x = [&](int64_t m0) {
     float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
     const int64_t m = m0 + t;
     if (m < n) {
       const float* src = prm.X + m * DIM;
-      x.x = src[0] * prm.scale;
-      if (DIM > 1) x.y = src[1] * prm.scale;
-      if (DIM > 2) x.z = src[2] * prm.scale;
-      if (DIM > 3) x.w = src[3] * prm.scale;
+      x.x = (src[0] - x0[0]) * prm.scale;
+      if (DIM > 1) x.y = (src[1] - x0[1]) * prm.scale;
+      if (DIM > 2) x.z = (src[2] - x0[2]) * prm.scale;
+      if (DIM > 3) x.w = (src[3] - x0[3]) * prm.scale;
     } else {
       x = make_float4(kFarPoint, kFarPoint, kFarPoint, kFarPoint);
     }
@@ -1375,12 +1382,16 @@ __global__ void __launch_bounds__(256)
   __shared__ float red[3][NP][kRowsRB];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int64_t j0 = n * blockIdx.y / gridDim.y, j1 = n * (blockIdx.y + 1) / gridDim.y;
+  // points relative to point 0, as in the other RBF kernels
+  float x0[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) x0[d] = n > 0 ? X[d] : 0.f;
   float xi[kRowsPerLane][DIM];
 #pragma unroll
   for (int u = 0; u < kRowsPerLane; ++u) {
     const int64_t row = int64_t(blockIdx.x) * kRowsRB + 64 * u + lane;
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) xi[u][d] = row < n ? X[row * DIM + d] * scale : 0.f;
+    for (int d = 0; d < DIM; ++d) xi[u][d] = row < n ? (X[row * DIM + d] - x0[d]) * scale : 0.f;
   }
   float acc[kRowsPerLane][NP];
 #pragma unroll
@@ -1395,7 +1406,7 @@ __global__ void __launch_bounds__(256)
       float c[4] = {0.f, 0.f, 0.f, 0.f};
       if (ok) {
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) c[d] = X[j * DIM + d] * scale;
+        for (int d = 0; d < DIM; ++d) c[d] = (X[j * DIM + d] - x0[d]) * scale;
         if (PACKV) c[3] = V[j];
       }
       xs[t] = make_float4(c[0], c[1], c[2], c[3]);

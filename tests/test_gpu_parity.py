@@ -109,6 +109,22 @@ def test_rbf_matvec_few_vectors_dim4(cuda):
     assert rel(W.cpu().numpy(), V.astype(np.float64) @ K) < 2e-6
 
 
+@pytest.mark.parametrize("precision,P,dim,tol", [
+    ("fp32", 8, 3, 2e-6), ("fp32", 2, 3, 2e-6), ("fp32", 2, 4, 2e-6), ("fp32", 40, 2, 2e-6),
+    ("bf16x2", 16, 3, 4e-3), ("bf16x2", 128, 3, 4e-3), ("bf16x3", 64, 3, 3e-5)])
+def test_rbf_matvec_far_from_origin(cuda, precision, P, dim, tol):
+    # points in [100, 101]^dim: every kernel takes them relative to point 0,
+    # so they keep the accuracy of the unit cube (scaled coordinates of ~170
+    # would lose ~1e-5 of each difference in FP32, and the BF16 kernels'
+    # expanded exponents ~1e-2)
+    n = 1500
+    X = (problems.rbf_points(n, seed=9, dim=dim) + np.float32(100.0)).astype(np.float32)
+    V = np.random.default_rng(5).standard_normal((P, n)).astype(np.float32)
+    W = operators.RBF(X, precision=precision)(torch.as_tensor(V, device=cuda), 0.5, 1e-2)
+    K = problems.rbf_dense64(X, 0.5, 1e-2)
+    assert rel(W.cpu().numpy(), V.astype(np.float64) @ K) < tol
+
+
 @pytest.mark.parametrize("precision,tol", [("bf16x3", 3e-5), ("bf16x2", 4e-3), ("bf16", 6e-3)])
 @pytest.mark.parametrize("n,P,dim", [(1000, 16, 3), (256, 128, 3), (130, 70, 3), (333, 40, 1),
                                      (96, 200, 2)])
